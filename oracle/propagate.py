@@ -1,0 +1,298 @@
+"""Oracle: time integration, Fourier transform and response-function drivers.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+* ``integrate_zvode`` restates qspectra/simulate/utils.py:16-109 literally
+  (scipy ``ode('zvode')`` per output time, serial loop over rows of a 2-D y0).
+  It is the "reference CPU path" of the benchmark.
+* ``integrate_rk4`` is the same driver with a fixed-step classical RK4 in
+  place of zvode (the product's discretisation), vectorised over leading axes.
+* the response drivers restate qspectra/simulate/response.py and the
+  decorators of qspectra/simulate/decorators.py.
+"""
+import numpy as np
+import scipy.integrate
+from scipy.fftpack import fft, fftfreq, fftshift, ifftshift
+
+from .system import (check_polarizations, invariant_weights_4th_order,
+                     invariant_polarizations, FOURTH_ORDER_INVARIANTS)
+
+
+class IntegratorError(Exception):
+    pass
+
+
+# ------------------------------------------------------------- integrators
+def integrate_zvode(f, y0, t, t0=None, save_func=None, **kwargs):
+    """simulate/utils.py:16-50 and :103-109 (rows of 2-D y0 looped serially)."""
+    y0 = np.asarray(y0)
+    if y0.ndim > 1:
+        out = [integrate_zvode(f, yi, t, t0, save_func, **kwargs) for yi in y0]
+        return np.array(out)
+    if t0 is None:
+        t0 = t[0]
+    solver = scipy.integrate.ode(f)
+    solver.set_integrator('zvode', **kwargs)
+    solver.set_initial_value(y0, t0)
+    save = save_func if save_func is not None else (lambda x: x)
+    ys = save(y0)
+    y = np.empty([len(t)] + list(np.shape(ys)), dtype=np.asarray(ys).dtype)
+    i0 = 0
+    if t[0] == t0:
+        y[0] = ys
+        i0 = 1
+    for i in range(i0, len(t)):
+        if not solver.successful():
+            raise IntegratorError('integration failed at time {}'.format(t[i]))
+        y[i] = save(solver.integrate(t[i]))
+    return y
+
+
+def rk4_schedule(t, t0=None, dt=None):
+    """Fixed-step schedule used by the RK4 oracle.
+
+    Interval i runs from the previous output time (t0 for the first) to
+    t[i]; it is split into n_i equal steps of length h_i <= dt (up to a
+    1e-6 relative tolerance).  dt defaults to the smallest nonzero interval.
+    """
+    t = np.asarray(t, float)
+    if t0 is None:
+        t0 = t[0]
+    prev = np.concatenate([[t0], t[:-1]])
+    dts = t - prev
+    if dt is None:
+        pos = dts[dts > 0]
+        dt = pos.min() if pos.size else 1.0
+    k = dts / dt
+    kr = np.round(k)
+    n = np.where(np.abs(k - kr) <= 1e-6 * np.maximum(1.0, k), kr, np.ceil(k)).astype(int)
+    n = np.where(dts == 0, 0, np.maximum(n, 1))
+    h = np.where(n > 0, dts / np.maximum(n, 1), 0.0)
+    return n, h
+
+
+def rk4_step(L, y, h):
+    """Classical RK4 on y' = L y; y has shape [..., M] (L applied on the
+    last axis, L may be dense ndarray or scipy sparse)."""
+    def f(v):
+        return (L @ v.reshape(-1, v.shape[-1]).T).T.reshape(v.shape)
+    k1 = f(y)
+    k2 = f(y + (h / 2) * k1)
+    k3 = f(y + (h / 2) * k2)
+    k4 = f(y + h * k3)
+    return y + (h / 6) * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+def integrate_rk4(L, y0, t, t0=None, save_matrix=None, dt=None):
+    """Fixed-step RK4 counterpart of ``integrate``.
+
+    Output layout follows simulate/utils.py:103-109: for y0 of shape
+    [..., M] the result is [..., len(t), K] (rows first), with K = M or the
+    row count of ``save_matrix`` (the matrix of a linear save_func).
+    """
+    y = np.array(y0, dtype=complex)
+    n, h = rk4_schedule(t, t0, dt)
+    out = []
+    for ni, hi in zip(n, h):
+        for _ in range(ni):
+            y = rk4_step(L, y, hi)
+        out.append(y if save_matrix is None else y @ np.asarray(save_matrix).T)
+    return np.stack(out, axis=-2)
+
+
+# ------------------------------------------------------------ Fourier tools
+def is_constant(x, atol=1e-7, positive=None):
+    """simulate/utils.py:122-125."""
+    x = np.asarray(x)
+    return (np.max(np.abs(x - x[0])) < atol
+            and (np.all((x > 0) == positive) if positive is not None else True))
+
+
+def symmetrize(t, x, axis=-1):
+    """simulate/utils.py:128-151."""
+    if not is_constant(np.diff(t), positive=True):
+        raise ValueError('sample times must differ by a positive constant')
+    t = np.asarray(t)
+    x = np.asarray(x)
+    T = max(t[-1], -t[0])
+    dt = t[1] - t[0]
+    n_plus = int((T - t[-1]) / dt) + 1
+    n_minus = int((T + t[0]) / dt) + 1
+    t_sym = np.concatenate([t[0] - dt * np.arange(1, n_minus)[::-1], t,
+                            t[-1] + dt * np.arange(1, n_plus)])
+    ax = axis % x.ndim
+    shape = list(x.shape)
+    shape[ax] = t_sym.size
+    x_sym = np.zeros(shape, dtype=x.dtype)
+    start, end = (np.searchsorted(t_sym, ti) for ti in (t[0], t[-1]))
+    sl = [slice(None)] * x.ndim
+    sl[ax] = slice(start, end + 1)
+    x_sym[tuple(sl)] = x
+    return t_sym, x_sym
+
+
+def fourier_transform(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1,
+                      convention='angular'):
+    """simulate/utils.py:154-219."""
+    if convention == 'angular':
+        unit_convert = unit_convert / (2 * np.pi)
+    t, x = symmetrize(t, x, axis)
+    N = x.shape[axis]
+    dt = t[1] - t[0]
+    f = fftshift(fftfreq(N, dt * unit_convert))
+    X = fftshift(fft(ifftshift(x * dt, axes=axis), axis=axis), axes=axis)
+    if sign == 1:
+        f = -f[::-1]
+        sl = [slice(None)] * X.ndim
+        sl[axis] = slice(None, None, -1)
+        X = X[tuple(sl)]
+    return f + rw_freq, X
+
+
+# --------------------------------------------------------- response drivers
+def _integrator(kind, dt, zvode_kw):
+    """Returns integrate(L, y0, t, t0, save_matrix) for 'rk4' or 'zvode'."""
+    def run(L, y0, t, t0=None, save_matrix=None):
+        if kind == 'rk4':
+            return integrate_rk4(L, y0, t, t0, save_matrix, dt)
+        save = None if save_matrix is None else (lambda y: np.asarray(save_matrix).dot(y))
+        return integrate_zvode(lambda tt, y: L.dot(y), y0, t, t0, save, **zvode_kw)
+    return run
+
+
+def linear_response(model, path, time_max, polarization='xx', initial_state=None,
+                    integrator='rk4', dt=None, **zvode_kw):
+    """response.py:15-43 (Heisenberg branch; the Liouville models implement it)."""
+    run = _integrator(integrator, dt, zvode_kw)
+    subs = path.split('->')
+    if initial_state is None:
+        initial_state = model.thermal_state(subs[0])
+    t = np.arange(0, time_max, model.time_step)
+    signal = 0
+    for sim in subs[1].split(','):
+        V = [model.dipole_operator('%s->%s' % (a, b), p, tr)
+             for a, b, p, tr in zip(subs[:-1], subs[1:], polarization, '+-')]
+        V_rho0 = np.apply_along_axis(V[0].commutator, -1, initial_state)
+        L = model.evolve_matrix(sim, heisenberg_picture=True)
+        V_Gt = run(L, -V[1].bra_vector, t)
+        signal = signal + np.tensordot(V_rho0, V_Gt, (-1, -1))
+    return t, signal
+
+
+def absorption_spectra(model, time_max, correlation_decay_time=None,
+                       polarization='xx', **kw):
+    """response.py:103-154 (returns Re FT of -S)."""
+    t, x = linear_response(model, 'gg->eg->gg', time_max, polarization, **kw)
+    if correlation_decay_time is not None:
+        x = x * np.exp(-t / correlation_decay_time)
+    f, X = fourier_transform(t, -x, rw_freq=model.rw_freq, unit_convert=model.unit_convert)
+    return f, X.real
+
+
+THIRD_ORDER_PATHWAYS = {
+    '-++': {'ESE': 'gg->ge->ee->eg->gg', 'GSB': 'gg->ge->gg->eg->gg',
+            'ESA': 'gg->ge->ee->fe->ee'},
+    '+-+': {'ESE': 'gg->eg->ee->eg->gg', 'GSB': 'gg->eg->gg->eg->gg',
+            'ESA': 'gg->eg->ee->fe->ee'},
+    '++-': {'ESA1': 'gg->eg->fg->fe->ee', 'ESA2': 'gg->eg->fg->eg->gg'},
+}
+"""response.py:252-264 (Abramavicius et al., Chem. Rev. 109, 2350)."""
+
+
+def parse_pathways(possible, include_signal):
+    """response.py:162-171 (dict order of the table)."""
+    sel = [v for k, v in possible.items() if include_signal is None or k in include_signal]
+    if not sel:
+        raise ValueError('at least one Liouville space pathway must be selected')
+    return sel
+
+
+def third_order_response(model, coherence_time_max, population_time_max=None,
+                         population_times=None, geometry='-++', polarization='xxxx',
+                         include_signal=None, integrator='rk4', dt=None, **zvode_kw):
+    """response.py:269-337 (Schroedinger t1,t2 + Heisenberg t3)."""
+    run = _integrator(integrator, dt, zvode_kw)
+    dt = model.time_step
+    t1 = np.arange(0, coherence_time_max, dt)
+    t2 = (np.arange(0, population_time_max, dt) if population_times is None
+          else np.asarray(population_times))
+    t3 = np.arange(0, coherence_time_max, dt)
+    rho0 = model.thermal_state('gg')
+    total = 0
+    for path in parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal):
+        subs = path.split('->')
+        V = [model.dipole_operator('%s->%s' % (a, b), p, tr)
+             for a, b, p, tr in zip(subs[:-1], subs[1:], polarization, geometry + '-')]
+        L = [model.evolve_matrix(s) for s in subs[1:-1]]
+        V_rho0 = V[0].commutator(rho0)
+        V_rho1 = run(L[0], V_rho0, t1, save_matrix=V[1].commutator_matrix())
+        V_rho2 = run(L[1], V_rho1, t2, t0=0, save_matrix=V[2].commutator_matrix())
+        Lh = model.evolve_matrix(subs[3], heisenberg_picture=True)
+        V_Gt3 = run(Lh, V[3].bra_vector, t3)
+        total = total + np.einsum('ci,abi', V_Gt3, V_rho2)
+    return (t1, t2, t3), total
+
+
+def two_dimensional_spectra(model, coherence_time_max, population_time_max=None,
+                            population_times=None, geometry='-++', polarization='xxxx',
+                            include_signal=None, **kw):
+    """response.py:430-455."""
+    (t1, t2, t3), X = third_order_response(model, coherence_time_max, population_time_max,
+                                           population_times, geometry, polarization,
+                                           include_signal, **kw)
+    f1, X = fourier_transform(t1, X, 0, rw_freq=model.rw_freq, sign=-1,
+                              unit_convert=model.unit_convert)
+    f3, X = fourier_transform(t3, X, 2, rw_freq=model.rw_freq,
+                              unit_convert=model.unit_convert)
+    return (f1, t2, f3), X
+
+
+def simulate_dynamics(model, initial_state, duration=None, times=None,
+                      liouville_subspace='ee', integrator='rk4', dt=None, **zvode_kw):
+    """simulate/eom.py:11-26 (psi -> outer(conj psi, psi), then reshape 'F')."""
+    run = _integrator(integrator, dt, zvode_kw)
+    L = model.evolve_matrix(liouville_subspace)
+    rho = np.asarray(initial_state)
+    if rho.ndim == 1:
+        rho = np.outer(rho.conj(), rho)
+    y0 = model.density_matrix_to_state_vector(rho, liouville_subspace)
+    t = np.arange(0, duration, model.time_step) if times is None else times
+    states = run(L, y0, t)
+    N = int(np.sqrt(states.shape[-1]))
+    return t, states.reshape(-1, N, N, order='F')
+
+
+# ------------------------------------------------------------------ decorators
+def ensemble_average(func, model, ensemble_size, *args, **kwargs):
+    """decorators.py:40-64: serial mean over model.sample(n), n < size."""
+    total = 0
+    for n in range(ensemble_size):
+        ticks, sig = func(model.sample(n), *args, **kwargs)
+        total = total + sig
+    return ticks, total / ensemble_size
+
+
+def isotropic_average_2nd(func, polarization, **kwargs):
+    """decorators.py:99-125."""
+    p = check_polarizations(polarization, 2)
+    w = np.dot(*p)
+    total = 0
+    for pol in ['xx', 'yy', 'zz']:
+        t, s = func(polarization=pol, **kwargs)
+        total = total + s
+    return t, total * (w / 3.0)
+
+
+def isotropic_average_4th(func, polarization, **kwargs):
+    """decorators.py:67-96 (weights <= 1e-8 skipped, signals cached)."""
+    weights = invariant_weights_4th_order(polarization)
+    cache = {}
+    total = 0
+    t = None
+    for inv, w in zip(FOURTH_ORDER_INVARIANTS, weights):
+        if w > 1e-8:
+            for p in invariant_polarizations(inv):
+                if p not in cache:
+                    t, cache[p] = func(polarization=p, **kwargs)
+                total = total + w * cache[p]
+    return t, total

@@ -1,0 +1,223 @@
+"""Writes the golden fixtures under tests/golden/ by running the reference.
+
+Run only in the build container, where the reference sources are present:
+
+    python tests/golden/make_golden.py /root/reference
+
+It imports qspectra from the given path and stores plain arrays (inputs and
+outputs) as .npz files.  Nothing here is imported by the tests or by the
+product; the fixtures are data.  Only outputs that no closed form or
+known-answer test pins are written:
+
+* Redfield Liouvillian blocks (non-secular, complex C(w), multi-site),
+* HEOM sparse Liouvillians (ADO order, low-temperature correction),
+* zvode-integrated trajectories and end-to-end spectra (absorption,
+  third-order response, 2-D spectra, isotropic and ensemble averages),
+* disorder samples and the complex bath correlation function.
+"""
+import os
+import sys
+import warnings
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TIGHT = dict(rtol=1e-12, atol=1e-14, nsteps=10 ** 7)
+
+
+def dimer_inputs():
+    th = np.deg2rad(30.0)
+    return dict(H1=np.array([[12881., 120.], [120., 12719.]]),
+                dipoles=np.array([[1., 0, 0], [np.cos(th), np.sin(th), 0]]),
+                T=0.69503476 * 77, lam=35., gamma=106.)
+
+
+def aggregate7_inputs(seed=1):
+    rs = np.random.RandomState(seed)
+    E = rs.uniform(12200, 12700, 7)
+    J = rs.normal(0, 60, (7, 7))
+    H1 = np.triu(J, 1)
+    H1 = H1 + H1.T + np.diag(E)
+    d = np.array([[3.019, 3.442, 0.797, 3.213, 2.969, 0.547, 1.983],
+                  [2.284, -2.023, -3.871, 2.145, -2.642, 3.562, 2.837],
+                  [1.506, 0.431, 0.853, 1.112, -0.661, -1.851, 2.015]]).T
+    d = d / np.linalg.norm(d, axis=1, keepdims=True)
+    return dict(H1=H1, dipoles=d, T=0.69503476 * 77, lam=35., gamma=106.)
+
+
+def extra_for_dt(qs, H1, dt):
+    """energy_spread_extra giving the model time step ``dt`` (fs)."""
+    ham = qs.ElectronicHamiltonian(H1).in_rotating_frame()
+    E = ham.E('gef')
+    return 1.0 / (2 * dt * qs.CM_FS) - max(E.max(), -E.min())
+
+
+def main(ref_path):
+    sys.path.insert(0, ref_path)
+    import qspectra as qs
+    from qspectra.dynamics.redfield import redfield_tensor
+    from qspectra.dynamics.heom import HEOMModel
+    from qspectra.simulate.utils import integrate
+    warnings.simplefilter('ignore')
+
+    def ham_of(inp, extra=None, disorder=None, seed=0):
+        return qs.ElectronicHamiltonian(
+            inp['H1'], bath=qs.DebyeBath(inp['T'], inp['lam'], inp['gamma']),
+            dipoles=inp['dipoles'], disorder=disorder, random_seed=seed,
+            energy_spread_extra=extra)
+
+    # ---------------------------------------------------------------- bath
+    b = qs.DebyeBath(0.69503476 * 77, 35., 106.)
+    x = np.concatenate([np.linspace(-900, 900, 61), [0.0, 1e-3, -1e-3]])
+    b300 = qs.DebyeBath(0.69503476 * 300, 35., 106.)
+    np.savez_compressed(os.path.join(HERE, 'bath.npz'), x=x,
+                        c77=np.array([b.corr_func_complex(v) for v in x]),
+                        r77=np.array([b.corr_func_real(v) for v in x]),
+                        c300=np.array([b300.corr_func_complex(v) for v in x]))
+
+    # ------------------------------------------------- Redfield dimer blocks
+    dim = dimer_inputs()
+    extra = extra_for_dt(qs, dim['H1'], 1.0)
+    ham = ham_of(dim, extra)
+    out = dict(extra=extra, **{'in_' + k: np.asarray(v) for k, v in dim.items()})
+    subspaces = ['gg', 'eg', 'ge', 'ee', 'fg', 'fe', 'gg,ge,eg,ee']
+    for sec in (True, False):
+        for dic in (True, False):
+            m = qs.RedfieldModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS,
+                                 secular=sec, discard_imag_corr=dic)
+            for s in subspaces:
+                idx = m.liouville_subspace_index(s)
+                key = 'L_%s_%d%d' % (s.replace(',', '_'), sec, dic)
+                out[key] = m.evolution_super_operator[np.ix_(idx, idx)]
+    m = qs.RedfieldModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS,
+                         secular=False, evolve_basis='eigen')
+    out['Leig_ee'] = m.evolution_super_operator[np.ix_(*[m.liouville_subspace_index('ee')] * 2)]
+    out['E_gef'], out['U_gef'] = m.hamiltonian.E('gef'), m.hamiltonian.U('gef')
+    out['rw_freq'] = m.rw_freq
+    out['time_step'] = m.time_step
+    out['R_ge'] = redfield_tensor(m.hamiltonian, 'ge', secular=False)
+    np.savez_compressed(os.path.join(HERE, 'redfield_dimer.npz'), **out)
+
+    # ---------------------------------------------- Redfield 7-site blocks
+    agg = aggregate7_inputs()
+    extra7 = extra_for_dt(qs, agg['H1'], 2.0)
+    ham7 = ham_of(agg, extra7, disorder=100 / qs.GAUSSIAN_SD_FWHM)
+    out = dict(extra=extra7, **{'in_' + k: np.asarray(v) for k, v in agg.items()})
+    m7 = qs.RedfieldModel(ham7, hilbert_subspace='gef', unit_convert=qs.CM_FS,
+                          secular=False)
+    for s in ['ge', 'eg', 'ee', 'fe', 'fg']:
+        idx = m7.liouville_subspace_index(s)
+        out['L_' + s] = m7.evolution_super_operator[np.ix_(idx, idx)]
+    m7s = qs.RedfieldModel(ham7, hilbert_subspace='gef', unit_convert=qs.CM_FS,
+                           secular=True)
+    idx = m7s.liouville_subspace_index('ee')
+    out['Lsec_ee'] = m7s.evolution_super_operator[np.ix_(idx, idx)]
+    out['rw_freq'] = m7.rw_freq
+    out['time_step'] = m7.time_step
+    for n in range(4):
+        out['sample_H1_%d' % n] = ham7.sample(n).H_1exc
+    hs = ham7.in_rotating_frame().sample(2)
+    out['sample2_rot_E_gef'] = hs.E('gef')
+    out['sample2_time_step'] = hs.time_step
+    np.savez_compressed(os.path.join(HERE, 'redfield_7site.npz'), **out)
+
+    # ------------------------------------------------------ spectra (zvode)
+    out = {}
+    mono = qs.ElectronicHamiltonian(np.array([[12500.]]), bath=qs.DebyeBath(qs.CM_K * 77, 35, 106),
+                                    dipoles=[[1., 0, 0]])
+    for dic in (True, False):
+        mm = qs.RedfieldModel(mono, discard_imag_corr=dic, unit_convert=qs.CM_FS)
+        f, X = qs.absorption_spectra(mm, 2000, **TIGHT)
+        out['mono_f_%d' % dic], out['mono_X_%d' % dic] = f, X
+        out['mono_dt_%d' % dic] = mm.time_step
+    md = qs.RedfieldModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS)
+    t, S = qs.linear_response(md, 'gg->eg->gg', 511.5, **TIGHT)
+    out['dimer_t'], out['dimer_S'] = t, S
+    f, X = qs.absorption_spectra(md, 511.5, **TIGHT)
+    out['dimer_f'], out['dimer_X'] = f, X
+    f, X = qs.absorption_spectra(md, 511.5)
+    out['dimer_X_default_tol'] = X
+    f, X = qs.absorption_spectra(md, 511.5, exact_isotropic_average=True, **TIGHT)
+    out['dimer_X_iso'] = X
+    f, X = qs.absorption_spectra(m7, 2 * 255.5, **TIGHT)
+    out['agg7_f'], out['agg7_X'] = f, X
+    f, X = qs.absorption_spectra(m7, 2 * 127.5, ensemble_size=8, **TIGHT)
+    out['agg7_ens8_f'], out['agg7_ens8_X'] = f, X
+    f, X = qs.absorption_spectra(m7, 2 * 127.5, correlation_decay_time=300.0,
+                                 polarization=[0.3, 1.1], **TIGHT)
+    out['agg7_decay_X'] = X
+    np.savez_compressed(os.path.join(HERE, 'spectra_linear.npz'), **out)
+
+    # --------------------------------------- third-order (reduced 7-site grid)
+    out = {}
+    t2s = np.array([0., 20., 40., 100.])
+    for geom in ('-++', '+-+', '++-'):
+        (t1, t2, t3), X = qs.third_order_response(m7, 2 * 15.5, population_times=t2s,
+                                                  geometry=geom, **TIGHT)
+        g = geom.replace('-', 'm').replace('+', 'p')
+        out['R3_' + g] = X
+        out['t1'], out['t2'], out['t3'] = t1, t2, t3
+    (f1, t2, f3), X = qs.two_dimensional_spectra(m7, 2 * 15.5, population_times=t2s,
+                                                 include_signal='GSB,ESE', **TIGHT)
+    out['S2d_f1'], out['S2d_f3'], out['S2d'] = f1, f3, X
+    (t1, t2, t3), X = qs.third_order_response(md, 8.5, population_times=[0., 3., 7.],
+                                              exact_isotropic_average=True,
+                                              polarization=[0, 0, qs.MAGIC_ANGLE, qs.MAGIC_ANGLE],
+                                              **TIGHT)
+    out['dimer_iso_R3'] = X
+    np.savez_compressed(os.path.join(HERE, 'spectra_third_order.npz'), **out)
+
+    # ------------------------------------------------------------------ HEOM
+    out = {}
+    dim300 = dict(dim, T=0.69503476 * 300)
+    hamh = ham_of(dim300, extra)
+    for name, kw in [('dimer', dict(level_cutoff=3, K=1)),
+                     ('dimer_mod', dict(level_cutoff=3, K=1, modified_HEOM=True)),
+                     ('dimer_nolt', dict(level_cutoff=3, K=2, low_temp_corr=False))]:
+        hm = HEOMModel(hamh, hilbert_subspace='ge', unit_convert=qs.CM_FS, **kw)
+        for s in ('ee', 'gg,ge,eg,ee', 'eg'):
+            L = (qs.CM_FS * hm.HEOM_tensor(s)).tocsr()
+            key = '%s_%s' % (name, s.replace(',', '_'))
+            out[key + '_data'], out[key + '_indices'], out[key + '_indptr'] = \
+                L.data, L.indices, L.indptr
+            out[key + '_shape'] = np.array(L.shape)
+        out[name + '_ados'] = np.array([a.flatten() for a in hm.ado_indices])
+    agg300 = dict(agg, T=0.69503476 * 300)
+    ham7h = ham_of(agg300, extra7)
+    hm7 = HEOMModel(ham7h, hilbert_subspace='ge', unit_convert=qs.CM_FS, level_cutoff=3, K=1)
+    L = (qs.CM_FS * hm7.HEOM_tensor('ee')).tocsr()
+    out['agg7_ee_data'], out['agg7_ee_indices'], out['agg7_ee_indptr'] = L.data, L.indices, L.indptr
+    out['agg7_ee_shape'] = np.array(L.shape)
+    out['agg7_ados'] = np.array([a.flatten() for a in hm7.ado_indices])
+    out['agg7_rw_freq'] = hm7.rw_freq
+    hm = HEOMModel(hamh, hilbert_subspace='ge', unit_convert=qs.CM_FS, level_cutoff=3, K=1)
+    rho0 = np.zeros((2, 2))
+    rho0[0, 0] = 1
+    y0 = hm.density_matrix_to_state_vector(rho0, 'ee')
+    t = np.arange(0, 200, hm.time_step)
+    y = integrate(hm.equation_of_motion('ee'), y0, t, **TIGHT)
+    out['dyn_t'], out['dyn_y'] = t, y
+    out['dyn_time_step'] = hm.time_step
+    np.savez_compressed(os.path.join(HERE, 'heom.npz'), **out)
+
+    # ------------------------------------------------ Redfield dynamics (zvode)
+    out = {}
+    t, rho = qs.simulate_dynamics(m7, [1, 0, 0, 0, 0, 0, 0], 200, **TIGHT)
+    out['t'], out['rho'] = t, rho
+    mu = qs.UnitaryModel(ham7, hilbert_subspace='gef', unit_convert=qs.CM_FS)
+    t, rho = qs.simulate_dynamics(mu, [0, 1, 0, 0, 0, 0, 0], 100, **TIGHT)
+    out['unitary_t'], out['unitary_rho'] = t, rho
+    np.savez_compressed(os.path.join(HERE, 'dynamics.npz'), **out)
+
+    with open(os.path.join(HERE, 'MANIFEST.txt'), 'w') as fh:
+        fh.write('written by tests/golden/make_golden.py from the reference at %s\n' % ref_path)
+        fh.write('numpy %s scipy %s python %s\n' % (np.__version__, __import__('scipy').__version__,
+                                                  sys.version.split()[0]))
+        fh.write('tight zvode: %r\n' % (TIGHT,))
+        for fn in sorted(os.listdir(HERE)):
+            if fn.endswith('.npz'):
+                fh.write('%s %d bytes\n' % (fn, os.path.getsize(os.path.join(HERE, fn))))
+
+
+if __name__ == '__main__':
+    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
