@@ -1,0 +1,97 @@
+/*
+ * qsx.h - C ABI of libqsx.so, the MI355X (gfx950) density-matrix propagation
+ * library behind qspectra_amd.
+ *
+ * The reference (qspectra) has no native boundary: its hot path is the Python
+ * operator API
+ *   DynamicalModel.equation_of_motion(subspace, heisenberg_picture) -> eom(t, y)
+ *       qspectra/dynamics/base.py:72-84, liouville_space.py:316-341,
+ *       heom.py:228-244
+ *   integrate(f, y0, t, t0, method_name, f_params, save_func, **kwargs)
+ *       qspectra/simulate/utils.py:16-109
+ *   RedfieldModel.evolution_super_operator (redfield.py:9-152),
+ *   Hamiltonian.eig (hamiltonian.py:310-328).
+ * The entry points below replace the arithmetic under that API; the Python
+ * package qspectra_amd binds them with ctypes.
+ *
+ * Conventions
+ *   - every function returns int32 status: QSX_OK or a negative QSX_E* code;
+ *     qsx_last_error() returns the message of the calling thread's last error;
+ *   - pointers are device pointers unless stated; complex numbers are
+ *     interleaved (re, im) pairs, double2 for complex128 and float2 for
+ *     complex64, the layout of numpy/torch complex arrays;
+ *   - all matrices are row-major; batch strides are in elements;
+ *   - work is enqueued on `stream` (a hipStream_t, NULL = default stream);
+ *     nothing synchronises the host and nothing allocates device memory
+ *     unless the comment says so;
+ *   - sizes are validated on the host before any launch.
+ */
+#ifndef QSX_H
+#define QSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QSX_OK 0
+#define QSX_EINVAL (-1)      /* bad argument (size, null pointer, unsupported mode) */
+#define QSX_EHIP (-2)        /* a HIP runtime call failed */
+
+#define QSX_FP64 0           /* complex128 Liouvillian products, FP64 state */
+#define QSX_MIXED 1          /* complex64 products (FP32 accumulate), FP64 state */
+
+/* Library version as 10000*major + 100*minor + patch. */
+int32_t qsx_version(void);
+
+/* Copies the calling thread's last error message (NUL-terminated) into buf. */
+int32_t qsx_last_error(char* buf, size_t len);
+
+/*
+ * out[b][r][:] = L_b . y[b][r][:]  (complex128).
+ * Replaces the eom(t, y) closure of LiouvilleSpaceModel.equation_of_motion
+ * (qspectra/dynamics/liouville_space.py:339-340: evolve_matrix.dot(rho)).
+ *   L    [B][M][M] with batch stride strideL (0: one L shared by the batch)
+ *   y    [B][R][M] contiguous, out [B][R][M] contiguous.
+ */
+int32_t qsx_apply_dense_c128(const void* L, int64_t strideL, int32_t M,
+                             const void* y, int32_t R, int32_t B, void* out,
+                             void* stream);
+
+/*
+ * Fused fixed-step classical RK4 propagation of R right-hand sides per
+ * problem, for B problems, with a linear projection at every save point.
+ * Replaces integrate(eom, y0, t, t0, save_func=V.commutator / bra, ...)
+ * (qspectra/simulate/utils.py:16-109; the per-row loop of :103-109 becomes
+ * the R axis, the ensemble loop of simulate/decorators.py:51-63 the B axis).
+ *
+ *   precision  QSX_FP64: L, proj complex128.
+ *              QSX_MIXED: L, proj complex64; the diagonal of L is taken in
+ *              FP64 from `diag` (complex128 [B][M], batch stride strideD) and
+ *              the state and RK update stay complex128.
+ *   L        [B][M][M], batch stride strideL (0 = shared)
+ *   y0       [B][R][M] complex128, contiguous
+ *   nsub,h   [nsave] int32 / double (device): save interval s is covered by
+ *            nsub[s] RK4 steps of size h[s]; nsub[s] = 0 saves the current
+ *            state again (the reference's y[0] = save(y0) when t[0] == t0).
+ *   proj     [B][K][M] (stride strideP, 0 = shared) or NULL for the identity
+ *            (then K must equal M)
+ *   out      [B][R][nsave][K] complex128 (rows first, as utils.py:103-109)
+ *   y_final  optional [B][R][M] complex128 final state (may be NULL)
+ *   status   device int32[1]; set to 1 if a saved state is not finite
+ *            (the host raises IntegratorError, simulate/utils.py:46-49)
+ */
+int32_t qsx_rk4_dense(int32_t precision, const void* L, int64_t strideL,
+                      const void* diag, int64_t strideD, int32_t M,
+                      const void* y0, int32_t R, int32_t B,
+                      const int32_t* nsub, const double* h, int32_t nsave,
+                      const void* proj, int64_t strideP, int32_t K,
+                      void* out, void* y_final, int32_t* status, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSX_H */

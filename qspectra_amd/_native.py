@@ -1,0 +1,157 @@
+"""ctypes binding of libqsx.so (include/qsx.h) and device-tensor plumbing.
+
+torch is imported first so that the HIP runtime it loads is the one libqsx
+binds to (both use the soname libamdhip64.so.7); device buffers are torch
+tensors and their ``data_ptr()`` values cross the C ABI.
+
+There is no CPU fallback: every compute entry point raises
+``NativeUnavailable`` when the library or a HIP device is missing.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libqsx.so')
+
+QSX_FP64 = 0
+QSX_MIXED = 1
+
+
+class NativeUnavailable(RuntimeError):
+    """libqsx.so or the HIP device it needs is not available."""
+
+
+class NativeError(RuntimeError):
+    """A libqsx entry point returned an error status."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
+                                   ctypes.c_void_p, ctypes.c_size_t)
+
+# name -> argtypes (restype is always int32)
+SIGNATURES = {
+    'qsx_version': [],
+    'qsx_last_error': [ctypes.c_char_p, c_sz],
+    'qsx_apply_dense_c128': [c_vp, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp],
+    'qsx_rk4_dense': [c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32, c_i32,
+                      c_vp, c_vp, c_i32, c_vp, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp],
+}
+
+
+def load_library(path=LIB_PATH):
+    """Loads libqsx.so (no device work); raises NativeUnavailable if absent."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise NativeUnavailable(
+                    '%s is not built; run __graft_entry__.build() or '
+                    'python -m qspectra_amd.csrc.build' % path)
+            lib = ctypes.CDLL(path)
+            for name, argtypes in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = argtypes
+                fn.restype = c_i32
+            _lib = lib
+    return _lib
+
+
+def require_native():
+    """Returns the library; raises NativeUnavailable without a HIP device."""
+    lib = load_library()
+    if not torch.cuda.is_available():
+        raise NativeUnavailable('no HIP device visible: qspectra_amd has no CPU path')
+    return lib
+
+
+def device():
+    require_native()
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(512)
+    load_library().qsx_last_error(buf, 512)
+    return buf.value.decode(errors='replace')
+
+
+def check(status, what):
+    if status != 0:
+        raise NativeError('%s failed (status %d): %s' % (what, status, last_error()))
+
+
+def stream_ptr(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def to_device(x, dtype=torch.complex128, dev=None):
+    """numpy/array-like/tensor -> contiguous device tensor of ``dtype``."""
+    dev = device() if dev is None else dev
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x)).to(device=dev, dtype=dtype).contiguous()
+
+
+# --------------------------------------------------------------- wrappers
+def apply_dense(L, y, stream=None):
+    """out[b, r] = L[b] @ y[b, r]  (complex128; L [B|1, M, M], y [B, R, M])."""
+    lib = require_native()
+    B, R, M = y.shape
+    assert L.dtype == torch.complex128 and y.dtype == torch.complex128
+    assert L.shape[-2:] == (M, M) and L.is_contiguous() and y.is_contiguous()
+    strideL = 0 if L.shape[0] == 1 else M * M
+    assert L.shape[0] in (1, B)
+    out = torch.empty_like(y)
+    check(lib.qsx_apply_dense_c128(ptr(L), strideL, M, ptr(y), R, B, ptr(out),
+                                   stream_ptr(stream)), 'qsx_apply_dense_c128')
+    return out
+
+
+def rk4_dense(L, y0, nsub, h, proj=None, precision=QSX_FP64, diag=None,
+              want_final=False, stream=None):
+    """Fused RK4 with saves; see include/qsx.h qsx_rk4_dense.
+
+    L     [B|1, M, M] complex128 (FP64) or complex64 off-diagonal (MIXED)
+    diag  [B|1, M] complex128 diagonal (MIXED only)
+    y0    [B, R, M] complex128
+    nsub  int32 [nsave], h float64 [nsave] (device tensors)
+    proj  [B|1, K, M] (dtype of L) or None
+    Returns (out [B, R, nsave, K], y_final or None, status tensor).
+    """
+    lib = require_native()
+    B, R, M = y0.shape
+    nsave = int(nsub.shape[0])
+    ldt = torch.complex128 if precision == QSX_FP64 else torch.complex64
+    assert L.dtype == ldt and L.is_contiguous() and L.shape[-2:] == (M, M)
+    assert y0.dtype == torch.complex128 and y0.is_contiguous()
+    assert nsub.dtype == torch.int32 and h.dtype == torch.float64
+    strideL = 0 if L.shape[0] == 1 else M * M
+    if proj is not None:
+        assert proj.dtype == ldt and proj.is_contiguous() and proj.shape[-1] == M
+        K = proj.shape[-2]
+        strideP = 0 if proj.shape[0] == 1 else K * M
+    else:
+        K, strideP = M, 0
+    strideD = 0
+    if precision == QSX_MIXED:
+        assert diag is not None and diag.dtype == torch.complex128 and diag.is_contiguous()
+        strideD = 0 if diag.shape[0] == 1 else M
+    out = torch.empty((B, R, nsave, K), dtype=torch.complex128, device=y0.device)
+    yfin = torch.empty_like(y0) if want_final else None
+    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    check(lib.qsx_rk4_dense(precision, ptr(L), strideL, ptr(diag), strideD, M, ptr(y0), R, B,
+                            ptr(nsub), ptr(h), nsave, ptr(proj), strideP, K, ptr(out),
+                            ptr(yfin), ptr(status), stream_ptr(stream)), 'qsx_rk4_dense')
+    return out, yfin, status

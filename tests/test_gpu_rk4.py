@@ -1,0 +1,146 @@
+"""Parity of the fused RK4 kernel (qsx_rk4_dense) and the Liouvillian apply
+kernel with the oracle, called through the C ABI.
+
+Liouvillians and dipole maps come from the oracle (which reproduces the
+reference's to 1e-12, tests/test_oracle_liouville.py); the kernels are
+checked against the oracle's fixed-step RK4 on the same schedule.
+FP64 mode: 1e-12 relative (max-norm over the whole output).  MIXED mode:
+2e-6 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import system as S
+from oracle import liouville as Lv
+from oracle import propagate as P
+from tests.conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _agg7_model(secular=False):
+    g = golden('redfield_7site')
+    ham = S.ElectronicHamiltonian(g['in_H1'], bath=S.DebyeBath(S.CM_K * 77, 35, 106),
+                                  dipoles=g['in_dipoles'], energy_spread_extra=float(g['extra']))
+    return Lv.LiouvilleModel(ham, unit_convert=S.CM_FS, secular=secular)
+
+
+def _dimer_model():
+    g = golden('redfield_dimer')
+    ham = S.ElectronicHamiltonian(g['in_H1'], bath=S.DebyeBath(S.CM_K * 77, 35, 106),
+                                  dipoles=g['in_dipoles'], energy_spread_extra=float(g['extra']))
+    return Lv.LiouvilleModel(ham, unit_convert=S.CM_FS)
+
+
+def _run(L, y0, t, t0=None, proj=None, dt=None, mixed=False, B=None):
+    from qspectra_amd import _native, engine
+    dev = _native.device()
+    nsub, h = engine.schedule_tensors(t, t0, dt, dev)
+    L = np.asarray(L)
+    if L.ndim == 2:
+        L = L[None]
+    y0 = np.asarray(y0, complex)
+    if y0.ndim == 1:
+        y0 = y0[None, None]
+    elif y0.ndim == 2:
+        y0 = y0[None]
+    if mixed:
+        d = np.ascontiguousarray(np.diagonal(L, axis1=1, axis2=2))
+        Loff = L.copy()
+        idx = np.arange(L.shape[-1])
+        Loff[:, idx, idx] = 0
+        Ld = _native.to_device(Loff, torch.complex64)
+        dd = _native.to_device(d, torch.complex128)
+        pj = None if proj is None else _native.to_device(np.asarray(proj)[None] if np.ndim(proj) == 2
+                                                         else proj, torch.complex64)
+        out, _, st = _native.rk4_dense(Ld, _native.to_device(y0), nsub, h, pj,
+                                       _native.QSX_MIXED, dd)
+    else:
+        pj = None if proj is None else _native.to_device(np.asarray(proj)[None] if np.ndim(proj) == 2
+                                                         else proj)
+        out, _, st = _native.rk4_dense(_native.to_device(L), _native.to_device(y0), nsub, h, pj)
+    torch.cuda.synchronize()
+    assert int(st.item()) == 0
+    return out.cpu().numpy()
+
+
+def test_apply_dense_matches_numpy(gpu):
+    from qspectra_amd import _native
+    rs = np.random.RandomState(0)
+    for B, R, M in [(1, 1, 2), (3, 5, 49), (2, 1, 147)]:
+        L = rs.randn(B, M, M) + 1j * rs.randn(B, M, M)
+        y = rs.randn(B, R, M) + 1j * rs.randn(B, R, M)
+        out = _native.apply_dense(_native.to_device(L), _native.to_device(y)).cpu().numpy()
+        assert rel_err(out, np.einsum('bij,brj->bri', L, y)) < 1e-14
+    out = _native.apply_dense(_native.to_device(L[:1]), _native.to_device(y)).cpu().numpy()
+    assert rel_err(out, np.einsum('ij,brj->bri', L[0], y)) < 1e-14
+
+
+def test_dimer_absorption_trajectory_fp64(gpu):
+    m = _dimer_model()
+    Lh = m.evolve_matrix('eg', heisenberg_picture=True)
+    V = m.dipole_operator('eg->gg', 'x', '-')
+    y0 = -V.bra_vector
+    t = np.arange(0, 2048, m.time_step)
+    ref = P.integrate_rk4(Lh, y0, t)
+    got = _run(Lh, y0, t)[0, 0]
+    assert got.shape == ref.shape == (2048, 2)
+    assert rel_err(got, ref) < 1e-12
+    got = _run(Lh, y0, t, mixed=True)[0, 0]
+    assert rel_err(got, ref) < 2e-6
+
+
+def test_population_stage_multi_rhs_with_projection(gpu):
+    """t2 stage of the 7-site third-order response: 128 RHS share the 'ee'
+    Liouvillian; saves project with the 'ee->eg' / 'ee->fe' commutators."""
+    m = _agg7_model()
+    L = m.evolve_matrix('ee')
+    rs = np.random.RandomState(1)
+    y0 = rs.randn(128, 49) + 1j * rs.randn(128, 49)
+    t2 = np.array([0., 100., 200., 500.])
+    for sub in ('eg', 'fe'):
+        V = m.dipole_operator('ee->' + sub, 'x', '+')
+        Pm = V.commutator_matrix()
+        ref = P.integrate_rk4(L, y0, t2, t0=0, save_matrix=Pm, dt=2.0)
+        got = _run(L, y0, t2, t0=0, proj=Pm, dt=2.0)[0]
+        assert got.shape == ref.shape
+        assert rel_err(got, ref) < 1e-12, sub
+        got = _run(L, y0, t2, t0=0, proj=Pm, dt=2.0, mixed=True)[0]
+        assert rel_err(got, ref) < 2e-6, sub
+
+
+def test_large_block_from_global_memory(gpu):
+    """'fe' Heisenberg block (M = 147) does not fit LDS: read from L2/HBM."""
+    m = _agg7_model()
+    Lh = m.evolve_matrix('fe', heisenberg_picture=True)
+    y0 = m.dipole_operator('fe->ee', 'x', '-').bra_vector
+    t = np.arange(0, 254.5, 2.0)
+    ref = P.integrate_rk4(Lh, y0, t)
+    assert rel_err(_run(Lh, y0, t)[0, 0], ref) < 1e-12
+    assert rel_err(_run(Lh, y0, t, mixed=True)[0, 0], ref) < 2e-6
+
+
+def test_batched_disorder_members(gpu):
+    m = _agg7_model()
+    Ls, ys = [], []
+    for n in range(16):
+        mm = m.sample(n)
+        Ls.append(mm.evolve_matrix('eg', heisenberg_picture=True))
+        ys.append(-mm.dipole_operator('eg->gg', 'y', '-').bra_vector)
+    Ls, ys = np.array(Ls), np.array(ys)[:, None, :]
+    t = np.arange(0, 1024.5, 2.0)
+    got = _run(Ls, ys, t)
+    for n in range(16):
+        ref = P.integrate_rk4(Ls[n], ys[n, 0], t)
+        assert rel_err(got[n, 0], ref) < 1e-12, n
+
+
+def test_non_finite_state_sets_status(gpu):
+    from qspectra_amd import _native, engine
+    L = np.eye(3) * (-1j)
+    y0 = np.array([[[1.0, np.nan, 0]]], complex)
+    nsub, h = engine.schedule_tensors(np.arange(4.0))
+    out, _, st = _native.rk4_dense(_native.to_device(L[None]), _native.to_device(y0), nsub, h)
+    torch.cuda.synchronize()
+    assert int(st.item()) == 1
