@@ -90,6 +90,54 @@ int32_t qsx_rk4_dense(int32_t precision, const void* L, int64_t strideL,
                       const void* proj, int64_t strideP, int32_t K,
                       void* out, void* y_final, int32_t* status, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched FP64 construction of Liouvillian blocks.
+ *
+ * Hilbert blocks are numbered 0 = g (1 state), 1 = e (n sites), 2 = f
+ * (n(n-1)/2 site pairs i<j).  Inputs per problem b (B problems):
+ *   H1  [B][n][n] double, the non-rotating one-exciton Hamiltonian (real
+ *       symmetric; a disorder sample per problem),
+ *   rw  [B] double rotating-frame frequency (NULL = 0).
+ * The Liouville block XY acts on rho_ab, a in X, b in Y; M = nX*nY and the
+ * local index is a + nX*b (column stacking).  Hilbert blocks of up to 64
+ * states are supported.
+ */
+
+/* Eigensystem of Hilbert block X of H1 - X*rw (Hamiltonian.eig,
+ * qspectra/hamiltonian.py:310-328, per excitation block): E [B][nb]
+ * ascending, U [B][nb][nb] with eigenvectors in columns, each with its
+ * largest-magnitude component positive.  Batched two-sided Jacobi in LDS.
+ * workspace: qsx_redfield_workspace_bytes(n, X, X, B) bytes. */
+int32_t qsx_block_eigh(const double* H1, int32_t n, int32_t B, int32_t X,
+                       const double* rw, double* E, double* U, void* workspace,
+                       void* stream);
+
+/* Bytes of device workspace needed by qsx_redfield_block / qsx_block_eigh
+ * (or -1 for invalid arguments). */
+int64_t qsx_redfield_workspace_bytes(int32_t n, int32_t X, int32_t Y, int32_t B);
+
+#define QSX_SECULAR 1        /* secular (Bloch) approximation, redfield.py:77-83 */
+#define QSX_REAL_CORR 2      /* discard_imag_corr: real C(w), bath.py:17-31 */
+#define QSX_EIGEN_BASIS 4    /* return the block in the energy eigenbasis */
+#define QSX_HEISENBERG 8     /* return the transpose (Heisenberg picture) */
+
+/* Redfield Liouvillian block XY (complex128 [B][M][M]) for a Debye bath
+ * (temperature T, reorganisation energy lam, cutoff gam, one identical
+ * uncorrelated bath per site), times unit_convert.  Replaces
+ * RedfieldModel.evolution_super_operator[idx][:, idx]
+ * (qspectra/dynamics/redfield.py:9-152, liouville_space.py:316-341). */
+int32_t qsx_redfield_block(const double* H1, int32_t n, int32_t B, const double* rw,
+                           double T, double lam, double gam, int32_t flags,
+                           double unit_convert, int32_t X, int32_t Y, void* L,
+                           void* workspace, void* stream);
+
+/* Unitary Liouvillian block -i*unit_convert*[H, .] in the rotating frame
+ * (UnitaryModel.evolution_super_operator, qspectra/dynamics/unitary.py:5-9);
+ * heisenberg != 0 returns the transpose. */
+int32_t qsx_unitary_block(const double* H1, int32_t n, int32_t B, const double* rw,
+                          double unit_convert, int32_t X, int32_t Y, int32_t heisenberg,
+                          void* L, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

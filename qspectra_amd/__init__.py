@@ -9,3 +9,31 @@ from ._native import (NativeUnavailable, NativeError, require_native, load_libra
                       QSX_FP64, QSX_MIXED)
 
 __version__ = '0.1.0'
+from .bath import DebyeBath, ArbitraryBath, UncoupledBath
+from .constants import CM_FS, CM_K, GAUSSIAN_SD_FWHM
+from .dynamics.liouville_space import (n_excitations, matrix_to_ket_vec, ket_vec_to_matrix,
+                                       matrix_to_bra_vec)
+from .dynamics.redfield import RedfieldModel
+from .dynamics.unitary import UnitaryModel
+from .hamiltonian import Hamiltonian, ElectronicHamiltonian
+from .operator_tools import (unit_vec, basis_transform_operator, basis_transform_vector,
+                             all_states)
+from .polarization import (polarization_vector, check_polarizations,
+                           invariant_weights_4th_order, invariant_polarizations,
+                           FOURTH_ORDER_INVARIANTS, MAGIC_ANGLE)
+from .simulate.eom import simulate_dynamics
+from .simulate.response import (linear_response, absorption_spectra, third_order_response,
+                                two_dimensional_spectra, PUMP_PROBE_PATHWAYS,
+                                THIRD_ORDER_PATHWAYS)
+from .simulate.utils import fourier_transform, integrate, bound_signal, IntegratorError
+
+__all__ = ['DebyeBath', 'ArbitraryBath', 'UncoupledBath', 'CM_FS', 'CM_K', 'GAUSSIAN_SD_FWHM',
+           'Hamiltonian', 'ElectronicHamiltonian', 'n_excitations', 'matrix_to_ket_vec',
+           'ket_vec_to_matrix', 'matrix_to_bra_vec', 'unit_vec', 'basis_transform_operator',
+           'basis_transform_vector', 'all_states', 'polarization_vector',
+           'check_polarizations', 'invariant_weights_4th_order', 'invariant_polarizations',
+           'FOURTH_ORDER_INVARIANTS', 'MAGIC_ANGLE', 'RedfieldModel', 'UnitaryModel',
+           'simulate_dynamics', 'linear_response', 'absorption_spectra',
+           'third_order_response', 'two_dimensional_spectra', 'PUMP_PROBE_PATHWAYS',
+           'THIRD_ORDER_PATHWAYS', 'fourier_transform', 'integrate', 'bound_signal',
+           'IntegratorError', 'NativeUnavailable', 'NativeError', 'require_native']

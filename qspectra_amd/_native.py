@@ -14,6 +14,8 @@ import threading
 import numpy as np
 import torch
 
+from . import timing
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libqsx.so')
 
@@ -35,13 +37,19 @@ _lock = threading.Lock()
 c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
                                    ctypes.c_void_p, ctypes.c_size_t)
 
-# name -> argtypes (restype is always int32)
+# name -> argtypes (restype int32 unless listed in RESTYPES)
+RESTYPES = {'qsx_redfield_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
     'qsx_apply_dense_c128': [c_vp, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp],
     'qsx_rk4_dense': [c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32, c_i32,
                       c_vp, c_vp, c_i32, c_vp, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp],
+    'qsx_block_eigh': [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    'qsx_redfield_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
+    'qsx_redfield_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_dbl, c_dbl, c_i32, c_dbl,
+                           c_i32, c_i32, c_vp, c_vp, c_vp],
+    'qsx_unitary_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_i32, c_i32, c_i32, c_vp, c_vp],
 }
 
 
@@ -58,7 +66,7 @@ def load_library(path=LIB_PATH):
             for name, argtypes in SIGNATURES.items():
                 fn = getattr(lib, name)
                 fn.argtypes = argtypes
-                fn.restype = c_i32
+                fn.restype = RESTYPES.get(name, c_i32)
             _lib = lib
     return _lib
 
@@ -120,7 +128,7 @@ def apply_dense(L, y, stream=None):
 
 
 def rk4_dense(L, y0, nsub, h, proj=None, precision=QSX_FP64, diag=None,
-              want_final=False, stream=None):
+              want_final=False, stream=None, steps=None):
     """Fused RK4 with saves; see include/qsx.h qsx_rk4_dense.
 
     L     [B|1, M, M] complex128 (FP64) or complex64 off-diagonal (MIXED)
@@ -151,7 +159,68 @@ def rk4_dense(L, y0, nsub, h, proj=None, precision=QSX_FP64, diag=None,
     out = torch.empty((B, R, nsave, K), dtype=torch.complex128, device=y0.device)
     yfin = torch.empty_like(y0) if want_final else None
     status = torch.zeros(1, dtype=torch.int32, device=y0.device)
-    check(lib.qsx_rk4_dense(precision, ptr(L), strideL, ptr(diag), strideD, M, ptr(y0), R, B,
-                            ptr(nsub), ptr(h), nsave, ptr(proj), strideP, K, ptr(out),
-                            ptr(yfin), ptr(status), stream_ptr(stream)), 'qsx_rk4_dense')
+    label = 'rk4_dense %s M=%d R=%d B=%d K=%d steps=%s nsave=%d' % (
+        'fp64' if precision == QSX_FP64 else 'mixed', M, R, B, K, steps, nsave)
+    with timing.launch(label, stream):
+        st = lib.qsx_rk4_dense(precision, ptr(L), strideL, ptr(diag), strideD, M, ptr(y0), R, B,
+                               ptr(nsub), ptr(h), nsave, ptr(proj), strideP, K, ptr(out),
+                               ptr(yfin), ptr(status), stream_ptr(stream))
+    check(st, 'qsx_rk4_dense')
     return out, yfin, status
+
+
+# ------------------------------------------------------ Liouvillian builders
+HILBERT_BLOCK = {'g': 0, 'e': 1, 'f': 2}
+QSX_SECULAR, QSX_REAL_CORR, QSX_EIGEN_BASIS, QSX_HEISENBERG = 1, 2, 4, 8
+
+
+def block_size(X, n):
+    return (1, n, n * (n - 1) // 2)[X]
+
+
+def _workspace(n, X, Y, B, dev):
+    nbytes = load_library().qsx_redfield_workspace_bytes(n, X, Y, B)
+    if nbytes < 0:
+        raise NativeError('invalid workspace request n=%d X=%d Y=%d B=%d' % (n, X, Y, B))
+    return torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=dev)
+
+
+def block_eigh(H1, X, rw=None, stream=None):
+    """E [B, nb], U [B, nb, nb] of Hilbert block X of H1 - X*rw (float64)."""
+    lib = require_native()
+    B, n, _ = H1.shape
+    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    nb = block_size(X, n)
+    E = torch.empty((B, nb), dtype=torch.float64, device=H1.device)
+    U = torch.empty((B, nb, nb), dtype=torch.float64, device=H1.device)
+    ws = _workspace(n, X, X, B, H1.device)
+    check(lib.qsx_block_eigh(ptr(H1), n, B, X, ptr(rw), ptr(E), ptr(U), ptr(ws),
+                             stream_ptr(stream)), 'qsx_block_eigh')
+    return E, U
+
+
+def redfield_block(H1, rw, T, lam, gam, X, Y, flags, unit_convert, stream=None):
+    """Redfield Liouvillian block XY, complex128 [B, M, M]."""
+    lib = require_native()
+    B, n, _ = H1.shape
+    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    M = block_size(X, n) * block_size(Y, n)
+    L = torch.empty((B, M, M), dtype=torch.complex128, device=H1.device)
+    ws = _workspace(n, X, Y, B, H1.device)
+    check(lib.qsx_redfield_block(ptr(H1), n, B, ptr(rw), float(T), float(lam), float(gam),
+                                 int(flags), float(unit_convert), X, Y, ptr(L), ptr(ws),
+                                 stream_ptr(stream)), 'qsx_redfield_block')
+    return L
+
+
+def unitary_block(H1, rw, unit_convert, X, Y, heisenberg=False, stream=None):
+    """Unitary Liouvillian block XY, complex128 [B, M, M]."""
+    lib = require_native()
+    B, n, _ = H1.shape
+    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    M = block_size(X, n) * block_size(Y, n)
+    L = torch.empty((B, M, M), dtype=torch.complex128, device=H1.device)
+    check(lib.qsx_unitary_block(ptr(H1), n, B, ptr(rw), float(unit_convert), X, Y,
+                                int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
+          'qsx_unitary_block')
+    return L

@@ -1,0 +1,48 @@
+"""Redfield theory (qspectra/dynamics/redfield.py API) with the Liouvillian
+blocks built on the GPU in FP64 (qsx_redfield_block): eigensystem, Debye
+C(w), Redfield tensor (May & Kuhn Eqs. 3.319, 3.322) and the basis change
+to the site basis, for a whole batch of Hamiltonians per launch."""
+import numpy as np
+
+from .. import _native
+from ..bath import debye_parameters
+from .liouville_space import LiouvilleSpaceModel, all_liouville_subspaces
+
+
+class RedfieldModel(LiouvilleSpaceModel):
+    """Redfield dynamics; each site couples to an identical independent bath.
+
+    secular: keep only population transfer and coherence decay terms.
+    discard_imag_corr: use the real part of the bath correlation function.
+    """
+
+    def __init__(self, hamiltonian, rw_freq=None, hilbert_subspace='gef', unit_convert=1,
+                 secular=True, discard_imag_corr=False, evolve_basis='site',
+                 sparse_matrix=False):
+        super(RedfieldModel, self).__init__(hamiltonian, rw_freq, hilbert_subspace,
+                                            unit_convert, evolve_basis, sparse_matrix)
+        self.secular = secular
+        self.discard_imag_corr = discard_imag_corr
+
+    def _flags(self, heisenberg):
+        return ((_native.QSX_SECULAR if self.secular else 0)
+                | (_native.QSX_REAL_CORR if self.discard_imag_corr else 0)
+                | (_native.QSX_EIGEN_BASIS if self.evolve_basis == 'eigen' else 0)
+                | (_native.QSX_HEISENBERG if heisenberg else 0))
+
+    def _build_block(self, X, Y, heisenberg, H1, rw):
+        T, lam, gam = debye_parameters(self.hamiltonian.bath)
+        return _native.redfield_block(H1, rw, T, lam, gam, X, Y, self._flags(heisenberg),
+                                      self.unit_convert)
+
+
+def redfield_evolve(hamiltonian, subspace='ge', evolve_basis='site', **kwargs):
+    """Full Redfield Liouvillian on the Hilbert subspace (numpy), without
+    unit conversion (redfield.py:95-104)."""
+    model = RedfieldModel.__new__(RedfieldModel)
+    LiouvilleSpaceModel.__init__(model, hamiltonian, hamiltonian.rw_freq, subspace, 1,
+                                 evolve_basis)
+    model.hamiltonian = hamiltonian
+    model.secular = kwargs.get('secular', True)
+    model.discard_imag_corr = kwargs.get('discard_imag_corr', False)
+    return model.evolution_super_operator

@@ -1,0 +1,226 @@
+"""Linear and third-order response functions (qspectra/simulate/response.py
+API), computed on the GPU.
+
+Each Liouville-space pathway is propagated exactly as the reference
+structures it (response.py:15-43, 269-337) -- Schroedinger picture over t1
+and t2, Heisenberg picture over t3 -- but every stage is one fused RK4
+launch: all t1 branches of the t2 stage are right-hand sides of one launch,
+all ensemble members are batch entries of the same launch, and the dipole
+maps applied at each output time run inside the kernel.  Signals stay on
+the device until the end.
+
+Keyword options in addition to the reference's: ``dt`` (RK4 step),
+``precision`` ('fp64' | 'mixed'), ``rtol`` (default-step accuracy).
+"""
+import copy
+
+import numpy as np
+import torch
+
+from .. import _native
+from .decorators import (optional_ensemble_average, optional_2nd_order_isotropic_average,
+                         optional_4th_order_isotropic_average)
+from .utils import propagate, fourier_transform
+from ..utils import ZeroArray
+
+_PROPAGATE_KW = ('dt', 'precision', 'rtol')
+
+
+def _propagate_kw(kwargs):
+    kw = {k: kwargs[k] for k in _PROPAGATE_KW if k in kwargs}
+    # zvode tolerances other than rtol have no RK4 meaning; rtol sets the step
+    return kw
+
+
+def _members(model):
+    return model._hamiltonians() if hasattr(model, '_hamiltonians') else [model.hamiltonian]
+
+
+def _member_model(model, ham):
+    m = copy.copy(model)
+    m.__dict__.pop('_imemoize__cache', None)
+    m.__dict__.pop('_batch', None)
+    m.hamiltonian = ham
+    return m
+
+
+def _op_matrices(model, liouv_map, polarization, transition, what):
+    """Per-member matrices of a dipole map: what = 'commutator' -> [B|1, K, M],
+    'bra' -> [B|1, M].  One shared copy when every member has the same
+    dipoles in the site basis."""
+    hams = _members(model)
+    shared = (model.evolve_basis == 'site' if hasattr(model, 'evolve_basis') else True) and all(
+        np.array_equal(h.dipoles, hams[0].dipoles) for h in hams)
+    targets = hams[:1] if shared else hams
+    mats = []
+    for h in targets:
+        op = _member_model(model, h).dipole_operator(liouv_map, polarization, transition)
+        mats.append(op.commutator.matrix if what == 'commutator' else op.bra_vector)
+    return np.stack(mats)
+
+
+def _dev(x):
+    return _native.to_device(x)
+
+
+def _expand(x, B):
+    return x if x.shape[0] == B else x.expand((B,) + tuple(x.shape[1:])).contiguous()
+
+
+@optional_ensemble_average
+@optional_2nd_order_isotropic_average
+def _linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,
+                     polarization='xx', **integrate_kwargs):
+    model = dynamical_model
+    kw = _propagate_kw(integrate_kwargs)
+    subspaces = liouv_space_path.split('->')
+    B = len(_members(model))
+    if initial_state is None:
+        initial_state = model.thermal_state(subspaces[0])
+    init = np.asarray(initial_state)
+    t = np.arange(0, time_max, model.time_step)
+    signal = ZeroArray()
+    for sim_subspace in subspaces[1].split(','):
+        V0 = _op_matrices(model, '%s->%s' % (subspaces[0], subspaces[1]), polarization[0], '+',
+                          'commutator')                                  # [B|1, M1, M0]
+        bra = _op_matrices(model, '%s->%s' % (subspaces[1], subspaces[2]), polarization[1], '-',
+                           'bra')                                        # [B|1, M1]
+        V_rho0 = np.einsum('bij,...j->b...i', V0, init)                 # [B|1, ..., M1]
+        lead = V_rho0.shape[1:-1]
+        proj = _dev(V_rho0.reshape(V_rho0.shape[0], -1, V_rho0.shape[-1]))   # [B|1, K, M1]
+        eom = model.equation_of_motion(sim_subspace, heisenberg_picture=True)
+        y0 = _expand(_dev(-bra[:, None, :]), B)                          # [B, 1, M1]
+        out = propagate(eom, y0, t, proj=proj, **kw)                     # [B, 1, nt, K]
+        s = out[:, 0].permute(0, 2, 1).reshape((B,) + lead + (len(t),))
+        signal += s
+    return t, signal.cpu().numpy()
+
+
+def linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,
+                    polarization='xx', ensemble_size=None, ensemble_random_orientations=False,
+                    exact_isotropic_average=False, **integrate_kwargs):
+    """Linear response function S(t) for a path 'ab->cd->ef' (rotating
+    wave approximation); returns (t, S)."""
+    return _linear_response(dynamical_model, liouv_space_path, time_max, initial_state,
+                            polarization, ensemble_size=ensemble_size,
+                            ensemble_random_orientations=ensemble_random_orientations,
+                            exact_isotropic_average=exact_isotropic_average,
+                            **integrate_kwargs)
+
+
+def absorption_spectra(dynamical_model, time_max, correlation_decay_time=None,
+                       polarization='xx', ensemble_size=None, ensemble_random_orientations=False,
+                       exact_isotropic_average=False, **integrate_kwargs):
+    """Absorption spectrum: Re FT[-S(t)] of the 'gg->eg->gg' response,
+    optionally damped by exp(-t/correlation_decay_time); returns (f, X)."""
+    t, x = linear_response(dynamical_model, 'gg->eg->gg', time_max, polarization=polarization,
+                           ensemble_size=ensemble_size,
+                           ensemble_random_orientations=ensemble_random_orientations,
+                           exact_isotropic_average=exact_isotropic_average, **integrate_kwargs)
+    if correlation_decay_time is not None:
+        x = x * np.exp(-t / correlation_decay_time)
+    f, X = fourier_transform(t, -x, rw_freq=dynamical_model.rw_freq,
+                             unit_convert=dynamical_model.unit_convert)
+    return f, X.real
+
+
+PUMP_PROBE_PATHWAYS = {'GSB': 'gg->eg->gg', 'ESE': 'ee->eg->gg', 'ESA': 'ee->fe->ee'}
+
+# Abramavicius, Palmieri, Voronine, Sanda & Mukamel, Chem. Rev. 109, 2350
+# (2009), Figs. 4-6 (the pathway table of response.py:252-264).
+THIRD_ORDER_PATHWAYS = {
+    '-++': {'ESE': 'gg->ge->ee->eg->gg', 'GSB': 'gg->ge->gg->eg->gg',
+            'ESA': 'gg->ge->ee->fe->ee'},
+    '+-+': {'ESE': 'gg->eg->ee->eg->gg', 'GSB': 'gg->eg->gg->eg->gg',
+            'ESA': 'gg->eg->ee->fe->ee'},
+    '++-': {'ESA1': 'gg->eg->fg->fe->ee', 'ESA2': 'gg->eg->fg->eg->gg'},
+}
+
+
+def _parse_pathways(possible_pathways, include_signal):
+    selected = [v for k, v in possible_pathways.items()
+                if include_signal is None or k in include_signal]
+    if not selected:
+        raise ValueError('at least one Liouville space pathway must be selected, i.e., '
+                         'include_signal must include at least one of %r'
+                         % list(possible_pathways.keys()))
+    return selected
+
+
+def third_order_grids(model, coherence_time_max, population_time_max=None,
+                      population_times=None):
+    dt = model.time_step
+    t1 = np.arange(0, coherence_time_max, dt)
+    t2 = (np.arange(0, population_time_max, dt) if population_times is None
+          else np.asarray(population_times, dtype=float))
+    t3 = np.arange(0, coherence_time_max, dt)
+    return t1, t2, t3
+
+
+@optional_ensemble_average
+@optional_4th_order_isotropic_average
+def _third_order_response(dynamical_model, coherence_time_max, population_time_max,
+                          population_times, geometry, polarization, include_signal,
+                          **integrate_kwargs):
+    model = dynamical_model
+    kw = _propagate_kw(integrate_kwargs)
+    t1, t2, t3 = third_order_grids(model, coherence_time_max, population_time_max,
+                                   population_times)
+    B = len(_members(model))
+    rho0 = model.thermal_state('gg')
+    total = None
+    for path in _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal):
+        s = path.split('->')
+        trans = geometry + '-'
+        V0 = _op_matrices(model, '%s->%s' % (s[0], s[1]), polarization[0], trans[0], 'commutator')
+        V1 = _op_matrices(model, '%s->%s' % (s[1], s[2]), polarization[1], trans[1], 'commutator')
+        V2 = _op_matrices(model, '%s->%s' % (s[2], s[3]), polarization[2], trans[2], 'commutator')
+        b3 = _op_matrices(model, '%s->%s' % (s[3], s[4]), polarization[3], trans[3], 'bra')
+        eom1 = model.equation_of_motion(s[1])
+        eom2 = model.equation_of_motion(s[2])
+        eom3 = model.equation_of_motion(s[3], heisenberg_picture=True)
+        V_rho0 = _expand(_dev(np.einsum('bij,j->bi', V0, rho0)[:, None, :]), B)   # [B,1,M1]
+        V_rho1 = propagate(eom1, V_rho0, t1, proj=_dev(V1), **kw)              # [B,1,n1,M2]
+        V_rho1 = V_rho1.reshape(B, len(t1), -1)
+        V_rho2 = propagate(eom2, V_rho1, t2, t0=0, proj=_dev(V2), **kw)        # [B,n1,n2,M3]
+        G3 = propagate(eom3, _expand(_dev(b3[:, None, :]), B), t3, **kw)      # [B,1,n3,M3]
+        # signal[b, a, c, d] = sum_i G3[b, d, i] * V_rho2[b, a, c, i]
+        M3 = G3.shape[-1]
+        prod = torch.matmul(V_rho2.reshape(B, -1, M3),
+                            G3[:, 0].transpose(1, 2).contiguous())           # [B, n1*n2, n3]
+        prod = prod.reshape(B, len(t1), len(t2), len(t3))
+        total = prod if total is None else total + prod
+    return (t1, t2, t3), total.cpu().numpy()
+
+
+def third_order_response(dynamical_model, coherence_time_max, population_time_max=None,
+                         population_times=None, geometry='-++', polarization='xxxx',
+                         include_signal=None, ensemble_size=None,
+                         ensemble_random_orientations=False, exact_isotropic_average=False,
+                         **integrate_kwargs):
+    """Third-order response R(t1, t2, t3) summed over Liouville pathways
+    (photon echo '-++', non-rephasing '+-+', double quantum '++-');
+    returns ((t1, t2, t3), signal[len(t1), len(t2), len(t3)])."""
+    return _third_order_response(dynamical_model, coherence_time_max, population_time_max,
+                                 population_times, geometry, polarization, include_signal,
+                                 ensemble_size=ensemble_size,
+                                 ensemble_random_orientations=ensemble_random_orientations,
+                                 exact_isotropic_average=exact_isotropic_average,
+                                 **integrate_kwargs)
+
+
+def two_dimensional_spectra(dynamical_model, coherence_time_max, population_time_max=None,
+                            population_times=None, geometry='-++', polarization='xxxx',
+                            include_signal=None, ensemble_size=None,
+                            ensemble_random_orientations=False, exact_isotropic_average=False,
+                            **integrate_kwargs):
+    """2-D spectrum: the third-order response Fourier transformed along t1
+    (sign -1) and t3; returns ((f1, t2, f3), X)."""
+    (t1, t2, t3), X = third_order_response(
+        dynamical_model, coherence_time_max, population_time_max, population_times, geometry,
+        polarization, include_signal, ensemble_size, ensemble_random_orientations,
+        exact_isotropic_average, **integrate_kwargs)
+    rw, uc = dynamical_model.rw_freq, dynamical_model.unit_convert
+    f1, X = fourier_transform(t1, X, 0, rw_freq=rw, sign=-1, unit_convert=uc)
+    f3, X = fourier_transform(t3, X, 2, rw_freq=rw, unit_convert=uc)
+    return (f1, t2, f3), X

@@ -1,0 +1,191 @@
+"""Parity of the GPU Liouvillian builders and the drop-in model / simulate
+API with the oracle and with the reference's golden outputs.
+
+Tolerances: FP64 operators 1e-12 relative (max-norm); FP64 propagation
+against the oracle's RK4 on the same schedule 1e-10; against the
+reference's tight-tolerance zvode outputs, the RK4 truncation error at the
+step used (stated per test); MIXED precision 1e-6 relative per spectrum.
+"""
+import numpy as np
+import pytest
+import torch
+
+import qspectra_amd as qa
+from qspectra_amd import _native
+from oracle import system as S
+from oracle import liouville as Lv
+from oracle import propagate as P
+from tests.conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(name):
+    g = golden(name)
+    return g, dict(H1=g['in_H1'], dipoles=g['in_dipoles'], extra=float(g['extra']))
+
+
+def _product_ham(inp, T=S.CM_K * 77, **kw):
+    return qa.ElectronicHamiltonian(inp['H1'], bath=qa.DebyeBath(T, 35, 106),
+                                    dipoles=inp['dipoles'], energy_spread_extra=inp['extra'], **kw)
+
+
+def _oracle_ham(inp, T=S.CM_K * 77, **kw):
+    return S.ElectronicHamiltonian(inp['H1'], bath=S.DebyeBath(T, 35, 106),
+                                   dipoles=inp['dipoles'], energy_spread_extra=inp['extra'], **kw)
+
+
+def _canonical_signs(U):
+    idx = np.argmax(np.abs(U), axis=0)
+    return np.sign(U[idx, np.arange(U.shape[1])])
+
+
+def test_block_eigh_matches_lapack(gpu):
+    rs = np.random.RandomState(3)
+    for n, X in [(7, 1), (7, 2), (24, 1), (2, 1), (3, 2), (1, 1)]:
+        H = rs.randn(5, n, n) * 80
+        H = H + H.transpose(0, 2, 1) + np.diag(rs.uniform(12200, 12700, n))
+        rw = rs.uniform(12300, 12500, 5)
+        E, U = _native.block_eigh(torch.as_tensor(H).cuda(), X, torch.as_tensor(rw).cuda())
+        E, U = E.cpu().numpy(), U.cpu().numpy()
+        for b in range(5):
+            Hb = S.operator_extend(H[b], 'gef'[X])
+            Er, Ur = np.linalg.eigh(Hb)
+            assert np.max(np.abs(E[b] - (Er - X * rw[b]))) < 1e-10
+            Ur = Ur * _canonical_signs(Ur)
+            assert np.max(np.abs(U[b] - Ur)) < 1e-10, (n, X)
+
+
+def test_redfield_dimer_blocks_match_reference(gpu):
+    g, inp = _inputs('redfield_dimer')
+    ham = _product_ham(inp)
+    for sec in (True, False):
+        for dic in (True, False):
+            m = qa.RedfieldModel(ham, unit_convert=qa.CM_FS, secular=sec, discard_imag_corr=dic)
+            for s in ['gg', 'eg', 'ge', 'ee', 'fg', 'fe', 'gg,ge,eg,ee']:
+                ref = g['L_%s_%d%d' % (s.replace(',', '_'), sec, dic)]
+                got = m.liouvillian(s)[0].cpu().numpy()
+                assert rel_err(got, ref) < 1e-12, (s, sec, dic)
+                got = m.liouvillian(s, heisenberg_picture=True)[0].cpu().numpy()
+                assert rel_err(got, ref.T) < 1e-12, (s, sec, dic, 'heisenberg')
+
+
+def test_redfield_eigen_basis_up_to_eigenvector_signs(gpu):
+    g, inp = _inputs('redfield_dimer')
+    m = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=False,
+                         evolve_basis='eigen')
+    sg = _canonical_signs(g['U_gef'])          # reference U -> canonical signs
+    e = sg[1:3]
+    sign = np.outer(e, e).reshape(-1, order='F')   # s_a s_b for (a, b) in 'ee'
+    ref = g['Leig_ee'] * np.outer(sign, sign)
+    assert rel_err(m.liouvillian('ee')[0].cpu().numpy(), ref) < 1e-12
+    assert np.max(np.abs(m.hamiltonian.E('gef') - g['E_gef'])) < 1e-9
+
+
+def test_redfield_7site_blocks_and_batch(gpu):
+    g, inp = _inputs('redfield_7site')
+    m = qa.RedfieldModel(_product_ham(inp, disorder=100 / qa.GAUSSIAN_SD_FWHM),
+                         unit_convert=qa.CM_FS, secular=False)
+    for s in ['ge', 'eg', 'ee', 'fe', 'fg']:
+        assert rel_err(m.liouvillian(s)[0].cpu().numpy(), g['L_' + s]) < 1e-12, s
+    ms = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=True)
+    assert rel_err(ms.liouvillian('ee')[0].cpu().numpy(), g['Lsec_ee']) < 1e-12
+    # a batch of disorder samples in one launch vs the oracle member by member
+    hams = [m.hamiltonian.sample(n) for n in range(6)]
+    mb = m.with_hamiltonians(hams)
+    om = Lv.LiouvilleModel(_oracle_ham(inp, disorder=100 / S.GAUSSIAN_SD_FWHM),
+                           unit_convert=S.CM_FS, secular=False)
+    for s in ('eg', 'ee'):
+        Lb = mb.liouvillian(s).cpu().numpy()
+        for n in range(6):
+            assert rel_err(Lb[n], om.sample(n).evolve_matrix(s)) < 1e-12, (s, n)
+
+
+def test_unitary_blocks_match_oracle(gpu):
+    _, inp = _inputs('redfield_7site')
+    m = qa.UnitaryModel(_product_ham(inp), unit_convert=qa.CM_FS)
+    om = Lv.LiouvilleModel(_oracle_ham(inp), unit_convert=S.CM_FS, kind='unitary')
+    for s in ['eg', 'ee', 'fe', 'gg,ge,eg,ee']:
+        assert rel_err(m.liouvillian(s)[0].cpu().numpy(), om.evolve_matrix(s)) < 1e-12, s
+
+
+def test_integrate_matches_oracle_and_reference(gpu):
+    g, inp = _inputs('redfield_7site')
+    m = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=False)
+    om = Lv.LiouvilleModel(_oracle_ham(inp), unit_convert=S.CM_FS, secular=False)
+    eom = m.equation_of_motion('ee')
+    y = np.random.RandomState(0).randn(3, 49) + 0j
+    assert rel_err(eom(0, y), y @ om.evolve_matrix('ee').T) < 1e-13
+    V = m.dipole_operator('ee->eg', 'x', '-')
+    t = np.arange(0, 100.0, 2.0)
+    got = qa.integrate(eom, y, t, save_func=V.commutator, dt=2.0)
+    ref = P.integrate_rk4(om.evolve_matrix('ee'), y, t, save_matrix=V.commutator.matrix, dt=2.0)
+    assert got.shape == (3, 50, 7)
+    assert rel_err(got, ref) < 1e-10
+    gd = golden('dynamics')
+    t, rho = qa.simulate_dynamics(m, [1, 0, 0, 0, 0, 0, 0], 200, dt=0.25)
+    assert rel_err(rho, gd['rho']) < 1e-8          # RK4 at 0.25 fs vs tight zvode
+    t, rho = qa.simulate_dynamics(m, [1, 0, 0, 0, 0, 0, 0], 200)   # default rtol step
+    assert rel_err(rho, gd['rho']) < 1e-6
+
+
+def test_absorption_spectra_match_reference(gpu):
+    gs = golden('spectra_linear')
+    g, inp = _inputs('redfield_dimer')
+    md = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS)
+    f, X = qa.absorption_spectra(md, 511.5, dt=1.0 / 8)
+    assert np.allclose(f, gs['dimer_f'])
+    assert rel_err(X, gs['dimer_X']) < 1e-8
+    om = Lv.LiouvilleModel(_oracle_ham(inp), unit_convert=S.CM_FS)
+    fo, Xo = P.absorption_spectra(om, 511.5, dt=1.0)
+    f, X = qa.absorption_spectra(md, 511.5, dt=1.0)
+    assert rel_err(X, Xo) < 1e-10
+    f, X = qa.absorption_spectra(md, 511.5, dt=1.0, precision='mixed')
+    assert rel_err(X, Xo) < 1e-6
+    f, X = qa.absorption_spectra(md, 511.5, exact_isotropic_average=True, dt=1.0 / 8)
+    assert rel_err(X, gs['dimer_X_iso']) < 1e-8
+    _, inp7 = _inputs('redfield_7site')
+    m7 = qa.RedfieldModel(_product_ham(inp7, disorder=100 / qa.GAUSSIAN_SD_FWHM),
+                          unit_convert=qa.CM_FS, secular=False)
+    f, X = qa.absorption_spectra(m7, 2 * 255.5, dt=2.0 / 8)
+    assert rel_err(X, gs['agg7_X']) < 1e-7
+    f, X = qa.absorption_spectra(m7, 2 * 127.5, ensemble_size=8, dt=2.0 / 8)
+    assert rel_err(X, gs['agg7_ens8_X']) < 1e-7
+    f, X = qa.absorption_spectra(m7, 2 * 127.5, correlation_decay_time=300.0,
+                                 polarization=[0.3, 1.1], dt=2.0 / 8)
+    assert rel_err(X, gs['agg7_decay_X']) < 1e-7
+
+
+def test_third_order_and_2d_match_reference(gpu):
+    gt = golden('spectra_third_order')
+    _, inp = _inputs('redfield_7site')
+    m7 = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=False)
+    om = Lv.LiouvilleModel(_oracle_ham(inp), unit_convert=S.CM_FS, secular=False)
+    for geom in ('-++', '+-+', '++-'):
+        key = 'R3_' + geom.replace('-', 'm').replace('+', 'p')
+        (t1, t2, t3), X = qa.third_order_response(m7, 2 * 15.5, population_times=gt['t2'],
+                                                  geometry=geom, dt=2.0)
+        _, Xo = P.third_order_response(om, 2 * 15.5, population_times=gt['t2'],
+                                       geometry=geom, dt=2.0)
+        assert rel_err(X, Xo) < 1e-10, geom
+        (t1, t2, t3), X = qa.third_order_response(m7, 2 * 15.5, population_times=gt['t2'],
+                                                  geometry=geom, dt=2.0 / 8)
+        assert rel_err(X, gt[key]) < 1e-6, geom
+    (f1, t2, f3), X = qa.two_dimensional_spectra(m7, 2 * 15.5, population_times=gt['t2'],
+                                                 include_signal='GSB,ESE', dt=2.0 / 8)
+    assert np.allclose(f1, gt['S2d_f1']) and np.allclose(f3, gt['S2d_f3'])
+    assert rel_err(X, gt['S2d']) < 1e-6
+    (f1, t2, f3), Xm = qa.two_dimensional_spectra(m7, 2 * 15.5, population_times=gt['t2'],
+                                                  include_signal='GSB,ESE', dt=2.0 / 8,
+                                                  precision='mixed')
+    assert rel_err(Xm, X) < 1e-6
+
+
+def test_isotropic_third_order_dimer(gpu):
+    gt = golden('spectra_third_order')
+    g, inp = _inputs('redfield_dimer')
+    md = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS)
+    (t1, t2, t3), X = qa.third_order_response(
+        md, 8.5, population_times=[0., 3., 7.], exact_isotropic_average=True,
+        polarization=[0, 0, qa.MAGIC_ANGLE, qa.MAGIC_ANGLE], dt=1.0 / 16)
+    assert rel_err(X, gt['dimer_iso_R3']) < 1e-7

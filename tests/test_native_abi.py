@@ -12,7 +12,7 @@ def declared_functions():
         if fn.endswith('.h'):
             text = open(os.path.join(ROOT, 'include', fn)).read()
             text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
-            names += re.findall(r'^\s*int32_t\s+(qsx_\w+)\s*\(', text, flags=re.M)
+            names += re.findall(r'^\s*int(?:32|64)_t\s+(qsx_\w+)\s*\(', text, flags=re.M)
     return names
 
 
