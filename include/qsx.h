@@ -138,6 +138,78 @@ int32_t qsx_unitary_block(const double* H1, int32_t n, int32_t B, const double* 
                           double unit_convert, int32_t X, int32_t Y, int32_t heisenberg,
                           void* L, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Register-resident fixed-step propagation (one workgroup per right-hand
+ * side; blocks up to M = 160).  Replaces the zvode loop of
+ * qspectra/simulate/utils.py:16-109 like qsx_rk4_dense, but saves full
+ * states (projections are applied afterwards with qsx_zgemm).
+ *
+ *   mode   QSX_MODE_RK4: A = L, four products per step of size h[s];
+ *          QSX_MODE_PROPAGATOR: A = P = P4(h L) (FP64) or P - I (MIXED),
+ *          one product per step; every h[s] must equal the h of P.
+ *   precision / diag as in qsx_rk4_dense (MIXED: A off-diagonal complex64,
+ *          diag the complex128 diagonal of L or of P - I).
+ *   out    [B][R][nsave][M] complex128.
+ */
+#define QSX_MODE_RK4 0
+#define QSX_MODE_PROPAGATOR 1
+int32_t qsx_propagate(int32_t mode, int32_t precision, const void* A, int64_t strideA,
+                      const void* diag, int64_t strideD, int32_t M, const void* y0,
+                      int32_t R, int32_t B, const int32_t* nsub, const double* h,
+                      int32_t nsave, void* out, void* y_final, int32_t* status, void* stream);
+
+/* Batched complex GEMM, row-major:
+ *   C[b] = alpha * A[b] (m x k) * B[b] (k x n) + beta * C[b] + gamma * I
+ * (gamma only for m == n).  precision QSX_FP64: complex128 operands;
+ * QSX_MIXED: complex64 A and B, FP32 products, complex128 C.  Leading
+ * dimensions lda/ldb/ldc (elements), batch strides sA/sB/sC (0 = shared;
+ * sC must be nonzero when batch > 1).  alpha, beta, gamma are complex
+ * (re, im) pairs passed by value.  Used for the RK4 propagator (Horner:
+ * P = I + hL(I + hL/2 (I + hL/3 (I + hL/4)))), its powers, and the
+ * save-point dipole maps and response contractions of response.py. */
+int32_t qsx_zgemm(int32_t precision, int32_t m, int32_t n, int32_t k, double alpha_re,
+                  double alpha_im, const void* A, int64_t lda, int64_t sA, const void* B,
+                  int64_t ldb, int64_t sB, double beta_re, double beta_im, void* C,
+                  int64_t ldc, int64_t sC, double gamma_re, double gamma_im, int32_t batch,
+                  void* stream);
+
+/* ---------------------------------------------------------------------------
+ * HEOM for a Drude bath (qspectra/dynamics/heom.py:177-443), batched over B
+ * samples that share the hierarchy structure.  State per sample: ADO-major
+ * [n_ado][M] complex128.  Operator tables (device pointers):
+ *   Hop    n > 0: [B][n][n] = -i*unit*H_X (its transpose for the Heisenberg
+ *          picture), the subspace being the square block XX, M = n*n;
+ *          n = 0: [B][M][M] dense -i*unit*[H, .] block (M <= 64)
+ *   shift  [n_ado] double, -unit * sum_jk m_jk nu_k
+ *   dvec   [M] complex, -unit * Delta * sum_j (n_j(r) + n_j(c) - 2 n_j(r) n_j(c))
+ *   edge_src/edge_w/edge_coef [n_ado][E]: incoming neighbour couplings
+ *          (source ADO or -1 after the last, weight-vector id, coefficient)
+ *   W      [nw][M] complex weight vectors (diagonal neighbour superoperators)
+ */
+
+/* out[b] = F(y[b]) (the hierarchy's eom(t, y), heom.py:241-242). */
+int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n, const void* Hop,
+                       const double* shift, const void* dvec, int32_t E,
+                       const int32_t* edge_src, const int32_t* edge_w, const void* edge_coef,
+                       const void* W, const void* y, void* out, void* stream);
+
+/* Device workspace bytes of qsx_heom_propagate (-1 for invalid arguments). */
+int64_t qsx_heom_workspace_bytes(int32_t precision, int32_t B, int32_t n_ado, int32_t M);
+
+/* Classical RK4 over the hierarchy (integrate on the HEOM eom,
+ * simulate/utils.py:16-109): save interval s has nsub_host[s] steps of size
+ * h_host[s] (HOST arrays); out [B][nsave][M] receives ADO 0 (the reduced
+ * density matrix) at each save point; y_final (optional) the final states.
+ * Each step is four stage launches; MIXED keeps the stage vector in
+ * complex64 (state and update complex128).  Allocates nothing: workspace
+ * holds qsx_heom_workspace_bytes bytes. */
+int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t M, int32_t n,
+                           const void* Hop, const double* shift, const void* dvec, int32_t E,
+                           const int32_t* edge_src, const int32_t* edge_w, const void* edge_coef,
+                           const void* W, const void* y0, const int32_t* nsub_host,
+                           const double* h_host, int32_t nsave, void* out, void* y_final,
+                           void* workspace, int32_t* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

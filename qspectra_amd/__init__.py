@@ -15,6 +15,7 @@ from .dynamics.liouville_space import (n_excitations, matrix_to_ket_vec, ket_vec
                                        matrix_to_bra_vec)
 from .dynamics.redfield import RedfieldModel
 from .dynamics.unitary import UnitaryModel
+from .dynamics.heom import HEOMModel
 from .hamiltonian import Hamiltonian, ElectronicHamiltonian
 from .operator_tools import (unit_vec, basis_transform_operator, basis_transform_vector,
                              all_states)
@@ -32,7 +33,7 @@ __all__ = ['DebyeBath', 'ArbitraryBath', 'UncoupledBath', 'CM_FS', 'CM_K', 'GAUS
            'ket_vec_to_matrix', 'matrix_to_bra_vec', 'unit_vec', 'basis_transform_operator',
            'basis_transform_vector', 'all_states', 'polarization_vector',
            'check_polarizations', 'invariant_weights_4th_order', 'invariant_polarizations',
-           'FOURTH_ORDER_INVARIANTS', 'MAGIC_ANGLE', 'RedfieldModel', 'UnitaryModel',
+           'FOURTH_ORDER_INVARIANTS', 'MAGIC_ANGLE', 'RedfieldModel', 'UnitaryModel', 'HEOMModel',
            'simulate_dynamics', 'linear_response', 'absorption_spectra',
            'third_order_response', 'two_dimensional_spectra', 'PUMP_PROBE_PATHWAYS',
            'THIRD_ORDER_PATHWAYS', 'fourier_transform', 'integrate', 'bound_signal',

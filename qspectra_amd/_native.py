@@ -38,7 +38,7 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
                                    ctypes.c_void_p, ctypes.c_size_t)
 
 # name -> argtypes (restype int32 unless listed in RESTYPES)
-RESTYPES = {'qsx_redfield_workspace_bytes': c_i64}
+RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -50,6 +50,16 @@ SIGNATURES = {
     'qsx_redfield_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_dbl, c_dbl, c_i32, c_dbl,
                            c_i32, c_i32, c_vp, c_vp, c_vp],
     'qsx_unitary_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_i32, c_i32, c_i32, c_vp, c_vp],
+    'qsx_propagate': [c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32, c_i32,
+                      c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp],
+    'qsx_zgemm': [c_i32, c_i32, c_i32, c_i32, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_vp, c_i64,
+                  c_i64, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_i32, c_vp],
+    'qsx_heom_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
+    'qsx_heom_apply': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp,
+                       c_vp, c_vp, c_vp, c_vp],
+    'qsx_heom_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp,
+                           c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
+                           c_vp],
 }
 
 
@@ -224,3 +234,116 @@ def unitary_block(H1, rw, unit_convert, X, Y, heisenberg=False, stream=None):
                                 int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
           'qsx_unitary_block')
     return L
+
+
+# -------------------------------------------------- register stepper / GEMM
+QSX_MODE_RK4 = 0
+QSX_MODE_PROPAGATOR = 1
+MAX_REGISTER_BLOCK = 160
+
+
+def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_final=False,
+                     stream=None, steps=None):
+    """qsx_propagate: full states at every save point, out [B, R, nsave, M]."""
+    lib = require_native()
+    B, R, M = y0.shape
+    nsave = int(nsub.shape[0])
+    adt = torch.complex128 if precision == QSX_FP64 else torch.complex64
+    assert A.dtype == adt and A.is_contiguous() and A.shape[-2:] == (M, M)
+    assert A.shape[0] in (1, B) and y0.dtype == torch.complex128 and y0.is_contiguous()
+    assert nsub.dtype == torch.int32 and h.dtype == torch.float64
+    strideA = 0 if A.shape[0] == 1 else M * M
+    strideD = 0
+    if precision == QSX_MIXED:
+        assert diag is not None and diag.dtype == torch.complex128 and diag.is_contiguous()
+        strideD = 0 if diag.shape[0] == 1 else M
+    out = torch.empty((B, R, nsave, M), dtype=torch.complex128, device=y0.device)
+    yfin = torch.empty_like(y0) if want_final else None
+    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    label = 'propagate %s %s M=%d R=%d B=%d steps=%s nsave=%d' % (
+        'rk4' if mode == QSX_MODE_RK4 else 'prop', 'fp64' if precision == QSX_FP64 else 'mixed',
+        M, R, B, steps, nsave)
+    with timing.launch(label, stream):
+        st = lib.qsx_propagate(mode, precision, ptr(A), strideA, ptr(diag), strideD, M, ptr(y0),
+                               R, B, ptr(nsub), ptr(h), nsave, ptr(out), ptr(yfin), ptr(status),
+                               stream_ptr(stream))
+    check(st, 'qsx_propagate')
+    return out, yfin, status
+
+
+def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stream=None):
+    """C = alpha A B + beta C + gamma I over a leading batch axis (1 = shared).
+
+    A [b|1, m, k], B [b|1, k, n] (complex128, or complex64 for MIXED),
+    C [b, m, n] complex128 (allocated when None)."""
+    lib = require_native()
+    assert A.dim() == 3 and B.dim() == 3 and A.is_contiguous() and B.is_contiguous()
+    bA, m, k = A.shape
+    bB, k2, n = B.shape
+    assert k == k2
+    batch = max(bA, bB) if C is None else C.shape[0]
+    if C is None:
+        C = torch.empty((batch, m, n), dtype=torch.complex128, device=A.device)
+        beta = 0.0
+    assert C.dtype == torch.complex128 and C.is_contiguous() and C.shape == (batch, m, n)
+    assert bA in (1, batch) and bB in (1, batch)
+    dt = torch.complex128 if precision == QSX_FP64 else torch.complex64
+    assert A.dtype == dt and B.dtype == dt
+    al, be, ga = complex(alpha), complex(beta), complex(gamma)
+    with timing.launch('zgemm %s m=%d n=%d k=%d batch=%d' % (
+            'fp64' if precision == QSX_FP64 else 'mixed', m, n, k, batch), stream):
+        st = lib.qsx_zgemm(precision, m, n, k, al.real, al.imag, ptr(A), k,
+                           0 if bA == 1 else m * k, ptr(B), n, 0 if bB == 1 else k * n,
+                           be.real, be.imag, ptr(C), n, m * n if batch > 1 else 0,
+                           ga.real, ga.imag, batch, stream_ptr(stream))
+    check(st, 'qsx_zgemm')
+    return C
+
+
+# ------------------------------------------------------------------- HEOM
+def _heom_ptrs(eom):
+    return (eom.n_ado, eom.M, eom.n_comm, ptr(eom.Hop), ptr(eom.shift), ptr(eom.dvec), eom.E,
+            ptr(eom.esrc), ptr(eom.ew), ptr(eom.ecoef), ptr(eom.W))
+
+
+def heom_apply(eom, y, stream=None):
+    """F(y) for hierarchy states y [B, n_ado * M] (complex128)."""
+    lib = require_native()
+    B = eom.Hop.shape[0]
+    assert y.shape == (B, eom.n_ado * eom.M) and y.dtype == torch.complex128
+    y = y.contiguous()
+    out = torch.empty_like(y)
+    n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W = _heom_ptrs(eom)
+    check(lib.qsx_heom_apply(B, n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W, ptr(y), ptr(out),
+                             stream_ptr(stream)), 'qsx_heom_apply')
+    return out
+
+
+def heom_propagate(eom, y0, nsub, h, precision=QSX_FP64, want_final=False, stream=None):
+    """RK4 over the hierarchy; nsub, h are host arrays.  Returns ADO 0 at
+    every save point, out [B, nsave, M], the final states and the status."""
+    lib = require_native()
+    B = eom.Hop.shape[0]
+    S = eom.n_ado * eom.M
+    assert y0.shape == (B, S) and y0.dtype == torch.complex128
+    y0 = y0.contiguous()
+    nsub = np.ascontiguousarray(nsub, dtype=np.int32)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    nsave = int(nsub.size)
+    nbytes = lib.qsx_heom_workspace_bytes(precision, B, eom.n_ado, eom.M)
+    if nbytes < 0:
+        raise NativeError('invalid HEOM workspace request')
+    ws = torch.empty((nbytes + 15) // 16, dtype=torch.complex128, device=y0.device)
+    out = torch.empty((B, nsave, eom.M), dtype=torch.complex128, device=y0.device)
+    yfin = torch.empty_like(y0) if want_final else None
+    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W = _heom_ptrs(eom)
+    label = 'heom %s B=%d n_ado=%d M=%d steps=%d' % (
+        'fp64' if precision == QSX_FP64 else 'mixed', B, n_ado, M, int(nsub.sum()))
+    with timing.launch(label, stream):
+        st = lib.qsx_heom_propagate(
+            precision, B, n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W, ptr(y0),
+            nsub.ctypes.data_as(c_vp), h.ctypes.data_as(c_vp), nsave, ptr(out), ptr(yfin),
+            ptr(ws), ptr(status), stream_ptr(stream))
+    check(st, 'qsx_heom_propagate')
+    return out, yfin, status

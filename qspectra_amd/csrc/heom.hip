@@ -1,0 +1,282 @@
+// HEOM (hierarchical equations of motion) right-hand side and fused RK4
+// stages for a Drude bath, batched over static-disorder samples (gfx950).
+//
+// State layout (qspectra/dynamics/heom.py:137-151, 278-286): per sample,
+// ADO-major [n_ado][M] with ADO 0 the reduced density matrix.  In the site
+// basis every bath term of heom.py:395-442 is diagonal in the Liouville
+// index p = (r, c):  [P_j, rho]_p = (n_j(r) - n_j(c)) rho_p,
+// (c_k P_j^L - c_k^* P_j^R) rho = (c_k n_j(r) - c_k^* n_j(c)) rho_p and the
+// low-temperature double commutator is Delta sum_j (n_j(r) + n_j(c) -
+// 2 n_j(r) n_j(c)).  So
+//   d rho_n / dt = Lh rho_n + (shift_n + dvec) o rho_n
+//                  + sum_{edges e into n} coef_e W_{w_e} o rho_{src_e}
+// with Lh the (sample-dependent) -i unit [H, .] block, shift_n the ADO
+// damping, W the per-element weight vectors and an edge table built on the
+// host (+/-1 neighbours, or their transpose for the Heisenberg picture).
+//
+// Lh is applied either as a dense M x M block (any subspace, M <= 64) or,
+// for a square subspace XX (M = n^2), as the commutator -i unit (H rho -
+// rho H) with the n x n block H_X (2n instead of n^2 products per element).
+//
+// One RK4 stage is one launch over (sample, ADO tile); the four stages of
+// classical RK4 use buffers y (FP64 state), ynew (FP64 accumulated update)
+// and a double-buffered stage vector zA/zB (FP64, or FP32 in MIXED mode).
+// Work-groups are mapped so that every tile of one sample lands on the same
+// XCD (work-groups are dispatched round-robin over the 8 XCDs): a sample's
+// hierarchy and its neighbour re-reads stay in one L2.
+
+#include "qsx_common.h"
+
+#include <vector>
+
+namespace qsx {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kXcds = 8;
+
+struct HeomArgs {
+  int B, n_ado, M, n, dense, E, tiles, ado_per_tile, xcd_map;
+  const double2* Hop;      // COMM: [B][n][n] = -i unit H_X (or its transpose); DENSE: [B][M][M]
+  const double* shift;     // [n_ado]   -unit * sum m nu
+  const double2* dvec;     // [M]       -unit * Delta * D_p
+  const int* esrc;         // [n_ado][E] source ADO (-1 = none)
+  const int* ew;           // [n_ado][E] weight vector id
+  const double2* ecoef;    // [n_ado][E] scalar coefficient (includes unit)
+  const double2* W;        // [nw][M]
+};
+
+template <typename T>
+__device__ inline double2 ld(const cplx_t<T>* p) { return to_d2(*p); }
+
+// Stage st of classical RK4: z_in = input stage vector, writes z_out and
+// updates ynew (st 0..2) or y (st 3); st = 4 only writes z_out = F(z_in).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void heom_stage_kernel(
+    HeomArgs a, const cplx_t<T>* __restrict__ zin, cplx_t<T>* __restrict__ zout,
+    double2* __restrict__ y, double2* __restrict__ ynew, double h, int st) {
+  using T2 = cplx_t<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* Hs = reinterpret_cast<double2*>(smem);                 // n*n (COMM) or M*M (DENSE)
+  const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  T2* zt = reinterpret_cast<T2*>(Hs + opsz);                      // [ado_per_tile][M]
+
+  int b, tile;
+  if (a.xcd_map) {
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    b = x + kXcds * (j / a.tiles);
+    tile = j % a.tiles;
+  } else {
+    b = blockIdx.x / a.tiles;
+    tile = blockIdx.x % a.tiles;
+  }
+  if (b >= a.B) return;                      // uniform per work-group
+  const int tid = threadIdx.x;
+  const int M = a.M;
+  const int ado0 = tile * a.ado_per_tile;
+  const int nloc = min(a.ado_per_tile, a.n_ado - ado0);
+  const int64_t base = ((int64_t)b * a.n_ado + ado0) * M;
+  const double2* Hb = a.Hop + (int64_t)b * opsz;
+  for (int e = tid; e < opsz; e += kThreads) Hs[e] = Hb[e];
+  for (int e = tid; e < nloc * M; e += kThreads) zt[e] = zin[base + e];
+  __syncthreads();
+
+  const int e = tid;
+  if (e < nloc * M) {
+    const int la = e / M, p = e % M, ado = ado0 + la;
+    const T2* z = zt + la * M;
+    double2 k = make_double2(0.0, 0.0);
+    if (a.dense) {
+      const double2* row = Hs + p * M;
+      for (int q = 0; q < M; ++q) k = cadd(k, cmul(row[q], ld<T>(z + q)));
+    } else {   // (H rho - rho H)_{rc}, p = r + n c, column-major rho
+      const int n = a.n, r = p % n, c = p / n;
+      for (int q = 0; q < n; ++q) {
+        k = cadd(k, cmul(Hs[r * n + q], ld<T>(z + q + n * c)));
+        const double2 t = cmul(ld<T>(z + r + n * q), Hs[q * n + c]);
+        k.x -= t.x;
+        k.y -= t.y;
+      }
+    }
+    const double2 zp = ld<T>(z + p);
+    const double2 dg = make_double2(a.shift[ado] + a.dvec[p].x, a.dvec[p].y);
+    k = cadd(k, cmul(dg, zp));
+    const int* src = a.esrc + (int64_t)ado * a.E;
+    const int* wid = a.ew + (int64_t)ado * a.E;
+    const double2* cf = a.ecoef + (int64_t)ado * a.E;
+    const T2* zs = zin + (int64_t)b * a.n_ado * M;
+    for (int q = 0; q < a.E; ++q) {
+      const int s = src[q];
+      if (s < 0) break;                       // edges are packed, -1 terminated
+      const double2 w = cmul(cf[q], a.W[(int64_t)wid[q] * M + p]);
+      k = cadd(k, cmul(w, ld<T>(zs + (int64_t)s * M + p)));
+    }
+    const int64_t gi = base + e;
+    if (st == 4) {                           // plain application: zout = F(zin)
+      zout[gi] = from_d2<T>(k);
+      return;
+    }
+    const double2 yv = y[gi];
+    if (st == 0) {
+      ynew[gi] = caxpy(h / 6.0, k, yv);
+      zout[gi] = from_d2<T>(caxpy(0.5 * h, k, yv));
+    } else if (st < 3) {
+      ynew[gi] = caxpy(h / 3.0, k, ynew[gi]);
+      zout[gi] = from_d2<T>(caxpy(st == 1 ? 0.5 * h : h, k, yv));
+    } else {
+      const double2 yn = caxpy(h / 6.0, k, ynew[gi]);
+      y[gi] = yn;
+      zout[gi] = from_d2<T>(yn);
+    }
+  }
+}
+
+template <typename T>
+__global__ void heom_init_kernel(const double2* __restrict__ y0, double2* __restrict__ y,
+                                 cplx_t<T>* __restrict__ z, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < total) {
+    y[i] = y0[i];
+    z[i] = from_d2<T>(y0[i]);
+  }
+}
+
+// out[b][s][0:M] = ADO 0 of sample b; status on non-finite values
+__global__ void heom_save_kernel(const double2* __restrict__ y, int n_ado, int M, int B, int s,
+                                 int nsave, double2* __restrict__ out, int* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * M) return;
+  const int b = i / M, p = i % M;
+  const double2 v = y[(int64_t)b * n_ado * M + p];
+  out[((int64_t)b * nsave + s) * M + p] = v;
+  if (!cfinite(v)) status[0] = 1;
+}
+
+template <typename T>
+int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const double* h,
+             int nsave, double2* out, double2* yfin, void* workspace, int* status,
+             hipStream_t stream) {
+  using T2 = cplx_t<T>;
+  HeomArgs a = a0;
+  const int64_t total = (int64_t)a.B * a.n_ado * a.M;
+  double2* y = static_cast<double2*>(workspace);
+  double2* ynew = y + total;
+  T2* zA = reinterpret_cast<T2*>(ynew + total);
+  T2* zB = zA + total;
+  const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  const size_t shmem = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * sizeof(T2);
+  const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;
+  hipLaunchKernelGGL(heom_init_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     stream, y0, y, zA, total);
+  const int sblocks = (a.B * a.M + 255) / 256;
+  for (int s = 0; s < nsave; ++s) {
+    for (int step = 0; step < nsub[s]; ++step) {
+      const T2* zi = zA;
+      T2* zo = zB;
+      for (int st = 0; st < 4; ++st) {
+        hipLaunchKernelGGL(heom_stage_kernel<T>, dim3(ngroups), dim3(kThreads), shmem, stream, a,
+                           zi, zo, y, ynew, h[s], st);
+        const T2* t = zi;
+        zi = zo;
+        zo = const_cast<T2*>(t);
+      }
+    }
+    hipLaunchKernelGGL(heom_save_kernel, dim3(sblocks), dim3(256), 0, stream, y, a.n_ado, a.M,
+                       a.B, s, nsave, out, status);
+  }
+  QSX_CHECK_HIP(hipGetLastError());
+  if (yfin != nullptr)
+    QSX_CHECK_HIP(hipMemcpyAsync(yfin, y, total * sizeof(double2), hipMemcpyDeviceToDevice,
+                                 stream));
+  return QSX_OK;
+}
+
+}  // namespace
+}  // namespace qsx
+
+static int fill_args(qsx::HeomArgs& a, int B, int n_ado, int M, int n, const void* Hop,
+                     const double* shift, const void* dvec, int E, const int32_t* edge_src,
+                     const int32_t* edge_w, const void* edge_coef, const void* W) {
+  using namespace qsx;
+  QSX_CHECK_ARG(Hop && shift && dvec && edge_src && edge_w && edge_coef && W,
+                "qsx_heom: null operator pointer");
+  QSX_CHECK_ARG(B >= 1 && n_ado >= 1 && M >= 1 && E >= 1 && B <= (1 << 24),
+                "qsx_heom: bad sizes B=%d n_ado=%d M=%d E=%d", B, n_ado, M, E);
+  const bool dense = n == 0;
+  QSX_CHECK_ARG(dense ? M <= 64 : (n >= 1 && n * n == M),
+                "qsx_heom: commutator form needs M = n^2, dense form M <= 64");
+  QSX_CHECK_ARG(M <= kThreads, "qsx_heom: M=%d exceeds one work-group", M);
+  a.B = B;
+  a.n_ado = n_ado;
+  a.M = M;
+  a.n = n;
+  a.dense = dense;
+  a.E = E;
+  a.ado_per_tile = std::max(1, kThreads / M);
+  a.tiles = (n_ado + a.ado_per_tile - 1) / a.ado_per_tile;
+  a.xcd_map = B >= kXcds;
+  a.Hop = (const double2*)Hop;
+  a.shift = shift;
+  a.dvec = (const double2*)dvec;
+  a.esrc = edge_src;
+  a.ew = edge_w;
+  a.ecoef = (const double2*)edge_coef;
+  a.W = (const double2*)W;
+  const int64_t groups = a.xcd_map ? (int64_t)kXcds * ((B + kXcds - 1) / kXcds) * a.tiles
+                                   : (int64_t)B * a.tiles;
+  QSX_CHECK_ARG(groups < (1LL << 31), "qsx_heom: too many work-groups");
+  return QSX_OK;
+}
+
+extern "C" int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n, const void* Hop,
+                                  const double* shift, const void* dvec, int32_t E,
+                                  const int32_t* edge_src, const int32_t* edge_w,
+                                  const void* edge_coef, const void* W, const void* y, void* out,
+                                  void* stream) {
+  using namespace qsx;
+  HeomArgs a;
+  int st = fill_args(a, B, n_ado, M, n, Hop, shift, dvec, E, edge_src, edge_w, edge_coef, W);
+  if (st) return st;
+  QSX_CHECK_ARG(y && out, "qsx_heom_apply: null pointer");
+  const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  const size_t shmem = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * 16;
+  const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;
+  hipLaunchKernelGGL(heom_stage_kernel<double>, dim3(ngroups), dim3(kThreads), shmem,
+                     as_stream(stream), a, (const double2*)y, (double2*)out, nullptr, nullptr,
+                     0.0, 4);
+  QSX_CHECK_HIP(hipGetLastError());
+  return QSX_OK;
+}
+
+extern "C" int64_t qsx_heom_workspace_bytes(int32_t precision, int32_t B, int32_t n_ado,
+                                            int32_t M) {
+  if (B < 1 || n_ado < 1 || M < 1 || (precision != QSX_FP64 && precision != QSX_MIXED)) return -1;
+  const int64_t total = (int64_t)B * n_ado * M;
+  return total * (2 * 16 + 2 * (precision == QSX_FP64 ? 16 : 8));
+}
+
+extern "C" int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t M,
+                                      int32_t n, const void* Hop, const double* shift,
+                                      const void* dvec, int32_t E, const int32_t* edge_src,
+                                      const int32_t* edge_w, const void* edge_coef,
+                                      const void* W, const void* y0, const int32_t* nsub_host,
+                                      const double* h_host, int32_t nsave, void* out,
+                                      void* y_final, void* workspace, int32_t* status,
+                                      void* stream) {
+  using namespace qsx;
+  QSX_CHECK_ARG(precision == QSX_FP64 || precision == QSX_MIXED, "qsx_heom_propagate: precision");
+  QSX_CHECK_ARG(y0 && nsub_host && h_host && out && workspace && status,
+                "qsx_heom_propagate: null pointer");
+  QSX_CHECK_ARG(nsave >= 1, "qsx_heom_propagate: nsave=%d", nsave);
+  HeomArgs a;
+  int rc = fill_args(a, B, n_ado, M, n, Hop, shift, dvec, E, edge_src, edge_w, edge_coef, W);
+  if (rc) return rc;
+  for (int s = 0; s < nsave; ++s)
+    QSX_CHECK_ARG(nsub_host[s] >= 0, "qsx_heom_propagate: negative step count");
+  hipStream_t st = as_stream(stream);
+  if (precision == QSX_FP64)
+    return run_heom<double>(a, (const double2*)y0, nsub_host, h_host, nsave, (double2*)out,
+                            (double2*)y_final, workspace, status, st);
+  return run_heom<float>(a, (const double2*)y0, nsub_host, h_host, nsave, (double2*)out,
+                         (double2*)y_final, workspace, status, st);
+}

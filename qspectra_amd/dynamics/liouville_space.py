@@ -312,6 +312,7 @@ class LiouvilleSpaceModel(DynamicalModel):
             L[:, s[:, None], s[None, :]] = blk[:, l[:, None], l[None, :]]
         return L
 
+    @imemoize
     def equation_of_motion(self, liouville_subspace, heisenberg_picture=False):
         return DeviceEOM(self.liouvillian(liouville_subspace, heisenberg_picture),
                          liouville_subspace, heisenberg_picture)

@@ -4,8 +4,9 @@ import numpy as np
 
 from .. import _native
 from .decorators import optional_ensemble_average
-from .utils import propagate
+from .utils import propagate, propagate_heom
 from ..dynamics.liouville_space import LinearMap
+from ..dynamics.heom import DeviceHEOM
 
 
 @optional_ensemble_average
@@ -20,7 +21,18 @@ def _simulate_dynamics(dynamical_model, initial_state, duration, times, liouvill
         rho = np.outer(rho.conj(), rho)
     y0 = model.density_matrix_to_state_vector(rho, liouville_subspace)
     t = np.arange(0, duration, model.time_step) if times is None else np.asarray(times)
-    kw = {k: integrate_kwargs[k] for k in ('dt', 'precision', 'rtol') if k in integrate_kwargs}
+    kw = {k: integrate_kwargs[k] for k in ('dt', 'precision', 'rtol', 'schedule')
+          if k in integrate_kwargs}
+    if isinstance(eom, DeviceHEOM):
+        # hierarchy: the reduced density matrix is ADO 0 of the state
+        B = eom.batch
+        y = _native.to_device(np.broadcast_to(y0, (B, y0.shape[-1])).copy())
+        ado0 = propagate_heom(eom, y, t, dt=kw.get('dt'),
+                              precision=kw.get('precision', 'fp64')).cpu().numpy()
+        if save_func is not None:
+            return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in ado0])
+        lm = model.lspace_model
+        return t, np.stack([lm.state_vector_to_density_matrix(ob) for ob in ado0])
     proj = None
     if isinstance(save_func, LinearMap):
         proj = _native.to_device(np.atleast_2d(save_func.matrix)[None])

@@ -10,7 +10,10 @@ maps applied at each output time run inside the kernel.  Signals stay on
 the device until the end.
 
 Keyword options in addition to the reference's: ``dt`` (RK4 step),
-``precision`` ('fp64' | 'mixed'), ``rtol`` (default-step accuracy).
+``precision`` ('fp64' | 'mixed'), ``rtol`` (default-step accuracy),
+``schedule`` ('auto' | 'rk4' | 'propagator'), and for third-order responses
+``plan`` ('fused', the default, see fused.py; or 'pathways', one pathway at a
+time as the reference does).
 """
 import copy
 
@@ -20,10 +23,11 @@ import torch
 from .. import _native
 from .decorators import (optional_ensemble_average, optional_2nd_order_isotropic_average,
                          optional_4th_order_isotropic_average)
-from .utils import propagate, fourier_transform
+from .utils import propagate, propagate_heom, fourier_transform
+from ..dynamics.heom import DeviceHEOM
 from ..utils import ZeroArray
 
-_PROPAGATE_KW = ('dt', 'precision', 'rtol')
+_PROPAGATE_KW = ('dt', 'precision', 'rtol', 'schedule')
 
 
 def _propagate_kw(kwargs):
@@ -55,6 +59,7 @@ def _op_matrices(model, liouv_map, polarization, transition, what):
     mats = []
     for h in targets:
         op = _member_model(model, h).dipole_operator(liouv_map, polarization, transition)
+        op = getattr(op, 'lspace_op', op)       # hierarchy operators act per ADO
         mats.append(op.commutator.matrix if what == 'commutator' else op.bra_vector)
     return np.stack(mats)
 
@@ -89,9 +94,18 @@ def _linear_response(dynamical_model, liouv_space_path, time_max, initial_state=
         lead = V_rho0.shape[1:-1]
         proj = _dev(V_rho0.reshape(V_rho0.shape[0], -1, V_rho0.shape[-1]))   # [B|1, K, M1]
         eom = model.equation_of_motion(sim_subspace, heisenberg_picture=True)
-        y0 = _expand(_dev(-bra[:, None, :]), B)                          # [B, 1, M1]
-        out = propagate(eom, y0, t, proj=proj, **kw)                     # [B, 1, nt, K]
-        s = out[:, 0].permute(0, 2, 1).reshape((B,) + lead + (len(t),))
+        if isinstance(eom, DeviceHEOM):
+            # the bra and the initial state live on ADO 0 (heom.py:22-58)
+            y0 = np.zeros((B, eom.size), dtype=complex)
+            y0[:, :bra.shape[-1]] = -bra
+            ado0 = propagate_heom(eom, _dev(y0), t, dt=kw.get('dt'),
+                                  precision=kw.get('precision', 'fp64'))   # [B, nt, M1]
+            out = torch.matmul(ado0, _expand(proj, B).transpose(1, 2).contiguous())
+            s = out.permute(0, 2, 1).reshape((B,) + lead + (len(t),))
+        else:
+            y0 = _expand(_dev(-bra[:, None, :]), B)                      # [B, 1, M1]
+            out = propagate(eom, y0, t, proj=proj, **kw)                 # [B, 1, nt, K]
+            s = out[:, 0].permute(0, 2, 1).reshape((B,) + lead + (len(t),))
         signal += s
     return t, signal.cpu().numpy()
 
@@ -164,6 +178,24 @@ def _third_order_response(dynamical_model, coherence_time_max, population_time_m
                           **integrate_kwargs):
     model = dynamical_model
     kw = _propagate_kw(integrate_kwargs)
+    plan = integrate_kwargs.get('plan', 'fused')
+    if plan == 'pathways':
+        return _pathway_by_pathway(model, coherence_time_max, population_time_max,
+                                   population_times, geometry, polarization, include_signal, kw)
+    if plan != 'fused':
+        raise ValueError("plan must be 'fused' or 'pathways'")
+    from .fused import ThirdOrderPlan
+    p = ThirdOrderPlan(model, coherence_time_max, population_time_max, population_times,
+                       geometry, polarization, include_signal, dt=kw.get('dt'),
+                       precision=kw.get('precision', 'fp64'), schedule=kw.get('schedule', 'auto'))
+    X = p.run()
+    p.check_status()
+    return (p.t1, p.t2, p.t3), X.cpu().numpy()
+
+
+def _pathway_by_pathway(model, coherence_time_max, population_time_max, population_times,
+                        geometry, polarization, include_signal, kw):
+    """Each pathway evaluated on its own, in the reference's order."""
     t1, t2, t3 = third_order_grids(model, coherence_time_max, population_time_max,
                                    population_times)
     B = len(_members(model))

@@ -14,13 +14,16 @@ Extra keyword arguments (all optional):
   rtol       target relative accuracy for the default step (1e-6, zvode's
              default); atol, nsteps and other zvode options are accepted and
              ignored.
+  schedule   'auto' (default), 'rk4' or 'propagator' (see propagate)
 """
 import numpy as np
 import torch
 
 from .. import _native
-from ..engine import schedule_tensors, total_steps
+from ..engine import rk4_schedule, schedule_tensors
+from ..lti import rk4_propagator, rk4_increment, split_diagonal
 from ..dynamics.liouville_space import DeviceEOM, LinearMap
+from ..dynamics.heom import DeviceHEOM
 
 ZVODE_OPTIONS = ('rtol', 'atol', 'nsteps', 'method', 'with_jacobian', 'lband', 'uband',
                  'order', 'first_step', 'min_step', 'max_step')
@@ -47,20 +50,53 @@ def _prepare_precision(L, proj, precision):
         return L, None, proj, _native.QSX_FP64
     if precision != 'mixed':
         raise ValueError("precision must be 'fp64' or 'mixed'")
-    M = L.shape[-1]
-    diag = torch.diagonal(L, dim1=-2, dim2=-1).contiguous()
-    off = L.clone()
-    off.diagonal(dim1=-2, dim2=-1).zero_()
-    Lm = off.to(torch.complex64).contiguous()
+    Lm, diag = split_diagonal(L)
     pm = None if proj is None else proj.to(torch.complex64).contiguous()
-    assert diag.shape[-1] == M
     return Lm, diag, pm, _native.QSX_MIXED
 
 
+def _mode(precision):
+    if precision in ('fp64', 'double', None):
+        return _native.QSX_FP64
+    if precision != 'mixed':
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    return _native.QSX_MIXED
+
+
+def stepper_operator(eom, schedule, h0, precision):
+    """Operator (and FP64 diagonal for MIXED) the register stepper applies,
+    cached on the equation of motion: L for 'rk4', P4(h0 L) (or P - I for
+    MIXED) for 'propagator'."""
+    key = (schedule, float(h0) if schedule == 'propagator' else None, precision)
+    cache = eom.__dict__.setdefault('_operators', {})
+    if key not in cache:
+        if schedule == 'propagator':
+            A = (rk4_propagator(eom.L, h0) if _mode(precision) == _native.QSX_FP64
+                 else rk4_increment(eom.L, h0))
+        else:
+            A = eom.L
+        cache[key] = (A, None) if _mode(precision) == _native.QSX_FP64 else split_diagonal(A)
+    return cache[key]
+
+
+def project_states(states, proj):
+    """out[b, r, s, :] = proj[b] @ states[b, r, s, :] as one batched complex128
+    GEMM (in every precision: dipole maps are a small share of the work)."""
+    B, R, ns, M = states.shape
+    K = proj.shape[-2]
+    projT = proj.transpose(-1, -2).to(torch.complex128).contiguous()
+    out = _native.zgemm(states.reshape(B, R * ns, M).contiguous(), projT)
+    return out.reshape(B, R, ns, K)
+
+
 def propagate(eom, y0, t, t0=None, proj=None, dt=None, precision='fp64', rtol=1e-6,
-              want_final=False):
+              want_final=False, schedule='auto'):
     """Device-level driver: y0 [B, R, M] complex128 tensor, proj [B|1, K, M]
-    tensor or None.  Returns out [B, R, len(t), K] (and the final state)."""
+    tensor or None.  Returns out [B, R, len(t), K] (and the final state).
+
+    schedule: 'rk4' (four Liouvillian products per step), 'propagator' (one
+    product per step with the RK4 propagator; needs a uniform step) or
+    'auto' (the propagator whenever every step has the same length)."""
     t = np.asarray(t, dtype=float).reshape(-1)
     if t0 is None:
         t0 = float(t[0])
@@ -72,14 +108,56 @@ def propagate(eom, y0, t, t0=None, proj=None, dt=None, precision='fp64', rtol=1e
             dt = min(dt, float(spacing.min()))
         if not np.isfinite(dt):
             dt = None
+    n_host, h_host = rk4_schedule(t, t0, dt)
     nsub, h = schedule_tensors(t, t0, dt, eom.L.device)
-    L, diag, pj, mode = _prepare_precision(eom.L, proj, precision)
-    out, yfin, status = _native.rk4_dense(L, y0, nsub, h, pj, mode, diag, want_final,
-                                          steps=total_steps(t, t0, dt))
+    steps = int(n_host.sum())
+    M = eom.L.shape[-1]
+    if M > _native.MAX_REGISTER_BLOCK:
+        L, diag, pj, mode = _prepare_precision(eom.L, proj, precision)
+        out, yfin, status = _native.rk4_dense(L, y0, nsub, h, pj, mode, diag, want_final,
+                                              steps=steps)
+    else:
+        hs = h_host[n_host > 0]
+        uniform = hs.size == 0 or np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0]))
+        if schedule == 'propagator' and not uniform:
+            raise ValueError('the propagator schedule needs a uniform step')
+        if schedule not in ('auto', 'rk4', 'propagator'):
+            raise ValueError("schedule must be 'auto', 'rk4' or 'propagator'")
+        use_prop = schedule == 'propagator' or (schedule == 'auto' and uniform and hs.size)
+        h0 = float(hs[0]) if hs.size else 0.0
+        A, diag = stepper_operator(eom, 'propagator' if use_prop else 'rk4', h0, precision)
+        states, yfin, status = _native.propagate_states(
+            _native.QSX_MODE_PROPAGATOR if use_prop else _native.QSX_MODE_RK4, A, y0, nsub, h,
+            _mode(precision), diag, want_final, steps=steps)
+        out = states if proj is None else project_states(states, proj)
     if int(status.item()) != 0:
         raise IntegratorError('integration failed: non-finite state between t=%g and t=%g'
                               % (t0, t[-1]))
     return (out, yfin) if want_final else out
+
+
+def propagate_heom(eom, y0, t, t0=None, dt=None, precision='fp64', full_states=False):
+    """Hierarchy propagation: y0 [B, n_ado * M] complex128 tensor.  Returns
+    ADO 0 at every output time, [B, len(t), M], or with full_states the
+    whole hierarchy, [B, len(t), n_ado * M] (one launch per output time)."""
+    t = np.asarray(t, dtype=float).reshape(-1)
+    if t0 is None:
+        t0 = float(t[0])
+    n, h = rk4_schedule(t, t0, dt)
+    mode = _mode(precision)
+    if not full_states:
+        out, _, status = _native.heom_propagate(eom, y0, n, h, mode)
+        if int(status.item()) != 0:
+            raise IntegratorError('integration failed: non-finite state')
+        return out
+    states = []
+    y = y0
+    for ni, hi in zip(n, h):
+        _, y, status = _native.heom_propagate(eom, y, [ni], [hi], mode, want_final=True)
+        if int(status.item()) != 0:
+            raise IntegratorError('integration failed: non-finite state')
+        states.append(y)
+    return torch.stack(states, dim=1)
 
 
 def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=None, **kwargs):
@@ -89,6 +167,8 @@ def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=N
 
     ``f`` must be an equation of motion produced by a qspectra_amd model
     (the GPU needs the operator, not a Python callback)."""
+    if isinstance(f, DeviceHEOM):
+        return _integrate_heom(f, y0, t, t0, save_func, kwargs)
     if not isinstance(f, DeviceEOM):
         raise TypeError('integrate needs an equation of motion from a qspectra_amd model '
                         '(DynamicalModel.equation_of_motion); arbitrary Python callables '
@@ -98,6 +178,7 @@ def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=N
     dt = kwargs.pop('dt', None)
     precision = kwargs.pop('precision', 'fp64')
     rtol = kwargs.pop('rtol', 1e-6)
+    schedule = kwargs.pop('schedule', 'auto')
     unknown = set(kwargs) - set(ZVODE_OPTIONS)
     if unknown:
         raise TypeError('unexpected integrate options: %s' % sorted(unknown))
@@ -121,7 +202,8 @@ def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=N
         proj = _native.to_device(np.atleast_2d(mat)[None], dev=dev)
     elif save_func is not None:
         host_save = save_func
-    out = propagate(f, _native.to_device(y, dev=dev), t, t0, proj, dt, precision, rtol)
+    out = propagate(f, _native.to_device(y, dev=dev), t, t0, proj, dt, precision, rtol,
+                    schedule=schedule)
     out = out.cpu().numpy()                      # [B, R, nt, K]
     nt = out.shape[2]
     if scalar:
@@ -132,6 +214,29 @@ def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=N
         saved = np.asarray(saved)
         result = saved.reshape(lead + (nt,) + saved.shape[1:])
     return result
+
+
+def _integrate_heom(f, y0, t, t0, save_func, kwargs):
+    dt = kwargs.pop('dt', None)
+    precision = kwargs.pop('precision', 'fp64')
+    kwargs.pop('rtol', None)
+    kwargs.pop('schedule', None)
+    unknown = set(kwargs) - set(ZVODE_OPTIONS)
+    if unknown:
+        raise TypeError('unexpected integrate options: %s' % sorted(unknown))
+    y0 = np.asarray(y0)
+    lead = y0.shape[:-1]
+    B = f.batch
+    y = y0.reshape(B, f.size) if B > 1 else y0.reshape(-1, f.size)
+    if y.shape[0] != B:
+        raise ValueError('hierarchy states must have a leading axis of %d samples' % B)
+    out = propagate_heom(f, _native.to_device(y), t, t0, dt, precision, full_states=True)
+    out = out.cpu().numpy().reshape(lead + (len(np.atleast_1d(t)), f.size))
+    if save_func is not None:
+        flat = out.reshape(-1, f.size)
+        saved = np.asarray([save_func(v) for v in flat])
+        out = saved.reshape(lead + (len(np.atleast_1d(t)),) + saved.shape[1:])
+    return out
 
 
 # ------------------------------------------------------------ Fourier tools
@@ -166,6 +271,39 @@ def _symmetrize(t, x, axis=-1):
     return t_sym, x_sym
 
 
+def fourier_transform_device(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1,
+                             convention='angular'):
+    """fourier_transform on a complex128 device tensor x (result stays on the
+    device): zero-pad to the symmetric grid, ifftshift, FFT, fftshift."""
+    t = np.asarray(t, dtype=float)
+    if t.ndim != 1:
+        raise ValueError('t must be one dimensional')
+    ax = axis % x.dim()
+    if t.size != x.shape[ax]:
+        raise ValueError('t must have the same length as the shape of x along the given axis')
+    if sign not in (-1, 1):
+        raise ValueError('sign must be +1 or -1')
+    if convention == 'angular':
+        unit_convert = unit_convert / (2 * np.pi)
+    elif convention != 'linear':
+        raise ValueError("convention must be 'angular' or 'linear'")
+    t_sym, _ = _symmetrize(t, np.zeros(t.size), 0)
+    lo = int(np.searchsorted(t_sym, t[0]))
+    N = t_sym.size
+    dt = t_sym[1] - t_sym[0]
+    shape = list(x.shape)
+    shape[ax] = N
+    xs = torch.zeros(shape, dtype=torch.complex128, device=x.device)
+    xs.narrow(ax, lo, t.size).copy_(x)
+    xs *= dt
+    X = torch.fft.fftshift(torch.fft.fft(torch.fft.ifftshift(xs, dim=ax), dim=ax), dim=ax)
+    f = np.fft.fftshift(np.fft.fftfreq(N, dt * unit_convert))
+    if sign == 1:
+        f = -f[::-1]
+        X = torch.flip(X, dims=[ax])
+    return f + rw_freq, X
+
+
 def fourier_transform(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1, convention='angular'):
     """X(w) = int exp(i sign (w - rw) t) x(t) dt on the symmetric, zero-padded
     grid (qspectra/simulate/utils.py:154-219); the FFT runs on the GPU."""
@@ -175,23 +313,11 @@ def fourier_transform(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1, conventi
     x = np.asarray(x)
     if t.size != x.shape[axis]:
         raise ValueError('t must have the same length as the shape of x along the given axis')
-    if sign not in (-1, 1):
-        raise ValueError('sign must be +1 or -1')
-    if convention == 'angular':
-        unit_convert = unit_convert / (2 * np.pi)
-    elif convention != 'linear':
-        raise ValueError("convention must be 'angular' or 'linear'")
-    t, x = _symmetrize(t, x, axis)
-    N = x.shape[axis]
-    dt = t[1] - t[0]
-    f = np.fft.fftshift(np.fft.fftfreq(N, dt * unit_convert))
-    xd = _native.to_device(x * dt)
-    ax = axis % x.ndim
-    X = torch.fft.fftshift(torch.fft.fft(torch.fft.ifftshift(xd, dim=ax), dim=ax), dim=ax)
-    if sign == 1:
-        f = -f[::-1]
-        X = torch.flip(X, dims=[ax])
-    return f + rw_freq, X.cpu().numpy()
+    if not is_constant(np.diff(t), positive=True):
+        raise ValueError('sample times must differ by a positive constant')
+    f, X = fourier_transform_device(t, _native.to_device(x), axis, rw_freq, unit_convert, sign,
+                                    convention)
+    return f, X.cpu().numpy()
 
 
 def bound_signal(ticks, signal, bounds, axis=0):

@@ -189,3 +189,34 @@ def test_isotropic_third_order_dimer(gpu):
         md, 8.5, population_times=[0., 3., 7.], exact_isotropic_average=True,
         polarization=[0, 0, qa.MAGIC_ANGLE, qa.MAGIC_ANGLE], dt=1.0 / 16)
     assert rel_err(X, gt['dimer_iso_R3']) < 1e-7
+
+
+def test_fused_plan_and_graph_replay(gpu):
+    """The fused plan equals the pathway-by-pathway evaluation; its HIP-graph
+    replays equal the eager launch every time, with eager work in between."""
+    from qspectra_amd.simulate.fused import ThirdOrderPlan
+    gt = golden('spectra_third_order')
+    _, inp = _inputs('redfield_7site')
+    m7 = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=False)
+    _, ref = qa.third_order_response(m7, 2 * 15.5, population_times=gt['t2'], dt=2.0,
+                                     schedule='rk4', plan='pathways')
+    plan = ThirdOrderPlan(m7, 2 * 15.5, population_times=gt['t2'], dt=2.0)
+    eager = plan.run()[0].cpu().numpy()
+    plan.check_status()
+    assert rel_err(eager, ref) < 1e-10
+    plan.capture()
+    for _ in range(3):
+        X = plan.run()[0].cpu().numpy()
+        assert rel_err(X, eager) < 1e-13
+        qa.absorption_spectra(m7, 2 * 31.5, dt=2.0)       # unrelated eager work
+    pm = ThirdOrderPlan(m7, 2 * 15.5, population_times=gt['t2'], dt=2.0,
+                        precision='mixed').capture()
+    Xm = pm.run()[0].cpu().numpy()
+    pm.check_status()
+    assert rel_err(Xm, ref) < 1e-6
+    for geom in ('+-+', '++-'):
+        _, a = qa.third_order_response(m7, 2 * 15.5, population_times=gt['t2'], dt=2.0,
+                                       geometry=geom, plan='pathways', schedule='rk4')
+        _, b = qa.third_order_response(m7, 2 * 15.5, population_times=gt['t2'], dt=2.0,
+                                       geometry=geom)
+        assert rel_err(b, a) < 1e-10, geom

@@ -144,3 +144,59 @@ def test_non_finite_state_sets_status(gpu):
     out, _, st = _native.rk4_dense(_native.to_device(L[None]), _native.to_device(y0), nsub, h)
     torch.cuda.synchronize()
     assert int(st.item()) == 1
+
+
+@pytest.mark.parametrize('M', [1, 2, 7, 13, 24, 49, 100, 147])
+def test_register_stepper_all_shapes(gpu, M):
+    """qsx_propagate in RK4 and propagator modes, FP64 and MIXED, for every
+    row-segment shape (M = 147 keeps 19 rows in LDS)."""
+    from qspectra_amd import _native, engine, lti
+    rs = np.random.RandomState(M)
+    L = (rs.randn(2, M, M) + 1j * rs.randn(2, M, M)) * 0.02 - 0.05j * np.eye(M)
+    y0 = rs.randn(2, 3, M) + 1j * rs.randn(2, 3, M)
+    t = np.arange(0, 40.0, 1.0)
+    nsub, h = engine.schedule_tensors(t, 0.0, 0.5)
+    ref = np.stack([P.integrate_rk4(L[b], y0[b], t, t0=0.0, dt=0.5) for b in range(2)])
+    Ld = _native.to_device(L)
+    y = _native.to_device(y0)
+    out, _, st = _native.propagate_states(_native.QSX_MODE_RK4, Ld, y, nsub, h)
+    assert int(st.item()) == 0
+    assert rel_err(out.cpu().numpy(), ref) < 1e-12, 'rk4 fp64'
+    off, dg = lti.split_diagonal(Ld)
+    out, _, _ = _native.propagate_states(_native.QSX_MODE_RK4, off, y, nsub, h,
+                                         _native.QSX_MIXED, dg)
+    assert rel_err(out.cpu().numpy(), ref) < 2e-6, 'rk4 mixed'
+    Pm = lti.rk4_propagator(Ld, 0.5)
+    out, _, _ = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, Pm, y, nsub, h)
+    assert rel_err(out.cpu().numpy(), ref) < 1e-11, 'propagator fp64'
+    off, dg = lti.split_diagonal(lti.rk4_increment(Ld, 0.5))
+    out, _, _ = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, off, y, nsub, h,
+                                         _native.QSX_MIXED, dg)
+    assert rel_err(out.cpu().numpy(), ref) < 2e-6, 'propagator mixed'
+
+
+def test_zgemm_and_propagator(gpu):
+    from qspectra_amd import _native, lti
+    rs = np.random.RandomState(5)
+    for (b, m, n, k) in [(1, 1, 1, 1), (3, 49, 128, 49), (2, 147, 49, 147), (1, 33, 65, 17)]:
+        A = rs.randn(b, m, k) + 1j * rs.randn(b, m, k)
+        B = rs.randn(b, k, n) + 1j * rs.randn(b, k, n)
+        C0 = rs.randn(b, m, n) + 1j * rs.randn(b, m, n)
+        C = _native.zgemm(_native.to_device(A), _native.to_device(B), _native.to_device(C0),
+                          alpha=0.5 - 1j, beta=2.0, gamma=(0.25 if m == n else 0.0))
+        ref = (0.5 - 1j) * A @ B + 2.0 * C0 + (0.25 * np.eye(m) if m == n else 0)
+        assert rel_err(C.cpu().numpy(), ref) < 1e-14
+        Cm = _native.zgemm(_native.to_device(A, torch.complex64),
+                           _native.to_device(B, torch.complex64), precision=_native.QSX_MIXED)
+        assert rel_err(Cm.cpu().numpy(), A @ B) < 2e-6
+    A = rs.randn(1, 20, 20) + 1j * rs.randn(1, 20, 20)
+    B = rs.randn(4, 20, 8) + 1j * rs.randn(4, 20, 8)
+    C = _native.zgemm(_native.to_device(A), _native.to_device(B))
+    assert rel_err(C.cpu().numpy(), A @ B) < 1e-14
+    L = (rs.randn(2, 30, 30) + 1j * rs.randn(2, 30, 30)) * 0.1
+    h = 0.3
+    hL = h * L
+    P4 = np.eye(30) + hL + hL @ hL / 2 + hL @ hL @ hL / 6 + hL @ hL @ hL @ hL / 24
+    assert rel_err(lti.rk4_propagator(_native.to_device(L), h).cpu().numpy(), P4) < 1e-14
+    assert rel_err(lti.rk4_increment(_native.to_device(L), h).cpu().numpy(),
+                   P4 - np.eye(30)) < 1e-14
