@@ -54,8 +54,9 @@ def run_plain(qa, model, work):
 
 class FusedSpectrum(object):
     """The batched path: fused plan (merged pathways, t3 on a second stream,
-    one-product-per-step RK4 propagator, complex64 products with FP64 state)
-    replayed from a HIP graph, then the two FFTs on the device."""
+    RK4 propagator schedules: cached propagator powers for the t2 stage and
+    time-chunked stepping for t1/t3, complex64 products with FP64 state), the
+    whole 2-D spectrum including both FFTs replayed from one HIP graph."""
 
     def __init__(self, model, work, precision='mixed'):
         from qspectra_amd.simulate.fused import ThirdOrderPlan
@@ -64,15 +65,10 @@ class FusedSpectrum(object):
                                    population_times=work['population_times'],
                                    geometry=work['geometry'],
                                    include_signal=work['include_signal'], dt=work['dt'],
-                                   precision=precision).capture()
+                                   precision=precision).capture(two_dimensional=True)
 
     def __call__(self):
-        from qspectra_amd.simulate.utils import fourier_transform_device
-        p, m = self.plan, self.model
-        X = p.run()[0]
-        _, X = fourier_transform_device(p.t1, X, 0, m.rw_freq, m.unit_convert, sign=-1)
-        _, X = fourier_transform_device(p.t3, X, 2, m.rw_freq, m.unit_convert)
-        return X
+        return self.plan.run()[0]
 
 
 def time_leg(torch, fn, steps):

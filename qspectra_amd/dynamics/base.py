@@ -58,7 +58,12 @@ class DynamicalModel(metaclass=ABCMeta):
 
     @property
     def time_step(self):
-        """Default output time step (rotating frame), in time units."""
+        """Default output time step (rotating frame), in time units; a
+        ``time_step_override`` attribute (set by batched drivers so that
+        systems of different size share one grid) takes precedence."""
+        override = self.__dict__.get('time_step_override')
+        if override is not None:
+            return override
         return self.hamiltonian.time_step / self.unit_convert
 
     def hilbert_subspace_index(self, subspace):

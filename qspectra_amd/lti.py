@@ -32,6 +32,51 @@ def rk4_increment(L, h):
     return _native.zgemm(L, T, alpha=h)
 
 
+def _cache(eom):
+    return eom.__dict__.setdefault('_operators', {})
+
+
+def cached_propagator(eom, h):
+    """P4(h L) of an equation of motion, computed once (FP64)."""
+    c = _cache(eom)
+    key = ('P', float(h))
+    if key not in c:
+        c[key] = rk4_propagator(eom.L, h)
+    return c[key]
+
+
+def propagator_power(eom, h, n):
+    """P4(h L)^n by repeated squaring (FP64), cached; equal to n classical
+    RK4 steps of size h up to rounding."""
+    c = _cache(eom)
+    key = ('Pn', float(h), int(n))
+    if key in c:
+        return c[key]
+    if n < 1:
+        raise ValueError('power must be positive')
+    result, k = None, 0
+    while (1 << k) <= n:
+        sq_key = ('P2k', float(h), k)
+        if sq_key not in c:
+            c[sq_key] = (cached_propagator(eom, h) if k == 0
+                         else _native.zgemm(c[('P2k', float(h), k - 1)],
+                                            c[('P2k', float(h), k - 1)]))
+        if n & (1 << k):
+            result = c[sq_key] if result is None else _native.zgemm(c[sq_key], result)
+        k += 1
+    c[key] = result
+    return result
+
+
+def transposed(eom, key, A):
+    """Cached contiguous transpose (for right-multiplying row states)."""
+    c = _cache(eom)
+    tkey = ('T',) + tuple(key)
+    if tkey not in c:
+        c[tkey] = A.transpose(-1, -2).contiguous()
+    return c[tkey]
+
+
 def split_diagonal(A):
     """(off-diagonal part as complex64, diagonal as complex128 [B, M])."""
     diag = torch.diagonal(A, dim1=-2, dim2=-1).contiguous()

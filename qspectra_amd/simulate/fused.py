@@ -23,36 +23,8 @@ from .. import _native
 from ..engine import rk4_schedule
 from .response import (_members, _op_matrices, _parse_pathways, THIRD_ORDER_PATHWAYS,
                        third_order_grids)
-from .utils import stepper_operator, _mode
-
-
-class _Stage(object):
-    """One propagation: operator, schedule and launch mode on the device."""
-
-    def __init__(self, eom, t, t0, dt, precision, schedule):
-        n, h = rk4_schedule(t, t0, dt)
-        hs = h[n > 0]
-        uniform = hs.size == 0 or np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0]))
-        prop = schedule == 'propagator' or (schedule == 'auto' and uniform and hs.size)
-        if prop and not uniform:
-            raise ValueError('the propagator schedule needs a uniform step')
-        if eom.L.shape[-1] > _native.MAX_REGISTER_BLOCK:
-            raise ValueError('block of %d states exceeds the register stepper'
-                             % eom.L.shape[-1])
-        self.mode = _native.QSX_MODE_PROPAGATOR if prop else _native.QSX_MODE_RK4
-        self.A, self.diag = stepper_operator(eom, 'propagator' if prop else 'rk4',
-                                             float(hs[0]) if hs.size else 0.0, precision)
-        dev = eom.L.device
-        self.nsub = torch.as_tensor(n.astype(np.int32)).to(dev)
-        self.h = torch.as_tensor(h.astype(np.float64)).to(dev)
-        self.steps = int(n.sum())
-        self.precision = _mode(precision)
-
-    def run(self, y0, statuses):
-        out, _, st = _native.propagate_states(self.mode, self.A, y0, self.nsub, self.h,
-                                              self.precision, self.diag, steps=self.steps)
-        statuses.append(st)
-        return out
+from .utils import _mode
+from ..schedules import Stage as _Stage
 
 
 class ThirdOrderPlan(object):
@@ -92,10 +64,10 @@ class ThirdOrderPlan(object):
                 self.groups[key12] = dict(
                     y1=dev_t(y1),
                     stage1=_Stage(model.equation_of_motion(s[1]), self.t1, None, dt, precision,
-                                  schedule),
+                                  schedule, 1),
                     V1T=dev_t(V1.transpose(0, 2, 1), self.gemm_dt),
                     stage2=_Stage(model.equation_of_motion(s[2]), self.t2, 0.0, dt, precision,
-                                  schedule),
+                                  schedule, len(self.t1)),
                     branches=[])
             key3 = tuple(s[3:])
             if key3 not in self.t3_stages:
@@ -104,7 +76,7 @@ class ThirdOrderPlan(object):
                 b3 = np.broadcast_to(b3, (B,) + b3.shape[1:])
                 self.t3_stages[key3] = (
                     _Stage(model.equation_of_motion(s[3], heisenberg_picture=True), self.t3,
-                           None, dt, precision, schedule), dev_t(b3))
+                           None, dt, precision, schedule, 1), dev_t(b3))
             V2 = _op_matrices(model, '%s->%s' % (s[2], s[3]), polarization[2], trans[2],
                               'commutator')
             self.groups[key12]['branches'].append((dev_t(V2.transpose(0, 2, 1), self.gemm_dt),
@@ -147,22 +119,47 @@ class ThirdOrderPlan(object):
         self._statuses = statuses
         return total.reshape(B, n1, n2, n3)
 
-    def capture(self):
-        """Records the launch sequence into a HIP graph (replayed by run)."""
-        self._launch()                      # warm up allocations and caches
+    def _launch_2d(self):
+        """The response followed by the two Fourier transforms of
+        two_dimensional_spectra (t1 with sign -1, then t3), on the device."""
+        from .utils import fourier_transform_device
+        m = self.model
+        X = self._launch()
+        _, X = fourier_transform_device(self.t1, X, 1, m.rw_freq, m.unit_convert, sign=-1)
+        _, X = fourier_transform_device(self.t3, X, 3, m.rw_freq, m.unit_convert)
+        return X
+
+    def capture(self, two_dimensional=False):
+        """Records the launch sequence (optionally with the 2-D Fourier
+        transforms) into a HIP graph, replayed by run()."""
+        fn = self._launch_2d if two_dimensional else self._launch
+        fn()                                # warm up allocations, caches, FFT plans
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self._out = self._launch()
+            self._out = fn()
         self._graph = g
+        self._captured_2d = two_dimensional
         return self
 
     def run(self):
-        """Device tensor [B, n1, n2, n3]; check_status() raises on overflow."""
+        """Device tensor [B, n1, n2, n3] (or the 2-D spectrum [B, N1, n2, N3]
+        when captured with two_dimensional=True); check_status() raises on
+        a non-finite state."""
         if self._graph is not None:
             self._graph.replay()
             return self._out
         return self._launch()
+
+    def frequencies(self):
+        """(f1, t2, f3) axes of the captured 2-D spectrum."""
+        from .utils import fourier_transform_device
+        m = self.model
+        z1 = torch.zeros(len(self.t1), dtype=torch.complex128, device=self._side.device)
+        f1, _ = fourier_transform_device(self.t1, z1, 0, m.rw_freq, m.unit_convert, sign=-1)
+        z3 = torch.zeros(len(self.t3), dtype=torch.complex128, device=self._side.device)
+        f3, _ = fourier_transform_device(self.t3, z3, 0, m.rw_freq, m.unit_convert)
+        return f1, self.t2, f3
 
     def check_status(self):
         from .utils import IntegratorError

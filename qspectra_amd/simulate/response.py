@@ -72,10 +72,9 @@ def _expand(x, B):
     return x if x.shape[0] == B else x.expand((B,) + tuple(x.shape[1:])).contiguous()
 
 
-@optional_ensemble_average
-@optional_2nd_order_isotropic_average
-def _linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,
-                     polarization='xx', **integrate_kwargs):
+def _batched_linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,
+                             polarization='xx', **integrate_kwargs):
+    """Linear response of a (possibly batched) model: (t, S [B, ..., nt])."""
     model = dynamical_model
     kw = _propagate_kw(integrate_kwargs)
     subspaces = liouv_space_path.split('->')
@@ -108,6 +107,10 @@ def _linear_response(dynamical_model, liouv_space_path, time_max, initial_state=
             s = out[:, 0].permute(0, 2, 1).reshape((B,) + lead + (len(t),))
         signal += s
     return t, signal.cpu().numpy()
+
+
+_linear_response = optional_ensemble_average(
+    optional_2nd_order_isotropic_average(_batched_linear_response))
 
 
 def linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,

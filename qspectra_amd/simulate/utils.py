@@ -21,7 +21,8 @@ import torch
 
 from .. import _native
 from ..engine import rk4_schedule, schedule_tensors
-from ..lti import rk4_propagator, rk4_increment, split_diagonal
+from ..lti import split_diagonal
+from ..schedules import Stage, stepper_operator, _mode
 from ..dynamics.liouville_space import DeviceEOM, LinearMap
 from ..dynamics.heom import DeviceHEOM
 
@@ -55,30 +56,6 @@ def _prepare_precision(L, proj, precision):
     return Lm, diag, pm, _native.QSX_MIXED
 
 
-def _mode(precision):
-    if precision in ('fp64', 'double', None):
-        return _native.QSX_FP64
-    if precision != 'mixed':
-        raise ValueError("precision must be 'fp64' or 'mixed'")
-    return _native.QSX_MIXED
-
-
-def stepper_operator(eom, schedule, h0, precision):
-    """Operator (and FP64 diagonal for MIXED) the register stepper applies,
-    cached on the equation of motion: L for 'rk4', P4(h0 L) (or P - I for
-    MIXED) for 'propagator'."""
-    key = (schedule, float(h0) if schedule == 'propagator' else None, precision)
-    cache = eom.__dict__.setdefault('_operators', {})
-    if key not in cache:
-        if schedule == 'propagator':
-            A = (rk4_propagator(eom.L, h0) if _mode(precision) == _native.QSX_FP64
-                 else rk4_increment(eom.L, h0))
-        else:
-            A = eom.L
-        cache[key] = (A, None) if _mode(precision) == _native.QSX_FP64 else split_diagonal(A)
-    return cache[key]
-
-
 def project_states(states, proj):
     """out[b, r, s, :] = proj[b] @ states[b, r, s, :] as one batched complex128
     GEMM (in every precision: dipole maps are a small share of the work)."""
@@ -95,8 +72,9 @@ def propagate(eom, y0, t, t0=None, proj=None, dt=None, precision='fp64', rtol=1e
     tensor or None.  Returns out [B, R, len(t), K] (and the final state).
 
     schedule: 'rk4' (four Liouvillian products per step), 'propagator' (one
-    product per step with the RK4 propagator; needs a uniform step) or
-    'auto' (the propagator whenever every step has the same length)."""
+    product per step with the RK4 propagator; needs a uniform step),
+    'blocked' (propagator powers / time chunks, see schedules.Stage) or
+    'auto' (blocked whenever every step has the same length)."""
     t = np.asarray(t, dtype=float).reshape(-1)
     if t0 is None:
         t0 = float(t[0])
@@ -117,18 +95,16 @@ def propagate(eom, y0, t, t0=None, proj=None, dt=None, precision='fp64', rtol=1e
         out, yfin, status = _native.rk4_dense(L, y0, nsub, h, pj, mode, diag, want_final,
                                               steps=steps)
     else:
-        hs = h_host[n_host > 0]
-        uniform = hs.size == 0 or np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0]))
-        if schedule == 'propagator' and not uniform:
-            raise ValueError('the propagator schedule needs a uniform step')
-        if schedule not in ('auto', 'rk4', 'propagator'):
-            raise ValueError("schedule must be 'auto', 'rk4' or 'propagator'")
-        use_prop = schedule == 'propagator' or (schedule == 'auto' and uniform and hs.size)
-        h0 = float(hs[0]) if hs.size else 0.0
-        A, diag = stepper_operator(eom, 'propagator' if use_prop else 'rk4', h0, precision)
-        states, yfin, status = _native.propagate_states(
-            _native.QSX_MODE_PROPAGATOR if use_prop else _native.QSX_MODE_RK4, A, y0, nsub, h,
-            _mode(precision), diag, want_final, steps=steps)
+        if schedule not in ('auto', 'rk4', 'propagator', 'blocked'):
+            raise ValueError("schedule must be 'auto', 'rk4', 'propagator' or 'blocked'")
+        stage = Stage(eom, t, t0, dt, precision, schedule, y0.shape[1])
+        statuses = []
+        states = stage.run(y0, statuses)
+        yfin = states[:, :, -1].contiguous() if want_final else None
+        status = sum(statuses) if statuses else torch.zeros(1, dtype=torch.int32,
+                                                            device=y0.device)
+        if not statuses and not bool(torch.isfinite(torch.view_as_real(states[:, :, -1])).all()):
+            status = torch.ones(1, dtype=torch.int32, device=y0.device)
         out = states if proj is None else project_states(states, proj)
     if int(status.item()) != 0:
         raise IntegratorError('integration failed: non-finite state between t=%g and t=%g'
