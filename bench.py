@@ -40,6 +40,8 @@ def parse_args(argv=None):
     ap.add_argument('--cpu-baseline', choices=('auto', 'off'), default='auto')
     ap.add_argument('--cpu-budget', type=float, default=30.0,
                     help='seconds of CPU work allowed for the baseline sample')
+    ap.add_argument('--workloads', choices=('all', 'none'), default='all',
+                    help='also measure the ensemble-absorption, gradient and HEOM workloads')
     return ap.parse_args(argv)
 
 
@@ -53,10 +55,11 @@ def run_plain(qa, model, work):
 
 
 class FusedSpectrum(object):
-    """The batched path: fused plan (merged pathways, t3 on a second stream,
-    RK4 propagator schedules: cached propagator powers for the t2 stage and
-    time-chunked stepping for t1/t3, complex64 products with FP64 state), the
-    whole 2-D spectrum including both FFTs replayed from one HIP graph."""
+    """The batched path: fused plan (one shared t1 trajectory; the t2 stage
+    on the rows of the V1 map with cached RK4 propagator powers; t3
+    trajectories on a second stream; time-chunked stepping with complex64
+    products and FP64 state; both Fourier transforms as GEMMs with the
+    transform matrices), the whole 2-D spectrum replayed from one HIP graph."""
 
     def __init__(self, model, work, precision='mixed'):
         from qspectra_amd.simulate.fused import ThirdOrderPlan
@@ -65,7 +68,7 @@ class FusedSpectrum(object):
                                    population_times=work['population_times'],
                                    geometry=work['geometry'],
                                    include_signal=work['include_signal'], dt=work['dt'],
-                                   precision=precision).capture(two_dimensional=True)
+                                   precision=precision, fourier=True).capture()
 
     def __call__(self):
         return self.plan.run()[0]
@@ -132,6 +135,129 @@ def cpu_baseline(budget, steps_per_spectrum, min_seconds=10.0):
                        'one host thread' % (runs, done_steps, elapsed))
 
 
+HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
+
+
+def _sync_time(torch, fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def ensemble_absorption_leg(torch):
+    """4096 static-disorder members (sigma = 100 cm^-1) of the 7-site aggregate
+    plus 1024 aggregates of 2..24 sites, Redfield 'ge', absorption over 4096
+    points at 1 fs, one batch per site count; end to end from the
+    Hamiltonians (GPU eigensolver + Redfield build + propagation + FFT)."""
+    from qspectra_amd import workloads as W
+    from qspectra_amd.batched import absorption_batch
+    hams = W.ensemble_absorption_hamiltonians()
+    rw, tmax = 12450.0, 4096.0
+    absorption_batch(hams[:32] + hams[-32:], 64.0, rw_freq=rw, dt=1.0, precision='mixed')
+    res = {}
+    for prec in ('mixed', 'fp64'):
+        res[prec] = _sync_time(torch, lambda: absorption_batch(hams, tmax, rw_freq=rw, dt=1.0,
+                                                               precision=prec))
+    X32, X64 = res['mixed'][1][1], res['fp64'][1][1]
+    err = np.max(np.abs(X32 - X64), axis=1) / np.max(np.abs(X64), axis=1)
+    steps = len(hams) * (int(tmax) - 1)
+    return {'workload': 'absorption of 4096 disorder members (sigma 100 cm^-1) of the 7-site '
+                        'aggregate + 1024 aggregates of N ~ U{2..24} sites, Redfield ge, 4096 '
+                        'points at 1 fs',
+            'rk4_steps': steps, 'value': steps / res['mixed'][0], 'unit': 'steps/s',
+            'seconds_mixed': res['mixed'][0], 'seconds_fp64': res['fp64'][0],
+            'fp64_value': steps / res['fp64'][0],
+            'rel_err_vs_fp64_max': float(err.max()), 'rel_err_vs_fp64_median':
+                float(np.median(err)), 'scope': 'end to end from the Hamiltonians'}
+
+
+def gradient_leg(torch):
+    """Gradient of sum_w (S - S_target)^2 for the ensemble-averaged absorption
+    spectrum (512 disorder members, 4096 RK4 steps at 1 fs, checkpoints every
+    64 steps) with respect to the 7 site energies, 21 couplings and lambda;
+    checked against central differences (h = 1e-3 cm^-1) of the FP64
+    forward pass."""
+    from qspectra_amd import gradients as G, workloads as W
+    from qspectra_amd.constants import CM_K, GAUSSIAN_SD_FWHM
+    (E0, J0, l0), (E1, J1, l1) = W.gradient_target_parameters()
+    _, d, _ = W.aggregate7_hamiltonian()
+    spec = G.RedfieldAbsorption(d, 106.0, CM_K * 77, 12450.0, 4096.0, 1.0,
+                                disorder=100.0 / GAUSSIAN_SD_FWHM, ensemble_size=512, ckpt=64)
+    dev = torch.device('cuda')
+    mk = lambda x, g=False: torch.tensor(np.asarray(x, float), device=dev, requires_grad=g)
+    with torch.no_grad():
+        target = spec(mk(E1), mk(J1), mk(l1))[1]
+
+    def loss(E, J, lam):
+        return ((spec(E, J, lam)[1] - target) ** 2).sum()
+
+    def fwd_bwd():
+        th = [mk(E0, True), mk(J0, True), mk(l0, True)]
+        loss(*th).backward()
+        return torch.cat([th[0].grad, th[1].grad, th[2].grad[None]])
+
+    fwd_bwd()
+    t_fb, grad = _sync_time(torch, fwd_bwd)
+    t_f, _ = _sync_time(torch, lambda: loss(mk(E0), mk(J0), mk(l0)).item())
+    theta = [mk(E0), mk(J0), mk(l0)]
+    fd, h = [], 1e-3
+    with torch.no_grad():
+        for w, tt in enumerate(theta):
+            for i in range(tt.numel()):
+                vals = []
+                for sgn in (1.0, -1.0):
+                    th = [x.clone() for x in theta]
+                    th[w].reshape(-1)[i] += sgn * h
+                    vals.append(float(loss(*th)))
+                fd.append((vals[0] - vals[1]) / (2 * h))
+    fd = np.array(fd)
+    g = grad.cpu().numpy()
+    steps = 512 * 4095
+    return {'workload': 'gradient of the squared spectral error of a 512-member ensemble '
+                        'absorption spectrum (7-site Redfield ge, 4096 RK4 steps at 1 fs, '
+                        'checkpoints every 64) w.r.t. 7 E, 21 J, lambda',
+            'seconds_forward_backward': t_fb, 'seconds_forward': t_f,
+            'value': 3 * steps / t_fb, 'unit': 'steps/s (forward + reverse, 3 per step)',
+            'grad_rel_err_vs_central_fd': float(np.abs(g - fd).max() / np.abs(fd).max()),
+            'fd_step_cm': h, 'parameters': int(g.size)}
+
+
+def heom_leg(torch):
+    """HEOM dynamics of the 7-site aggregate (300 K, K = 1, depth 4: 3060
+    ADOs, 'ee' block, M = 49), 256 disorder members, 1 ps at 0.5 fs from a
+    site-1 excitation, FP64 and mixed."""
+    from qspectra_amd import workloads as W
+    from qspectra_amd.simulate.utils import propagate_heom
+    m = W.heom_aggregate7_model()
+    members = [m.hamiltonian.sample(n) for n in range(256)]
+    eom = m.with_hamiltonians(members).equation_of_motion('ee')
+    y0 = torch.zeros((256, eom.size), dtype=torch.complex128, device='cuda')
+    y0[:, 0] = 1.0                                # |1><1|: element (0, 0) of ADO 0
+    t = np.arange(0, 1000.0 + 1e-9, 10.0)
+    propagate_heom(eom, y0, t[:2], dt=0.5)
+    res = {}
+    for prec in ('fp64', 'mixed'):
+        res[prec] = _sync_time(torch, lambda: propagate_heom(eom, y0, t, dt=0.5,
+                                                             precision=prec))
+    r64, r32 = res['fp64'][1], res['mixed'][1]
+    err = float((r32 - r64).abs().max() / r64.abs().max())
+    S = eom.size
+    steps = 2000
+    out = {'workload': 'HEOM 7-site aggregate, 300 K, K=1, depth 4 (3060 ADOs x 49), 256 '
+                       'disorder members, 2000 RK4 steps at 0.5 fs',
+           'ados': int(eom.n_ado), 'state_elements': int(S),
+           'hierarchy_steps': 256 * steps, 'rel_err_mixed_vs_fp64': err}
+    for prec in ('fp64', 'mixed'):
+        sec = res[prec][0]
+        gbs = 256 * S * HEOM_BYTES_PER_ELEMENT_STEP[prec] * steps / sec / 1e9
+        out[prec] = {'seconds': sec, 'hierarchy_steps_per_s': 256 * steps / sec,
+                     'achieved_GBps': gbs, 'hbm_frac': gbs / 8000.0,
+                     'bytes_per_element_step': HEOM_BYTES_PER_ELEMENT_STEP[prec]}
+    return out
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
@@ -187,6 +313,12 @@ def main(argv=None):
         'kernels': {k: {'count': v['count'], 'mean_ms': round(v['mean_ms'], 5)}
                     for k, v in summary.items()},
     }
+    if args.workloads == 'all':
+        line['workloads'] = {}
+        for name, leg in (('ensemble_absorption', ensemble_absorption_leg),
+                          ('adjoint_gradient', gradient_leg), ('heom', heom_leg)):
+            print('bench: %s' % name, file=sys.stderr, flush=True)
+            line['workloads'][name] = leg(torch)
     if args.cpu_baseline == 'auto':
         line['cpu_baseline'] = cpu_baseline(args.cpu_budget, steps_per_spectrum)
         line['vs_cpu'] = value / line['cpu_baseline']['value']

@@ -158,20 +158,43 @@ int32_t qsx_propagate(int32_t mode, int32_t precision, const void* A, int64_t st
                       int32_t R, int32_t B, const int32_t* nsub, const double* h,
                       int32_t nsave, void* out, void* y_final, int32_t* status, void* stream);
 
-/* Batched complex GEMM, row-major:
- *   C[b] = alpha * A[b] (m x k) * B[b] (k x n) + beta * C[b] + gamma * I
- * (gamma only for m == n).  precision QSX_FP64: complex128 operands;
- * QSX_MIXED: complex64 A and B, FP32 products, complex128 C.  Leading
- * dimensions lda/ldb/ldc (elements), batch strides sA/sB/sC (0 = shared;
- * sC must be nonzero when batch > 1).  alpha, beta, gamma are complex
- * (re, im) pairs passed by value.  Used for the RK4 propagator (Horner:
- * P = I + hL(I + hL/2 (I + hL/3 (I + hL/4)))), its powers, and the
- * save-point dipole maps and response contractions of response.py. */
-int32_t qsx_zgemm(int32_t precision, int32_t m, int32_t n, int32_t k, double alpha_re,
-                  double alpha_im, const void* A, int64_t lda, int64_t sA, const void* B,
-                  int64_t ldb, int64_t sB, double beta_re, double beta_im, void* C,
-                  int64_t ldc, int64_t sC, double gamma_re, double gamma_im, int32_t batch,
-                  void* stream);
+/* Reverse mode of qsx_propagate in QSX_MODE_PROPAGATOR (FP64): given the
+ * one-step propagator A = P [B|1][M][M], y0 [B][R][M], the step counts nsub
+ * (device) and the cotangents gout [B][R][nsave][M] of the saved states
+ * (dLoss/d conj(y)), writes gA [B][R][M][M] = sum over steps of
+ * lambda y^H (one gradient per right-hand side; the caller sums over R,
+ * and over B when A is shared) and gy0 [B][R][M] (optional).  The forward
+ * states are recomputed from checkpoints every ckpt steps held in the
+ * workspace (qsx_adjoint_workspace_bytes; total_steps = sum of nsub).
+ * status: 1 non-finite gradient, 2 workspace smaller than the checkpoints.
+ * Blocks up to M = 64; ckpt * M * 16 bytes must fit the LDS.  The gradient
+ * path of the reference's spectra (absorption_spectra,
+ * qspectra/simulate/response.py:103-154), which has no reverse mode. */
+int64_t qsx_adjoint_workspace_bytes(int32_t M, int32_t R, int32_t B, int64_t total_steps,
+                                    int32_t ckpt);
+int32_t qsx_propagate_adjoint(const void* A, int64_t strideA, int32_t M, const void* y0,
+                              int32_t R, int32_t B, const int32_t* nsub, int32_t nsave,
+                              const void* gout, int32_t ckpt, void* workspace,
+                              int64_t workspace_bytes, void* gA, void* gy0, int32_t* status,
+                              void* stream);
+
+/* Batched complex GEMM, row-major storage:
+ *   C[b] = alpha * op(A[b]) (m x k) * op(B[b]) (k x n) + beta * C[b] + gamma * I
+ * with op(A) = A^T when trans & 1 (A stored k x m) and op(B) = B^T when
+ * trans & 2 (B stored n x k); gamma only for m == n.  precision QSX_FP64:
+ * complex128 operands; QSX_MIXED: complex64 A and B, FP32 products,
+ * complex128 C.  Leading dimensions lda/ldb/ldc (elements, of the stored
+ * operands), batch strides sA/sB/sC (0 = shared; sC must be nonzero when
+ * batch > 1).  alpha, beta, gamma are complex (re, im) pairs passed by value.
+ * Used for the RK4 propagator (Horner: P = I + hL(I + hL/2 (I + hL/3 (I +
+ * hL/4)))), its powers, the dipole maps and response contractions of
+ * response.py:309-336, and the Fourier transforms of two_dimensional_spectra
+ * (simulate/utils.py:154-219) as products with the transform matrices. */
+int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_t n, int32_t k,
+                  double alpha_re, double alpha_im, const void* A, int64_t lda, int64_t sA,
+                  const void* B, int64_t ldb, int64_t sB, double beta_re, double beta_im,
+                  void* C, int64_t ldc, int64_t sC, double gamma_re, double gamma_im,
+                  int32_t batch, void* stream);
 
 /* ---------------------------------------------------------------------------
  * HEOM for a Drude bath (qspectra/dynamics/heom.py:177-443), batched over B

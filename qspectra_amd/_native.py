@@ -38,7 +38,8 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
                                    ctypes.c_void_p, ctypes.c_size_t)
 
 # name -> argtypes (restype int32 unless listed in RESTYPES)
-RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64}
+RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64,
+            'qsx_adjoint_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -52,7 +53,10 @@ SIGNATURES = {
     'qsx_unitary_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_i32, c_i32, c_i32, c_vp, c_vp],
     'qsx_propagate': [c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32, c_i32,
                       c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp],
-    'qsx_zgemm': [c_i32, c_i32, c_i32, c_i32, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_vp, c_i64,
+    'qsx_adjoint_workspace_bytes': [c_i32, c_i32, c_i32, c_i64, c_i32],
+    'qsx_propagate_adjoint': [c_vp, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_i32,
+                              c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    'qsx_zgemm': [c_i32, c_i32, c_i32, c_i32, c_i32, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_vp, c_i64,
                   c_i64, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_i32, c_vp],
     'qsx_heom_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
     'qsx_heom_apply': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp,
@@ -243,8 +247,10 @@ MAX_REGISTER_BLOCK = 160
 
 
 def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_final=False,
-                     stream=None, steps=None):
-    """qsx_propagate: full states at every save point, out [B, R, nsave, M]."""
+                     stream=None, steps=None, status=None):
+    """qsx_propagate: full states at every save point, out [B, R, nsave, M].
+    status: optional int32 device slot (set to 1 on a non-finite state; the
+    caller clears it), allocated and cleared here when None."""
     lib = require_native()
     B, R, M = y0.shape
     nsave = int(nsub.shape[0])
@@ -259,7 +265,8 @@ def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_f
         strideD = 0 if diag.shape[0] == 1 else M
     out = torch.empty((B, R, nsave, M), dtype=torch.complex128, device=y0.device)
     yfin = torch.empty_like(y0) if want_final else None
-    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    if status is None:
+        status = torch.zeros(1, dtype=torch.int32, device=y0.device)
     label = 'propagate %s %s M=%d R=%d B=%d steps=%s nsave=%d' % (
         'rk4' if mode == QSX_MODE_RK4 else 'prop', 'fp64' if precision == QSX_FP64 else 'mixed',
         M, R, B, steps, nsave)
@@ -271,31 +278,87 @@ def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_f
     return out, yfin, status
 
 
-def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stream=None):
-    """C = alpha A B + beta C + gamma I over a leading batch axis (1 = shared).
-
-    A [b|1, m, k], B [b|1, k, n] (complex128, or complex64 for MIXED),
-    C [b, m, n] complex128 (allocated when None)."""
+def propagate_adjoint(A, y0, nsub, total_steps, gout, ckpt=64, stream=None):
+    """qsx_propagate_adjoint: (gA [B, R, M, M], gy0 [B, R, M], status) for
+    the propagator schedule y <- A y (A = P, complex128) with step counts
+    nsub (device int32, summing to total_steps) and saved-state cotangents
+    gout [B, R, nsave, M]."""
     lib = require_native()
-    assert A.dim() == 3 and B.dim() == 3 and A.is_contiguous() and B.is_contiguous()
-    bA, m, k = A.shape
-    bB, k2, n = B.shape
-    assert k == k2
+    B, R, M = y0.shape
+    nsave = int(nsub.shape[0])
+    if A.dtype != torch.complex128 or y0.dtype != torch.complex128:
+        raise TypeError('propagate_adjoint needs complex128 operands')
+    if A.shape[-2:] != (M, M) or A.shape[0] not in (1, B):
+        raise ValueError('operator shape %s does not match states %s' % (tuple(A.shape),
+                                                                          tuple(y0.shape)))
+    if tuple(gout.shape) != (B, R, nsave, M):
+        raise ValueError('cotangent shape %s, expected %s' % (tuple(gout.shape),
+                                                              (B, R, nsave, M)))
+    strideA = 0 if A.shape[0] == 1 else M * M
+    A, y0, gout = A.contiguous(), y0.contiguous(), gout.contiguous()
+    nbytes = lib.qsx_adjoint_workspace_bytes(M, R, B, int(total_steps), int(ckpt))
+    if nbytes < 0:
+        raise NativeError('invalid adjoint workspace request')
+    ws = torch.empty((nbytes + 15) // 16, dtype=torch.complex128, device=y0.device)
+    gA = torch.empty((B, R, M, M), dtype=torch.complex128, device=y0.device)
+    gy0 = torch.empty((B, R, M), dtype=torch.complex128, device=y0.device)
+    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    with timing.launch('adjoint M=%d R=%d B=%d steps=%d ckpt=%d' % (M, R, B, total_steps, ckpt),
+                       stream):
+        st = lib.qsx_propagate_adjoint(ptr(A), strideA, M, ptr(y0), R, B, ptr(nsub), nsave,
+                                       ptr(gout), int(ckpt),
+                                       ptr(ws), ws.numel() * 16, ptr(gA), ptr(gy0), ptr(status),
+                                       stream_ptr(stream))
+    check(st, 'qsx_propagate_adjoint')
+    return gA, gy0, status
+
+
+def _mat_strides(X):
+    """(leading dimension, batch stride) of a [b, rows, cols] operand whose
+    last axis is contiguous (views such as column slices are accepted)."""
+    assert X.dim() == 3 and X.stride(2) == 1, 'operand rows must be contiguous'
+    ld = X.stride(1) if X.shape[1] > 1 else X.shape[2]
+    assert ld >= X.shape[2]
+    return ld, (X.stride(0) if X.shape[0] > 1 else 0)
+
+
+def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stream=None,
+          transa=False, transb=False):
+    """C = alpha op(A) op(B) + beta C + gamma I over a leading batch axis
+    (1 = shared); op(X) = X^T when trans is set.
+
+    A [b|1, m, k] ([b|1, k, m] when transa), B [b|1, k, n] ([b|1, n, k] when
+    transb), complex128 (or complex64 for MIXED); C [b, m, n] complex128
+    (allocated when None).  Operands may be strided views whose last axis is
+    contiguous (for example column slices of a wider matrix)."""
+    lib = require_native()
+    assert A.dim() == 3 and B.dim() == 3
+    bA = A.shape[0]
+    m, k = (A.shape[2], A.shape[1]) if transa else (A.shape[1], A.shape[2])
+    bB = B.shape[0]
+    k2, n = (B.shape[2], B.shape[1]) if transb else (B.shape[1], B.shape[2])
+    assert k == k2, (tuple(A.shape), tuple(B.shape), transa, transb)
     batch = max(bA, bB) if C is None else C.shape[0]
     if C is None:
         C = torch.empty((batch, m, n), dtype=torch.complex128, device=A.device)
         beta = 0.0
-    assert C.dtype == torch.complex128 and C.is_contiguous() and C.shape == (batch, m, n)
+    assert C.dtype == torch.complex128 and C.shape == (batch, m, n)
     assert bA in (1, batch) and bB in (1, batch)
     dt = torch.complex128 if precision == QSX_FP64 else torch.complex64
     assert A.dtype == dt and B.dtype == dt
+    lda, sA = _mat_strides(A)
+    ldb, sB = _mat_strides(B)
+    ldc, sC = _mat_strides(C)
+    if batch > 1:
+        assert sC != 0
     al, be, ga = complex(alpha), complex(beta), complex(gamma)
-    with timing.launch('zgemm %s m=%d n=%d k=%d batch=%d' % (
-            'fp64' if precision == QSX_FP64 else 'mixed', m, n, k, batch), stream):
-        st = lib.qsx_zgemm(precision, m, n, k, al.real, al.imag, ptr(A), k,
-                           0 if bA == 1 else m * k, ptr(B), n, 0 if bB == 1 else k * n,
-                           be.real, be.imag, ptr(C), n, m * n if batch > 1 else 0,
-                           ga.real, ga.imag, batch, stream_ptr(stream))
+    trans = (1 if transa else 0) | (2 if transb else 0)
+    with timing.launch('zgemm %s m=%d n=%d k=%d batch=%d%s' % (
+            'fp64' if precision == QSX_FP64 else 'mixed', m, n, k, batch,
+            ' trans=%d' % trans if trans else ''), stream):
+        st = lib.qsx_zgemm(precision, trans, m, n, k, al.real, al.imag, ptr(A), lda, sA,
+                           ptr(B), ldb, sB, be.real, be.imag, ptr(C), ldc, sC, ga.real,
+                           ga.imag, batch, stream_ptr(stream))
     check(st, 'qsx_zgemm')
     return C
 

@@ -96,8 +96,14 @@ __global__ __launch_bounds__(NT) void step_kernel(
         if (j < M) cfma<T>(s, a[c], z[j]);
       }
     } else if (has1) {
+      // overflow rows use the same column split (S * W >= M), unrolled so
+      // that the LDS loads of a row segment are all in flight together
       const T2* Ar = Aov + (int64_t)(tid / S) * M;
-      for (int j = k; j < M; j += S) cfma<T>(s, Ar[j], z[j]);
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        const int j = k + S * c;
+        if (j < M) cfma<T>(s, Ar[j], z[j]);
+      }
     }
 #pragma unroll
     for (int m = S / 2; m > 0; m >>= 1) {

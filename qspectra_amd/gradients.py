@@ -1,0 +1,314 @@
+"""Gradients of Redfield absorption spectra with respect to site energies,
+couplings and the reorganization energy (reverse mode on the GPU).
+
+The reference computes spectra only (response.py:103-154); this module makes
+the same computation differentiable:
+
+    theta = (E [N], J [N(N-1)/2], lam)
+      -> H_n = diag(E) + J + diag(delta_n)       (member n of the static-
+         disorder ensemble; delta_n is drawn as ElectronicHamiltonian.sample(n)
+         draws it, hamiltonian.py:552-578, and does not depend on theta)
+      -> L_n  Redfield 'eg' block in the site basis (redfield.py:9-104)
+      -> P_n = P4(h L_n^T)                        (Heisenberg picture, one RK4 step)
+      -> x_n(t_k) = (P_n^k y_0) . v               (linear response, response.py:15-43)
+      -> S(w) = Re FT[-mean_n x_n]                (absorption, response.py:103-154)
+
+Values come from the product kernels (qsx_redfield_block, qsx_zgemm,
+qsx_propagate); the reverse pass uses
+  * qsx_propagate_adjoint (checkpointed recomputation of the forward states)
+    for the time propagation,
+  * the closed-form vector-Jacobian product of the Horner polynomial P4 (on
+    qsx_zgemm),
+  * torch's autograd through a torch restatement of the Redfield block
+    (eigh, C(w), Redfield tensor, basis change) for dL/dtheta.
+"""
+import numpy as np
+import torch
+
+from . import _native
+from .constants import CM_FS
+from .hamiltonian import ElectronicHamiltonian, diagonal_gaussian_disorder
+from .bath import DebyeBath
+from .dynamics.redfield import RedfieldModel
+from .simulate.response import _op_matrices
+from .simulate.utils import fourier_transform_device
+from .utils import check_random_state
+
+_BLOCKS = {'g': 0, 'e': 1}
+
+
+# --------------------------------------------------------- time propagation
+class _Propagate(torch.autograd.Function):
+    """states [B, R, nsave, M] of y <- A y (A = P, the one-step propagator),
+    with nsub[s] steps before save s; reverse mode by qsx_propagate_adjoint."""
+
+    @staticmethod
+    def forward(ctx, A, y0, nsub_host, ckpt):
+        dev = y0.device
+        nsub = torch.as_tensor(np.asarray(nsub_host, dtype=np.int32)).to(dev)
+        h = torch.ones(len(nsub_host), dtype=torch.float64, device=dev)
+        total = int(np.sum(nsub_host))
+        states, _, status = _native.propagate_states(
+            _native.QSX_MODE_PROPAGATOR, A.contiguous(), y0.contiguous(), nsub, h,
+            steps=total)
+        ctx.save_for_backward(A, y0, nsub)
+        ctx.total, ctx.ckpt = total, ckpt
+        ctx.mark_non_differentiable(status)
+        return states, status
+
+    @staticmethod
+    def backward(ctx, gstates, _gstatus):
+        A, y0, nsub = ctx.saved_tensors
+        gA, gy0, status = _native.propagate_adjoint(A, y0, nsub, ctx.total,
+                                                    gstates.contiguous(), ctx.ckpt)
+        if int(status.item()) != 0:
+            raise FloatingPointError('adjoint propagation failed (status %d)'
+                                     % int(status.item()))
+        gA = gA.sum(dim=1)
+        if A.shape[0] == 1 and gA.shape[0] > 1:
+            gA = gA.sum(dim=0, keepdim=True)
+        return gA, gy0, None, None
+
+
+class _RK4Propagator(torch.autograd.Function):
+    """P = P4(h L) = I + hL + (hL)^2/2 + (hL)^3/6 + (hL)^4/24 for L [B, M, M].
+
+    Reverse mode: with X = h L^H and G = dLoss/dP,
+        dLoss/dL = h (T1 + T2/2 + T3/6 + T4/24),  T1 = G,
+        T_{m+1} = X T_m + G X^m
+    (T_m = sum_i X^i G X^(m-1-i), the adjoint of d(L^m) = sum_i L^i dL L^(m-1-i))."""
+
+    @staticmethod
+    def forward(ctx, L, h):
+        from .lti import rk4_propagator
+        L = L.contiguous()
+        ctx.save_for_backward(L)
+        ctx.h = float(h)
+        return rk4_propagator(L, ctx.h)
+
+    @staticmethod
+    def backward(ctx, G):
+        (L,) = ctx.saved_tensors
+        h = ctx.h
+        G = G.contiguous()
+        X = (L.conj().transpose(-1, -2) * h).contiguous()
+        T = G
+        Xm = X
+        acc = G.clone()
+        for m, fact in ((2, 2.0), (3, 6.0), (4, 24.0)):
+            T = _native.zgemm(X, T)
+            T = _native.zgemm(G, Xm, T, beta=1.0)
+            acc = acc + T / fact
+            if m < 4:
+                Xm = _native.zgemm(Xm, X)
+        return acc * h, None
+
+
+def rk4_propagator(L, h):
+    """Differentiable P4(h L) for L [B, M, M] complex128 on the device."""
+    return _RK4Propagator.apply(L, h)
+
+
+def propagate(A, y0, nsub, ckpt=64):
+    """Differentiable propagation with the one-step propagator A [B|1, M, M]:
+    states [B, R, len(nsub), M] (nsub: host step counts per save interval).
+    The reverse pass recomputes the forward states from checkpoints every
+    ckpt steps (blocks up to M = 64)."""
+    states, status = _Propagate.apply(A, y0, nsub, int(ckpt))
+    if int(status.item()) != 0:
+        raise FloatingPointError('propagation produced a non-finite state')
+    return states
+
+
+# ---------------------------------------------------- Redfield block (torch)
+def debye_correlation(x, lam, gamma, T, real=False, matsubara_cutoff=1000):
+    """Debye bath correlation C(x) on a real tensor x (bath.py:70-102): the
+    complex form with the reference's truncated Matsubara sum (the k = 0
+    term is zero) or, for real=True, (1/expm1(x/T) + 1) J(x) with
+    J(x) = 2 lam gamma x / (gamma^2 + x^2); C(0) = lam (2T/gamma - i) or
+    2 lam T / gamma.  Differentiable in x and lam."""
+    zero = x == 0
+    xs = torch.where(zero, torch.ones_like(x), x)
+    if real:
+        J = 2 * lam * gamma * xs / (gamma ** 2 + xs ** 2)
+        val = (1 / torch.expm1(xs / T) + 1) * J
+        c0 = 2 * lam * T / gamma
+        return torch.where(zero, c0 * torch.ones_like(x), val).to(torch.complex128)
+    nu = 2 * np.pi * T * torch.arange(1, matsubara_cutoff, dtype=torch.float64, device=x.device)
+    a = nu / (nu ** 2 - gamma ** 2)
+    xc = xs.to(torch.complex128)[..., None]
+    msum = (a / (nu - 1j * xc)).sum(-1)
+    cot = 1.0 / np.tan(gamma / (2 * T))
+    val = lam * gamma * ((cot - 1j) / (gamma - 1j * xs.to(torch.complex128)) + 4 * T * msum)
+    c0 = lam * (2 * T / gamma - 1j)
+    return torch.where(zero, c0 * torch.ones_like(val), val)
+
+
+def _block_eig(H1, X, rw):
+    """(E rotating frame [B, nX], U [B, nX, nX], number operators' K [B, N, nX, nX])
+    for Hilbert block X ('g' or 'e') of the one-exciton Hamiltonians H1."""
+    B, N, _ = H1.shape
+    if X == 'g':
+        z = torch.zeros((B, 1), dtype=torch.float64, device=H1.device)
+        return z, torch.ones((B, 1, 1), dtype=torch.float64, device=H1.device), \
+            torch.zeros((B, N, 1, 1), dtype=torch.float64, device=H1.device)
+    E, U = torch.linalg.eigh(H1)
+    K = U[:, :, None, :] * U[:, :, :, None]          # K_i[a][c] = U[i,a] U[i,c]
+    return E - rw, U, K
+
+
+def redfield_block(H1, lam, gamma, T, rw, X, Y, secular=False, discard_imag_corr=False,
+                   unit_convert=CM_FS):
+    """Redfield Liouvillian block XY (X, Y in 'g', 'e') in the site basis for
+    one-exciton Hamiltonians H1 [B, N, N] (lab frame, float64 tensor):
+    L[(a + nX b), (c + nX d)] of -i[H, .] - R (redfield.py:9-104,
+    liouville_space.py:68-131), times unit_convert.  Differentiable in H1
+    and lam (torch autograd)."""
+    EX, UX, KX = _block_eig(H1, X, rw)
+    EY, UY, KY = (EX, UX, KX) if Y == X else _block_eig(H1, Y, rw)
+
+    def corr(E):
+        return debye_correlation(E[:, :, None] - E[:, None, :], lam, gamma, T,
+                                 real=discard_imag_corr)           # C[c][a] = C(E_c - E_a)
+
+    def gs(K, C):
+        # Gs[a][c] = sum_i sum_b K_i[a][b] K_i[b][c] C[c][b]
+        Kc = K.to(torch.complex128)
+        return torch.einsum('niab,nibc,ncb->nac', Kc, Kc, C)
+
+    CX = corr(EX)
+    CY = CX if Y == X else corr(EY)
+    GX, GY = gs(KX, CX), (gs(KX, CX) if Y == X else gs(KY, CY))
+    nX, nY = EX.shape[1], EY.shape[1]
+    dev = H1.device
+    IX = torch.eye(nX, dtype=torch.complex128, device=dev)
+    IY = torch.eye(nY, dtype=torch.complex128, device=dev)
+    KXc, KYc = KX.to(torch.complex128), KY.to(torch.complex128)
+    R = (torch.einsum('bd,nac->nabcd', IY, GX)
+         + torch.einsum('ac,nbd->nabcd', IX, GY.conj())
+         - torch.einsum('niac,nibd,ndb->nabcd', KXc, KYc, CY.conj())
+         - torch.einsum('niac,nibd,nca->nabcd', KXc, KYc, CX))
+    w = (EX[:, :, None] - EY[:, None, :]).to(torch.complex128)       # [n, a, b]
+    L = -1j * torch.einsum('nab,ac,bd->nabcd', w, IX, IY) - R
+    if secular:
+        ia = torch.arange(nX, device=dev)
+        ib = torch.arange(nY, device=dev)
+        keep = (ia[:, None, None, None] == ia[None, None, :, None]) & \
+               (ib[None, :, None, None] == ib[None, None, None, :])
+        if X == Y:
+            keep = keep | ((ia[:, None, None, None] == ib[None, :, None, None])
+                           & (ia[None, None, :, None] == ib[None, None, None, :]))
+        L = L * keep
+    UXc, UYc = UX.to(torch.complex128), UY.to(torch.complex128)
+    L = torch.einsum('npa,nqb,nabcd,nrc,nsd->npqrs', UXc, UYc, L, UXc, UYc)
+    B = L.shape[0]
+    L = L.permute(0, 2, 1, 4, 3).reshape(B, nX * nY, nX * nY)
+    return L * unit_convert
+
+
+# ------------------------------------------------------------ spectra
+def site_hamiltonian(E, J):
+    """H1 = diag(E) + J with J the upper triangle (i < j, row-major order)
+    mirrored below the diagonal; E [N], J [N(N-1)/2] tensors."""
+    N = E.shape[0]
+    iu = torch.triu_indices(N, N, offset=1, device=E.device)
+    H = torch.zeros((N, N), dtype=torch.float64, device=E.device)
+    H = H.index_put((iu[0], iu[1]), J).index_put((iu[1], iu[0]), J)
+    return H + torch.diag(E)
+
+
+def disorder_shifts(n_sites, fwhm, ensemble_size, random_seed=0):
+    """Diagonal static disorder of ensemble members 0..ensemble_size-1 as
+    ElectronicHamiltonian.sample_ensemble draws it: member n uses
+    RandomState(list(atleast_1d(random_seed)) + [n])."""
+    out = np.zeros((ensemble_size, n_sites))
+    if fwhm is None:
+        return out
+    draw = diagonal_gaussian_disorder(fwhm, n_sites)
+    for n in range(ensemble_size):
+        rs = check_random_state(list(np.atleast_1d(random_seed)) + [n])
+        out[n] = np.diag(draw(rs))
+    return out
+
+
+class RedfieldAbsorption(object):
+    """Differentiable absorption spectrum of a Redfield aggregate ('ge'
+    Hilbert space, 'eg' coherence), optionally averaged over a static-
+    disorder ensemble evaluated as one batch.
+
+        spec = RedfieldAbsorption(dipoles, cutoff_freq, temperature, rw_freq,
+                                  time_max, dt, disorder=100 / GAUSSIAN_SD_FWHM,
+                                  ensemble_size=512)
+        f, S = spec(E, J, lam)        # S: torch tensor, differentiable
+
+    E [N] site energies, J [N(N-1)/2] couplings (upper triangle), lam the
+    reorganization energy (cm^-1), each a float64 tensor on the device.
+    rw_freq is fixed (so the frequency axis does not move with theta).  The
+    forward values equal absorption_spectra(model, time_max, dt=dt,
+    ensemble_size=...) of the RedfieldModel with the same parameters."""
+
+    def __init__(self, dipoles, cutoff_freq, temperature, rw_freq, time_max, dt,
+                 disorder=None, ensemble_size=None, random_seed=0, secular=False,
+                 discard_imag_corr=False, polarization='xx', unit_convert=CM_FS, ckpt=64,
+                 correlation_decay_time=None):
+        self.dipoles = np.asarray(dipoles, dtype=float)
+        self.N = self.dipoles.shape[0]
+        self.gamma, self.T = float(cutoff_freq), float(temperature)
+        self.rw_freq, self.dt = float(rw_freq), float(dt)
+        self.secular, self.discard_imag_corr = secular, discard_imag_corr
+        self.polarization, self.unit_convert, self.ckpt = polarization, unit_convert, ckpt
+        self.size = ensemble_size or 1
+        self.shifts = disorder_shifts(self.N, disorder if ensemble_size else None, self.size,
+                                      random_seed)
+        self.t = np.arange(0, time_max, dt)
+        self.decay = correlation_decay_time
+        dev = _native.device()
+        self.dev = dev
+        # dipole maps in the site basis do not depend on theta
+        ham = ElectronicHamiltonian(np.eye(self.N), bath=DebyeBath(self.T, 1.0, self.gamma),
+                                    dipoles=self.dipoles)
+        model = RedfieldModel(ham, rw_freq=self.rw_freq, hilbert_subspace='ge',
+                              unit_convert=unit_convert)
+        V0 = _op_matrices(model, 'gg->eg', polarization[0], '+', 'commutator')[0]   # [M, 1]
+        bra = _op_matrices(model, 'eg->gg', polarization[1], '-', 'bra')[0]          # [M]
+        self.v = torch.as_tensor(V0[:, 0] * model.thermal_state('gg')[0]).to(dev)
+        self.y0 = torch.as_tensor(-bra).to(dev).reshape(1, 1, -1).expand(
+            self.size, 1, -1).contiguous()
+        self.nsub = np.ones(len(self.t), dtype=np.int64)
+        self.nsub[0] = 0
+
+    def hamiltonians(self, E, J):
+        H = site_hamiltonian(E, J)
+        return H[None] + torch.diag_embed(torch.as_tensor(self.shifts).to(H.device))
+
+    def liouvillian(self, H1, lam):
+        """Heisenberg 'eg' blocks [B, M, M]: values from qsx_redfield_block,
+        derivatives from the torch restatement."""
+        flags = ((_native.QSX_SECULAR if self.secular else 0)
+                 | (_native.QSX_REAL_CORR if self.discard_imag_corr else 0)
+                 | _native.QSX_HEISENBERG)
+        rw = torch.full((H1.shape[0],), self.rw_freq, dtype=torch.float64, device=H1.device)
+        L_dev = _native.redfield_block(H1.detach().contiguous(), rw, self.T, float(lam),
+                                       self.gamma, 1, 0, flags, self.unit_convert)
+        if not (H1.requires_grad or (torch.is_tensor(lam) and lam.requires_grad)):
+            return L_dev
+        L_t = redfield_block(H1, lam, self.gamma, self.T, self.rw_freq, 'e', 'g',
+                             self.secular, self.discard_imag_corr,
+                             self.unit_convert).transpose(-1, -2)
+        return L_dev + (L_t - L_t.detach())
+
+    def signal(self, E, J, lam):
+        """Ensemble-mean linear response x(t) [len(t)] (complex128)."""
+        H1 = self.hamiltonians(E, J)
+        L = self.liouvillian(H1, lam)
+        P = rk4_propagator(L, self.dt)
+        states = propagate(P, self.y0, self.nsub, self.ckpt)       # [B, 1, nt, M]
+        x = torch.einsum('bkm,m->bk', states[:, 0], self.v)
+        return x.mean(dim=0)
+
+    def __call__(self, E, J, lam):
+        x = self.signal(E, J, lam)
+        if self.decay is not None:
+            x = x * torch.as_tensor(np.exp(-self.t / self.decay)).to(x.device)
+        f, X = fourier_transform_device(self.t, -x, -1, self.rw_freq, self.unit_convert)
+        return f, X.real

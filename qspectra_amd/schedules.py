@@ -97,8 +97,10 @@ class Stage(object):
             self.nsub = torch.as_tensor(n.astype(np.int32)).to(dev)
             self.h = torch.as_tensor(h.astype(np.float64)).to(dev)
 
-    def run(self, y0, statuses):
-        """y0 [B, R, M] -> states [B, R, nsave, M]."""
+    def run(self, y0, statuses, status=None):
+        """y0 [B, R, M] -> states [B, R, nsave, M] (for 'chunks' a view of the
+        padded [B, 1, nc * c, M] output).  status: optional device slot shared
+        by the launches (cleared by the caller)."""
         if self.kind == 'powers':
             B, R, M = y0.shape
             outs, y = [], y0
@@ -111,14 +113,15 @@ class Stage(object):
             B, R, M = y0.shape
             st0, _, s0 = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, self.Pc, y0,
                                                    self.starts_n, self.starts_h,
-                                                   steps=self.nc - 1)
+                                                   steps=self.nc - 1, status=status)
             starts = st0.reshape(B, self.nc, M)
             st1, _, s1 = _native.propagate_states(self.mode, self.A, starts, self.chunk_n,
                                                    self.chunk_h, self.precision, self.diag,
-                                                   steps=self.steps)
-            statuses += [s0, s1]
+                                                   steps=self.steps, status=status)
+            statuses += [s0] if status is not None else [s0, s1]
             return st1.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
         out, _, st = _native.propagate_states(self.mode, self.A, y0, self.nsub, self.h,
-                                              self.precision, self.diag, steps=self.steps)
+                                              self.precision, self.diag, steps=self.steps,
+                                              status=status)
         statuses.append(st)
         return out

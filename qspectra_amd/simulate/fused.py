@@ -1,165 +1,251 @@
-"""Fused execution plan for third-order response functions.
+"""Fused execution plan for third-order response functions and 2-D spectra.
 
 The reference evaluates every Liouville pathway independently
-(response.py:309-336) and notes the redundancy itself (:292-295).  A plan
-built here, once per model and grid:
-  * merges pathways that share their first propagations (ESE and ESA share
-    V0, V1, the t1 and the t2 stage; only V2 and t3 differ),
-  * evaluates each distinct t3 Heisenberg propagation once (GSB and ESE
-    share the 'eg' one) on a second stream, concurrently with t1 -> t2,
-  * keeps every operator matrix, RK4 propagator and step schedule resident on
-    the device, so a spectrum is a fixed sequence of kernel launches:
-        register stepper (t1)  -> GEMM (V1 map)  -> register stepper (t2, all
-        t1 branches as right-hand sides) -> GEMM (V2 map) -> GEMM (contract
-        with the t3 bra trajectories)
-  * can be captured once into a HIP graph and replayed.
-Numerically identical to the pathway-by-pathway evaluation (same kernels and
-operands, same summation of pathway contributions up to order).
+(response.py:269-337): t1 propagation, then all t1 branches propagated
+through t2 as separate right-hand sides, then the dipole maps and the
+contraction with the t3 Heisenberg trajectories; it notes the redundancy
+itself (:292-295).  The response is a product of linear maps,
+
+    R_p(t1, t2_s, t3) = S1(t1) . V1^T . (P2^T)^N_s . V2_p^T . G3_p(t3)^T
+
+(S1: the t1 trajectory as rows; P2: the one-step RK4 propagator of the t2
+block; N_s: RK4 steps up to t2_s; G3_p: the t3 Heisenberg trajectory of the
+bra of pathway p), so a plan evaluates it in the cheapest association:
+
+  * one t1 trajectory per distinct (s0, s1) pair (GSB, ESE and ESA of the
+    '-++' geometry share it), computed by the register stepper;
+  * the t2 stage acts on the M1 rows of V1^T instead of on the n1 t1
+    branches: Q_s = V1^T (P2^N_s)^T, one batched GEMM with the RK4
+    propagator powers (cached on the equation of motion, like the
+    Liouvillian);
+  * W_s = Q_s V2_p^T for every pathway, written side by side so that all
+    pathways contract with their t3 trajectories in one GEMM;
+  * each distinct t3 Heisenberg trajectory once, on a second stream;
+  * the Fourier transforms of two_dimensional_spectra (simulate/utils.py:
+    154-219) as products with the transform matrices F1, F3 (the FFT
+    pipeline applied to unit vectors), folded into the t1 and t3 factors:
+        X_s = (F1 S1) . [sum_p W_s^p (F3 G3_p)^T].
+Exact algebra: the result equals the pathway-by-pathway evaluation up to
+rounding (the same RK4 schedule, only the order of the linear maps
+differs).  The launch sequence can be captured into a HIP graph.
 """
 import numpy as np
 import torch
 
 from .. import _native
 from ..engine import rk4_schedule
+from ..lti import propagator_power
 from .response import (_members, _op_matrices, _parse_pathways, THIRD_ORDER_PATHWAYS,
                        third_order_grids)
-from .utils import _mode
 from ..schedules import Stage as _Stage
 
 
+def transform_matrix(t, rw_freq, unit_convert, sign, device):
+    """(f, F) with F [N, len(t)] the linear map x -> fourier_transform(t, x)
+    (zero padding, ifftshift, FFT, fftshift, sign), obtained by applying the
+    device FFT pipeline to the unit vectors."""
+    from .utils import fourier_transform_device
+    eye = torch.eye(len(t), dtype=torch.complex128, device=device)
+    f, F = fourier_transform_device(t, eye, 0, rw_freq, unit_convert, sign=sign)
+    return f, F.contiguous()
+
+
 class ThirdOrderPlan(object):
+    """Third-order response (fourier=False, run() -> [B, n1, n2, n3]) or 2-D
+    spectrum (fourier=True, run() -> [B, N1, n2, N3], frequencies()) of a
+    model; run() launches on the current stream, capture() records a graph.
+    schedule 'rk4' (no propagator) is served by the pathway-by-pathway
+    evaluation and not by a plan."""
+
     def __init__(self, model, coherence_time_max, population_time_max=None,
                  population_times=None, geometry='-++', polarization='xxxx',
-                 include_signal=None, dt=None, precision='fp64', schedule='auto'):
+                 include_signal=None, dt=None, precision='fp64', schedule='auto',
+                 fourier=False):
+        if schedule == 'rk4':
+            raise ValueError("the fused plan needs the RK4 propagator; use plan='pathways' "
+                             "for schedule='rk4'")
         self.model = model
         self.t1, self.t2, self.t3 = third_order_grids(model, coherence_time_max,
                                                       population_time_max, population_times)
         self.B = B = len(_members(model))
         self.precision = precision
-        # dipole maps and contractions run in FP64 in every precision: the
-        # contraction sums oscillating terms that cancel, and these GEMMs are
-        # a small share of the work
-        self.gemm_dt = torch.complex128
+        self.fourier = fourier
         dt = dt if dt is not None else model.time_step
         rho0 = model.thermal_state('gg')
         trans = geometry + '-'
         paths = _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal)
         dev = _native.device()
+        n1, n2, n3 = len(self.t1), len(self.t2), len(self.t3)
 
-        def dev_t(x, dtype=torch.complex128):
-            return torch.as_tensor(np.array(x)).to(dev, dtype).contiguous()
+        def dev_t(x):
+            return torch.as_tensor(np.array(x)).to(dev, torch.complex128).contiguous()
 
-        self.groups = {}      # (s0, s1, s2) -> dict(t1 stage, V1T, t2 stage, branches)
-        self.t3_stages = {}   # (s3, s4) -> (stage, bra)
+        def per_batch(x, reps):
+            """[B|1, ...] -> [B*reps | 1, ...] (member-major)."""
+            if x.shape[0] == 1:
+                return x
+            return x.repeat_interleave(reps, dim=0).contiguous()
+
+        # t2 schedule: cumulative RK4 steps at each population time
+        n, h = rk4_schedule(self.t2, 0.0, dt)
+        hs = h[n > 0]
+        if hs.size and not np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0])):
+            raise ValueError('the fused plan needs a uniform t2 step')
+        self.N2 = np.cumsum(n)
+        h2 = float(hs[0]) if hs.size else float(dt)
+
+        self.t1_stages = {}      # (s0, s1) -> stage, initial states
+        self.t3_stages = {}      # (s3, s4) -> stage, bra, column offset in G
+        self.groups = {}         # (s0, s1, s2) -> Q operands, branches
+        width = 0
+        branches = []
         for path in paths:
             s = path.split('->')
-            key12 = tuple(s[:3])
-            if key12 not in self.groups:
+            k1 = (s[0], s[1])
+            if k1 not in self.t1_stages:
                 V0 = _op_matrices(model, '%s->%s' % (s[0], s[1]), polarization[0], trans[0],
-                                  'commutator')
-                V1 = _op_matrices(model, '%s->%s' % (s[1], s[2]), polarization[1], trans[1],
                                   'commutator')
                 y1 = np.einsum('bij,j->bi', V0, rho0)[:, None, :]
                 y1 = np.broadcast_to(y1, (B,) + y1.shape[1:])
-                self.groups[key12] = dict(
-                    y1=dev_t(y1),
-                    stage1=_Stage(model.equation_of_motion(s[1]), self.t1, None, dt, precision,
-                                  schedule, 1),
-                    V1T=dev_t(V1.transpose(0, 2, 1), self.gemm_dt),
-                    stage2=_Stage(model.equation_of_motion(s[2]), self.t2, 0.0, dt, precision,
-                                  schedule, len(self.t1)),
-                    branches=[])
-            key3 = tuple(s[3:])
-            if key3 not in self.t3_stages:
+                stage = _Stage(model.equation_of_motion(s[1]), self.t1, None, dt, precision,
+                               schedule, 1)
+                self.t1_stages[k1] = (stage, dev_t(y1))
+            k12 = tuple(s[:3])
+            if k12 not in self.groups:
+                V1 = _op_matrices(model, '%s->%s' % (s[1], s[2]), polarization[1], trans[1],
+                                  'commutator')                        # [B|1, M2, M1]
+                eom2 = model.equation_of_motion(s[2])
+                M2 = eom2.L.shape[-1]
+                pw = []
+                for N in self.N2:
+                    if N == 0:
+                        pw.append(torch.eye(M2, dtype=torch.complex128, device=dev)
+                                  .expand(eom2.L.shape[0], M2, M2))
+                    else:
+                        pw.append(propagator_power(eom2, h2, int(N)))
+                Pn = torch.stack(pw, dim=1).reshape(-1, M2, M2).contiguous()   # [B*n2|n2, ..]
+                if Pn.shape[0] != B * n2:
+                    Pn = Pn.repeat(B, 1, 1).contiguous() if Pn.shape[0] == n2 else Pn
+                V1T = per_batch(dev_t(V1.transpose(0, 2, 1)), n2)           # [B*n2|1, M1, M2]
+                self.groups[k12] = dict(t1=k1, V1T=V1T, Pn=Pn)
+            k3 = tuple(s[3:])
+            if k3 not in self.t3_stages:
                 b3 = _op_matrices(model, '%s->%s' % (s[3], s[4]), polarization[3], trans[3],
                                   'bra')[:, None, :]
                 b3 = np.broadcast_to(b3, (B,) + b3.shape[1:])
-                self.t3_stages[key3] = (
-                    _Stage(model.equation_of_motion(s[3], heisenberg_picture=True), self.t3,
-                           None, dt, precision, schedule, 1), dev_t(b3))
+                stage = _Stage(model.equation_of_motion(s[3], heisenberg_picture=True),
+                               self.t3, None, dt, precision, schedule, 1)
+                self.t3_stages[k3] = dict(stage=stage, bra=dev_t(b3), col=None)
             V2 = _op_matrices(model, '%s->%s' % (s[2], s[3]), polarization[2], trans[2],
-                              'commutator')
-            self.groups[key12]['branches'].append((dev_t(V2.transpose(0, 2, 1), self.gemm_dt),
-                                                   key3))
-        self.steps = sum(g['stage1'].steps + len(self.t1) * g['stage2'].steps * len(g['branches'])
-                         + sum(self.t3_stages[k][0].steps for _, k in g['branches'])
-                         for g in self.groups.values())
+                              'commutator')                            # [B|1, M3, M2]
+            V2T = per_batch(dev_t(V2.transpose(0, 2, 1)), n2)           # [B*n2|1, M2, M3]
+            M3 = V2T.shape[-1]
+            st3 = self.t3_stages[k3]
+            first = st3['col'] is None
+            if first:
+                st3['col'] = width
+                width += M3
+            branches.append((k12, k3, V2T, st3['col'], M3, first))
+        # pathways contract with their t3 trajectory through one column range
+        # per distinct trajectory: W blocks of pathways sharing it are summed
+        self.width = width
+        self.branches = branches
+        M1s = set(int(g['V1T'].shape[1]) for g in self.groups.values())
+        if len(M1s) != 1:
+            raise ValueError('pathways of one plan must share the t1 block')
+        self.M1 = M1s.pop()
+        if len(self.t1_stages) != 1:
+            raise ValueError('pathways of one plan must share the t1 propagation')
+
+        rw, uc = model.rw_freq, model.unit_convert
+        if fourier:
+            self.f1, self.F1 = transform_matrix(self.t1, rw, uc, -1, dev)
+            self.f3, self.F3 = transform_matrix(self.t3, rw, uc, 1, dev)
+        else:
+            self.f1, self.f3 = self.t1, self.t3
+            self.F1 = torch.eye(n1, dtype=torch.complex128, device=dev)
+            self.F3 = torch.eye(n3, dtype=torch.complex128, device=dev)
+        self.F1, self.F3 = self.F1[None].contiguous(), self.F3[None].contiguous()
+        N1, N3 = self.F1.shape[1], self.F3.shape[1]
+        # device buffers reused by every run (fixed addresses for graph replay)
+        self.G = torch.empty((B, N3, width), dtype=torch.complex128, device=dev)
+        self.W = torch.empty((B * n2, self.M1, width), dtype=torch.complex128, device=dev)
+        self.Q = {k: torch.empty((B * n2, self.M1, g['V1T'].shape[2]), dtype=torch.complex128,
+                                 device=dev) for k, g in self.groups.items()}
+        self.Z = torch.empty((B * n2, self.M1, N3), dtype=torch.complex128, device=dev)
+        self.S1F = torch.empty((B, N1, self.M1), dtype=torch.complex128, device=dev)
+        self.X = torch.empty((B * n2, N1, N3), dtype=torch.complex128, device=dev)
+        # status slots of the stepper launches (t1, then each t3), cleared per run
+        self.status = torch.zeros(1 + len(self.t3_stages), dtype=torch.int32, device=dev)
+        self.steps = self._count_steps(paths)
         self._side = torch.cuda.Stream()
         self._graph = None
         self._statuses = []
 
+    def _count_steps(self, paths):
+        """RK4 steps of the reference's pathway-by-pathway algorithm (one
+        right-hand side per t1 branch in the t2 stage)."""
+        n1 = len(self.t1)
+        s1 = next(iter(self.t1_stages.values()))[0].steps
+        s2 = int(self.N2[-1])
+        s3 = {k: v['stage'].steps for k, v in self.t3_stages.items()}
+        return sum(s1 + n1 * s2 + s3[tuple(p.split('->')[3:])] for p in paths)
+
     # ------------------------------------------------------------------ run
-    def _gemm(self, A, B, C=None, beta=0.0):
-        return _native.zgemm(A.contiguous(), B, C, beta=beta)
-
     def _launch(self):
-        B, n1, n2, n3 = self.B, len(self.t1), len(self.t2), len(self.t3)
-        statuses = []
+        B, n2 = self.B, len(self.t2)
+        n1, n3 = len(self.t1), len(self.t3)
         main = torch.cuda.current_stream()
+        self.status.zero_()
+        statuses = []
         self._side.wait_stream(main)
-        G3 = {}
         with torch.cuda.stream(self._side):
-            for key, (stage, bra) in self.t3_stages.items():
-                G = stage.run(bra, statuses)                                  # [B, 1, n3, M3]
-                G3[key] = G[:, 0].transpose(1, 2).to(self.gemm_dt).contiguous()   # [B, M3, n3]
-                G3[key].record_stream(main)      # consumed on the main stream
-        total = torch.zeros((B, n1 * n2, n3), dtype=torch.complex128, device=main.device)
-        first = True
-        for g in self.groups.values():
-            s1 = g['stage1'].run(g['y1'], statuses)                           # [B, 1, n1, M1]
-            Y1 = self._gemm(s1.reshape(B, n1, -1), g['V1T'])                  # [B, n1, M2]
-            s2 = g['stage2'].run(Y1, statuses)                                # [B, n1, n2, M2]
-            for V2T, key3 in g['branches']:
-                Y2 = self._gemm(s2.reshape(B, n1 * n2, -1), V2T)              # [B, n1 n2, M3]
-                if first:
-                    main.wait_stream(self._side)
-                    first = False
-                self._gemm(Y2, G3[key3], total, beta=1.0)
+            # t3 Heisenberg trajectories, transformed along t3, side by side in G
+            for i, st in enumerate(self.t3_stages.values()):
+                M3 = st['bra'].shape[-1]
+                G3 = st['stage'].run(st['bra'], statuses,
+                                     self.status[1 + i:2 + i])[:, 0, :n3]      # [B, n3, M3]
+                _native.zgemm(self.F3, G3, self.G[:, :, st['col']:st['col'] + M3])
+        # t1 trajectory, transformed along t1
+        stage1, y1 = next(iter(self.t1_stages.values()))
+        S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
+        _native.zgemm(self.F1, S1, self.S1F)
+        # t2 stage on the rows of V1^T, then the V2 maps of every pathway
+        for k12, g in self.groups.items():
+            _native.zgemm(g['V1T'], g['Pn'], self.Q[k12], transb=True)
+        for k12, k3, V2T, col, M3, first in self.branches:
+            _native.zgemm(self.Q[k12], V2T, self.W[:, :, col:col + M3],
+                          beta=0.0 if first else 1.0)
         main.wait_stream(self._side)
+        G = self.G if B == 1 else self.G.repeat_interleave(n2, dim=0)
+        _native.zgemm(self.W, G, self.Z, transb=True)                            # [B n2, M1, N3]
+        S1F = self.S1F if B == 1 else self.S1F.repeat_interleave(n2, dim=0)
+        _native.zgemm(S1F, self.Z, self.X)                                       # [B n2, N1, N3]
         self._statuses = statuses
-        return total.reshape(B, n1, n2, n3)
+        N1, N3 = self.X.shape[1], self.X.shape[2]
+        return self.X.view(B, n2, N1, N3).permute(0, 2, 1, 3)
 
-    def _launch_2d(self):
-        """The response followed by the two Fourier transforms of
-        two_dimensional_spectra (t1 with sign -1, then t3), on the device."""
-        from .utils import fourier_transform_device
-        m = self.model
-        X = self._launch()
-        _, X = fourier_transform_device(self.t1, X, 1, m.rw_freq, m.unit_convert, sign=-1)
-        _, X = fourier_transform_device(self.t3, X, 3, m.rw_freq, m.unit_convert)
-        return X
-
-    def capture(self, two_dimensional=False):
-        """Records the launch sequence (optionally with the 2-D Fourier
-        transforms) into a HIP graph, replayed by run()."""
-        fn = self._launch_2d if two_dimensional else self._launch
-        fn()                                # warm up allocations, caches, FFT plans
+    def capture(self):
+        """Records the launch sequence into a HIP graph, replayed by run()."""
+        self._launch()                      # warm up allocations and caches
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self._out = fn()
+            self._out = self._launch()
         self._graph = g
-        self._captured_2d = two_dimensional
         return self
 
     def run(self):
-        """Device tensor [B, n1, n2, n3] (or the 2-D spectrum [B, N1, n2, N3]
-        when captured with two_dimensional=True); check_status() raises on
-        a non-finite state."""
+        """Device tensor [B, n1, n2, n3] (time domain) or [B, N1, n2, N3]
+        (fourier=True); check_status() raises on a non-finite state."""
         if self._graph is not None:
             self._graph.replay()
             return self._out
         return self._launch()
 
     def frequencies(self):
-        """(f1, t2, f3) axes of the captured 2-D spectrum."""
-        from .utils import fourier_transform_device
-        m = self.model
-        z1 = torch.zeros(len(self.t1), dtype=torch.complex128, device=self._side.device)
-        f1, _ = fourier_transform_device(self.t1, z1, 0, m.rw_freq, m.unit_convert, sign=-1)
-        z3 = torch.zeros(len(self.t3), dtype=torch.complex128, device=self._side.device)
-        f3, _ = fourier_transform_device(self.t3, z3, 0, m.rw_freq, m.unit_convert)
-        return f1, self.t2, f3
+        """(f1, t2, f3) of a fourier=True plan ((t1, t2, t3) otherwise)."""
+        return self.f1, self.t2, self.f3
 
     def check_status(self):
         from .utils import IntegratorError

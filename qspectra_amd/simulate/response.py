@@ -182,7 +182,7 @@ def _third_order_response(dynamical_model, coherence_time_max, population_time_m
     model = dynamical_model
     kw = _propagate_kw(integrate_kwargs)
     plan = integrate_kwargs.get('plan', 'fused')
-    if plan == 'pathways':
+    if plan == 'pathways' or (plan == 'fused' and kw.get('schedule') == 'rk4'):
         return _pathway_by_pathway(model, coherence_time_max, population_time_max,
                                    population_times, geometry, polarization, include_signal, kw)
     if plan != 'fused':

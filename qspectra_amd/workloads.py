@@ -78,3 +78,55 @@ def rk4_steps_2d(n1=128, t2=(0., 100., 200., 500.), n3=128, dt=2.0, pathways=3):
     the t1 grid, n1 RHS over the t2 grid (from t0 = 0), 1 RHS over t3."""
     s2 = int(round(max(t2) / dt))
     return pathways * ((n1 - 1) + n1 * s2 + (n3 - 1))
+
+
+def ensemble_absorption_hamiltonians(n_disorder=4096, n_aggregates=1024, sigma=100.0):
+    """Absorption ensemble: n_disorder static-disorder members (diagonal
+    Gaussian, standard deviation sigma) of the 7-site aggregate, drawn as
+    ElectronicHamiltonian.sample_ensemble draws them (member n from
+    RandomState([0, n])), plus n_aggregates aggregates of N ~ U{2..24}
+    sites with E ~ U(12200, 12700), symmetric J ~ N(0, 80) and random unit
+    dipoles, all from RandomState(2).  Returns ElectronicHamiltonians."""
+    from .hamiltonian import ElectronicHamiltonian
+    H1, d, _ = aggregate7_hamiltonian()
+    parent = ElectronicHamiltonian(H1, bath=drude_bath(), dipoles=d,
+                                   disorder=sigma / GAUSSIAN_SD_FWHM)
+    hams = list(parent.sample_ensemble(n_disorder)) if n_disorder else []
+    rs = np.random.RandomState(2)
+    for _ in range(n_aggregates):
+        n = rs.randint(2, 25)
+        E = rs.uniform(12200, 12700, n)
+        J = rs.normal(0, 80, (n, n))
+        H = np.triu(J, 1)
+        H = H + H.T + np.diag(E)
+        dip = rs.normal(size=(n, 3))
+        dip /= np.linalg.norm(dip, axis=1, keepdims=True)
+        hams.append(ElectronicHamiltonian(H, bath=drude_bath(), dipoles=dip))
+    return hams
+
+
+def gradient_target_parameters(seed=3):
+    """Perturbed parameters of the 7-site aggregate for the gradient
+    workload: site energies +-50, couplings +-10, lambda +-5 cm^-1 (uniform,
+    RandomState(seed)).  Returns (E, J upper triangle, lam) for the
+    unperturbed and the perturbed system."""
+    H1, _, _ = aggregate7_hamiltonian()
+    iu = np.triu_indices(7, 1)
+    E0, J0, lam0 = np.diag(H1).copy(), H1[iu].copy(), 35.0
+    rs = np.random.RandomState(seed)
+    return ((E0, J0, lam0),
+            (E0 + rs.uniform(-50, 50, 7), J0 + rs.uniform(-10, 10, len(J0)),
+             lam0 + rs.uniform(-5, 5)))
+
+
+def heom_aggregate7_model(kelvin=300.0, level_cutoff=5):
+    """HEOM model of the 7-site aggregate: Drude bath at 300 K, K = 1
+    Matsubara term, hierarchy depth 4 (level_cutoff 5 -> 3060 ADOs), 'ge'
+    Hilbert space, disorder standard deviation 100 cm^-1."""
+    from .dynamics.heom import HEOMModel
+    from .hamiltonian import ElectronicHamiltonian
+    H1, d, _ = aggregate7_hamiltonian()
+    ham = ElectronicHamiltonian(H1, bath=drude_bath(kelvin), dipoles=d,
+                                disorder=100.0 / GAUSSIAN_SD_FWHM)
+    return HEOMModel(ham, hilbert_subspace='ge', unit_convert=CM_FS, level_cutoff=level_cutoff,
+                     K=1)

@@ -223,8 +223,9 @@ def test_fused_plan_and_graph_replay(gpu):
 
 
 def test_blocked_schedules_and_2d_graph(gpu):
-    """Time-chunked (t1, t3) and power (t2) schedules equal step-by-step RK4;
-    the whole 2-D spectrum replayed from one graph equals the eager API."""
+    """Time-chunked t1/t3 trajectories and the reassociated t2 stage equal
+    step-by-step RK4 pathway by pathway; the 2-D spectrum of a fourier plan,
+    replayed from one graph, equals the eager API every time."""
     from qspectra_amd.simulate.fused import ThirdOrderPlan
     _, inp = _inputs('redfield_7site')
     m7 = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=False)
@@ -232,10 +233,9 @@ def test_blocked_schedules_and_2d_graph(gpu):
     t2 = [0.0, 30.0, 62.0]
     (t1, _, t3), ref = P.third_order_response(om, 2 * 69.5, population_times=t2, dt=2.0)
     plan = ThirdOrderPlan(m7, 2 * 69.5, population_times=t2, dt=2.0)
-    kinds = sorted(set([g['stage1'].kind for g in plan.groups.values()]
-                       + [g['stage2'].kind for g in plan.groups.values()]
-                       + [st.kind for st, _ in plan.t3_stages.values()]))
-    assert kinds == ['chunks', 'powers'], kinds
+    kinds = sorted(set([st.kind for st, _ in plan.t1_stages.values()]
+                       + [st['stage'].kind for st in plan.t3_stages.values()]))
+    assert kinds == ['chunks'], kinds
     X = plan.run()[0].cpu().numpy()
     plan.check_status()
     assert len(t1) == 70 and rel_err(X, ref) < 1e-10
@@ -243,10 +243,28 @@ def test_blocked_schedules_and_2d_graph(gpu):
     assert rel_err(pm.run()[0].cpu().numpy(), ref) < 1e-6
     (f1, _, f3), S2 = qa.two_dimensional_spectra(m7, 2 * 69.5, population_times=t2, dt=2.0,
                                                  plan='pathways', schedule='rk4')
-    g2 = ThirdOrderPlan(m7, 2 * 69.5, population_times=t2, dt=2.0).capture(True)
+    g2 = ThirdOrderPlan(m7, 2 * 69.5, population_times=t2, dt=2.0, fourier=True).capture()
     ff1, _, ff3 = g2.frequencies()
     assert np.allclose(ff1, f1) and np.allclose(ff3, f3)
     for _ in range(3):
         Y = g2.run()[0].cpu().numpy()
         assert rel_err(Y, S2) < 1e-10
         qa.absorption_spectra(m7, 2 * 31.5, dt=2.0)       # eager work between replays
+
+
+def test_fused_plan_ensemble_and_isotropic(gpu):
+    """Plans over a batch of disorder members (per-member Liouvillians) and
+    the 4th-order isotropic average equal the pathway-by-pathway path."""
+    _, inp = _inputs('redfield_7site')
+    ham = _product_ham(inp, disorder=80)
+    m7 = qa.RedfieldModel(ham, unit_convert=qa.CM_FS, secular=False)
+    kw = dict(population_times=[0.0, 40.0], dt=2.0, ensemble_size=3)
+    _, a = qa.third_order_response(m7, 2 * 11.5, plan='pathways', schedule='rk4', **kw)
+    _, b = qa.third_order_response(m7, 2 * 11.5, **kw)
+    assert rel_err(b, a) < 1e-10
+    (_, _, _), c = qa.two_dimensional_spectra(m7, 2 * 11.5, population_times=[0.0, 40.0],
+                                              dt=2.0, exact_isotropic_average=True)
+    (_, _, _), d = qa.two_dimensional_spectra(m7, 2 * 11.5, population_times=[0.0, 40.0],
+                                              dt=2.0, exact_isotropic_average=True,
+                                              plan='pathways', schedule='rk4')
+    assert rel_err(c, d) < 1e-10

@@ -200,3 +200,19 @@ def test_zgemm_and_propagator(gpu):
     assert rel_err(lti.rk4_propagator(_native.to_device(L), h).cpu().numpy(), P4) < 1e-14
     assert rel_err(lti.rk4_increment(_native.to_device(L), h).cpu().numpy(),
                    P4 - np.eye(30)) < 1e-14
+
+
+def test_zgemm_transposed_operands_and_tile_shapes(gpu):
+    from qspectra_amd import _native
+    rs = np.random.RandomState(9)
+    for (b, m, n, k) in [(1, 7, 255, 128), (3, 255, 7, 128), (2, 147, 147, 147),
+                         (1, 300, 200, 40), (5, 1, 3, 2)]:
+        A = rs.randn(b, m, k) + 1j * rs.randn(b, m, k)
+        B = rs.randn(b, k, n) + 1j * rs.randn(b, k, n)
+        ref = A @ B
+        for ta in (False, True):
+            for tb in (False, True):
+                Ad = _native.to_device(A.transpose(0, 2, 1).copy() if ta else A)
+                Bd = _native.to_device(B.transpose(0, 2, 1).copy() if tb else B)
+                C = _native.zgemm(Ad, Bd, transa=ta, transb=tb)
+                assert rel_err(C.cpu().numpy(), ref) < 1e-14, (b, m, n, k, ta, tb)
