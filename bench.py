@@ -42,7 +42,36 @@ def parse_args(argv=None):
                     help='seconds of CPU work allowed for the baseline sample')
     ap.add_argument('--workloads', choices=('all', 'none'), default='all',
                     help='also measure the ensemble-absorption, gradient and HEOM workloads')
+    ap.add_argument('--counters', action='store_true',
+                    help='no timing: run a fixed kernel sequence for hardware-counter '
+                         'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
+                         'launch sequence, two HEOM RK4 steps of 256 members)')
     return ap.parse_args(argv)
+
+
+def counter_sequence(torch, qa):
+    """Kernel sequence profiled with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE):
+    the calibration copy reads and writes exactly 256 MiB per dispatch."""
+    from qspectra_amd import workloads as W
+    from qspectra_amd.simulate.utils import propagate_heom
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.complex64, device='cuda')   # 256 MiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    fused = FusedSpectrum(W.aggregate7_model(secular=False), W.AGG7_2D, 'mixed')
+    for _ in range(3):
+        fused.plan._launch()
+    torch.cuda.synchronize()
+    m = W.heom_aggregate7_model()
+    eom = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(256)]) \
+        .equation_of_motion('ee')
+    y0 = torch.zeros((256, eom.size), dtype=torch.complex128, device='cuda')
+    y0[:, 0] = 1.0
+    for prec in ('fp64', 'mixed'):
+        propagate_heom(eom, y0, np.array([0.0, 1.0]), dt=0.5, precision=prec)
+    torch.cuda.synchronize()
+    print('counter sequence done', file=sys.stderr)
 
 
 def run_plain(qa, model, work):
@@ -268,6 +297,9 @@ def main(argv=None):
     import qspectra_amd as qa
     from qspectra_amd import workloads as W, timing
     qa.require_native()
+    if args.counters:
+        counter_sequence(torch, qa)
+        return 0
     model = W.aggregate7_model(secular=False)
     work = W.AGG7_2D
     steps_per_spectrum = W.rk4_steps_2d()

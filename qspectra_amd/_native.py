@@ -316,7 +316,8 @@ def propagate_adjoint(A, y0, nsub, total_steps, gout, ckpt=64, stream=None):
 def _mat_strides(X):
     """(leading dimension, batch stride) of a [b, rows, cols] operand whose
     last axis is contiguous (views such as column slices are accepted)."""
-    assert X.dim() == 3 and X.stride(2) == 1, 'operand rows must be contiguous'
+    assert X.dim() == 3 and (X.shape[2] == 1 or X.stride(2) == 1), \
+        'operand rows must be contiguous'
     ld = X.stride(1) if X.shape[1] > 1 else X.shape[2]
     assert ld >= X.shape[2]
     return ld, (X.stride(0) if X.shape[0] > 1 else 0)

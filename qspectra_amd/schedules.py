@@ -1,6 +1,10 @@
 """Step schedules of the propagation engine (exact classical RK4 up to
 rounding; P = P4(h L) is the one-step RK4 propagator of a time-invariant L).
 """
+import copy
+import json
+import os
+
 import numpy as np
 import torch
 
@@ -38,15 +42,21 @@ class Stage(object):
     """One propagation with its launch schedule resident on the device.
 
     Schedules (all exact classical RK4 up to rounding, with P = P4(hL)):
-      'powers'  few save points: Y_s = P^(n_s - n_{s-1}) Y_{s-1}, one batched
-                GEMM per save interval with a cached power (FP64);
-      'chunks'  one right-hand side saved at every one of many equally spaced
-                points: chunk starts y_{kc} = (P^c)^k y_0 by the register
-                stepper with P^c, then all chunks stepped in parallel with P,
-                so the sequential chain is n/c + c products instead of n;
-      'stepper' the register stepper with P (or L for schedule 'rk4')."""
+      'powers'   few save points: Y_s = P^(n_s - n_{s-1}) Y_{s-1}, one batched
+                 GEMM per save interval with a cached power (FP64);
+      'chunks'   one right-hand side saved at every one of many equally spaced
+                 points: chunk starts y_{kc} = (P^c)^k y_0 by the register
+                 stepper with P^c, then all chunks stepped in parallel with P,
+                 so the sequential chain is n/c + c products instead of n;
+      'doubling' the same trajectory by doubling: rows [y_0 .. y_{m-1}] give
+                 rows [y_m .. y_{2m-1}] in one GEMM with the cached power
+                 (P^q)^m, so ceil(log2 n) GEMMs (FP64);
+      'stepper'  the register stepper with P (or L for schedule 'rk4').
+    With schedule 'auto' a trajectory of one right-hand side picks among
+    'doubling', 'chunks' (several chunk lengths) and 'stepper' by timing
+    them on the device (schedules.autotune), once per problem shape."""
 
-    def __init__(self, eom, t, t0, dt, precision, schedule, R, chunk=None):
+    def __init__(self, eom, t, t0, dt, precision, schedule, R, chunk=None, kind=None):
         n, h = rk4_schedule(t, t0, dt)
         hs = h[n > 0]
         uniform = hs.size == 0 or np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0]))
@@ -60,26 +70,64 @@ class Stage(object):
         self.eom = eom
         self.h0 = float(hs[0]) if hs.size else 0.0
         self.n = n
+        self.h = h
         self.nsave = len(n)
         self.steps = int(n.sum())
         self.precision = _mode(precision)
         self.prec_name = precision
-        dev = eom.L.device
-        self.kind = 'stepper'
-        if chunk is None:
-            chunk = max(4, int(np.ceil(np.sqrt(self.nsave))))
-        if prop and schedule in ('auto', 'blocked') and hs.size:
-            if self.nsave <= 8:
-                self.kind = 'powers'
-            elif (R == 1 and n[0] == 0 and np.all(n[1:] == n[1]) and n[1] > 0
-                  and self.nsave > 2 * chunk):
-                self.kind = 'chunks'
         self.mode = _native.QSX_MODE_PROPAGATOR if prop else _native.QSX_MODE_RK4
-        if self.kind == 'powers':
+        trajectory = (prop and hs.size and R == 1 and n[0] == 0 and np.all(n[1:] == n[1])
+                      and n[1] > 0 and self.nsave > 8)
+        if kind is None:
+            kind, chunk = 'stepper', chunk
+            if prop and schedule in ('auto', 'blocked') and hs.size:
+                if self.nsave <= 8:
+                    kind = 'powers'
+                elif trajectory:
+                    kind, chunk = self._choose(schedule, chunk)
+        elif kind in ('chunks', 'doubling') and not trajectory:
+            raise ValueError("schedule %r needs one right-hand side saved at equally "
+                             "spaced points" % kind)
+        self._setup(kind, chunk)
+
+    # ---------------------------------------------------------- selection
+    def _candidates(self):
+        cands = [('doubling', None)]
+        for c in (4, 8, 16, 32, 64):
+            if 2 * c < self.nsave:
+                cands.append(('chunks', c))
+        if self.steps <= 512:
+            cands.append(('stepper', None))
+        return cands
+
+    def _choose(self, schedule, chunk):
+        if chunk is not None:
+            return 'chunks', chunk
+        default = ('chunks', max(4, int(np.ceil(np.sqrt(self.nsave)))))
+        if schedule != 'auto' or not autotune_enabled():
+            return default
+        M = self.eom.L.shape[-1]
+        key = 'traj M=%d B=%d nsave=%d q=%d %s' % (M, self.eom.L.shape[0], self.nsave,
+                                                  int(self.n[1]), self.prec_name)
+
+        def trial(cand):
+            st = copy.copy(self)
+            st._setup(cand[0], cand[1])
+            y0 = torch.ones((self.eom.L.shape[0], 1, M), dtype=torch.complex128,
+                            device=self.eom.L.device) / np.sqrt(M)
+            return lambda: st.run(y0, [])
+        best = autotune(key, self._candidates(), trial)
+        return best[0], best[1]
+
+    def _setup(self, kind, chunk):
+        eom, n, h = self.eom, self.n, self.h
+        dev = eom.L.device
+        self.kind = kind
+        if kind == 'powers':
             self.mats = [None if k == 0 else
                          transposed(eom, ('Pn', self.h0, int(k)),
                                     propagator_power(eom, self.h0, int(k))) for k in n]
-        elif self.kind == 'chunks':
+        elif kind == 'chunks':
             q = int(n[1])                       # steps per output interval
             self.c = chunk
             self.nc = -(-self.nsave // chunk)
@@ -90,13 +138,24 @@ class Stage(object):
                                        device=dev)
             self.chunk_n = torch.tensor([0] + [q] * (chunk - 1), dtype=torch.int32, device=dev)
             self.chunk_h = torch.full((chunk,), self.h0, dtype=torch.float64, device=dev)
-            self.A, self.diag = stepper_operator(eom, 'propagator', self.h0, precision)
-        else:
+            self.A, self.diag = stepper_operator(eom, 'propagator', self.h0, self.prec_name)
+        elif kind == 'doubling':
+            q = int(n[1])
+            self.pw = []
+            m = 1
+            while m < self.nsave:
+                self.pw.append(propagator_power(eom, self.h0, q * m))
+                m *= 2
+        elif kind == 'stepper':
+            prop = self.mode == _native.QSX_MODE_PROPAGATOR
             self.A, self.diag = stepper_operator(eom, 'propagator' if prop else 'rk4',
-                                                 self.h0, precision)
+                                                 self.h0, self.prec_name)
             self.nsub = torch.as_tensor(n.astype(np.int32)).to(dev)
-            self.h = torch.as_tensor(h.astype(np.float64)).to(dev)
+            self.hs = torch.as_tensor(h.astype(np.float64)).to(dev)
+        else:
+            raise ValueError('unknown schedule kind %r' % kind)
 
+    # ---------------------------------------------------------------- run
     def run(self, y0, statuses, status=None):
         """y0 [B, R, M] -> states [B, R, nsave, M] (for 'chunks' a view of the
         padded [B, 1, nc * c, M] output).  status: optional device slot shared
@@ -109,6 +168,17 @@ class Stage(object):
                     y = _native.zgemm(y, A)           # rows: y <- y P^T
                 outs.append(y)
             return torch.stack(outs, dim=2)
+        if self.kind == 'doubling':
+            B, R, M = y0.shape
+            out = torch.empty((B, 1, self.nsave, M), dtype=torch.complex128, device=y0.device)
+            o = out[:, 0]
+            o[:, 0].copy_(y0[:, 0])
+            m = 1
+            for Pm in self.pw:                        # rows m..2m-1 = rows 0..m-1 (P^m)^T
+                r = min(m, self.nsave - m)
+                _native.zgemm(o[:, :r], Pm, o[:, m:m + r], transb=True)
+                m += r
+            return out
         if self.kind == 'chunks':
             B, R, M = y0.shape
             st0, _, s0 = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, self.Pc, y0,
@@ -120,8 +190,79 @@ class Stage(object):
                                                    steps=self.steps, status=status)
             statuses += [s0] if status is not None else [s0, s1]
             return st1.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
-        out, _, st = _native.propagate_states(self.mode, self.A, y0, self.nsub, self.h,
+        out, _, st = _native.propagate_states(self.mode, self.A, y0, self.nsub, self.hs,
                                               self.precision, self.diag, steps=self.steps,
                                               status=status)
         statuses.append(st)
         return out
+
+
+# ---------------------------------------------------------------- autotuner
+_TUNED = {}
+
+
+def autotune_enabled():
+    return os.environ.get('QSX_AUTOTUNE', '1') != '0'
+
+
+def _cache_path():
+    return os.environ.get('QSX_TUNE_CACHE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'qspectra_amd',
+                                       'schedules.json'))
+
+
+def _load_cache():
+    if _TUNED:
+        return
+    _TUNED['_loaded'] = True
+    try:
+        with open(_cache_path()) as f:
+            _TUNED.update(json.load(f))
+    except (OSError, ValueError):
+        pass
+
+
+def _save_cache():
+    try:
+        path = _cache_path()
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = {k: v for k, v in _TUNED.items() if not k.startswith('_')}
+        with open(path + '.tmp', 'w') as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(path + '.tmp', path)
+    except OSError:
+        pass
+
+
+def autotune(key, candidates, make_trial, reps=5):
+    """The fastest of ``candidates`` for problem ``key`` on this device:
+    each candidate's launch sequence (make_trial(candidate) -> callable) is
+    run once to warm up, then timed ``reps`` times with device events; the
+    median decides.  Decisions are cached in memory and in a JSON file
+    (QSX_TUNE_CACHE, default ~/.cache/qspectra_amd/schedules.json), keyed by
+    device name and problem shape."""
+    _load_cache()
+    full = '%s | %s' % (torch.cuda.get_device_name(), key)
+    hit = _TUNED.get(full)
+    if hit is not None and tuple(hit['choice']) in [tuple(c) for c in candidates]:
+        return tuple(hit['choice'])
+    times = {}
+    for cand in candidates:
+        fn = make_trial(cand)
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        times[cand] = float(np.median(ms))
+    best = min(times, key=times.get)
+    _TUNED[full] = {'choice': list(best),
+                    'ms': {'%s:%s' % c: round(v, 5) for c, v in times.items()}}
+    _save_cache()
+    return best

@@ -50,6 +50,11 @@ def transform_matrix(t, rw_freq, unit_convert, sign, device):
     return f, F.contiguous()
 
 
+class PlanNotApplicable(ValueError):
+    """The response cannot be evaluated by a fused plan (non-uniform t2 step,
+    schedule 'rk4', pathways with different t1 propagations)."""
+
+
 class ThirdOrderPlan(object):
     """Third-order response (fourier=False, run() -> [B, n1, n2, n3]) or 2-D
     spectrum (fourier=True, run() -> [B, N1, n2, N3], frequencies()) of a
@@ -62,8 +67,8 @@ class ThirdOrderPlan(object):
                  include_signal=None, dt=None, precision='fp64', schedule='auto',
                  fourier=False):
         if schedule == 'rk4':
-            raise ValueError("the fused plan needs the RK4 propagator; use plan='pathways' "
-                             "for schedule='rk4'")
+            raise PlanNotApplicable("the fused plan needs the RK4 propagator; use "
+                                    "plan='pathways' for schedule='rk4'")
         self.model = model
         self.t1, self.t2, self.t3 = third_order_grids(model, coherence_time_max,
                                                       population_time_max, population_times)
@@ -90,7 +95,7 @@ class ThirdOrderPlan(object):
         n, h = rk4_schedule(self.t2, 0.0, dt)
         hs = h[n > 0]
         if hs.size and not np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0])):
-            raise ValueError('the fused plan needs a uniform t2 step')
+            raise PlanNotApplicable('the fused plan needs a uniform t2 step')
         self.N2 = np.cumsum(n)
         h2 = float(hs[0]) if hs.size else float(dt)
 
@@ -152,10 +157,10 @@ class ThirdOrderPlan(object):
         self.branches = branches
         M1s = set(int(g['V1T'].shape[1]) for g in self.groups.values())
         if len(M1s) != 1:
-            raise ValueError('pathways of one plan must share the t1 block')
+            raise PlanNotApplicable('pathways of one plan must share the t1 block')
         self.M1 = M1s.pop()
         if len(self.t1_stages) != 1:
-            raise ValueError('pathways of one plan must share the t1 propagation')
+            raise PlanNotApplicable('pathways of one plan must share the t1 propagation')
 
         rw, uc = model.rw_freq, model.unit_convert
         if fourier:

@@ -182,15 +182,21 @@ def _third_order_response(dynamical_model, coherence_time_max, population_time_m
     model = dynamical_model
     kw = _propagate_kw(integrate_kwargs)
     plan = integrate_kwargs.get('plan', 'fused')
-    if plan == 'pathways' or (plan == 'fused' and kw.get('schedule') == 'rk4'):
+    if plan not in ('fused', 'pathways'):
+        raise ValueError("plan must be 'fused' or 'pathways'")
+    p = None
+    if plan == 'fused':
+        from .fused import ThirdOrderPlan, PlanNotApplicable
+        try:
+            p = ThirdOrderPlan(model, coherence_time_max, population_time_max, population_times,
+                               geometry, polarization, include_signal, dt=kw.get('dt'),
+                               precision=kw.get('precision', 'fp64'),
+                               schedule=kw.get('schedule', 'auto'))
+        except PlanNotApplicable:
+            p = None
+    if p is None:
         return _pathway_by_pathway(model, coherence_time_max, population_time_max,
                                    population_times, geometry, polarization, include_signal, kw)
-    if plan != 'fused':
-        raise ValueError("plan must be 'fused' or 'pathways'")
-    from .fused import ThirdOrderPlan
-    p = ThirdOrderPlan(model, coherence_time_max, population_time_max, population_times,
-                       geometry, polarization, include_signal, dt=kw.get('dt'),
-                       precision=kw.get('precision', 'fp64'), schedule=kw.get('schedule', 'auto'))
     X = p.run()
     p.check_status()
     return (p.t1, p.t2, p.t3), X.cpu().numpy()
