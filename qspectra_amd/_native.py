@@ -270,10 +270,12 @@ def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_f
     label = 'propagate %s %s M=%d R=%d B=%d steps=%s nsave=%d' % (
         'rk4' if mode == QSX_MODE_RK4 else 'prop', 'fp64' if precision == QSX_FP64 else 'mixed',
         M, R, B, steps, nsave)
-    with timing.launch(label, stream):
-        st = lib.qsx_propagate(mode, precision, ptr(A), strideA, ptr(diag), strideD, M, ptr(y0),
-                               R, B, ptr(nsub), ptr(h), nsave, ptr(out), ptr(yfin), ptr(status),
-                               stream_ptr(stream))
+    def go():
+        return lib.qsx_propagate(mode, precision, ptr(A), strideA, ptr(diag), strideD, M,
+                                 ptr(y0), R, B, ptr(nsub), ptr(h), nsave, ptr(out), ptr(yfin),
+                                 ptr(status), stream_ptr(stream))
+    with timing.launch(label, stream, go):
+        st = go()
     check(st, 'qsx_propagate')
     return out, yfin, status
 
@@ -354,12 +356,14 @@ def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stre
         assert sC != 0
     al, be, ga = complex(alpha), complex(beta), complex(gamma)
     trans = (1 if transa else 0) | (2 if transb else 0)
+    def go():
+        return lib.qsx_zgemm(precision, trans, m, n, k, al.real, al.imag, ptr(A), lda, sA,
+                             ptr(B), ldb, sB, be.real, be.imag, ptr(C), ldc, sC, ga.real,
+                             ga.imag, batch, stream_ptr(stream))
     with timing.launch('zgemm %s m=%d n=%d k=%d batch=%d%s' % (
             'fp64' if precision == QSX_FP64 else 'mixed', m, n, k, batch,
-            ' trans=%d' % trans if trans else ''), stream):
-        st = lib.qsx_zgemm(precision, trans, m, n, k, al.real, al.imag, ptr(A), lda, sA,
-                           ptr(B), ldb, sB, be.real, be.imag, ptr(C), ldc, sC, ga.real,
-                           ga.imag, batch, stream_ptr(stream))
+            ' trans=%d' % trans if trans else ''), stream, go):
+        st = go()
     check(st, 'qsx_zgemm')
     return C
 

@@ -51,15 +51,35 @@ __device__ inline double2 ld(const cplx_t<T>* p) { return to_d2(*p); }
 
 // Stage st of classical RK4: z_in = input stage vector, writes z_out and
 // updates ynew (st 0..2) or y (st 3); st = 4 only writes z_out = F(z_in).
+// LDS layout of a stage work-group: operator block, the tile's stage
+// vectors, and the tile's edge tables (so that the neighbour gathers do not
+// wait on dependent global loads of the table).
+inline size_t heom_lds_bytes(int opsz, int ado_per_tile, int M, int E, size_t zbytes) {
+  size_t off = (size_t)opsz * sizeof(double2);
+  off += (size_t)ado_per_tile * M * zbytes;
+  off = (off + 15) & ~(size_t)15;
+  off += (size_t)ado_per_tile * E * sizeof(double2);
+  off += 2 * (size_t)ado_per_tile * E * sizeof(int);
+  return off;
+}
+
+// Stage st of classical RK4: z_in = input stage vector, writes z_out and
+// updates ynew (st 0..2) or y (st 3); st = 4 only writes z_out = F(z_in).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
     HeomArgs a, const cplx_t<T>* __restrict__ zin, cplx_t<T>* __restrict__ zout,
     double2* __restrict__ y, double2* __restrict__ ynew, double h, int st) {
   using T2 = cplx_t<T>;
   extern __shared__ __align__(16) unsigned char smem[];
-  double2* Hs = reinterpret_cast<double2*>(smem);                 // n*n (COMM) or M*M (DENSE)
   const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  const int E = a.E;
+  double2* Hs = reinterpret_cast<double2*>(smem);                 // n*n (COMM) or M*M (DENSE)
   T2* zt = reinterpret_cast<T2*>(Hs + opsz);                      // [ado_per_tile][M]
+  size_t off = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * sizeof(T2);
+  off = (off + 15) & ~(size_t)15;
+  double2* ecf = reinterpret_cast<double2*>(smem + off);          // [ado_per_tile][E]
+  int* esrc = reinterpret_cast<int*>(ecf + a.ado_per_tile * E);   // [ado_per_tile][E]
+  int* ewid = esrc + a.ado_per_tile * E;                          // [ado_per_tile][E]
 
   int b, tile;
   if (a.xcd_map) {
@@ -79,6 +99,12 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
   const double2* Hb = a.Hop + (int64_t)b * opsz;
   for (int e = tid; e < opsz; e += kThreads) Hs[e] = Hb[e];
   for (int e = tid; e < nloc * M; e += kThreads) zt[e] = zin[base + e];
+  for (int e = tid; e < nloc * E; e += kThreads) {
+    const int64_t g = (int64_t)ado0 * E + e;
+    esrc[e] = a.esrc[g];
+    ewid[e] = a.ew[g];
+    ecf[e] = a.ecoef[g];
+  }
   __syncthreads();
 
   const int e = tid;
@@ -101,26 +127,31 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
     const double2 zp = ld<T>(z + p);
     const double2 dg = make_double2(a.shift[ado] + a.dvec[p].x, a.dvec[p].y);
     k = cadd(k, cmul(dg, zp));
-    const int* src = a.esrc + (int64_t)ado * a.E;
-    const int* wid = a.ew + (int64_t)ado * a.E;
-    const double2* cf = a.ecoef + (int64_t)ado * a.E;
-    const T2* zs = zin + (int64_t)b * a.n_ado * M;
-    for (int q = 0; q < a.E; ++q) {
-      const int s = src[q];
-      if (s < 0) break;                       // edges are packed, -1 terminated
-      const double2 w = cmul(cf[q], a.W[(int64_t)wid[q] * M + p]);
-      k = cadd(k, cmul(w, ld<T>(zs + (int64_t)s * M + p)));
+    const int* sl = esrc + la * E;
+    const int* wl = ewid + la * E;
+    const double2* cl = ecf + la * E;
+    const T2* zs = zin + (int64_t)b * a.n_ado * M + p;
+    // edges are packed and -1 terminated; predicated (no early exit) so that
+    // the unrolled neighbour gathers are all in flight together
+#pragma unroll 4
+    for (int q = 0; q < E; ++q) {
+      const int s = sl[q];
+      if (s >= 0) {
+        const double2 w = cmul(cl[q], a.W[(int64_t)wl[q] * M + p]);
+        k = cadd(k, cmul(w, ld<T>(zs + (int64_t)s * M)));
+      }
     }
     const int64_t gi = base + e;
     if (st == 4) {                           // plain application: zout = F(zin)
       zout[gi] = from_d2<T>(k);
       return;
     }
-    const double2 yv = y[gi];
     if (st == 0) {
+      const double2 yv = y[gi];
       ynew[gi] = caxpy(h / 6.0, k, yv);
       zout[gi] = from_d2<T>(caxpy(0.5 * h, k, yv));
     } else if (st < 3) {
+      const double2 yv = y[gi];
       ynew[gi] = caxpy(h / 3.0, k, ynew[gi]);
       zout[gi] = from_d2<T>(caxpy(st == 1 ? 0.5 * h : h, k, yv));
     } else {
@@ -164,7 +195,11 @@ int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const d
   T2* zA = reinterpret_cast<T2*>(ynew + total);
   T2* zB = zA + total;
   const int opsz = a.dense ? a.M * a.M : a.n * a.n;
-  const size_t shmem = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * sizeof(T2);
+  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.E, sizeof(T2));
+  QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_heom_propagate: %zu bytes of LDS", shmem);
+  if (shmem > 64 * 1024)
+    QSX_CHECK_HIP(hipFuncSetAttribute((const void*)heom_stage_kernel<T>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;
   hipLaunchKernelGGL(heom_init_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      stream, y0, y, zA, total);
@@ -239,7 +274,8 @@ extern "C" int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n
   if (st) return st;
   QSX_CHECK_ARG(y && out, "qsx_heom_apply: null pointer");
   const int opsz = a.dense ? a.M * a.M : a.n * a.n;
-  const size_t shmem = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * 16;
+  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.E, sizeof(double2));
+  QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_heom_apply: %zu bytes of LDS", shmem);
   const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;
   hipLaunchKernelGGL(heom_stage_kernel<double>, dim3(ngroups), dim3(kThreads), shmem,
                      as_stream(stream), a, (const double2*)y, (double2*)out, nullptr, nullptr,

@@ -1,10 +1,11 @@
 // Batched complex GEMM with an identity term (gfx950):
 //   C[b] = alpha * A[b] * B[b] + beta * C[b] + gamma * I
-// Row-major storage, op(A) and op(B) optionally transposed; 32 x 32 output
-// tiles per 256-thread workgroup (16 x 16 for small problems), K staged
-// through LDS in chunks of 16, each thread accumulating a 2 x 2 (1 x 1) block.
-// FP64: complex128 throughout.  MIXED: complex64 A and B with FP32
-// accumulation, complex128 C.  Used for the Horner construction of the RK4
+// Row-major storage, op(A) and op(B) optionally transposed, K staged through
+// LDS.  FP64: complex128 throughout; large products on the FP64 matrix cores
+// (16 x 16 tiles per wave, 32 x 32 per 4-wave work-group), small ones on the
+// vector units.  MIXED: complex64 A and B with FP32 accumulation on the
+// vector units.  Vector path: 32 x 32 or 16 x 16 tiles per 256-thread
+// work-group, 2 x 2 or 1 x 1 block per thread, K in chunks of 16.  Used for the Horner construction of the RK4
 // propagator, its powers, save-point dipole maps and response contractions:
 // operands here are at most a few hundred wide, so the kernel favours many
 // small tiles over deep register blocking.
@@ -88,6 +89,96 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
   }
 }
 
+// FP64 path on the matrix cores: each wave owns a 16 x 16 complex output
+// tile held as two v_mfma_f64_16x16x4f64 accumulators (real, imaginary);
+// a work-group is WM x WN waves.  Operand fragments (gfx950 layout): A lane
+// l = A[l % 16][k0 + l / 16], B lane l = B[k0 + l / 16][l % 16]; result lane
+// l, element i = D[l / 16 + 4 i][l % 16].  Complex product = four real MFMAs
+// (re re, -im im, re im, im re) on real/imaginary planes staged in LDS, K
+// in chunks of TK (64 for one-wave, 32 for four-wave work-groups) so that
+// the operand loads of a chunk are all in flight together.
+typedef double qsx_double4 __attribute__((ext_vector_type(4)));
+
+template <bool TA, bool TB, int WM, int WN, int TK>
+__global__ __launch_bounds__(64 * WM * WN) void zgemm_mfma_kernel(
+    int m, int n, int k, double2 alpha, const double2* __restrict__ A, int64_t lda, int64_t sA,
+    const double2* __restrict__ B, int64_t ldb, int64_t sB, double2 beta,
+    double2* __restrict__ C, int64_t ldc, int64_t sC, double2 gamma) {
+  constexpr int NT = 64 * WM * WN, TM = 16 * WM, TN = 16 * WN;
+  __shared__ double Ar[TK][TM + 1], Ai[TK][TM + 1];
+  __shared__ double Br[TK][TN + 1], Bi[TK][TN + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid % 64, wave = tid / 64;
+  const int wm = wave % WM, wn = wave / WM;
+  const int bm = blockIdx.x * TM, bn = blockIdx.y * TN, bz = blockIdx.z;
+  const double2* Ab = A + (int64_t)bz * sA;
+  const double2* Bb = B + (int64_t)bz * sB;
+  double2* Cb = C + (int64_t)bz * sC;
+  qsx_double4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  const int r16 = lane % 16, q4 = lane / 16;
+  for (int k0 = 0; k0 < k; k0 += TK) {
+    for (int e = tid; e < TM * TK; e += NT) {
+      int mm, kk;
+      if (TA) { kk = e / TM; mm = e % TM; }
+      else { mm = e / TK; kk = e % TK; }
+      const int gi = bm + mm, gk = k0 + kk;
+      const int64_t off = TA ? (int64_t)gk * lda + gi : (int64_t)gi * lda + gk;
+      const double2 v = (gi < m && gk < k) ? Ab[off] : make_double2(0.0, 0.0);
+      Ar[kk][mm] = v.x;
+      Ai[kk][mm] = v.y;
+    }
+    for (int e = tid; e < TK * TN; e += NT) {
+      int kk, nn;
+      if (TB) { nn = e / TK; kk = e % TK; }
+      else { kk = e / TN; nn = e % TN; }
+      const int gk = k0 + kk, gj = bn + nn;
+      const int64_t off = TB ? (int64_t)gj * ldb + gk : (int64_t)gk * ldb + gj;
+      const double2 v = (gk < k && gj < n) ? Bb[off] : make_double2(0.0, 0.0);
+      Br[kk][nn] = v.x;
+      Bi[kk][nn] = v.y;
+    }
+    __syncthreads();
+    const int kend = min(TK, k - k0);
+    for (int kq = 0; kq < kend; kq += 4) {
+      const double ar = Ar[kq + q4][wm * 16 + r16], ai = Ai[kq + q4][wm * 16 + r16];
+      const double br = Br[kq + q4][wn * 16 + r16], bi = Bi[kq + q4][wn * 16 + r16];
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = bm + wm * 16 + q4 + 4 * i, gj = bn + wn * 16 + r16;
+    if (gi >= m || gj >= n) continue;
+    double2 v = cmul(alpha, make_double2(cr[i], ci[i]));
+    double2* cp = Cb + (int64_t)gi * ldc + gj;
+    if (beta.x != 0.0 || beta.y != 0.0) v = cadd(v, cmul(beta, *cp));
+    if (gi == gj) v = cadd(v, gamma);
+    *cp = v;
+  }
+}
+
+template <int WM, int WN, int TK>
+void launch_zgemm_mfma(int trans, hipStream_t s, int m, int n, int k, double2 al, const void* A,
+                       int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB,
+                       double2 be, void* C, int64_t ldc, int64_t sC, double2 ga, int batch) {
+  dim3 grid((m + 16 * WM - 1) / (16 * WM), (n + 16 * WN - 1) / (16 * WN), batch);
+#define QSX_ZM(TA_, TB_)                                                                      \
+  hipLaunchKernelGGL((zgemm_mfma_kernel<TA_, TB_, WM, WN, TK>), grid, dim3(64 * WM * WN), 0, s, \
+                     m, n, k, al, (const double2*)A, lda, sA, (const double2*)B, ldb, sB, be,  \
+                     (double2*)C, ldc, sC, ga)
+  switch (trans & 3) {
+    case 0: QSX_ZM(false, false); break;
+    case 1: QSX_ZM(true, false); break;
+    case 2: QSX_ZM(false, true); break;
+    default: QSX_ZM(true, true); break;
+  }
+#undef QSX_ZM
+}
+
 template <typename T, int TM, int TN>
 void launch_zgemm(int trans, dim3 grid, hipStream_t s, int m, int n, int k, double2 al,
                   const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB,
@@ -129,6 +220,16 @@ extern "C" int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_
   hipStream_t s = as_stream(stream);
   // small problems: 16 x 16 tiles so that more workgroups share the work
   const int64_t tiles32 = (int64_t)((m + 31) / 32) * ((n + 31) / 32) * batch;
+  // FP64 with enough 32 x 32 tiles to fill the chip: matrix cores; small and
+  // skinny products are latency-bound and run faster on the vector path
+  // (measured on MI355X: 255 x 255 x 154 x 4 16 us vs 21 us; 255 x 147 x 128
+  // 16 us vs 9 us)
+  if (precision == QSX_FP64 && tiles32 >= 192) {
+    launch_zgemm_mfma<2, 2, 32>(trans, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC,
+                                ga, batch);
+    QSX_CHECK_HIP(hipGetLastError());
+    return QSX_OK;
+  }
   if (tiles32 < 192) {
     dim3 grid((m + 15) / 16, (n + 15) / 16, batch);
     if (precision == QSX_FP64)

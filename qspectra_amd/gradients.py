@@ -288,7 +288,8 @@ class RedfieldAbsorption(object):
                  | (_native.QSX_REAL_CORR if self.discard_imag_corr else 0)
                  | _native.QSX_HEISENBERG)
         rw = torch.full((H1.shape[0],), self.rw_freq, dtype=torch.float64, device=H1.device)
-        L_dev = _native.redfield_block(H1.detach().contiguous(), rw, self.T, float(lam),
+        lam_v = float(lam.detach()) if torch.is_tensor(lam) else float(lam)
+        L_dev = _native.redfield_block(H1.detach().contiguous(), rw, self.T, lam_v,
                                        self.gamma, 1, 0, flags, self.unit_convert)
         if not (H1.requires_grad or (torch.is_tensor(lam) and lam.requires_grad)):
             return L_dev

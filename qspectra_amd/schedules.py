@@ -51,12 +51,15 @@ class Stage(object):
       'doubling' the same trajectory by doubling: rows [y_0 .. y_{m-1}] give
                  rows [y_m .. y_{2m-1}] in one GEMM with the cached power
                  (P^q)^m, so ceil(log2 n) GEMMs (FP64);
+      'dchunks'  'chunks' with the chunk starts computed by doubling (powers
+                 of P^c, log2(n/c) GEMMs) instead of a stepper chain;
       'stepper'  the register stepper with P (or L for schedule 'rk4').
     With schedule 'auto' a trajectory of one right-hand side picks among
     'doubling', 'chunks' (several chunk lengths) and 'stepper' by timing
     them on the device (schedules.autotune), once per problem shape."""
 
-    def __init__(self, eom, t, t0, dt, precision, schedule, R, chunk=None, kind=None):
+    def __init__(self, eom, t, t0, dt, precision, schedule, R, chunk=None, kind=None,
+                 graph=False):
         n, h = rk4_schedule(t, t0, dt)
         hs = h[n > 0]
         uniform = hs.size == 0 or np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0]))
@@ -76,6 +79,7 @@ class Stage(object):
         self.precision = _mode(precision)
         self.prec_name = precision
         self.mode = _native.QSX_MODE_PROPAGATOR if prop else _native.QSX_MODE_RK4
+        self.graph = graph
         trajectory = (prop and hs.size and R == 1 and n[0] == 0 and np.all(n[1:] == n[1])
                       and n[1] > 0 and self.nsave > 8)
         if kind is None:
@@ -85,7 +89,7 @@ class Stage(object):
                     kind = 'powers'
                 elif trajectory:
                     kind, chunk = self._choose(schedule, chunk)
-        elif kind in ('chunks', 'doubling') and not trajectory:
+        elif kind in ('chunks', 'dchunks', 'doubling') and not trajectory:
             raise ValueError("schedule %r needs one right-hand side saved at equally "
                              "spaced points" % kind)
         self._setup(kind, chunk)
@@ -96,6 +100,7 @@ class Stage(object):
         for c in (4, 8, 16, 32, 64):
             if 2 * c < self.nsave:
                 cands.append(('chunks', c))
+                cands.append(('dchunks', c))
         if self.steps <= 512:
             cands.append(('stepper', None))
         return cands
@@ -107,8 +112,9 @@ class Stage(object):
         if schedule != 'auto' or not autotune_enabled():
             return default
         M = self.eom.L.shape[-1]
-        key = 'traj M=%d B=%d nsave=%d q=%d %s' % (M, self.eom.L.shape[0], self.nsave,
-                                                  int(self.n[1]), self.prec_name)
+        key = 'traj M=%d B=%d nsave=%d q=%d %s %s' % (
+            M, self.eom.L.shape[0], self.nsave, int(self.n[1]), self.prec_name,
+            'graph' if self.graph else 'eager')
 
         def trial(cand):
             st = copy.copy(self)
@@ -116,7 +122,7 @@ class Stage(object):
             y0 = torch.ones((self.eom.L.shape[0], 1, M), dtype=torch.complex128,
                             device=self.eom.L.device) / np.sqrt(M)
             return lambda: st.run(y0, [])
-        best = autotune(key, self._candidates(), trial)
+        best = autotune(key, self._candidates(), trial, graph=self.graph)
         return best[0], best[1]
 
     def _setup(self, kind, chunk):
@@ -136,6 +142,18 @@ class Stage(object):
                                          device=dev)
             self.starts_h = torch.full((self.nc,), self.h0 * q * chunk, dtype=torch.float64,
                                        device=dev)
+            self.chunk_n = torch.tensor([0] + [q] * (chunk - 1), dtype=torch.int32, device=dev)
+            self.chunk_h = torch.full((chunk,), self.h0, dtype=torch.float64, device=dev)
+            self.A, self.diag = stepper_operator(eom, 'propagator', self.h0, self.prec_name)
+        elif kind == 'dchunks':
+            q = int(n[1])
+            self.c = chunk
+            self.nc = -(-self.nsave // chunk)
+            self.pw = []
+            m = 1
+            while m < self.nc:
+                self.pw.append(propagator_power(eom, self.h0, q * chunk * m))
+                m *= 2
             self.chunk_n = torch.tensor([0] + [q] * (chunk - 1), dtype=torch.int32, device=dev)
             self.chunk_h = torch.full((chunk,), self.h0, dtype=torch.float64, device=dev)
             self.A, self.diag = stepper_operator(eom, 'propagator', self.h0, self.prec_name)
@@ -171,14 +189,17 @@ class Stage(object):
         if self.kind == 'doubling':
             B, R, M = y0.shape
             out = torch.empty((B, 1, self.nsave, M), dtype=torch.complex128, device=y0.device)
-            o = out[:, 0]
-            o[:, 0].copy_(y0[:, 0])
-            m = 1
-            for Pm in self.pw:                        # rows m..2m-1 = rows 0..m-1 (P^m)^T
-                r = min(m, self.nsave - m)
-                _native.zgemm(o[:, :r], Pm, o[:, m:m + r], transb=True)
-                m += r
+            _doubling(out[:, 0], y0[:, 0], self.pw)
             return out
+        if self.kind == 'dchunks':
+            B, R, M = y0.shape
+            starts = torch.empty((B, self.nc, M), dtype=torch.complex128, device=y0.device)
+            _doubling(starts, y0[:, 0], self.pw)
+            st1, _, s1 = _native.propagate_states(self.mode, self.A, starts, self.chunk_n,
+                                                   self.chunk_h, self.precision, self.diag,
+                                                   steps=self.steps, status=status)
+            statuses.append(s1)
+            return st1.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
         if self.kind == 'chunks':
             B, R, M = y0.shape
             st0, _, s0 = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, self.Pc, y0,
@@ -195,6 +216,18 @@ class Stage(object):
                                               status=status)
         statuses.append(st)
         return out
+
+
+def _doubling(o, y0, powers):
+    """Rows of o [B, n, M]: o[:, k] = A^k y0 where powers[j] = A^(2^j):
+    rows m..2m-1 are rows 0..m-1 times (A^m)^T, one GEMM per doubling."""
+    n = o.shape[1]
+    o[:, 0].copy_(y0)
+    m = 1
+    for Pm in powers:
+        r = min(m, n - m)
+        _native.zgemm(o[:, :r], Pm, o[:, m:m + r], transb=True)
+        m += r
 
 
 # ---------------------------------------------------------------- autotuner
@@ -234,13 +267,15 @@ def _save_cache():
         pass
 
 
-def autotune(key, candidates, make_trial, reps=5):
+def autotune(key, candidates, make_trial, reps=5, graph=False):
     """The fastest of ``candidates`` for problem ``key`` on this device:
     each candidate's launch sequence (make_trial(candidate) -> callable) is
-    run once to warm up, then timed ``reps`` times with device events; the
-    median decides.  Decisions are cached in memory and in a JSON file
-    (QSX_TUNE_CACHE, default ~/.cache/qspectra_amd/schedules.json), keyed by
-    device name and problem shape."""
+    run once to warm up, then timed ``reps`` times with device events -- as
+    eager launches, or (graph=True) as replays of a captured HIP graph, the
+    way a plan runs it; the median decides.  Decisions are cached in memory
+    and in a JSON file (QSX_TUNE_CACHE, default
+    ~/.cache/qspectra_amd/schedules.json), keyed by device name and problem
+    shape."""
     _load_cache()
     full = '%s | %s' % (torch.cuda.get_device_name(), key)
     hit = _TUNED.get(full)
@@ -251,6 +286,13 @@ def autotune(key, candidates, make_trial, reps=5):
         fn = make_trial(cand)
         fn()
         torch.cuda.synchronize()
+        if graph:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                fn()
+            fn = g.replay
+            fn()
+            torch.cuda.synchronize()
         ms = []
         for _ in range(reps):
             a = torch.cuda.Event(enable_timing=True)

@@ -20,11 +20,13 @@ bra of pathway p), so a plan evaluates it in the cheapest association:
     Liouvillian);
   * W_s = Q_s V2_p^T for every pathway, written side by side so that all
     pathways contract with their t3 trajectories in one GEMM;
-  * each distinct t3 Heisenberg trajectory once, on a second stream;
+  * each distinct t3 Heisenberg trajectory once, each on a stream of its own;
   * the Fourier transforms of two_dimensional_spectra (simulate/utils.py:
     154-219) as products with the transform matrices F1, F3 (the FFT
     pipeline applied to unit vectors), folded into the t1 and t3 factors:
-        X_s = (F1 S1) . [sum_p W_s^p (F3 G3_p)^T].
+        X_s = [(F1 S1) W_s] . [F3 G3]^T
+    with W_s = [sum_p W_s^p] side by side over the distinct t3 trajectories,
+    so that only the last GEMM waits for the t3 streams.
 Exact algebra: the result equals the pathway-by-pathway evaluation up to
 rounding (the same RK4 schedule, only the order of the linear maps
 differs).  The launch sequence can be captured into a HIP graph.
@@ -113,7 +115,7 @@ class ThirdOrderPlan(object):
                 y1 = np.einsum('bij,j->bi', V0, rho0)[:, None, :]
                 y1 = np.broadcast_to(y1, (B,) + y1.shape[1:])
                 stage = _Stage(model.equation_of_motion(s[1]), self.t1, None, dt, precision,
-                               schedule, 1)
+                               schedule, 1, graph=True)
                 self.t1_stages[k1] = (stage, dev_t(y1))
             k12 = tuple(s[:3])
             if k12 not in self.groups:
@@ -139,7 +141,7 @@ class ThirdOrderPlan(object):
                                   'bra')[:, None, :]
                 b3 = np.broadcast_to(b3, (B,) + b3.shape[1:])
                 stage = _Stage(model.equation_of_motion(s[3], heisenberg_picture=True),
-                               self.t3, None, dt, precision, schedule, 1)
+                               self.t3, None, dt, precision, schedule, 1, graph=True)
                 self.t3_stages[k3] = dict(stage=stage, bra=dev_t(b3), col=None)
             V2 = _op_matrices(model, '%s->%s' % (s[2], s[3]), polarization[2], trans[2],
                               'commutator')                            # [B|1, M3, M2]
@@ -152,9 +154,19 @@ class ThirdOrderPlan(object):
                 width += M3
             branches.append((k12, k3, V2T, st3['col'], M3, first))
         # pathways contract with their t3 trajectory through one column range
-        # per distinct trajectory: W blocks of pathways sharing it are summed
+        # per distinct trajectory: per t2 group, the V2 maps of its pathways
+        # are placed at their trajectory's columns (summed where pathways
+        # share one), so that W = sum_g Q_g V2_g^T is one GEMM per group
         self.width = width
         self.branches = branches
+        for k12, g in self.groups.items():
+            mine = [br for br in branches if br[0] == k12]
+            bsz = max(br[2].shape[0] for br in mine)
+            M2 = mine[0][2].shape[1]
+            full = torch.zeros((bsz, M2, width), dtype=torch.complex128, device=dev)
+            for _, _, V2T, col, M3, _ in mine:
+                full[:, :, col:col + M3] += V2T
+            g['V2T'] = full.contiguous()
         M1s = set(int(g['V1T'].shape[1]) for g in self.groups.values())
         if len(M1s) != 1:
             raise PlanNotApplicable('pathways of one plan must share the t1 block')
@@ -177,13 +189,14 @@ class ThirdOrderPlan(object):
         self.W = torch.empty((B * n2, self.M1, width), dtype=torch.complex128, device=dev)
         self.Q = {k: torch.empty((B * n2, self.M1, g['V1T'].shape[2]), dtype=torch.complex128,
                                  device=dev) for k, g in self.groups.items()}
-        self.Z = torch.empty((B * n2, self.M1, N3), dtype=torch.complex128, device=dev)
+        self.U = torch.empty((B * n2, N1, width), dtype=torch.complex128, device=dev)
         self.S1F = torch.empty((B, N1, self.M1), dtype=torch.complex128, device=dev)
         self.X = torch.empty((B * n2, N1, N3), dtype=torch.complex128, device=dev)
         # status slots of the stepper launches (t1, then each t3), cleared per run
         self.status = torch.zeros(1 + len(self.t3_stages), dtype=torch.int32, device=dev)
         self.steps = self._count_steps(paths)
-        self._side = torch.cuda.Stream()
+        self._sides = [torch.cuda.Stream() for _ in self.t3_stages]
+        self._t2_stream = torch.cuda.Stream()
         self._graph = None
         self._statuses = []
 
@@ -203,29 +216,34 @@ class ThirdOrderPlan(object):
         main = torch.cuda.current_stream()
         self.status.zero_()
         statuses = []
-        self._side.wait_stream(main)
-        with torch.cuda.stream(self._side):
-            # t3 Heisenberg trajectories, transformed along t3, side by side in G
-            for i, st in enumerate(self.t3_stages.values()):
+        # each t3 Heisenberg trajectory on its own stream, transformed along t3
+        # and written to its column range of G
+        for i, (st, side) in enumerate(zip(self.t3_stages.values(), self._sides)):
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
                 M3 = st['bra'].shape[-1]
                 G3 = st['stage'].run(st['bra'], statuses,
                                      self.status[1 + i:2 + i])[:, 0, :n3]      # [B, n3, M3]
                 _native.zgemm(self.F3, G3, self.G[:, :, st['col']:st['col'] + M3])
-        # t1 trajectory, transformed along t1
+        # t2 stage on the rows of V1^T and the V2 maps of every pathway, on a
+        # stream of its own (independent of the trajectories)
+        t2s = self._t2_stream
+        t2s.wait_stream(main)
+        with torch.cuda.stream(t2s):
+            for i, (k12, g) in enumerate(self.groups.items()):
+                _native.zgemm(g['V1T'], g['Pn'], self.Q[k12], transb=True)
+                _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=0.0 if i == 0 else 1.0)
+        # t1 trajectory, transformed along t1, then U_s = (F1 S1) W_s
         stage1, y1 = next(iter(self.t1_stages.values()))
         S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
         _native.zgemm(self.F1, S1, self.S1F)
-        # t2 stage on the rows of V1^T, then the V2 maps of every pathway
-        for k12, g in self.groups.items():
-            _native.zgemm(g['V1T'], g['Pn'], self.Q[k12], transb=True)
-        for k12, k3, V2T, col, M3, first in self.branches:
-            _native.zgemm(self.Q[k12], V2T, self.W[:, :, col:col + M3],
-                          beta=0.0 if first else 1.0)
-        main.wait_stream(self._side)
-        G = self.G if B == 1 else self.G.repeat_interleave(n2, dim=0)
-        _native.zgemm(self.W, G, self.Z, transb=True)                            # [B n2, M1, N3]
+        main.wait_stream(t2s)
         S1F = self.S1F if B == 1 else self.S1F.repeat_interleave(n2, dim=0)
-        _native.zgemm(S1F, self.Z, self.X)                                       # [B n2, N1, N3]
+        _native.zgemm(S1F, self.W, self.U)                                       # [B n2, N1, w]
+        for side in self._sides:
+            main.wait_stream(side)
+        G = self.G if B == 1 else self.G.repeat_interleave(n2, dim=0)
+        _native.zgemm(self.U, G, self.X, transb=True)                            # [B n2, N1, N3]
         self._statuses = statuses
         N1, N3 = self.X.shape[1], self.X.shape[2]
         return self.X.view(B, n2, N1, N3).permute(0, 2, 1, 3)

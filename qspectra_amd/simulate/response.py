@@ -50,18 +50,30 @@ def _member_model(model, ham):
 
 def _op_matrices(model, liouv_map, polarization, transition, what):
     """Per-member matrices of a dipole map: what = 'commutator' -> [B|1, K, M],
-    'bra' -> [B|1, M].  One shared copy when every member has the same
-    dipoles in the site basis."""
+    'bra' -> [B|1, M].  In the site basis the map depends on the member only
+    through its dipoles, so it is evaluated once per distinct dipole set (one
+    shared copy when all members have the same dipoles)."""
     hams = _members(model)
-    shared = (model.evolve_basis == 'site' if hasattr(model, 'evolve_basis') else True) and all(
-        np.array_equal(h.dipoles, hams[0].dipoles) for h in hams)
-    targets = hams[:1] if shared else hams
-    mats = []
-    for h in targets:
-        op = _member_model(model, h).dipole_operator(liouv_map, polarization, transition)
+    site = model.evolve_basis == 'site' if hasattr(model, 'evolve_basis') else True
+    if site:
+        keys, first = [], {}
+        for i, h in enumerate(hams):
+            d = np.ascontiguousarray(h.dipoles)
+            k = (d.shape, d.tobytes())
+            first.setdefault(k, i)
+            keys.append(first[k])
+        uniq = sorted(set(keys))
+    else:
+        keys = list(range(len(hams)))
+        uniq = keys
+    mats = {}
+    for i in uniq:
+        op = _member_model(model, hams[i]).dipole_operator(liouv_map, polarization, transition)
         op = getattr(op, 'lspace_op', op)       # hierarchy operators act per ADO
-        mats.append(op.commutator.matrix if what == 'commutator' else op.bra_vector)
-    return np.stack(mats)
+        mats[i] = op.commutator.matrix if what == 'commutator' else op.bra_vector
+    if len(uniq) == 1:
+        return np.stack([mats[uniq[0]]])
+    return np.stack([mats[k] for k in keys])
 
 
 def _dev(x):

@@ -15,6 +15,7 @@ _active = []
 class KernelTimes(object):
     def __init__(self):
         self.records = []
+        self.relaunch = {}       # label -> callable repeating the last such launch
 
     def summary(self):
         torch.cuda.synchronize()
@@ -37,7 +38,9 @@ def kernel_timer():
 
 
 @contextlib.contextmanager
-def launch(label, stream=None):
+def launch(label, stream=None, relaunch=None):
+    """Brackets one libqsx launch; ``relaunch`` (optional) repeats exactly
+    that launch (same operands), for back-to-back duration measurements."""
     if not _active:
         yield
         return
@@ -49,3 +52,30 @@ def launch(label, stream=None):
     b.record(s)
     for kt in _active:
         kt.records.append((label, a, b))
+        if relaunch is not None:
+            kt.relaunch[label] = relaunch
+
+
+def back_to_back_ms(fn, n=20, reps=5):
+    """Mean device time of one call of ``fn`` (a single kernel launch) when
+    n calls are replayed back to back from a captured HIP graph, so that
+    host launch overhead and idle gaps do not enter: median over reps."""
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(n):
+            fn()
+    g.replay()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        g.replay()
+        b.record()
+        b.synchronize()
+        ms.append(a.elapsed_time(b) / n)
+    ms.sort()
+    return ms[len(ms) // 2]

@@ -272,6 +272,7 @@ def test_fused_plan_ensemble_and_isotropic(gpu):
 
 @pytest.mark.parametrize('kind,chunk,precision', [('doubling', None, 'fp64'),
                                                   ('chunks', 4, 'fp64'), ('chunks', 16, 'mixed'),
+                                                  ('dchunks', 8, 'mixed'), ('dchunks', 16, 'fp64'),
                                                   ('stepper', None, 'fp64'),
                                                   ('stepper', None, 'mixed')])
 def test_trajectory_schedules_match_oracle(gpu, kind, chunk, precision):
