@@ -205,16 +205,23 @@ int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_t n, int32_
  *          n = 0: [B][M][M] dense -i*unit*[H, .] block (M <= 64)
  *   shift  [n_ado] double, -unit * sum_jk m_jk nu_k
  *   dvec   [M] complex, -unit * Delta * sum_j (n_j(r) + n_j(c) - 2 n_j(r) n_j(c))
- *   edge_src/edge_w/edge_coef [n_ado][E]: incoming neighbour couplings
- *          (source ADO or -1 after the last, weight-vector id, coefficient)
- *   W      [nw][M] complex weight vectors (diagonal neighbour superoperators)
+ *   bin_src/bin_coef [n_ado][nbins][2] (nbins = N * K1, bin b = j * K1 + k):
+ *          the neighbour coupled through bin b (kind 0: n + e_b with weight
+ *          n_j(r) - n_j(c); kind 1: n - e_b with weight c_k n_j(r) - c_k^*
+ *          n_j(c); in the Heisenberg picture the transposed edges), -1 for
+ *          none, and its coefficient (includes unit)
+ *   ck     [K1] complex correlation-function coefficients c_k (heom.py:69-89)
+ *   rel_site [M][nrel] sites occupied in the row or the column state of
+ *          element p (-1 padded); rel_occ [M][nrel][2] their occupations
+ *          (row, column): the weights vanish for every other site
  */
 
 /* out[b] = F(y[b]) (the hierarchy's eom(t, y), heom.py:241-242). */
 int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n, const void* Hop,
-                       const double* shift, const void* dvec, int32_t E,
-                       const int32_t* edge_src, const int32_t* edge_w, const void* edge_coef,
-                       const void* W, const void* y, void* out, void* stream);
+                       const double* shift, const void* dvec, int32_t nbins, int32_t K1,
+                       const int32_t* bin_src, const void* bin_coef, const void* ck,
+                       int32_t nrel, const int32_t* rel_site, const double* rel_occ,
+                       const void* y, void* out, void* stream);
 
 /* Device workspace bytes of qsx_heom_propagate (-1 for invalid arguments). */
 int64_t qsx_heom_workspace_bytes(int32_t precision, int32_t B, int32_t n_ado, int32_t M);
@@ -227,9 +234,10 @@ int64_t qsx_heom_workspace_bytes(int32_t precision, int32_t B, int32_t n_ado, in
  * complex64 (state and update complex128).  Allocates nothing: workspace
  * holds qsx_heom_workspace_bytes bytes. */
 int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t M, int32_t n,
-                           const void* Hop, const double* shift, const void* dvec, int32_t E,
-                           const int32_t* edge_src, const int32_t* edge_w, const void* edge_coef,
-                           const void* W, const void* y0, const int32_t* nsub_host,
+                           const void* Hop, const double* shift, const void* dvec, int32_t nbins,
+                           int32_t K1, const int32_t* bin_src, const void* bin_coef,
+                           const void* ck, int32_t nrel, const int32_t* rel_site,
+                           const double* rel_occ, const void* y0, const int32_t* nsub_host,
                            const double* h_host, int32_t nsave, void* out, void* y_final,
                            void* workspace, int32_t* status, void* stream);
 

@@ -59,11 +59,11 @@ SIGNATURES = {
     'qsx_zgemm': [c_i32, c_i32, c_i32, c_i32, c_i32, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_vp, c_i64,
                   c_i64, c_dbl, c_dbl, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_i32, c_vp],
     'qsx_heom_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
-    'qsx_heom_apply': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp,
-                       c_vp, c_vp, c_vp, c_vp],
-    'qsx_heom_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp,
-                           c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
-                           c_vp],
+    'qsx_heom_apply': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp,
+                       c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    'qsx_heom_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32,
+                           c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp,
+                           c_vp, c_vp, c_vp, c_vp],
 }
 
 
@@ -370,8 +370,9 @@ def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stre
 
 # ------------------------------------------------------------------- HEOM
 def _heom_ptrs(eom):
-    return (eom.n_ado, eom.M, eom.n_comm, ptr(eom.Hop), ptr(eom.shift), ptr(eom.dvec), eom.E,
-            ptr(eom.esrc), ptr(eom.ew), ptr(eom.ecoef), ptr(eom.W))
+    return (eom.n_ado, eom.M, eom.n_comm, ptr(eom.Hop), ptr(eom.shift), ptr(eom.dvec),
+            eom.nbins, eom.K1, ptr(eom.bsrc), ptr(eom.bcoef), ptr(eom.ck), eom.nrel,
+            ptr(eom.rel_site), ptr(eom.rel_occ))
 
 
 def heom_apply(eom, y, stream=None):
@@ -381,9 +382,8 @@ def heom_apply(eom, y, stream=None):
     assert y.shape == (B, eom.n_ado * eom.M) and y.dtype == torch.complex128
     y = y.contiguous()
     out = torch.empty_like(y)
-    n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W = _heom_ptrs(eom)
-    check(lib.qsx_heom_apply(B, n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W, ptr(y), ptr(out),
-                             stream_ptr(stream)), 'qsx_heom_apply')
+    check(lib.qsx_heom_apply(B, *_heom_ptrs(eom), ptr(y), ptr(out), stream_ptr(stream)),
+          'qsx_heom_apply')
     return out
 
 
@@ -405,13 +405,12 @@ def heom_propagate(eom, y0, nsub, h, precision=QSX_FP64, want_final=False, strea
     out = torch.empty((B, nsave, eom.M), dtype=torch.complex128, device=y0.device)
     yfin = torch.empty_like(y0) if want_final else None
     status = torch.zeros(1, dtype=torch.int32, device=y0.device)
-    n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W = _heom_ptrs(eom)
     label = 'heom %s B=%d n_ado=%d M=%d steps=%d' % (
-        'fp64' if precision == QSX_FP64 else 'mixed', B, n_ado, M, int(nsub.sum()))
+        'fp64' if precision == QSX_FP64 else 'mixed', B, eom.n_ado, eom.M, int(nsub.sum()))
     with timing.launch(label, stream):
         st = lib.qsx_heom_propagate(
-            precision, B, n_ado, M, n, Hop, shift, dvec, E, es, ew, ec, W, ptr(y0),
-            nsub.ctypes.data_as(c_vp), h.ctypes.data_as(c_vp), nsave, ptr(out), ptr(yfin),
-            ptr(ws), ptr(status), stream_ptr(stream))
+            precision, B, *_heom_ptrs(eom), ptr(y0), nsub.ctypes.data_as(c_vp),
+            h.ctypes.data_as(c_vp), nsave, ptr(out), ptr(yfin), ptr(ws), ptr(status),
+            stream_ptr(stream))
     check(st, 'qsx_heom_propagate')
     return out, yfin, status

@@ -9,10 +9,16 @@
 // low-temperature double commutator is Delta sum_j (n_j(r) + n_j(c) -
 // 2 n_j(r) n_j(c)).  So
 //   d rho_n / dt = Lh rho_n + (shift_n + dvec) o rho_n
-//                  + sum_{edges e into n} coef_e W_{w_e} o rho_{src_e}
+//                  + sum_{bins b = (j, k)} [ coef_{n,b,0} (n_j(r) - n_j(c)) rho_{src_{n,b,0}}
+//                       + coef_{n,b,1} (c_k n_j(r) - c_k^* n_j(c)) rho_{src_{n,b,1}} ]_p
 // with Lh the (sample-dependent) -i unit [H, .] block, shift_n the ADO
-// damping, W the per-element weight vectors and an edge table built on the
-// host (+/-1 neighbours, or their transpose for the Heisenberg picture).
+// damping and binned neighbour tables built on the host: kind 0 the n + e_b
+// neighbour, kind 1 the n - e_b neighbour (or, in the Heisenberg picture,
+// the ADO that has n as that neighbour; one edge per ADO, bin and kind
+// either way).  The weights vanish unless site j is occupied in the row or
+// the column state of element p, so each element walks only the bins of its
+// occupied sites (a host list per element): <= 8 of the 28 edges of a
+// 7-site, K = 1 hierarchy on the 'ee' block.
 //
 // Lh is applied either as a dense M x M block (any subspace, M <= 64) or,
 // for a square subspace XX (M = n^2), as the commutator -i unit (H rho -
@@ -34,16 +40,23 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
+#ifndef QSX_HEOM_ELEMS_PER_THREAD
+#define QSX_HEOM_ELEMS_PER_THREAD 1
+#endif
+// Liouville elements per thread and stage: a work-group covers
+// kElemsPerThread * 256 / M ADOs (with their edge tables in LDS)
+constexpr int kElemsPerThread = QSX_HEOM_ELEMS_PER_THREAD;
 
 struct HeomArgs {
-  int B, n_ado, M, n, dense, E, tiles, ado_per_tile, xcd_map;
+  int B, n_ado, M, n, dense, nbins, K1, nrel, tiles, ado_per_tile, xcd_map;
   const double2* Hop;      // COMM: [B][n][n] = -i unit H_X (or its transpose); DENSE: [B][M][M]
   const double* shift;     // [n_ado]   -unit * sum m nu
   const double2* dvec;     // [M]       -unit * Delta * D_p
-  const int* esrc;         // [n_ado][E] source ADO (-1 = none)
-  const int* ew;           // [n_ado][E] weight vector id
-  const double2* ecoef;    // [n_ado][E] scalar coefficient (includes unit)
-  const double2* W;        // [nw][M]
+  const int* bsrc;         // [n_ado][nbins][2] neighbour ADO (-1 = none)
+  const double2* bcoef;    // [n_ado][nbins][2] coefficient (includes unit)
+  const double2* ck;       // [K1]      correlation-function coefficients c_k
+  const int* rsite;        // [M][nrel] sites occupied in the row or column state (-1 pad)
+  const double* rocc;      // [M][nrel][2] their occupations (row, column)
 };
 
 template <typename T>
@@ -54,12 +67,12 @@ __device__ inline double2 ld(const cplx_t<T>* p) { return to_d2(*p); }
 // LDS layout of a stage work-group: operator block, the tile's stage
 // vectors, and the tile's edge tables (so that the neighbour gathers do not
 // wait on dependent global loads of the table).
-inline size_t heom_lds_bytes(int opsz, int ado_per_tile, int M, int E, size_t zbytes) {
+inline size_t heom_lds_bytes(int opsz, int ado_per_tile, int M, int nbins, size_t zbytes) {
   size_t off = (size_t)opsz * sizeof(double2);
   off += (size_t)ado_per_tile * M * zbytes;
   off = (off + 15) & ~(size_t)15;
-  off += (size_t)ado_per_tile * E * sizeof(double2);
-  off += 2 * (size_t)ado_per_tile * E * sizeof(int);
+  off += 2 * (size_t)ado_per_tile * nbins * sizeof(double2);
+  off += 2 * (size_t)ado_per_tile * nbins * sizeof(int);
   return off;
 }
 
@@ -72,14 +85,13 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
   using T2 = cplx_t<T>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int opsz = a.dense ? a.M * a.M : a.n * a.n;
-  const int E = a.E;
+  const int nb2 = 2 * a.nbins;
   double2* Hs = reinterpret_cast<double2*>(smem);                 // n*n (COMM) or M*M (DENSE)
   T2* zt = reinterpret_cast<T2*>(Hs + opsz);                      // [ado_per_tile][M]
   size_t off = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * sizeof(T2);
   off = (off + 15) & ~(size_t)15;
-  double2* ecf = reinterpret_cast<double2*>(smem + off);          // [ado_per_tile][E]
-  int* esrc = reinterpret_cast<int*>(ecf + a.ado_per_tile * E);   // [ado_per_tile][E]
-  int* ewid = esrc + a.ado_per_tile * E;                          // [ado_per_tile][E]
+  double2* tcf = reinterpret_cast<double2*>(smem + off);          // [ado_per_tile][nbins][2]
+  int* tsrc = reinterpret_cast<int*>(tcf + a.ado_per_tile * nb2);  // [ado_per_tile][nbins][2]
 
   int b, tile;
   if (a.xcd_map) {
@@ -99,16 +111,14 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
   const double2* Hb = a.Hop + (int64_t)b * opsz;
   for (int e = tid; e < opsz; e += kThreads) Hs[e] = Hb[e];
   for (int e = tid; e < nloc * M; e += kThreads) zt[e] = zin[base + e];
-  for (int e = tid; e < nloc * E; e += kThreads) {
-    const int64_t g = (int64_t)ado0 * E + e;
-    esrc[e] = a.esrc[g];
-    ewid[e] = a.ew[g];
-    ecf[e] = a.ecoef[g];
+  for (int e = tid; e < nloc * nb2; e += kThreads) {
+    const int64_t g = (int64_t)ado0 * nb2 + e;
+    tsrc[e] = a.bsrc[g];
+    tcf[e] = a.bcoef[g];
   }
   __syncthreads();
 
-  const int e = tid;
-  if (e < nloc * M) {
+  for (int e = tid; e < nloc * M; e += kThreads) {
     const int la = e / M, p = e % M, ado = ado0 + la;
     const T2* z = zt + la * M;
     double2 k = make_double2(0.0, 0.0);
@@ -127,24 +137,34 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
     const double2 zp = ld<T>(z + p);
     const double2 dg = make_double2(a.shift[ado] + a.dvec[p].x, a.dvec[p].y);
     k = cadd(k, cmul(dg, zp));
-    const int* sl = esrc + la * E;
-    const int* wl = ewid + la * E;
-    const double2* cl = ecf + la * E;
+    // neighbour terms: only the bins of the sites occupied in row or column
+    const int* ts = tsrc + la * nb2;
+    const double2* tc = tcf + la * nb2;
     const T2* zs = zin + (int64_t)b * a.n_ado * M + p;
-    // edges are packed and -1 terminated; predicated (no early exit) so that
-    // the unrolled neighbour gathers are all in flight together
-#pragma unroll 4
-    for (int q = 0; q < E; ++q) {
-      const int s = sl[q];
-      if (s >= 0) {
-        const double2 w = cmul(cl[q], a.W[(int64_t)wl[q] * M + p]);
-        k = cadd(k, cmul(w, ld<T>(zs + (int64_t)s * M)));
+    for (int t = 0; t < a.nrel; ++t) {
+      const int j = a.rsite[p * a.nrel + t];
+      if (j < 0 || (j + 1) * a.K1 > a.nbins) continue;
+      const double orow = a.rocc[(p * a.nrel + t) * 2], ocol = a.rocc[(p * a.nrel + t) * 2 + 1];
+      const double wp = orow - ocol;
+      for (int kk = 0; kk < a.K1; ++kk) {
+        const int bi = 2 * (j * a.K1 + kk);
+        const int s0 = ts[bi], s1 = ts[bi + 1];
+        if (s0 >= 0 && s0 < a.n_ado && wp != 0.0) {
+          const double2 w = make_double2(tc[bi].x * wp, tc[bi].y * wp);
+          k = cadd(k, cmul(w, ld<T>(zs + (int64_t)s0 * M)));
+        }
+        if (s1 >= 0 && s1 < a.n_ado) {
+          const double2 c = a.ck[kk];
+          // c_k n_j(r) - conj(c_k) n_j(c)
+          const double2 wm = make_double2(c.x * orow - c.x * ocol, c.y * orow + c.y * ocol);
+          k = cadd(k, cmul(cmul(tc[bi + 1], wm), ld<T>(zs + (int64_t)s1 * M)));
+        }
       }
     }
     const int64_t gi = base + e;
     if (st == 4) {                           // plain application: zout = F(zin)
       zout[gi] = from_d2<T>(k);
-      return;
+      continue;
     }
     if (st == 0) {
       const double2 yv = y[gi];
@@ -195,7 +215,7 @@ int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const d
   T2* zA = reinterpret_cast<T2*>(ynew + total);
   T2* zB = zA + total;
   const int opsz = a.dense ? a.M * a.M : a.n * a.n;
-  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.E, sizeof(T2));
+  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(T2));
   QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_heom_propagate: %zu bytes of LDS", shmem);
   if (shmem > 64 * 1024)
     QSX_CHECK_HIP(hipFuncSetAttribute((const void*)heom_stage_kernel<T>,
@@ -230,13 +250,16 @@ int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const d
 }  // namespace qsx
 
 static int fill_args(qsx::HeomArgs& a, int B, int n_ado, int M, int n, const void* Hop,
-                     const double* shift, const void* dvec, int E, const int32_t* edge_src,
-                     const int32_t* edge_w, const void* edge_coef, const void* W) {
+                     const double* shift, const void* dvec, int nbins, int K1,
+                     const int32_t* bin_src, const void* bin_coef, const void* ck, int nrel,
+                     const int32_t* rel_site, const double* rel_occ) {
   using namespace qsx;
-  QSX_CHECK_ARG(Hop && shift && dvec && edge_src && edge_w && edge_coef && W,
+  QSX_CHECK_ARG(Hop && shift && dvec && bin_src && bin_coef && ck && rel_site && rel_occ,
                 "qsx_heom: null operator pointer");
-  QSX_CHECK_ARG(B >= 1 && n_ado >= 1 && M >= 1 && E >= 1 && B <= (1 << 24),
-                "qsx_heom: bad sizes B=%d n_ado=%d M=%d E=%d", B, n_ado, M, E);
+  QSX_CHECK_ARG(B >= 1 && n_ado >= 1 && M >= 1 && B <= (1 << 24),
+                "qsx_heom: bad sizes B=%d n_ado=%d M=%d", B, n_ado, M);
+  QSX_CHECK_ARG(K1 >= 1 && nbins >= K1 && nbins % K1 == 0 && nrel >= 1,
+                "qsx_heom: bad bin tables nbins=%d K1=%d nrel=%d", nbins, K1, nrel);
   const bool dense = n == 0;
   QSX_CHECK_ARG(dense ? M <= 64 : (n >= 1 && n * n == M),
                 "qsx_heom: commutator form needs M = n^2, dense form M <= 64");
@@ -246,17 +269,20 @@ static int fill_args(qsx::HeomArgs& a, int B, int n_ado, int M, int n, const voi
   a.M = M;
   a.n = n;
   a.dense = dense;
-  a.E = E;
-  a.ado_per_tile = std::max(1, kThreads / M);
+  a.nbins = nbins;
+  a.K1 = K1;
+  a.nrel = nrel;
+  a.ado_per_tile = std::max(1, kElemsPerThread * kThreads / M);
   a.tiles = (n_ado + a.ado_per_tile - 1) / a.ado_per_tile;
   a.xcd_map = B >= kXcds;
   a.Hop = (const double2*)Hop;
   a.shift = shift;
   a.dvec = (const double2*)dvec;
-  a.esrc = edge_src;
-  a.ew = edge_w;
-  a.ecoef = (const double2*)edge_coef;
-  a.W = (const double2*)W;
+  a.bsrc = bin_src;
+  a.bcoef = (const double2*)bin_coef;
+  a.ck = (const double2*)ck;
+  a.rsite = rel_site;
+  a.rocc = rel_occ;
   const int64_t groups = a.xcd_map ? (int64_t)kXcds * ((B + kXcds - 1) / kXcds) * a.tiles
                                    : (int64_t)B * a.tiles;
   QSX_CHECK_ARG(groups < (1LL << 31), "qsx_heom: too many work-groups");
@@ -264,17 +290,18 @@ static int fill_args(qsx::HeomArgs& a, int B, int n_ado, int M, int n, const voi
 }
 
 extern "C" int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n, const void* Hop,
-                                  const double* shift, const void* dvec, int32_t E,
-                                  const int32_t* edge_src, const int32_t* edge_w,
-                                  const void* edge_coef, const void* W, const void* y, void* out,
-                                  void* stream) {
+                                  const double* shift, const void* dvec, int32_t nbins,
+                                  int32_t K1, const int32_t* bin_src, const void* bin_coef,
+                                  const void* ck, int32_t nrel, const int32_t* rel_site,
+                                  const double* rel_occ, const void* y, void* out, void* stream) {
   using namespace qsx;
   HeomArgs a;
-  int st = fill_args(a, B, n_ado, M, n, Hop, shift, dvec, E, edge_src, edge_w, edge_coef, W);
+  int st = fill_args(a, B, n_ado, M, n, Hop, shift, dvec, nbins, K1, bin_src, bin_coef, ck, nrel,
+                     rel_site, rel_occ);
   if (st) return st;
   QSX_CHECK_ARG(y && out, "qsx_heom_apply: null pointer");
   const int opsz = a.dense ? a.M * a.M : a.n * a.n;
-  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.E, sizeof(double2));
+  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(double2));
   QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_heom_apply: %zu bytes of LDS", shmem);
   const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;
   hipLaunchKernelGGL(heom_stage_kernel<double>, dim3(ngroups), dim3(kThreads), shmem,
@@ -293,19 +320,21 @@ extern "C" int64_t qsx_heom_workspace_bytes(int32_t precision, int32_t B, int32_
 
 extern "C" int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t M,
                                       int32_t n, const void* Hop, const double* shift,
-                                      const void* dvec, int32_t E, const int32_t* edge_src,
-                                      const int32_t* edge_w, const void* edge_coef,
-                                      const void* W, const void* y0, const int32_t* nsub_host,
-                                      const double* h_host, int32_t nsave, void* out,
-                                      void* y_final, void* workspace, int32_t* status,
-                                      void* stream) {
+                                      const void* dvec, int32_t nbins, int32_t K1,
+                                      const int32_t* bin_src, const void* bin_coef,
+                                      const void* ck, int32_t nrel, const int32_t* rel_site,
+                                      const double* rel_occ, const void* y0,
+                                      const int32_t* nsub_host, const double* h_host,
+                                      int32_t nsave, void* out, void* y_final, void* workspace,
+                                      int32_t* status, void* stream) {
   using namespace qsx;
   QSX_CHECK_ARG(precision == QSX_FP64 || precision == QSX_MIXED, "qsx_heom_propagate: precision");
   QSX_CHECK_ARG(y0 && nsub_host && h_host && out && workspace && status,
                 "qsx_heom_propagate: null pointer");
   QSX_CHECK_ARG(nsave >= 1, "qsx_heom_propagate: nsave=%d", nsave);
   HeomArgs a;
-  int rc = fill_args(a, B, n_ado, M, n, Hop, shift, dvec, E, edge_src, edge_w, edge_coef, W);
+  int rc = fill_args(a, B, n_ado, M, n, Hop, shift, dvec, nbins, K1, bin_src, bin_coef, ck, nrel,
+                     rel_site, rel_occ);
   if (rc) return rc;
   for (int s = 0; s < nsave; ++s)
     QSX_CHECK_ARG(nsub_host[s] >= 0, "qsx_heom_propagate: negative step count");

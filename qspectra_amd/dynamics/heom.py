@@ -11,9 +11,14 @@ structure is reduced to tables the GPU kernel (qsx_heom_propagate) reads:
     +1 neighbours, -i [P_j, .], and the -1 neighbours,
     -i m_jk (c_k P_j^L - c_k^* P_j^R) (heom.py:410-442), both diagonal in p in
     the site basis; the Heisenberg picture uses the transposed edges,
+  * the same edges binned by (ADO, bin (j, k), kind) -- the form the kernel
+    reads -- and, per element p, the sites occupied in its row or column
+    state: the weights of every other site vanish, so the kernel walks only
+    those bins,
   * per sample: the system Hamiltonian block for -i [H, .].
-``HEOM_tensor`` assembles the same tables into the reference's sparse
-matrix, which pins the tables on the host.
+``HEOM_tensor`` assembles the packed edges into the reference's sparse
+matrix, which pins the tables on the host; the binned form is checked
+against the packed one.
 """
 import numpy as np
 import torch
@@ -139,9 +144,6 @@ class DeviceHEOM(object):
     def size(self):
         return self.n_ado * self.M
 
-    def args(self):
-        return (self.n_ado, self.M, self.n_comm, self.Hop, self.shift, self.dvec, self.E,
-                self.esrc, self.ew, self.ecoef, self.W)
 
     def __call__(self, t, y):
         y = np.asarray(y, dtype=complex)
@@ -214,6 +216,18 @@ class HEOMModel(DynamicalModel):
         D = (occ_r + occ_c - 2 * occ_r * occ_c).sum(axis=1)
         dvec = -uc * delta * D if self.low_temp_corr else np.zeros(M, complex)
         edges = [[] for _ in range(len(ados))]
+        # the same edges binned by (ADO, bin b, kind): kind 0 couples to n + e_b
+        # with weight W_j, kind 1 to n - e_b with weight W_{N+b}; every
+        # (ADO, bin, kind) holds at most one edge, also after transposition
+        bsrc = -np.ones((len(ados), bins, 2), dtype=np.int32)
+        bcoef = np.zeros((len(ados), bins, 2), dtype=complex)
+
+        def binned(n, s, b, kind, cf):
+            if heisenberg_picture:
+                bsrc[s, b, kind], bcoef[s, b, kind] = n, cf
+            else:
+                bsrc[n, b, kind], bcoef[n, b, kind] = s, cf
+
         for n, m in enumerate(ados):
             for b in range(bins):
                 j, k = divmod(b, K + 1)
@@ -223,12 +237,23 @@ class HEOMModel(DynamicalModel):
                 if p is not None:
                     cf = (np.sqrt((m[b] + 1) * abs(c[k])) if self.modified_HEOM else 1.0)
                     edges[n].append((p, j, -1j * uc * cf))
+                    binned(n, p, b, 0, -1j * uc * cf)
                 if m[b] > 0:
                     dn = m.copy()
                     dn[b] -= 1
                     q = lookup[tuple(dn)]
                     cf = (np.sqrt(m[b] / abs(c[k])) if self.modified_HEOM else m[b])
                     edges[n].append((q, N + j * (K + 1) + k, -1j * uc * cf))
+                    binned(n, q, b, 1, -1j * uc * cf)
+        occ = occ_r + occ_c
+        nrel = max(1, int((occ > 0).sum(axis=1).max()))
+        rel_site = -np.ones((M, nrel), dtype=np.int32)
+        rel_occ = np.zeros((M, nrel, 2))
+        for p_ in range(M):
+            js = np.flatnonzero(occ[p_] > 0)
+            rel_site[p_, :js.size] = js
+            rel_occ[p_, :js.size, 0] = occ_r[p_, js]
+            rel_occ[p_, :js.size, 1] = occ_c[p_, js]
         if heisenberg_picture:
             t_edges = [[] for _ in range(len(ados))]
             for n, lst in enumerate(edges):
@@ -243,7 +268,9 @@ class HEOMModel(DynamicalModel):
             for e, (s, w, cf) in enumerate(lst):
                 esrc[n, e], ew[n, e], ecoef[n, e] = s, w, cf
         return dict(n_ado=len(ados), M=M, shift=shift, dvec=dvec.astype(complex), W=W, E=E,
-                    esrc=esrc, ew=ew, ecoef=ecoef, idx=idx)
+                    esrc=esrc, ew=ew, ecoef=ecoef, idx=idx, nbins=bins, K1=K + 1,
+                    bsrc=bsrc, bcoef=bcoef, ck=np.asarray(c[:K + 1], dtype=complex),
+                    nrel=nrel, rel_site=rel_site, rel_occ=rel_occ)
 
     def _square_block(self, liouville_subspace):
         """Hilbert manifold X when the subspace is the single block 'XX'."""
@@ -323,8 +350,10 @@ class HEOMModel(DynamicalModel):
             n_ado=tb['n_ado'], M=tb['M'], n_comm=n_comm, Hop=Hop.contiguous(),
             shift=torch.as_tensor(tb['shift'], dtype=torch.float64).to(dev),
             dvec=_native.to_device(tb['dvec']), E=tb['E'],
-            esrc=torch.as_tensor(tb['esrc']).to(dev), ew=torch.as_tensor(tb['ew']).to(dev),
-            ecoef=_native.to_device(tb['ecoef']), W=_native.to_device(tb['W']),
+            nbins=tb['nbins'], K1=tb['K1'], bsrc=torch.as_tensor(tb['bsrc']).to(dev),
+            bcoef=_native.to_device(tb['bcoef']), ck=_native.to_device(tb['ck']),
+            nrel=tb['nrel'], rel_site=torch.as_tensor(tb['rel_site']).to(dev),
+            rel_occ=torch.as_tensor(tb['rel_occ'], dtype=torch.float64).to(dev),
             liouville_subspace=liouville_subspace, heisenberg_picture=heisenberg_picture))
         cache[key] = eom
         return eom

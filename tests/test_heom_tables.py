@@ -60,3 +60,54 @@ def test_heom_tensor_7site():
     assert m5.ado_count == 3060
     tb = m5.tables('ee')
     assert tb['E'] <= 28 and tb['M'] == 49
+
+
+def _coupling_from_edges(tb):
+    """Dense hierarchy coupling (neighbour terms only) from the packed edges."""
+    n, M = tb['n_ado'], tb['M']
+    C = np.zeros((n * M, n * M), complex)
+    for a in range(n):
+        for s, w, cf in zip(tb['esrc'][a], tb['ew'][a], tb['ecoef'][a]):
+            if s < 0:
+                break
+            C[a * M:(a + 1) * M, s * M:(s + 1) * M] += np.diag(cf * tb['W'][w])
+    return C
+
+
+def _coupling_from_bins(tb):
+    """The same from the binned tables and the per-element site lists, with the
+    weights recomputed from the occupations as the GPU kernel does."""
+    n, M, K1 = tb['n_ado'], tb['M'], tb['K1']
+    C = np.zeros((n * M, n * M), complex)
+    for a in range(n):
+        for p in range(M):
+            for t, j in enumerate(tb['rel_site'][p]):
+                if j < 0:
+                    continue
+                o_r, o_c = tb['rel_occ'][p, t]
+                for k in range(K1):
+                    b = j * K1 + k
+                    for kind, w in ((0, o_r - o_c),
+                                    (1, tb['ck'][k] * o_r - np.conj(tb['ck'][k]) * o_c)):
+                        s = tb['bsrc'][a, b, kind]
+                        if s >= 0:
+                            C[a * M + p, s * M + p] += tb['bcoef'][a, b, kind] * w
+    return C
+
+
+@pytest.mark.parametrize('kw', [dict(level_cutoff=3, K=1), dict(level_cutoff=3, K=2),
+                                dict(level_cutoff=3, K=1, modified_HEOM=True)])
+@pytest.mark.parametrize('subspace', ['ee', 'gg,ge,eg,ee', 'eg', 'fe', 'ff'])
+@pytest.mark.parametrize('heisenberg', [False, True])
+def test_binned_tables_equal_packed_edges(kw, subspace, heisenberg):
+    """The GPU's binned neighbour tables (at most one edge per ADO, bin and
+    kind; weights from the occupations of the element's row and column
+    states, only for the sites they occupy) give the same coupling as the
+    packed edge lists that assemble to the reference's generator."""
+    _, ham = _ham('redfield_dimer', qa.CM_K * 300)
+    hs = 'gef' if 'f' in subspace else 'ge'
+    m = qa.HEOMModel(ham, rw_freq=12800.0, hilbert_subspace=hs, unit_convert=qa.CM_FS, **kw)
+    tb = m.tables(subspace, heisenberg)
+    A, B = _coupling_from_edges(tb), _coupling_from_bins(tb)
+    assert np.abs(A - B).max() <= 1e-14 * np.abs(A).max()
+    assert np.count_nonzero(A) == np.count_nonzero(B)
