@@ -241,6 +241,30 @@ int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t 
                            const double* h_host, int32_t nsave, void* out, void* y_final,
                            void* workspace, int32_t* status, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Fixed-step RK4 under classical fields (rotating wave approximation):
+ *     y' = L y + sum_i f_i(t) C_i y,   f_i(t) = -i E_i(t) (conjugated for a
+ * '+' interaction), C_i the commutator superoperator of field term i's
+ * dipole operator.  Replaces the zvode integration of the field-driven
+ * equation of motion in simulate_with_fields / simulate_pump
+ * (qspectra/simulate/eom.py:79-103, the callback f at :86-93).
+ *
+ *   A      L [B|1][M][M] (strideA 0 = shared); MIXED: off-diagonal complex64
+ *          plus diag, the complex128 diagonal [B|1][M] (strideD).
+ *   C      [B|1][P][M][M] (strideC), complex64 for MIXED; 1 <= P <= 4.
+ *   F      complex128 [B|1][nF][3][P] (strideF): f_i at the stage times
+ *          t_g, t_g + h/2, t_g + h of global step g (steps numbered across
+ *          the save intervals in order).
+ *   nsub, h, nsave, y0 [B][R][M], out [B][R][nsave][M] as qsx_propagate.
+ *   status 1: non-finite state; 2: the schedule has more steps than nF.
+ * Blocks up to M = 64 (one work-group per right-hand side). */
+int32_t qsx_propagate_driven(int32_t precision, const void* A, int64_t strideA,
+                             const void* diag, int64_t strideD, const void* C,
+                             int64_t strideC, int32_t P, const void* F, int64_t strideF,
+                             int64_t nF, int32_t M, const void* y0, int32_t R, int32_t B,
+                             const int32_t* nsub, const double* h, int32_t nsave, void* out,
+                             int32_t* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

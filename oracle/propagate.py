@@ -7,8 +7,12 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
   It is the "reference CPU path" of the benchmark.
 * ``integrate_rk4`` is the same driver with a fixed-step classical RK4 in
   place of zvode (the product's discretisation), vectorised over leading axes.
+* ``integrate_rk4_driven`` is classical RK4 for the field-driven equation
+  y' = L y + sum_i f_i(t) C_i y on the same schedule (stage times t,
+  t + h/2, t + h of every step), the discretisation of the driven kernel.
 * the response drivers restate qspectra/simulate/response.py and the
-  decorators of qspectra/simulate/decorators.py.
+  decorators of qspectra/simulate/decorators.py; simulate_with_fields and
+  simulate_pump restate qspectra/simulate/eom.py:79-227.
 """
 import numpy as np
 import scipy.integrate
@@ -98,6 +102,35 @@ def integrate_rk4(L, y0, t, t0=None, save_matrix=None, dt=None):
             y = rk4_step(L, y, hi)
         out.append(y if save_matrix is None else y @ np.asarray(save_matrix).T)
     return np.stack(out, axis=-2)
+
+
+def integrate_rk4_driven(L, fields, y0, t, t0=None, dt=None):
+    """RK4 on y' = L y + sum_i f_i(t) (C_i y), fields = [(f_i, C_i)] with
+    f_i a function of time; step k of interval s starts at t_{s-1} + k h_s
+    (t0 for s = 0).  Returns [len(t), M]."""
+    t = np.asarray(t, float)
+    if t0 is None:
+        t0 = t[0]
+    n, h = rk4_schedule(t, t0, dt)
+    starts = np.concatenate([[t0], t[:-1]])
+
+    def f(tt, v):
+        d = L.dot(v)
+        for fi, Ci in fields:
+            d = d + fi(tt) * Ci.dot(v)
+        return d
+    y = np.array(y0, dtype=complex)
+    out = []
+    for s, (ns, hs) in enumerate(zip(n, h)):
+        for k in range(ns):
+            tk = starts[s] + k * hs
+            k1 = f(tk, y)
+            k2 = f(tk + hs / 2, y + (hs / 2) * k1)
+            k3 = f(tk + hs / 2, y + (hs / 2) * k2)
+            k4 = f(tk + hs, y + hs * k3)
+            y = y + (hs / 6) * (k1 + 2 * k2 + 2 * k3 + k4)
+        out.append(y)
+    return np.array(out)
 
 
 # ------------------------------------------------------------ Fourier tools
@@ -199,6 +232,28 @@ THIRD_ORDER_PATHWAYS = {
 """response.py:252-264 (Abramavicius et al., Chem. Rev. 109, 2350)."""
 
 
+PUMP_PROBE_PATHWAYS = {'GSB': 'gg->eg->gg', 'ESE': 'ee->eg->gg', 'ESA': 'ee->fe->ee'}
+
+
+def impulsive_probe(model, state, time_max, polarization='xx',
+                    initial_liouv_subspace='gg,ge,eg,ee', include_signal='GSB,ESE,ESA', **kw):
+    """response.py:157-231: the second-order part of ``state`` (state minus
+    the thermal state), mapped to the first subspace of each pump-probe
+    pathway, as the initial state of a linear response; summed over the
+    pathways and Fourier transformed."""
+    initial_state = np.asarray(state) - model.thermal_state(initial_liouv_subspace)
+    total = 0
+    t = None
+    for path in parse_pathways(PUMP_PROBE_PATHWAYS, include_signal):
+        sub0 = path.split('->')[0]
+        portion = np.apply_along_axis(
+            lambda v: model.map_between_subspaces(v, initial_liouv_subspace, sub0), -1,
+            initial_state)
+        t, sig = linear_response(model, path, time_max, polarization, portion, **kw)
+        total = total + sig
+    return fourier_transform(t, total, rw_freq=model.rw_freq, unit_convert=model.unit_convert)
+
+
 def parse_pathways(possible, include_signal):
     """response.py:162-171 (dict order of the table)."""
     sel = [v for k, v in possible.items() if include_signal is None or k in include_signal]
@@ -260,6 +315,56 @@ def simulate_dynamics(model, initial_state, duration=None, times=None,
     states = run(L, y0, t)
     N = int(np.sqrt(states.shape[-1]))
     return t, states.reshape(-1, N, N, order='F')
+
+
+def simulate_with_fields(model, pulses, geometry='-+', polarization='xx', time_extra=0,
+                         times=None, liouville_subspace='gg,ge,eg,ee', save_func=None,
+                         integrator='rk4', dt=None, **zvode_kw):
+    """simulate/eom.py:79-103: thermal state at t0 = min t_init, derivative
+    L y + sum (-1j E (conj for '+'))[V, y], outputs on arange(t0, tf +
+    time_extra, time_step) or tf + times; states (or save_func of them)."""
+    L = model.evolve_matrix(liouville_subspace)
+    V = [model.dipole_operator(liouville_subspace, p, tr)
+         for p, tr in zip(polarization, geometry)]
+    info = list(zip(pulses, geometry, V))
+
+    def coef(pulse, trans):
+        def fi(tt):
+            E = pulse(tt, model.rw_freq)
+            return -1j * (np.conj(E) if trans == '+' else E)
+        return fi
+    fields = [(coef(p, tr), Vi.commutator_matrix()) for p, tr, Vi in info]
+    rho0 = model.thermal_state(liouville_subspace)
+    t0 = min(p.t_init for p in pulses)
+    tf = max(p.t_final for p in pulses)
+    t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
+         else tf + np.asarray(times))
+    if integrator == 'rk4':
+        states = integrate_rk4_driven(L, fields, rho0, t, t0, dt)
+        if save_func is not None:
+            states = np.array([save_func(v) for v in states])
+        return t, states
+
+    def rhs(tt, y):
+        d = L.dot(y)
+        for fi, Ci in fields:
+            d = d + fi(tt) * Ci.dot(y)
+        return d
+    return t, integrate_zvode(rhs, rho0, t, t0, save_func, **zvode_kw)
+
+
+def simulate_pump(model, pump, polarization='x', time_extra=0, times=None,
+                  liouville_subspace='gg,ge,eg,ee', save_func=None,
+                  exact_isotropic_average=False, **kw):
+    """simulate/eom.py:162-227: [pump, pump], geometry '-+', the same
+    polarization twice; optional 2nd-order isotropic average."""
+    def run(polarization):
+        return simulate_with_fields(model, [pump, pump], '-+', polarization, time_extra,
+                                    times, liouville_subspace, save_func, **kw)
+    pols = [polarization, polarization]
+    if exact_isotropic_average:
+        return isotropic_average_2nd(run, pols)
+    return run(pols)
 
 
 # ------------------------------------------------------------------ decorators

@@ -318,3 +318,33 @@ class ElectronicHamiltonian(object):
         """hamiltonian.py:163-175 on the non-rotating H."""
         T = getattr(self.bath, 'temperature', 0) if self.bath is not None else 0
         return thermal_state(self.H(subspace, rotating=False), T)
+
+
+# --------------------------------------------------------------------- pulses
+class CustomPulse(object):
+    """pulse.py:51-71: field given by call(t, rw_freq)."""
+
+    def __init__(self, t_init, t_final, call):
+        self.t_init, self.t_final, self.call = t_init, t_final, call
+
+    def __call__(self, t, rw_freq):
+        return self.call(t, rw_freq)
+
+
+class GaussianPulse(object):
+    """pulse.py:74-114: scale * exp(i c (w_c - rw) (t - t_p) - (t - t_p)^2 / (2 s^2)),
+    s = GAUSSIAN_SD_FWHM * fwhm, limits t_p -/+ t_limits_multiple * s."""
+
+    def __init__(self, carrier_freq, fwhm, t_peak=0, scale=1, freq_convert=1,
+                 t_limits_multiple=3):
+        sigma = GAUSSIAN_SD_FWHM * fwhm
+        self.two_sigma_squared = 2 * sigma ** 2
+        self.t_init = t_peak - t_limits_multiple * sigma
+        self.t_final = t_peak + t_limits_multiple * sigma
+        self.t_peak, self.carrier_freq = t_peak, carrier_freq
+        self.scale, self.freq_convert = scale, freq_convert
+
+    def __call__(self, t, rw_freq):
+        return self.scale * np.exp(1j * self.freq_convert * (self.carrier_freq - rw_freq)
+                                   * (t - self.t_peak)
+                                   - (t - self.t_peak) ** 2 / self.two_sigma_squared)

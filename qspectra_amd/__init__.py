@@ -22,9 +22,10 @@ from .operator_tools import (unit_vec, basis_transform_operator, basis_transform
 from .polarization import (polarization_vector, check_polarizations,
                            invariant_weights_4th_order, invariant_polarizations,
                            FOURTH_ORDER_INVARIANTS, MAGIC_ANGLE)
-from .simulate.eom import simulate_dynamics
+from .pulse import Pulse, CustomPulse, GaussianPulse
+from .simulate.eom import simulate_dynamics, simulate_with_fields, simulate_pump
 from .simulate.response import (linear_response, absorption_spectra, third_order_response,
-                                two_dimensional_spectra, PUMP_PROBE_PATHWAYS,
+                                two_dimensional_spectra, impulsive_probe, PUMP_PROBE_PATHWAYS,
                                 THIRD_ORDER_PATHWAYS)
 from .simulate.utils import fourier_transform, integrate, bound_signal, IntegratorError
 
@@ -34,7 +35,9 @@ __all__ = ['DebyeBath', 'ArbitraryBath', 'UncoupledBath', 'CM_FS', 'CM_K', 'GAUS
            'basis_transform_vector', 'all_states', 'polarization_vector',
            'check_polarizations', 'invariant_weights_4th_order', 'invariant_polarizations',
            'FOURTH_ORDER_INVARIANTS', 'MAGIC_ANGLE', 'RedfieldModel', 'UnitaryModel', 'HEOMModel',
-           'simulate_dynamics', 'linear_response', 'absorption_spectra',
-           'third_order_response', 'two_dimensional_spectra', 'PUMP_PROBE_PATHWAYS',
+           'simulate_dynamics', 'simulate_with_fields', 'simulate_pump', 'Pulse', 'CustomPulse',
+           'GaussianPulse', 'linear_response', 'absorption_spectra',
+           'third_order_response', 'two_dimensional_spectra', 'impulsive_probe',
+           'PUMP_PROBE_PATHWAYS',
            'THIRD_ORDER_PATHWAYS', 'fourier_transform', 'integrate', 'bound_signal',
            'IntegratorError', 'NativeUnavailable', 'NativeError', 'require_native']

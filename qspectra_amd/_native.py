@@ -64,6 +64,9 @@ SIGNATURES = {
     'qsx_heom_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32,
                            c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp,
                            c_vp, c_vp, c_vp, c_vp],
+    'qsx_propagate_driven': [c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64,
+                             c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp,
+                             c_vp],
 }
 
 
@@ -278,6 +281,58 @@ def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_f
         st = go()
     check(st, 'qsx_propagate')
     return out, yfin, status
+
+
+MAX_DRIVEN_BLOCK = 64
+MAX_FIELD_TERMS = 4
+
+
+def propagate_driven(A, C, F, y0, nsub, h, precision=QSX_FP64, diag=None, stream=None,
+                     status=None):
+    """qsx_propagate_driven: RK4 of y' = A y + sum_i f_i(t) C_i y with the
+    field values F [B|1, nF, 3, P] (complex128, at the three stage times of
+    every step); C [B|1, P, M, M]; returns (out [B, R, nsave, M], status)."""
+    lib = require_native()
+    B, R, M = y0.shape
+    nsave = int(nsub.shape[0])
+    adt = torch.complex128 if precision == QSX_FP64 else torch.complex64
+    P = int(C.shape[1])
+    if A.dtype != adt or C.dtype != adt or F.dtype != torch.complex128:
+        raise TypeError('propagate_driven: operand dtypes do not match the precision')
+    if (A.shape[-2:] != (M, M) or A.shape[0] not in (1, B) or tuple(C.shape[2:]) != (M, M)
+            or C.shape[0] not in (1, B) or F.dim() != 4 or F.shape[0] not in (1, B)
+            or F.shape[2] != 3 or F.shape[3] != P):
+        raise ValueError('propagate_driven: shapes A %s, C %s, F %s do not fit states %s'
+                         % (tuple(A.shape), tuple(C.shape), tuple(F.shape), tuple(y0.shape)))
+    if not 1 <= P <= MAX_FIELD_TERMS:
+        raise ValueError('propagate_driven: %d field terms (1..%d)' % (P, MAX_FIELD_TERMS))
+    if M > MAX_DRIVEN_BLOCK:
+        raise ValueError('propagate_driven: block of %d states exceeds %d' % (M, MAX_DRIVEN_BLOCK))
+    assert nsub.dtype == torch.int32 and h.dtype == torch.float64
+    A, C, F, y0 = A.contiguous(), C.contiguous(), F.contiguous(), y0.contiguous()
+    nF = int(F.shape[1])
+    strideA = 0 if A.shape[0] == 1 else M * M
+    strideC = 0 if C.shape[0] == 1 else P * M * M
+    strideF = 0 if F.shape[0] == 1 else nF * 3 * P
+    strideD = 0
+    if precision == QSX_MIXED:
+        assert diag is not None and diag.dtype == torch.complex128
+        diag = diag.contiguous()
+        strideD = 0 if diag.shape[0] == 1 else M
+    out = torch.empty((B, R, nsave, M), dtype=torch.complex128, device=y0.device)
+    if status is None:
+        status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    label = 'driven %s M=%d P=%d R=%d B=%d steps=%d nsave=%d' % (
+        'fp64' if precision == QSX_FP64 else 'mixed', M, P, R, B, nF, nsave)
+    def go():
+        return lib.qsx_propagate_driven(precision, ptr(A), strideA, ptr(diag), strideD, ptr(C),
+                                        strideC, P, ptr(F), strideF, nF, M, ptr(y0), R, B,
+                                        ptr(nsub), ptr(h), nsave, ptr(out), ptr(status),
+                                        stream_ptr(stream))
+    with timing.launch(label, stream, go):
+        st = go()
+    check(st, 'qsx_propagate_driven')
+    return out, status
 
 
 def propagate_adjoint(A, y0, nsub, total_steps, gout, ckpt=64, stream=None):

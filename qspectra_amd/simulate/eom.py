@@ -1,10 +1,15 @@
-"""Field-free propagation of density matrices (qspectra/simulate/eom.py,
-simulate_dynamics)."""
+"""Propagation of density matrices without and with classical fields
+(qspectra/simulate/eom.py: simulate_dynamics, simulate_with_fields,
+simulate_pump)."""
 import numpy as np
+import torch
 
 from .. import _native
-from .decorators import optional_ensemble_average
-from .utils import propagate, propagate_heom
+from ..engine import rk4_schedule, schedule_tensors
+from ..lti import split_diagonal
+from ..pulse import sample_field, field_rate
+from .decorators import optional_ensemble_average, optional_2nd_order_isotropic_average
+from .utils import propagate, propagate_heom, IntegratorError, ZVODE_OPTIONS
 from ..dynamics.liouville_space import LinearMap
 from ..dynamics.heom import DeviceHEOM
 
@@ -55,3 +60,162 @@ def simulate_dynamics(dynamical_model, initial_state, duration=None, times=None,
                               liouville_subspace, save_func, ensemble_size=ensemble_size,
                               ensemble_random_orientations=ensemble_random_orientations,
                               **integrate_kwargs)
+
+
+# ------------------------------------------------------------ with fields
+def stage_times(t, t0, n, h):
+    """Start time of every RK4 step of the schedule (n, h) from t0 to the
+    output times t: interval s starts at t[s-1] (t0 for the first) and its
+    k-th step at that start + k h[s]."""
+    starts = np.concatenate([[float(t0)], np.asarray(t, dtype=float)[:-1]])
+    n = np.asarray(n, dtype=np.int64)
+    first = np.concatenate([[0], np.cumsum(n)[:-1]])
+    k = np.arange(int(n.sum())) - np.repeat(first, n)
+    return np.repeat(starts, n) + k * np.repeat(h, n), np.repeat(h, n)
+
+
+def field_samples(terms, rw_freq, tg, hg):
+    """F [nF, 3, P] complex128: f_i = -i E_i (E_i conjugated for a '+'
+    interaction) at t_g, t_g + h/2 and t_g + h of every step g."""
+    grid = np.stack([tg, tg + hg / 2, tg + hg], axis=1)          # [nF, 3]
+    F = np.empty(grid.shape + (len(terms),), dtype=complex)
+    for i, (pulse, trans) in enumerate(terms):
+        E = sample_field(pulse, grid, rw_freq)
+        F[..., i] = -1j * (np.conj(E) if trans == '+' else E)
+    return F
+
+
+def _field_step(L, C, terms, rw_freq, span, rtol):
+    """RK4 step for the driven equation with estimated global error <= rtol
+    over ``span``: the rate bound is the largest absolute row sum of L plus,
+    for each field term, its peak amplitude times that of C_i, plus the
+    fastest relative rate of change of the fields."""
+    rho = float(torch.max(torch.sum(torch.abs(L), dim=-1)).item())
+    rate = 0.0
+    for i, (pulse, _) in enumerate(terms):
+        amp, r = field_rate(pulse, rw_freq)
+        rho += amp * float(np.abs(C[:, i]).sum(axis=-1).max())
+        rate = max(rate, r)
+    rho += rate
+    if rho == 0 or span <= 0:
+        return None
+    return (120.0 * rtol / (rho * span)) ** 0.25 / rho
+
+
+def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_extra, times,
+                          liouville_subspace, save_func, **integrate_kwargs):
+    """(t, states [B, len(t), ...]) for a (possibly batched) model; the
+    terms pair pulses, geometry and polarization in order (as zip does)."""
+    from .response import _members, _member_model, _op_matrices
+    model = dynamical_model
+    dt = integrate_kwargs.pop('dt', None)
+    precision = integrate_kwargs.pop('precision', 'fp64')
+    rtol = integrate_kwargs.pop('rtol', 1e-6)
+    unknown = set(integrate_kwargs) - set(ZVODE_OPTIONS)
+    if unknown:
+        raise TypeError('unexpected integrate options: %s' % sorted(unknown))
+    eom = model.equation_of_motion(liouville_subspace)
+    if isinstance(eom, DeviceHEOM):
+        raise NotImplementedError('field-driven propagation of hierarchy (HEOM) models is not '
+                                  'implemented; use a Redfield or unitary model')
+    pairs = list(zip(polarization, geometry))
+    terms = list(zip(pulses, geometry))[:len(pairs)]
+    pairs = pairs[:len(terms)]
+    if not terms:
+        raise ValueError('at least one pulse is needed')
+    if len(terms) > _native.MAX_FIELD_TERMS:
+        raise ValueError('at most %d field terms are supported' % _native.MAX_FIELD_TERMS)
+    M = eom.size
+    if M > _native.MAX_DRIVEN_BLOCK:
+        raise ValueError('field-driven propagation supports Liouville subspaces of at most %d '
+                         'states (%r has %d)' % (_native.MAX_DRIVEN_BLOCK, liouville_subspace, M))
+    hams = _members(model)
+    B = len(hams)
+    mats = [_op_matrices(model, liouville_subspace, pol, trans, 'commutator')
+            for pol, trans in pairs]                                   # each [B|1, M, M]
+    Bc = max(m.shape[0] for m in mats)
+    C = np.stack([np.broadcast_to(m, (Bc, M, M)) for m in mats], axis=1)   # [B|1, P, M, M]
+
+    if B == 1:
+        rho0 = np.asarray(model.thermal_state(liouville_subspace))[None]
+    else:
+        rho0 = np.stack([_member_model(model, h).thermal_state(liouville_subspace)
+                         for h in hams])
+    t0 = min(p.t_init for p in pulses)
+    tf = max(p.t_final for p in pulses)
+    t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
+         else tf + np.asarray(times, dtype=float))
+    t = np.atleast_1d(np.asarray(t, dtype=float))
+    if np.any(t < t0):
+        raise ValueError('output times must not precede the first pulse (t=%g)' % t0)
+    rw_freq = model.rw_freq
+    if dt is None:
+        dt = _field_step(eom.L, C, terms, rw_freq, float(t[-1]) - float(t0), rtol)
+        gaps = np.diff(np.concatenate([[t0], t]))
+        gaps = gaps[gaps > 0]
+        if gaps.size:
+            dt = gaps.min() if dt is None else min(dt, float(gaps.min()))
+    n, h = rk4_schedule(t, t0, dt)
+    tg, hg = stage_times(t, t0, n, h)
+    F = field_samples(terms, rw_freq, tg, hg)
+    if F.shape[0] == 0:
+        F = np.zeros((1, 3, len(terms)), dtype=complex)                # nothing to step
+    dev = eom.L.device
+    nsub, hs = schedule_tensors(t, t0, dt, dev)
+    y0 = _native.to_device(rho0[:, None, :], dev=dev)                  # [B, 1, M]
+    Ft = _native.to_device(F[None], dev=dev)
+    if precision in ('fp64', 'double', None):
+        out, status = _native.propagate_driven(eom.L, _native.to_device(C, dev=dev), Ft, y0,
+                                               nsub, hs, _native.QSX_FP64)
+    elif precision == 'mixed':
+        Lm, diag = split_diagonal(eom.L)
+        Cm = _native.to_device(C, dtype=torch.complex64, dev=dev)
+        out, status = _native.propagate_driven(Lm, Cm, Ft, y0, nsub, hs, _native.QSX_MIXED,
+                                               diag=diag)
+    else:
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    if int(status.item()) != 0:
+        raise IntegratorError('integration failed: non-finite state between t=%g and t=%g'
+                              % (t0, t[-1]))
+    states = out[:, 0]                                                 # [B, nt, M]
+    if isinstance(save_func, LinearMap):
+        mat = np.asarray(save_func.matrix)
+        proj = _native.to_device(np.atleast_2d(mat), dev=dev)
+        saved = torch.matmul(states, proj.transpose(0, 1).contiguous()).cpu().numpy()
+        return t, saved[..., 0] if mat.ndim == 1 else saved
+    states = states.cpu().numpy()
+    if save_func is not None:
+        return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in states])
+    return t, states
+
+
+def simulate_with_fields(dynamical_model, pulses, geometry='-+', polarization='xx',
+                         time_extra=0, times=None, liouville_subspace='gg,ge,eg,ee',
+                         save_func=None, ensemble_size=None, ensemble_random_orientations=False,
+                         **integrate_kwargs):
+    """Time evolution under classical pulses in the rotating wave
+    approximation (qspectra/simulate/eom.py:105-159): from the thermal
+    state at the start of the first pulse, y' = L y + sum_i f_i(t) [V_i, y]
+    with f_i = -i E_i (conjugated for '+').  Output times: every model time
+    step from the first pulse's start to the last pulse's end + time_extra,
+    or the last pulse's end + ``times``.  Returns (t, states) -- state
+    vectors, or save_func of them.  Keyword options: dt (RK4 step), rtol
+    (accuracy of the default step), precision ('fp64' | 'mixed')."""
+    return optional_ensemble_average(_simulate_with_fields)(
+        dynamical_model, pulses, geometry, polarization, time_extra, times, liouville_subspace,
+        save_func, ensemble_size=ensemble_size,
+        ensemble_random_orientations=ensemble_random_orientations, **integrate_kwargs)
+
+
+def simulate_pump(dynamical_model, pump, polarization='x', time_extra=0, times=None,
+                  liouville_subspace='gg,ge,eg,ee', save_func=None, ensemble_size=None,
+                  ensemble_random_orientations=False, exact_isotropic_average=False,
+                  **integrate_kwargs):
+    """Time evolution under one pump pulse acting twice ('-+', the same
+    polarization), qspectra/simulate/eom.py:162-227; exact_isotropic_average
+    averages over orientations to second order in the field."""
+    return optional_ensemble_average(optional_2nd_order_isotropic_average(_simulate_with_fields))(
+        dynamical_model, [pump, pump], '-+', [polarization, polarization], time_extra, times,
+        liouville_subspace, save_func, ensemble_size=ensemble_size,
+        ensemble_random_orientations=ensemble_random_orientations,
+        exact_isotropic_average=exact_isotropic_average, **integrate_kwargs)

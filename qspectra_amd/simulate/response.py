@@ -155,6 +155,34 @@ def absorption_spectra(dynamical_model, time_max, correlation_decay_time=None,
 
 PUMP_PROBE_PATHWAYS = {'GSB': 'gg->eg->gg', 'ESE': 'ee->eg->gg', 'ESA': 'ee->fe->ee'}
 
+def impulsive_probe(dynamical_model, state, time_max, polarization='xx',
+                    initial_liouv_subspace='gg,ge,eg,ee', include_signal='GSB,ESE,ESA',
+                    ensemble_size=None, ensemble_random_orientations=False,
+                    exact_isotropic_average=False, **integrate_kwargs):
+    """Probe the second-order part of ``state`` (one or several state
+    vectors in ``initial_liouv_subspace``) with an impulsive probe pulse:
+    for each selected pump-probe pathway (GSB, ESE, ESA) the part of
+    state - thermal state in its first subspace is the initial state of a
+    linear response; the summed signal is Fourier transformed.  Returns
+    (f, signal).  Several states are right-hand sides of the same launches."""
+    model = dynamical_model
+    initial_state = np.asarray(state) - model.thermal_state(initial_liouv_subspace)
+    total = ZeroArray()
+    t = None
+    for path in _parse_pathways(PUMP_PROBE_PATHWAYS, include_signal):
+        sub0 = path.split('->')[0]
+        portion = np.apply_along_axis(
+            lambda v: model.map_between_subspaces(v, initial_liouv_subspace, sub0), -1,
+            initial_state)
+        t, signal = linear_response(model, path, time_max, portion, polarization,
+                                    ensemble_size=ensemble_size,
+                                    ensemble_random_orientations=ensemble_random_orientations,
+                                    exact_isotropic_average=exact_isotropic_average,
+                                    **integrate_kwargs)
+        total += signal
+    return fourier_transform(t, total, rw_freq=model.rw_freq, unit_convert=model.unit_convert)
+
+
 # Abramavicius, Palmieri, Voronine, Sanda & Mukamel, Chem. Rev. 109, 2350
 # (2009), Figs. 4-6 (the pathway table of response.py:252-264).
 THIRD_ORDER_PATHWAYS = {

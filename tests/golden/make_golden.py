@@ -2,7 +2,8 @@
 
 Run only in the build container, where the reference sources are present:
 
-    python tests/golden/make_golden.py /root/reference
+    python tests/golden/make_golden.py /root/reference            # every file
+    python tests/golden/make_golden.py /root/reference fields     # fields.npz only
 
 It imports qspectra from the given path and stores plain arrays (inputs and
 outputs) as .npz files.  Nothing here is imported by the tests or by the
@@ -13,6 +14,9 @@ known-answer test pins are written:
 * HEOM sparse Liouvillians (ADO order, low-temperature correction),
 * zvode-integrated trajectories and end-to-end spectra (absorption,
   third-order response, 2-D spectra, isotropic and ensemble averages),
+* field-driven trajectories (simulate_with_fields, simulate_pump: Gaussian
+  pulses, complex amplitudes, three field terms, unitary / 7-site /
+  ensemble models) and the impulsive probe of a pumped state,
 * disorder samples and the complex bath correlation function.
 """
 import os
@@ -52,7 +56,55 @@ def extra_for_dt(qs, H1, dt):
     return 1.0 / (2 * dt * qs.CM_FS) - max(E.max(), -E.min())
 
 
-def main(ref_path):
+def write_fields(qs, ham, ham7):
+    """fields.npz: simulate_with_fields / simulate_pump / impulsive_probe."""
+    out = {}
+    md = qs.RedfieldModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS)
+    pump = qs.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=qs.CM_FS)
+    t, y = qs.simulate_pump(md, pump, 'x', time_extra=40., **TIGHT)
+    out['pump_t'], out['pump_y'] = t, y
+    t, y = qs.simulate_pump(md, pump, [1., 0.5, 0.], times=np.arange(0., 60., 3.),
+                            exact_isotropic_average=True, **TIGHT)
+    out['iso_t'], out['iso_y'] = t, y
+    p1 = qs.GaussianPulse(12700., 15., t_peak=-10., scale=0.015, freq_convert=qs.CM_FS)
+    p2 = qs.GaussianPulse(12900., 25., t_peak=5., scale=0.01 * np.exp(0.6j),
+                          freq_convert=qs.CM_FS, t_limits_multiple=2.5)
+    t, y = qs.simulate_with_fields(md, [p1, p2], '-+', 'xy', time_extra=20., **TIGHT)
+    out['two_t'], out['two_y'] = t, y
+    t, y = qs.simulate_with_fields(md, [p1, p2, p1], '-+-', ['x', 'y', 0.7], times=[0., 5., 12.5],
+                                   **TIGHT)
+    out['three_t'], out['three_y'] = t, y
+    mu = qs.UnitaryModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS)
+    t, y = qs.simulate_pump(mu, pump, 'y', time_extra=30., **TIGHT)
+    out['unitary_t'], out['unitary_y'] = t, y
+    m7 = qs.RedfieldModel(ham7, hilbert_subspace='gef', unit_convert=qs.CM_FS, secular=False)
+    pump7 = qs.GaussianPulse(12450., 20., t_peak=0., scale=0.01, freq_convert=qs.CM_FS)
+    t, y = qs.simulate_pump(m7, pump7, 'x', time_extra=20., **TIGHT)
+    out['agg7_t'], out['agg7_y'] = t, y
+    t, y = qs.simulate_pump(m7, pump7, 'z', times=np.array([0., 10., 20.]), ensemble_size=2,
+                            **TIGHT)
+    out['agg7_ens2_t'], out['agg7_ens2_y'] = t, y
+    states = out['pump_y'][[-1, -10, -20]]
+    f, X = qs.impulsive_probe(md, states, 300.5, **TIGHT)
+    out['probe_states'], out['probe_f'], out['probe_X'] = states, f, X
+    f, X = qs.impulsive_probe(md, states[0], 300.5, polarization='xy', include_signal='ESE,ESA',
+                              **TIGHT)
+    out['probe1_f'], out['probe1_X'] = f, X
+    np.savez_compressed(os.path.join(HERE, 'fields.npz'), **out)
+
+
+def write_manifest(ref_path):
+    with open(os.path.join(HERE, 'MANIFEST.txt'), 'w') as fh:
+        fh.write('written by tests/golden/make_golden.py from the reference at %s\n' % ref_path)
+        fh.write('numpy %s scipy %s python %s\n' % (np.__version__, __import__('scipy').__version__,
+                                                  sys.version.split()[0]))
+        fh.write('tight zvode: %r\n' % (TIGHT,))
+        for fn in sorted(os.listdir(HERE)):
+            if fn.endswith('.npz'):
+                fh.write('%s %d bytes\n' % (fn, os.path.getsize(os.path.join(HERE, fn))))
+
+
+def main(ref_path, only=None):
     sys.path.insert(0, ref_path)
     import qspectra as qs
     from qspectra.dynamics.redfield import redfield_tensor
@@ -65,6 +117,14 @@ def main(ref_path):
             inp['H1'], bath=qs.DebyeBath(inp['T'], inp['lam'], inp['gamma']),
             dipoles=inp['dipoles'], disorder=disorder, random_seed=seed,
             energy_spread_extra=extra)
+
+    if only == 'fields':
+        dim, agg = dimer_inputs(), aggregate7_inputs()
+        ham = ham_of(dim, extra_for_dt(qs, dim['H1'], 1.0))
+        ham7 = ham_of(agg, extra_for_dt(qs, agg['H1'], 2.0), disorder=100 / qs.GAUSSIAN_SD_FWHM)
+        write_fields(qs, ham, ham7)
+        write_manifest(ref_path)
+        return
 
     # ---------------------------------------------------------------- bath
     b = qs.DebyeBath(0.69503476 * 77, 35., 106.)
@@ -209,15 +269,10 @@ def main(ref_path):
     out['unitary_t'], out['unitary_rho'] = t, rho
     np.savez_compressed(os.path.join(HERE, 'dynamics.npz'), **out)
 
-    with open(os.path.join(HERE, 'MANIFEST.txt'), 'w') as fh:
-        fh.write('written by tests/golden/make_golden.py from the reference at %s\n' % ref_path)
-        fh.write('numpy %s scipy %s python %s\n' % (np.__version__, __import__('scipy').__version__,
-                                                  sys.version.split()[0]))
-        fh.write('tight zvode: %r\n' % (TIGHT,))
-        for fn in sorted(os.listdir(HERE)):
-            if fn.endswith('.npz'):
-                fh.write('%s %d bytes\n' % (fn, os.path.getsize(os.path.join(HERE, fn))))
+    write_fields(qs, ham, ham7)
+    write_manifest(ref_path)
 
 
 if __name__ == '__main__':
-    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')
+    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference',
+         sys.argv[2] if len(sys.argv) > 2 else None)

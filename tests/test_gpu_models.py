@@ -294,3 +294,24 @@ def test_trajectory_schedules_match_oracle(gpu, kind, chunk, precision):
     assert rel_err(out, ref) < tol, (kind, chunk, precision)
     tuned = Stage(eom, t, None, 2.0, precision, 'auto', 1)
     assert rel_err(tuned.run(y0, [])[0, 0].cpu().numpy(), ref) < tol, tuned.kind
+
+
+def test_impulsive_probe_matches_oracle(gpu):
+    """impulsive_probe (response.py:157-231) of several probe-time states of
+    the 7-site aggregate, as right-hand sides of one set of launches."""
+    _, inp = _inputs('redfield_7site')
+    m7 = qa.RedfieldModel(_product_ham(inp), unit_convert=qa.CM_FS, secular=False)
+    om = Lv.LiouvilleModel(_oracle_ham(inp), unit_convert=S.CM_FS, secular=False)
+    sub = 'gg,ge,eg,ee'
+    rs = np.random.RandomState(8)
+    n = len(om.liouville_subspace_index(sub))
+    states = om.thermal_state(sub) + 0.05 * (rs.randn(3, n) + 1j * rs.randn(3, n))
+    f, X = qa.impulsive_probe(m7, states, 2 * 63.5, dt=2.0)
+    fo, Xo = P.impulsive_probe(om, states, 2 * 63.5, dt=2.0)
+    assert np.allclose(f, fo) and X.shape == Xo.shape == (3, len(fo))
+    assert rel_err(X, Xo) < 1e-10
+    _, Xm = qa.impulsive_probe(m7, states, 2 * 63.5, dt=2.0, precision='mixed')
+    assert rel_err(Xm, Xo) < 1e-6
+    _, Xg = qa.impulsive_probe(m7, states[0], 2 * 63.5, dt=2.0, include_signal='GSB')
+    _, Xog = P.impulsive_probe(om, states[0], 2 * 63.5, dt=2.0, include_signal='GSB')
+    assert rel_err(Xg, Xog) < 1e-10

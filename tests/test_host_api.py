@@ -187,3 +187,43 @@ def test_rk4_schedule_matches_oracle():
         assert_allclose(h1, h2, rtol=1e-15)
     n, h = engine.rk4_schedule([0., 100., 200., 500.], 0.0, 2.0)
     assert_equal(n, [0, 50, 50, 150])
+
+
+def test_pulses_and_field_samples():
+    """qspectra tests/test_pulse.py:9-38 against qspectra_amd.pulse, and the
+    host side of field-driven propagation: stage times follow the RK4
+    schedule (the oracle's stepping), field samples are -i E (conj for '+')."""
+    from qspectra_amd import pulse
+    from qspectra_amd.simulate import eom
+    with pytest.raises(TypeError, match="Can't instantiate abstract class"):
+        pulse.Pulse()
+    pump = pulse.CustomPulse(0, 1, lambda x, r: x + r)
+    assert (pump.t_init, pump.t_final, pump(1, 2)) == (0, 1, 3)
+    pump = pulse.GaussianPulse(carrier_freq=10, fwhm=1, t_peak=5, scale=100, freq_convert=1,
+                               t_limits_multiple=2)
+    assert_allclose(pump(np.array([5, 5.5]), 10), [100, 50])
+    assert_allclose(pump(np.array([5, 5.5]), 5), [100, 50 * np.exp(2.5j)])
+    assert pump.t_init == 5 - 2 * S.GAUSSIAN_SD_FWHM and pump.t_final == 5 + 2 * S.GAUSSIAN_SD_FWHM
+    pump = pulse.GaussianPulse(carrier_freq=10, fwhm=2, t_peak=0, scale=100, freq_convert=2)
+    assert_allclose(pump(np.array([0, 1]), 10.0), [100, 50])
+    assert_allclose(pump(1, 0.0), 50 * np.exp(20.0j))
+    # a scalar-only pulse function is sampled one time at a time
+    scalar = pulse.CustomPulse(0, 2, lambda t, rw: complex(np.cos(float(t))) * (1 + rw))
+    ts = np.linspace(0, 2, 7)
+    assert_allclose(pulse.sample_field(scalar, ts, 1.0), 2 * np.cos(ts))
+    amp, rate = pulse.field_rate(pulse.GaussianPulse(0., 1., scale=3.), 0.)
+    sig = S.GAUSSIAN_SD_FWHM
+    assert abs(amp - 3) < 1e-12 and abs(rate - np.exp(-0.5) / sig) < 1e-3 / sig
+    # stage times: t0 = -1, outputs at -1 (no step), 0.5 (3 steps), 1.0 (1 step)
+    t = np.array([-1.0, 0.5, 1.0])
+    n, h = engine.rk4_schedule(t, -1.0, 0.5)
+    tg, hg = eom.stage_times(t, -1.0, n, h)
+    assert_allclose(tg, [-1.0, -0.5, 0.0, 0.5])
+    assert_allclose(hg, [0.5, 0.5, 0.5, 0.5])
+    no, ho = P.rk4_schedule(t, -1.0, 0.5)
+    assert_equal(n, no) and assert_allclose(h, ho)
+    g1 = pulse.GaussianPulse(3., 2., scale=0.5 + 0.2j)
+    F = eom.field_samples([(g1, '-'), (g1, '+')], 1.0, tg, hg)
+    for k, off in enumerate([0.0, 0.25, 0.5]):
+        assert_allclose(F[:, k, 0], -1j * g1(tg + off, 1.0))
+        assert_allclose(F[:, k, 1], -1j * np.conj(g1(tg + off, 1.0)))

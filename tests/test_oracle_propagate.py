@@ -129,3 +129,148 @@ def test_third_order_matches_reference():
     assert_allclose(f1, g['S2d_f1'])
     assert_allclose(f3, g['S2d_f3'])
     assert rel_err(X, g['S2d']) < 1e-6
+
+
+def test_impulsive_probe_closed_relations():
+    """impulsive_probe (response.py:157-231): zero for the thermal state;
+    linear in the state; a ground-population change c alone gives c times the
+    'gg->eg->gg' linear response (GSB pathway); a single-exciton population
+    enters through ESE and ESA only."""
+    g = golden('redfield_dimer')
+    ham = S.ElectronicHamiltonian(g['in_H1'], bath=S.DebyeBath(S.CM_K * 77, 35, 106),
+                                  dipoles=g['in_dipoles'], energy_spread_extra=float(g['extra']))
+    m = Lv.LiouvilleModel(ham, hilbert_subspace='gef', unit_convert=S.CM_FS)
+    sub = 'gg,ge,eg,ee'
+    rho0 = m.thermal_state(sub)
+    f, X0 = P.impulsive_probe(m, rho0, 64.0)
+    assert np.abs(X0).max() == 0
+    gg = m.liouville_subspace_index('gg')
+    full = m.liouville_subspace_index(sub)
+    d_g = np.zeros(len(full), complex)
+    d_g[np.flatnonzero(np.isin(full, gg))] = 0.3
+    _, Xg = P.impulsive_probe(m, rho0 + d_g, 64.0)
+    t, S_gsb = P.linear_response(m, 'gg->eg->gg', 64.0, 'xx', np.array([0.3]))
+    _, ref = P.fourier_transform(t, S_gsb, rw_freq=m.rw_freq, unit_convert=m.unit_convert)
+    assert rel_err(Xg, ref) < 1e-14
+    rs = np.random.RandomState(4)
+    a = rs.randn(len(full)) + 1j * rs.randn(len(full))
+    b = rs.randn(len(full)) + 1j * rs.randn(len(full))
+    _, Xa = P.impulsive_probe(m, rho0 + a, 64.0)
+    _, Xb = P.impulsive_probe(m, rho0 + b, 64.0)
+    _, Xab = P.impulsive_probe(m, rho0 + 2 * a - b, 64.0)
+    assert rel_err(Xab, 2 * Xa - Xb) < 1e-12
+    _, Xs = P.impulsive_probe(m, np.stack([rho0 + a, rho0 + b]), 64.0)
+    assert rel_err(Xs[0], Xa) < 1e-13 and rel_err(Xs[1], Xb) < 1e-13
+    ee = m.liouville_subspace_index('ee')
+    d_e = np.zeros(len(full), complex)
+    d_e[np.flatnonzero(np.isin(full, ee))[0]] = 0.5
+    _, Xe = P.impulsive_probe(m, rho0 + d_e, 64.0)
+    _, Xe_gsb = P.impulsive_probe(m, rho0 + d_e, 64.0, include_signal='GSB')
+    assert np.abs(Xe_gsb).max() == 0 and np.abs(Xe).max() > 0
+
+
+def test_pulses_known_answers():
+    # qspectra tests/test_pulse.py:15-38
+    pump = S.CustomPulse(0, 1, lambda x, r: x + r)
+    assert (pump.t_init, pump.t_final, pump(1, 2)) == (0, 1, 3)
+    pump = S.GaussianPulse(carrier_freq=10, fwhm=1, t_peak=5, scale=100, freq_convert=1,
+                           t_limits_multiple=2)
+    assert_allclose(pump(np.array([5, 5.5]), 10), [100, 50])
+    assert_allclose(pump(np.array([5, 5.5]), 5), [100, 50 * np.exp(2.5j)])
+    assert pump.t_init == 5 - 2 * S.GAUSSIAN_SD_FWHM and pump.t_final == 5 + 2 * S.GAUSSIAN_SD_FWHM
+    pump = S.GaussianPulse(carrier_freq=10, fwhm=2, t_peak=0, scale=100, freq_convert=2)
+    assert_allclose(pump(np.array([0, 1]), 10.0), [100, 50])
+    assert_allclose(pump(1, 0.0), 50 * np.exp(20.0j))
+
+
+def _two_level():
+    """Monomer at the rotating-wave frequency (H = 0 in the rotating frame),
+    unit dipole along x, no bath: the fields alone drive it."""
+    ham = S.ElectronicHamiltonian(np.array([[12500.]]), dipoles=[[1., 0, 0]])
+    return Lv.LiouvilleModel(ham, hilbert_subspace='ge', kind='unitary')
+
+
+def test_driven_rk4_rabi_closed_form():
+    """A resonant constant field E acting through '-' and '+' is the
+    Hamiltonian |E| (e^{i phi}|g><e| + h.c.): from the ground state the
+    excited population is sin^2(|E| t) whatever the phase phi, and the
+    coherence magnitude |sin cos| (Rabi oscillation)."""
+    m = _two_level()
+    sub = 'gg,ge,eg,ee'
+    T = 6.0
+    for E in (0.7, 0.7 * np.exp(1.1j)):
+        pulse = S.CustomPulse(0.0, T, lambda t, rw, E=E: E * np.ones_like(np.asarray(t, float)))
+        t, y = P.simulate_with_fields(m, [pulse, pulse], '-+', 'xx', times=np.linspace(-T, 0, 13),
+                                      liouville_subspace=sub, dt=T / 4000)
+        assert_allclose(t, np.linspace(0, T, 13))
+        ee = np.flatnonzero(np.isin(m.liouville_subspace_index(sub),
+                                    m.liouville_subspace_index('ee')))[0]
+        eg = np.flatnonzero(np.isin(m.liouville_subspace_index(sub),
+                                    m.liouville_subspace_index('eg')))[0]
+        assert np.max(np.abs(y[:, ee] - np.sin(abs(E) * t) ** 2)) < 1e-11
+        assert np.max(np.abs(np.abs(y[:, eg]) - np.abs(np.sin(abs(E) * t) * np.cos(abs(E) * t)))) < 1e-11
+        # a field through '-' only has no Hermitian partner: the ground state
+        # is its fixed point ([|g><e|, |g><g|] acts on ge/eg only via '+')
+        t, y1 = P.simulate_with_fields(m, [pulse], '+', 'x', times=np.linspace(-T, 0, 5),
+                                       liouville_subspace=sub, dt=T / 400)
+        assert np.abs(y1[:, ee]).max() < 1e-14
+
+
+def test_driven_rk4_fourth_order():
+    """Fixed-step RK4 for the time-dependent equation converges at fourth
+    order to the tight zvode solution (a dropped or misplaced stage time
+    degrades the order)."""
+    md = Lv.LiouvilleModel(_dimer(), hilbert_subspace='gef', unit_convert=S.CM_FS)
+    pump = S.GaussianPulse(12750., 20., scale=0.02, freq_convert=S.CM_FS)
+    times = np.arange(0., 20., 4.)
+    _, ref = P.simulate_pump(md, pump, 'x', times=times, integrator='zvode', **TIGHT)
+    errs = [rel_err(P.simulate_pump(md, pump, 'x', times=times, dt=dt)[1], ref)
+            for dt in (1.0, 0.5)]
+    assert 12 < errs[0] / errs[1] < 20
+
+
+def test_fields_match_reference():
+    """simulate_with_fields / simulate_pump / impulsive_probe against the
+    reference's own zvode outputs (tests/golden/fields.npz)."""
+    g = golden('fields')
+    md = Lv.LiouvilleModel(_dimer(), hilbert_subspace='gef', unit_convert=S.CM_FS)
+    pump = S.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=S.CM_FS)
+    t, y = P.simulate_pump(md, pump, 'x', time_extra=40., integrator='zvode', **TIGHT)
+    assert_allclose(t, g['pump_t'])
+    assert rel_err(y, g['pump_y']) < 1e-9
+    t, y = P.simulate_pump(md, pump, 'x', time_extra=40., dt=md.time_step / 4)
+    assert rel_err(y, g['pump_y']) < 1e-8
+    t, y = P.simulate_pump(md, pump, [1., 0.5, 0.], times=np.arange(0., 60., 3.),
+                           exact_isotropic_average=True, integrator='zvode', **TIGHT)
+    assert_allclose(t, g['iso_t'])
+    assert rel_err(y, g['iso_y']) < 1e-9
+    p1 = S.GaussianPulse(12700., 15., t_peak=-10., scale=0.015, freq_convert=S.CM_FS)
+    p2 = S.GaussianPulse(12900., 25., t_peak=5., scale=0.01 * np.exp(0.6j),
+                         freq_convert=S.CM_FS, t_limits_multiple=2.5)
+    t, y = P.simulate_with_fields(md, [p1, p2], '-+', 'xy', time_extra=20., integrator='zvode',
+                                  **TIGHT)
+    assert_allclose(t, g['two_t'])
+    assert rel_err(y, g['two_y']) < 1e-9
+    t, y = P.simulate_with_fields(md, [p1, p2, p1], '-+-', ['x', 'y', 0.7], times=[0., 5., 12.5],
+                                  integrator='zvode', **TIGHT)
+    assert_allclose(t, g['three_t'])
+    assert rel_err(y, g['three_y']) < 1e-9
+    mu = Lv.LiouvilleModel(_dimer(), hilbert_subspace='gef', unit_convert=S.CM_FS, kind='unitary')
+    t, y = P.simulate_pump(mu, pump, 'y', time_extra=30., integrator='zvode', **TIGHT)
+    assert rel_err(y, g['unitary_y']) < 1e-9
+    m7 = Lv.LiouvilleModel(_agg7(disorder=100 / S.GAUSSIAN_SD_FWHM), unit_convert=S.CM_FS,
+                           secular=False)
+    pump7 = S.GaussianPulse(12450., 20., t_peak=0., scale=0.01, freq_convert=S.CM_FS)
+    t, y = P.simulate_pump(m7, pump7, 'x', time_extra=20., integrator='zvode', **TIGHT)
+    assert_allclose(t, g['agg7_t'])
+    assert rel_err(y, g['agg7_y']) < 1e-9
+    t, y = P.ensemble_average(
+        lambda mm: P.simulate_pump(mm, pump7, 'z', times=np.array([0., 10., 20.]),
+                                   integrator='zvode', **TIGHT), m7, 2)
+    assert rel_err(y, g['agg7_ens2_y']) < 1e-9
+    f, X = P.impulsive_probe(md, g['probe_states'], 300.5, integrator='zvode', **TIGHT)
+    assert_allclose(f, g['probe_f'])
+    assert rel_err(X, g['probe_X']) < 1e-9
+    f, X = P.impulsive_probe(md, g['probe_states'][0], 300.5, polarization='xy',
+                             include_signal='ESE,ESA', integrator='zvode', **TIGHT)
+    assert rel_err(X, g['probe1_X']) < 1e-9
