@@ -40,8 +40,9 @@ def parse_args(argv=None):
     ap.add_argument('--cpu-baseline', choices=('auto', 'off'), default='auto')
     ap.add_argument('--cpu-budget', type=float, default=30.0,
                     help='seconds of CPU work allowed for the baseline sample')
-    ap.add_argument('--workloads', choices=('all', 'none'), default='all',
-                    help='also measure the ensemble-absorption, gradient and HEOM workloads')
+    ap.add_argument('--workloads', default='all',
+                    help="'all', 'none' or a comma list of ensemble_absorption, "
+                         "adjoint_gradient, heom, pump: the further workloads to measure")
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
                          'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
@@ -309,6 +310,46 @@ def heom_leg(torch):
     return out
 
 
+def pump_leg(torch):
+    """simulate_pump of 1024 static-disorder members (sigma = 100 cm^-1) of
+    the 7-site aggregate in 'gg,ge,eg,ee' (M = 64): Gaussian pump (FWHM 20 fs
+    at 12450 cm^-1, peak field 0.01 fs^-1), output every 2 fs up to 1 ps
+    after the pulse, at the default RK4 step; FP64 and mixed; end to end
+    from the Hamiltonians (Liouvillians, dipole maps, field samples, the
+    driven kernel)."""
+    import qspectra_amd as qa
+    from qspectra_amd import workloads as W, timing
+    m = W.aggregate7_model(disorder_sigma=100.0)
+    pump = qa.GaussianPulse(12450., 20., scale=0.01, freq_convert=qa.CM_FS)
+    kw = dict(time_extra=1000.0, ensemble_size=1024)
+    qa.simulate_pump(m, pump, 'x', **kw)               # warm-up
+    res, kern = {}, {}
+    for prec in ('fp64', 'mixed'):
+        with timing.kernel_timer() as kt:
+            res[prec] = _sync_time(torch, lambda: qa.simulate_pump(m, pump, 'x', precision=prec,
+                                                                   **kw))
+        (label, info), = [(k, v) for k, v in kt.summary().items() if k.startswith('driven')]
+        kern[prec] = (label, info['mean_ms'])
+    y64, y32 = res['fp64'][1][1], res['mixed'][1][1]
+    err = float(np.abs(y32 - y64).max() / np.abs(y64).max())
+    steps = int(re.search(r'steps=(\d+)', kern['fp64'][0]).group(1))
+    out = {'workload': 'simulate_pump, 1024 disorder members of the 7-site aggregate, '
+                       "'gg,ge,eg,ee' (M = 64), Gaussian pump FWHM 20 fs, 1 ps after the pulse "
+                       'at 2 fs output, default RK4 step',
+           'rk4_steps_per_member': steps, 'rel_err_mixed_vs_fp64': err}
+    for prec in ('fp64', 'mixed'):
+        sec = res[prec][0]
+        out[prec] = {'seconds_end_to_end': sec, 'kernel': kern[prec][0],
+                     'kernel_ms': kern[prec][1],
+                     'steps_per_s_end_to_end': 1024 * steps / sec,
+                     'steps_per_s_kernel': 1024 * steps / (kern[prec][1] * 1e-3)}
+    return out
+
+
+LEGS = (('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', gradient_leg),
+        ('heom', heom_leg), ('pump', pump_leg))
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
@@ -369,12 +410,17 @@ def main(argv=None):
         'kernels': {k: {'count': v['count'], 'mean_ms': round(v['mean_ms'], 5)}
                     for k, v in summary.items()},
     }
-    if args.workloads == 'all':
+    wanted = ([n for n, _ in LEGS] if args.workloads == 'all' else
+              [] if args.workloads == 'none' else args.workloads.split(','))
+    unknown = set(wanted) - set(n for n, _ in LEGS)
+    if unknown:
+        raise SystemExit('unknown workloads: %s' % sorted(unknown))
+    if wanted:
         line['workloads'] = {}
-        for name, leg in (('ensemble_absorption', ensemble_absorption_leg),
-                          ('adjoint_gradient', gradient_leg), ('heom', heom_leg)):
-            print('bench: %s' % name, file=sys.stderr, flush=True)
-            line['workloads'][name] = leg(torch)
+        for name, leg in LEGS:
+            if name in wanted:
+                print('bench: %s' % name, file=sys.stderr, flush=True)
+                line['workloads'][name] = leg(torch)
     if args.cpu_baseline == 'auto':
         line['cpu_baseline'] = cpu_baseline(args.cpu_budget, steps_per_spectrum)
         line['vs_cpu'] = value / line['cpu_baseline']['value']

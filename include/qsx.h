@@ -251,19 +251,25 @@ int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t 
  *
  *   A      L [B|1][M][M] (strideA 0 = shared); MIXED: off-diagonal complex64
  *          plus diag, the complex128 diagonal [B|1][M] (strideD).
- *   C      [B|1][P][M][M] (strideC), complex64 for MIXED; 1 <= P <= 4.
+ *   K, col, coef   the C_i in ELL form over the union of their sparsity
+ *          patterns (over all terms and batch entries): col int32 [M][K]
+ *          column of slot k of each row, coef [B|1][M][K][P] (complex64
+ *          for MIXED) the P coefficients of that slot (zero in padding
+ *          slots); strideE = slots per batch entry of coef (M*K, or 0 =
+ *          shared); 1 <= P <= 4.
  *   F      complex128 [B|1][nF][3][P] (strideF): f_i at the stage times
  *          t_g, t_g + h/2, t_g + h of global step g (steps numbered across
  *          the save intervals in order).
  *   nsub, h, nsave, y0 [B][R][M], out [B][R][nsave][M] as qsx_propagate.
  *   status 1: non-finite state; 2: the schedule has more steps than nF.
- * Blocks up to M = 64 (one work-group per right-hand side). */
+ * Blocks up to M = 64 (one work-group per right-hand side); K up to 16, or
+ * up to the row width. */
 int32_t qsx_propagate_driven(int32_t precision, const void* A, int64_t strideA,
-                             const void* diag, int64_t strideD, const void* C,
-                             int64_t strideC, int32_t P, const void* F, int64_t strideF,
-                             int64_t nF, int32_t M, const void* y0, int32_t R, int32_t B,
-                             const int32_t* nsub, const double* h, int32_t nsave, void* out,
-                             int32_t* status, void* stream);
+                             const void* diag, int64_t strideD, int32_t K, const int32_t* col,
+                             const void* coef, int64_t strideE, int32_t P, const void* F,
+                             int64_t strideF, int64_t nF, int32_t M, const void* y0, int32_t R,
+                             int32_t B, const int32_t* nsub, const double* h, int32_t nsave,
+                             void* out, int32_t* status, void* stream);
 
 #ifdef __cplusplus
 }

@@ -64,9 +64,9 @@ SIGNATURES = {
     'qsx_heom_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32,
                            c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp,
                            c_vp, c_vp, c_vp, c_vp],
-    'qsx_propagate_driven': [c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i64,
-                             c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp,
-                             c_vp],
+    'qsx_propagate_driven': [c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp, c_i64, c_i32,
+                             c_vp, c_i64, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32,
+                             c_vp, c_vp, c_vp],
 }
 
 
@@ -287,32 +287,59 @@ MAX_DRIVEN_BLOCK = 64
 MAX_FIELD_TERMS = 4
 
 
-def propagate_driven(A, C, F, y0, nsub, h, precision=QSX_FP64, diag=None, stream=None,
+def ell_pack(C):
+    """ELL form of field operators C [B|1, P, M, M] (numpy) over the union of
+    their sparsity patterns (all terms and batch entries): (col int32
+    [B|1, M, K], coef complex128 [B|1, M, K, P]); K is the largest number of
+    nonzero columns of a row (at least 1), padding slots point at column 0
+    with zero coefficients."""
+    C = np.asarray(C)
+    Bc, P, M, _ = C.shape
+    mask = np.any(C != 0, axis=(0, 1))                          # [M, M]
+    nnz = mask.sum(axis=1)
+    K = max(int(nnz.max()), 1)
+    col = np.zeros((M, K), dtype=np.int32)
+    for i in range(M):
+        cols = np.flatnonzero(mask[i])
+        col[i, :cols.size] = cols
+    valid = np.arange(K)[None, :] < nnz[:, None]                # [M, K]
+    coef = np.take_along_axis(C, col[None, None].repeat(Bc, 0).repeat(P, 1), axis=3)
+    coef = np.where(valid[None, None], coef, 0).transpose(0, 2, 3, 1)   # [B|1, M, K, P]
+    return np.ascontiguousarray(col[None]), np.ascontiguousarray(coef)
+
+
+def propagate_driven(A, col, coef, F, y0, nsub, h, precision=QSX_FP64, diag=None, stream=None,
                      status=None):
     """qsx_propagate_driven: RK4 of y' = A y + sum_i f_i(t) C_i y with the
-    field values F [B|1, nF, 3, P] (complex128, at the three stage times of
-    every step); C [B|1, P, M, M]; returns (out [B, R, nsave, M], status)."""
+    field operators in ELL form (col int32 [1, M, K], coef [B|1, M, K, P],
+    see ell_pack) and the field values F [B|1, nF, 3, P] (complex128, at
+    the three stage times of every step); returns (out [B, R, nsave, M],
+    status)."""
     lib = require_native()
     B, R, M = y0.shape
     nsave = int(nsub.shape[0])
     adt = torch.complex128 if precision == QSX_FP64 else torch.complex64
-    P = int(C.shape[1])
-    if A.dtype != adt or C.dtype != adt or F.dtype != torch.complex128:
+    K, P = int(coef.shape[2]), int(coef.shape[3])
+    if A.dtype != adt or coef.dtype != adt or F.dtype != torch.complex128 or \
+            col.dtype != torch.int32:
         raise TypeError('propagate_driven: operand dtypes do not match the precision')
-    if (A.shape[-2:] != (M, M) or A.shape[0] not in (1, B) or tuple(C.shape[2:]) != (M, M)
-            or C.shape[0] not in (1, B) or F.dim() != 4 or F.shape[0] not in (1, B)
-            or F.shape[2] != 3 or F.shape[3] != P):
-        raise ValueError('propagate_driven: shapes A %s, C %s, F %s do not fit states %s'
-                         % (tuple(A.shape), tuple(C.shape), tuple(F.shape), tuple(y0.shape)))
+    if (A.shape[-2:] != (M, M) or A.shape[0] not in (1, B) or tuple(col.shape) != (1, M, K)
+            or coef.dim() != 4 or coef.shape[0] not in (1, B) or coef.shape[1] != M
+            or F.dim() != 4 or F.shape[0] not in (1, B) or F.shape[2] != 3 or F.shape[3] != P):
+        raise ValueError('propagate_driven: shapes A %s, col %s, coef %s, F %s do not fit '
+                         'states %s' % (tuple(A.shape), tuple(col.shape), tuple(coef.shape),
+                                        tuple(F.shape), tuple(y0.shape)))
     if not 1 <= P <= MAX_FIELD_TERMS:
         raise ValueError('propagate_driven: %d field terms (1..%d)' % (P, MAX_FIELD_TERMS))
     if M > MAX_DRIVEN_BLOCK:
         raise ValueError('propagate_driven: block of %d states exceeds %d' % (M, MAX_DRIVEN_BLOCK))
     assert nsub.dtype == torch.int32 and h.dtype == torch.float64
-    A, C, F, y0 = A.contiguous(), C.contiguous(), F.contiguous(), y0.contiguous()
+    A, col, coef, F, y0 = (x.contiguous() for x in (A, col, coef, F, y0))
+    if col.numel() and (int(col.min()) < 0 or int(col.max()) >= M):
+        raise ValueError('propagate_driven: ELL column index outside 0..%d' % (M - 1))
     nF = int(F.shape[1])
     strideA = 0 if A.shape[0] == 1 else M * M
-    strideC = 0 if C.shape[0] == 1 else P * M * M
+    strideE = 0 if coef.shape[0] == 1 else M * K
     strideF = 0 if F.shape[0] == 1 else nF * 3 * P
     strideD = 0
     if precision == QSX_MIXED:
@@ -322,13 +349,13 @@ def propagate_driven(A, C, F, y0, nsub, h, precision=QSX_FP64, diag=None, stream
     out = torch.empty((B, R, nsave, M), dtype=torch.complex128, device=y0.device)
     if status is None:
         status = torch.zeros(1, dtype=torch.int32, device=y0.device)
-    label = 'driven %s M=%d P=%d R=%d B=%d steps=%d nsave=%d' % (
-        'fp64' if precision == QSX_FP64 else 'mixed', M, P, R, B, nF, nsave)
+    label = 'driven %s M=%d K=%d P=%d R=%d B=%d steps=%d nsave=%d' % (
+        'fp64' if precision == QSX_FP64 else 'mixed', M, K, P, R, B, nF, nsave)
     def go():
-        return lib.qsx_propagate_driven(precision, ptr(A), strideA, ptr(diag), strideD, ptr(C),
-                                        strideC, P, ptr(F), strideF, nF, M, ptr(y0), R, B,
-                                        ptr(nsub), ptr(h), nsave, ptr(out), ptr(status),
-                                        stream_ptr(stream))
+        return lib.qsx_propagate_driven(precision, ptr(A), strideA, ptr(diag), strideD, K,
+                                        ptr(col), ptr(coef), strideE, P, ptr(F), strideF, nF, M,
+                                        ptr(y0), R, B, ptr(nsub), ptr(h), nsave, ptr(out),
+                                        ptr(status), stream_ptr(stream))
     with timing.launch(label, stream, go):
         st = go()
     check(st, 'qsx_propagate_driven')

@@ -66,4 +66,49 @@ __device__ inline double2 caxpy(double s, double2 x, double2 y) {  // y + s*x
 }
 __device__ inline bool cfinite(double2 a) { return isfinite(a.x) && isfinite(a.y); }
 
+// A zero held in a VGPR that the compiler cannot see through.  Indexing a
+// wave-uniform load with it makes it a vector-memory load: s_barrier is
+// preceded by a wait for every outstanding scalar-memory load, so a scalar
+// prefetch would stall the next barrier, while a vector load is waited for
+// only at its first use.
+__device__ inline int opaque_zero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline double wave_uniform(double v) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+
+// Lane move within rows of 16 lanes by a DPP control (gfx9 encoding:
+// 0x00-0xFF quad permutation, 0x140 row mirror, 0x141 half-row mirror).
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over aligned segments of S lanes (S = 1, 2, 4, 8 or 16); every lane of
+// a segment receives the same total.  Butterfly over quad permutations
+// [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors (lane i with
+// 7-i, then with 15-i: after the previous levels all lanes of each half
+// hold that half's sum).  DPP moves stay in the VALU instead of routing
+// through the LDS crossbar as wave shuffles do.
+template <int S, typename F>
+__device__ inline F seg_sum(F v) {
+  static_assert(S == 1 || S == 2 || S == 4 || S == 8 || S == 16, "segment of 1..16 lanes");
+  if (S >= 2) v += dpp_move<0xB1>(v);
+  if (S >= 4) v += dpp_move<0x4E>(v);
+  if (S >= 8) v += dpp_move<0x141>(v);
+  if (S >= 16) v += dpp_move<0x140>(v);
+  return v;
+}
+
 }  // namespace qsx

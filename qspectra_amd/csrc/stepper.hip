@@ -6,10 +6,12 @@
 // j = k + S*c (k = t % S, c < W) of that row, held in registers for the
 // whole launch; rows beyond NT/S (only for the largest blocks) are read
 // from LDS by the first threads.  A matrix-vector product is W complex
-// FMAs per thread plus a shuffle reduction over the S lanes of a row
-// (S is a power of two, so a row's lanes are adjacent in one wavefront),
-// and the stage vector is exchanged through a double-buffered LDS vector:
-// one barrier per product.
+// FMAs per thread plus a DPP butterfly over the S lanes of a row (S is a
+// power of two, so a row's lanes are adjacent in one wavefront), and the
+// stage vector is exchanged through a double-buffered LDS vector padded
+// with zeros to S*W entries (so the row segments read it without bounds
+// tests): one barrier per product.  The step counts and lengths of the next
+// save interval are fetched with vector loads one interval ahead.
 //
 // Modes
 //   RK4  (PROP = false): y <- y + h/6 (k1 + 2 k2 + 2 k3 + k4), k = L z,
@@ -30,14 +32,6 @@
 namespace qsx {
 namespace {
 
-template <typename T>
-__device__ inline cplx_t<T> shfl_xor_c(cplx_t<T> v, int m) {
-  cplx_t<T> r;
-  r.x = __shfl_xor(v.x, m);
-  r.y = __shfl_xor(v.y, m);
-  return r;
-}
-
 __constant__ double kRkW[4] = {1.0, 2.0, 2.0, 1.0};
 __constant__ double kRkC[3] = {0.5, 0.5, 1.0};
 
@@ -49,9 +43,10 @@ __global__ __launch_bounds__(NT) void step_kernel(
     double2* __restrict__ out, double2* __restrict__ yfin, int* __restrict__ status) {
   using T2 = cplx_t<T>;
   constexpr int kRegRows = NT / S;
+  constexpr int MP = S * W;                            // padded row length (>= M)
   extern __shared__ __align__(16) unsigned char smem[];
-  T2* zb = reinterpret_cast<T2*>(smem);               // [2][M] stage vector
-  T2* Aov = zb + 2 * M;                                // [M - kRegRows][M] overflow rows
+  T2* zb = reinterpret_cast<T2*>(smem);               // [2][MP] stage vector
+  T2* Aov = zb + 2 * MP;                               // [M - kRegRows][MP] overflow rows
 
   const int tid = threadIdx.x;
   const int r = blockIdx.x, b = blockIdx.y;
@@ -72,7 +67,11 @@ __global__ __launch_bounds__(NT) void step_kernel(
     const int j = k + S * c;
     a[c] = (has0 && j < M) ? Ab[(int64_t)row0 * M + j] : T2{0, 0};
   }
-  for (int e = tid; e < nov * M; e += NT) Aov[e] = Ab[(int64_t)kRegRows * M + e];
+  for (int e = tid; e < nov * MP; e += NT) {
+    const int j = e % MP;
+    Aov[e] = j < M ? Ab[(int64_t)(kRegRows + e / MP) * M + j] : T2{0, 0};
+  }
+  for (int e = tid; e < 2 * MP; e += NT) zb[e] = T2{0, 0};
 
   const double2* yb0 = y0 + ((int64_t)b * R + r) * M;
   double2 y[2], acc[2], zi[2], dd[2];
@@ -83,34 +82,43 @@ __global__ __launch_bounds__(NT) void step_kernel(
     y[q] = on ? yb0[row] : make_double2(0.0, 0.0);
     dd[q] = (MIXED && on) ? diag[(int64_t)b * strideD + row] : make_double2(0.0, 0.0);
     acc[q] = zi[q] = make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int row = q == 0 ? row0 : row1;
+    const bool on = q == 0 ? has0 : has1;
     if (on && lead) zb[row] = from_d2<T>(y[q]);
   }
   __syncthreads();
 
   auto matvec = [&](const T2* z, int q) -> double2 {
-    T2 s = {0, 0};
+    // FP64 segments wider than 16 columns keep a per-column test (not
+    // needed for bounds: the stage vector is zero-padded): it serialises the
+    // LDS loads, which otherwise would all be hoisted and exceed the 128
+    // registers a 1024-thread block leaves beside the coefficients
+    constexpr bool kSerial = sizeof(T) == 8 && W > 16;
+    T2 s = {0, 0}, s2 = {0, 0};
     if (q == 0) {
 #pragma unroll
       for (int c = 0; c < W; ++c) {
         const int j = k + S * c;
-        if (j < M) cfma<T>(s, a[c], z[j]);
+        if (!kSerial || j < M) cfma<T>((c & 1) ? s2 : s, a[c], z[j]);
       }
     } else if (has1) {
-      // overflow rows use the same column split (S * W >= M), unrolled so
-      // that the LDS loads of a row segment are all in flight together
-      const T2* Ar = Aov + (int64_t)(tid / S) * M;
+      // overflow rows use the same column split
+      const T2* Ar = Aov + (int64_t)(tid / S) * MP;
 #pragma unroll
       for (int c = 0; c < W; ++c) {
         const int j = k + S * c;
-        if (j < M) cfma<T>(s, Ar[j], z[j]);
+        if (!kSerial || j < M) cfma<T>((c & 1) ? s2 : s, Ar[j], z[j]);
       }
     }
-#pragma unroll
-    for (int m = S / 2; m > 0; m >>= 1) {
-      const T2 o = shfl_xor_c<T>(s, m);
-      s.x += o.x;
-      s.y += o.y;
-    }
+    s.x = seg_sum<S>(s.x + s2.x);
+    s.y = seg_sum<S>(s.y + s2.y);
+    // keep the primary and overflow products apart: overlapping their LDS
+    // loads would exceed the register budget of the largest blocks
+    __builtin_amdgcn_sched_barrier(0);
     return to_d2(s);
   };
 
@@ -118,13 +126,20 @@ __global__ __launch_bounds__(NT) void step_kernel(
   bool bad = false;
   const int nslots = nov > 0 ? 2 : 1;
   double2* ob = out + ((int64_t)b * R + r) * nsave * M;
+  const int vz = opaque_zero();
+  int n_next = nsub[vz];
+  double h_next = hs[vz];
   for (int s = 0; s < nsave; ++s) {
-    const int n = nsub[s];
-    const double h = hs[s];
+    const int n = max(wave_uniform(n_next), 0);
+    const double h = wave_uniform(h_next);
+    if (s + 1 < nsave) {
+      n_next = nsub[s + 1 + vz];
+      h_next = hs[s + 1 + vz];
+    }
     for (int step = 0; step < n; ++step) {
       if (PROP) {
-        const T2* zc = zb + cur * M;
-        T2* zn = zb + (cur ^ 1) * M;
+        const T2* zc = zb + cur * MP;
+        T2* zn = zb + (cur ^ 1) * MP;
         double2 kq[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q)
@@ -146,8 +161,8 @@ __global__ __launch_bounds__(NT) void step_kernel(
       } else {
 #pragma unroll 1
         for (int st = 0; st < 4; ++st) {
-          const T2* zc = zb + cur * M;
-          T2* zn = zb + (cur ^ 1) * M;
+          const T2* zc = zb + cur * MP;
+          T2* zn = zb + (cur ^ 1) * MP;
           double2 kq[2];
 #pragma unroll
           for (int q = 0; q < 2; ++q)
@@ -223,7 +238,8 @@ int launch_step(const Shape& sh, const void* A, int64_t strideA, const void* dia
   using T2 = cplx_t<T>;
   const int rows_reg = sh.NT / sh.S;
   const int nov = M > rows_reg ? M - rows_reg : 0;
-  const size_t shmem = (size_t)(2 * M + nov * M) * sizeof(T2);
+  const int mp = sh.S * sh.W;
+  const size_t shmem = (size_t)(2 * mp + nov * mp) * sizeof(T2);
   QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_propagate: block M=%d needs %zu bytes of LDS", M, shmem);
   dim3 grid(R, B);
 #define QSX_STEP(NT_, S_, W_)                                                                  \

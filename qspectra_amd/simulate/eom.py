@@ -164,13 +164,15 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     nsub, hs = schedule_tensors(t, t0, dt, dev)
     y0 = _native.to_device(rho0[:, None, :], dev=dev)                  # [B, 1, M]
     Ft = _native.to_device(F[None], dev=dev)
+    col, coef = _native.ell_pack(C)
+    col = torch.as_tensor(col, device=dev)
     if precision in ('fp64', 'double', None):
-        out, status = _native.propagate_driven(eom.L, _native.to_device(C, dev=dev), Ft, y0,
-                                               nsub, hs, _native.QSX_FP64)
+        out, status = _native.propagate_driven(eom.L, col, _native.to_device(coef, dev=dev), Ft,
+                                               y0, nsub, hs, _native.QSX_FP64)
     elif precision == 'mixed':
         Lm, diag = split_diagonal(eom.L)
-        Cm = _native.to_device(C, dtype=torch.complex64, dev=dev)
-        out, status = _native.propagate_driven(Lm, Cm, Ft, y0, nsub, hs, _native.QSX_MIXED,
+        cm = _native.to_device(coef, dtype=torch.complex64, dev=dev)
+        out, status = _native.propagate_driven(Lm, col, cm, Ft, y0, nsub, hs, _native.QSX_MIXED,
                                                diag=diag)
     else:
         raise ValueError("precision must be 'fp64' or 'mixed'")

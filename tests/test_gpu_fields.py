@@ -154,21 +154,25 @@ def test_driven_kernel_status_and_errors(gpu):
     """The kernel stops with status 2 when the field record is shorter than
     the schedule; bad sizes are rejected on the host."""
     M, P_ = 4, 1
-    A = torch.zeros((1, M, M), dtype=torch.complex128, device='cuda')
-    C = torch.zeros((1, P_, M, M), dtype=torch.complex128, device='cuda')
-    F = torch.zeros((1, 3, 3, P_), dtype=torch.complex128, device='cuda')
-    y0 = torch.ones((1, 1, M), dtype=torch.complex128, device='cuda')
-    nsub = torch.tensor([2, 2], dtype=torch.int32, device='cuda')
-    h = torch.tensor([0.1, 0.1], dtype=torch.float64, device='cuda')
-    out, status = _native.propagate_driven(A, C, F, y0, nsub, h)
+    dev = 'cuda'
+    A = torch.zeros((1, M, M), dtype=torch.complex128, device=dev)
+    col, coef = _native.ell_pack(np.zeros((1, P_, M, M), complex))
+    col = torch.as_tensor(col, device=dev)
+    coef = torch.as_tensor(coef, device=dev)
+    F = torch.zeros((1, 3, 3, P_), dtype=torch.complex128, device=dev)
+    y0 = torch.ones((1, 1, M), dtype=torch.complex128, device=dev)
+    nsub = torch.tensor([2, 2], dtype=torch.int32, device=dev)
+    h = torch.tensor([0.1, 0.1], dtype=torch.float64, device=dev)
+    out, status = _native.propagate_driven(A, col, coef, F, y0, nsub, h)
     assert int(status.item()) == 2
-    F = torch.zeros((1, 4, 3, P_), dtype=torch.complex128, device='cuda')
-    out, status = _native.propagate_driven(A, C, F, y0, nsub, h)
+    F = torch.zeros((1, 4, 3, P_), dtype=torch.complex128, device=dev)
+    out, status = _native.propagate_driven(A, col, coef, F, y0, nsub, h)
     assert int(status.item()) == 0
     assert torch.equal(out[0, 0, 1], y0[0, 0])                  # L = 0, no field
+    big = torch.zeros((1, 65, 65), dtype=torch.complex128, device=dev)
+    c65, k65 = _native.ell_pack(np.zeros((1, 1, 65, 65), complex))
     with pytest.raises(ValueError):
-        _native.propagate_driven(torch.zeros((1, 65, 65), dtype=torch.complex128, device='cuda'),
-                                 torch.zeros((1, 1, 65, 65), dtype=torch.complex128,
-                                             device='cuda'), F,
-                                 torch.ones((1, 1, 65), dtype=torch.complex128, device='cuda'),
+        _native.propagate_driven(big, torch.as_tensor(c65, device=dev),
+                                 torch.as_tensor(k65, device=dev), F,
+                                 torch.ones((1, 1, 65), dtype=torch.complex128, device=dev),
                                  nsub, h)

@@ -227,3 +227,26 @@ def test_pulses_and_field_samples():
     for k, off in enumerate([0.0, 0.25, 0.5]):
         assert_allclose(F[:, k, 0], -1j * g1(tg + off, 1.0))
         assert_allclose(F[:, k, 1], -1j * np.conj(g1(tg + off, 1.0)))
+
+
+def test_ell_pack_of_field_operators():
+    """ELL form of the field operators reproduces every C_i (union pattern
+    over terms and batch entries, zero padding)."""
+    from qspectra_amd import _native
+    rs = np.random.RandomState(2)
+    C = np.zeros((3, 2, 9, 9), complex)
+    mask = rs.rand(2, 9, 9) < 0.3
+    C[:, mask] = rs.randn(3, int(mask.sum())) + 1j * rs.randn(3, int(mask.sum()))
+    C[1, 0, 4, 7] = 2.0                       # pattern entry of one batch entry only
+    col, coef = _native.ell_pack(C)
+    assert col.shape[0] == 1 and coef.shape[:3] == (3, 9, col.shape[2]) and coef.shape[3] == 2
+    D = np.zeros_like(C)
+    for b in range(3):
+        for i in range(9):
+            for k in range(col.shape[2]):
+                D[b, :, i, col[0, i, k]] += coef[b, i, k]
+    assert_allclose(D, C)
+    union = np.any(C != 0, axis=(0, 1)).sum(axis=1).max()
+    assert col.shape[2] == union
+    col0, coef0 = _native.ell_pack(np.zeros((1, 1, 3, 3)))
+    assert col0.shape == (1, 3, 1) and not coef0.any()
