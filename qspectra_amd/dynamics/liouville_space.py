@@ -263,6 +263,38 @@ class LiouvilleSpaceModel(DynamicalModel):
             rho = self.hamiltonian.transform_vector_to_eigenbasis(rho, liouville_subspace)
         return rho
 
+    def thermal_states(self, liouville_subspace):
+        """thermal_state of every batch member, [B, M]: site-basis models of
+        real electronic Hamiltonians take one batched eigensolver launch per
+        manifold; other models are evaluated member by member."""
+        from ..hamiltonian import ElectronicHamiltonian, electronic_thermal_states
+        from ..utils import copy_with_new_cache
+        hams = self._hamiltonians()
+        if len(hams) == 1:
+            return np.asarray(self.thermal_state(liouville_subspace))[None]
+        if liouville_subspace == 'gg':
+            return np.ones((len(hams), 1), dtype=complex)
+        batched = (self.evolve_basis == 'site'
+                   and all(type(h) is ElectronicHamiltonian for h in hams)
+                   and not any(np.iscomplexobj(h.H_1exc) and np.any(np.imag(h.H_1exc) != 0)
+                               for h in hams))
+        if not batched:
+            out = []
+            for h in hams:
+                member = copy_with_new_cache(self)
+                member.__dict__.pop('_batch', None)
+                member.hamiltonian = h
+                out.append(member.thermal_state(liouville_subspace))
+            return np.stack(out)
+        rho = electronic_thermal_states(hams, liouville_subspace)       # [B, N, N]
+        vec = rho.transpose(0, 2, 1).reshape(len(hams), -1)             # column-stacked
+        fi, ti = [self.liouville_subspace_index(s)
+                  for s in (full_liouville_subspace(liouville_subspace), liouville_subspace)]
+        N = self.hamiltonian.n_states(self.hilbert_subspace)
+        full = np.zeros((len(hams), N * N), dtype=complex)
+        full[:, fi] = vec
+        return full[:, ti]
+
     def map_between_subspaces(self, state, from_subspace, to_subspace):
         fi, ti = [self.liouville_subspace_index(s) for s in (from_subspace, to_subspace)]
         N = self.hamiltonian.n_states(self.hilbert_subspace)

@@ -46,6 +46,40 @@ def _state_from_eig(E, U, temperature):
     return rho.astype(complex)
 
 
+def electronic_thermal_states(hamiltonians, subspace):
+    """thermal_state(subspace) of several ElectronicHamiltonians at once,
+    [B, N, N]: one batched eigensolver launch per manifold of the
+    non-rotating one-exciton Hamiltonians, then the Boltzmann (T > 0) or
+    ground-state (T <= 0) weights of _state_from_eig per member."""
+    hams = list(hamiltonians)
+    n = hams[0].n_sites
+    dev = _native.device()
+    H1 = torch.as_tensor(np.stack([h.H1_nonrotating for h in hams])).to(dev)
+    rw = torch.zeros(len(hams), dtype=torch.float64, device=dev)
+    Es, Us = [], []
+    for s in 'gef':
+        X = 'gef'.index(s)
+        if s not in subspace or _native.block_size(X, n) == 0:
+            continue
+        E, U = _native.block_eigh(H1, X, rw)
+        Es.append(E.cpu().numpy())
+        Us.append(U.cpu().numpy())
+    E = np.concatenate(Es, axis=1)                                  # [B, N]
+    B, N = E.shape
+    U = np.zeros((B, N, N))
+    k = 0
+    for u in Us:
+        U[:, k:k + u.shape[1], k:k + u.shape[1]] = u
+        k += u.shape[1]
+    T = np.array([getattr(getattr(h._not_rotating, 'bath', None), 'temperature', 0) or 0
+                  for h in hams], dtype=float)
+    dE = E - E.min(axis=1, keepdims=True)
+    hot = T > 0
+    w = np.where(hot[:, None], np.exp(-dE / np.where(hot, T, 1.0)[:, None]), dE == 0)
+    w = w / w.sum(axis=1, keepdims=True)
+    return np.einsum('bij,bj,bkj->bik', U, w, U).astype(complex)
+
+
 def add_braket(labels):
     out = []
     for label in labels:

@@ -34,20 +34,27 @@ def _get_call_args(func, *args, **kwargs):
     return out
 
 
+def _host(x):
+    """numpy view of a result (device tensors are copied to the host)."""
+    return x.cpu().numpy() if hasattr(x, 'cpu') else x
+
+
 def optional_ensemble_average(func):
     """Adds ``ensemble_size`` / ``ensemble_random_orientations``: the first
     ensemble_size samples of the model's Hamiltonian are evaluated as one
-    batch and averaged."""
+    batch and averaged (on the device when the wrapped function returns a
+    device tensor, so only the mean crosses to the host)."""
     @wraps(func)
     def wrapper(dynamical_model, *args, **kwargs):
         size = kwargs.pop('ensemble_size', None)
         random_orientations = kwargs.pop('ensemble_random_orientations', False)
         if size is None:
             ticks, signal = func(dynamical_model, *args, **kwargs)
-            return ticks, signal[0]
+            return ticks, _host(signal[0])
         hams = list(dynamical_model.hamiltonian.sample_ensemble(size, random_orientations))
         ticks, signal = func(dynamical_model.with_hamiltonians(hams), *args, **kwargs)
-        return ticks, signal.sum(axis=0) / size
+        total = signal.sum(dim=0) if hasattr(signal, 'cpu') else signal.sum(axis=0)
+        return ticks, _host(total / size)
     return wrapper
 
 

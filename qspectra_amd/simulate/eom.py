@@ -106,7 +106,7 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
                           liouville_subspace, save_func, **integrate_kwargs):
     """(t, states [B, len(t), ...]) for a (possibly batched) model; the
     terms pair pulses, geometry and polarization in order (as zip does)."""
-    from .response import _members, _member_model, _op_matrices
+    from .response import _members, _op_matrices
     model = dynamical_model
     dt = integrate_kwargs.pop('dt', None)
     precision = integrate_kwargs.pop('precision', 'fp64')
@@ -136,11 +136,8 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     Bc = max(m.shape[0] for m in mats)
     C = np.stack([np.broadcast_to(m, (Bc, M, M)) for m in mats], axis=1)   # [B|1, P, M, M]
 
-    if B == 1:
-        rho0 = np.asarray(model.thermal_state(liouville_subspace))[None]
-    else:
-        rho0 = np.stack([_member_model(model, h).thermal_state(liouville_subspace)
-                         for h in hams])
+    rho0 = (model.thermal_states(liouville_subspace) if hasattr(model, 'thermal_states')
+            else np.asarray(model.thermal_state(liouville_subspace))[None])
     t0 = min(p.t_init for p in pulses)
     tf = max(p.t_final for p in pulses)
     t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
@@ -183,12 +180,12 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     if isinstance(save_func, LinearMap):
         mat = np.asarray(save_func.matrix)
         proj = _native.to_device(np.atleast_2d(mat), dev=dev)
-        saved = torch.matmul(states, proj.transpose(0, 1).contiguous()).cpu().numpy()
+        saved = torch.matmul(states, proj.transpose(0, 1).contiguous())
         return t, saved[..., 0] if mat.ndim == 1 else saved
-    states = states.cpu().numpy()
     if save_func is not None:
+        states = states.cpu().numpy()
         return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in states])
-    return t, states
+    return t, states                       # device tensor: ensembles average on the GPU
 
 
 def simulate_with_fields(dynamical_model, pulses, geometry='-+', polarization='xx',

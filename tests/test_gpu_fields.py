@@ -176,3 +176,24 @@ def test_driven_kernel_status_and_errors(gpu):
                                  torch.as_tensor(k65, device=dev), F,
                                  torch.ones((1, 1, 65), dtype=torch.complex128, device=dev),
                                  nsub, h)
+
+
+def test_batched_thermal_states(gpu):
+    """thermal_states of an ensemble (one eigensolver launch per manifold)
+    equals thermal_state member by member, at 77 K and 300 K and in
+    subspaces where the Boltzmann weights matter ('ee')."""
+    from tests.conftest import golden
+    g = golden('redfield_7site')
+    for T in (S.CM_K * 77, S.CM_K * 3000):
+        ham = qa.ElectronicHamiltonian(g['in_H1'], bath=qa.DebyeBath(T, 35, 106),
+                                       dipoles=g['in_dipoles'],
+                                       energy_spread_extra=float(g['extra']),
+                                       disorder=100 / S.GAUSSIAN_SD_FWHM)
+        m = qa.RedfieldModel(ham, hilbert_subspace='gef', unit_convert=qa.CM_FS)
+        from qspectra_amd.simulate.response import _member_model
+        hams = list(m.hamiltonian.sample_ensemble(5))
+        mb = m.with_hamiltonians(hams)
+        for sub in ('gg,ge,eg,ee', 'ee', 'gg', 'ge,eg,ee'):
+            got = mb.thermal_states(sub)
+            ref = np.stack([_member_model(mb, h).thermal_state(sub) for h in hams])
+            assert got.shape == ref.shape and np.abs(got - ref).max() < 1e-13, (T, sub)
