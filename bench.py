@@ -180,9 +180,9 @@ HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
 # scaled by 2.0 and WRITE by 1.0 (the factors that make a 256 MiB device copy
 # in the same runs read and write 256 MiB).  Keyed by launch label.
 PMC_TRAFFIC = {
-    'propagate prop mixed M=147 R=8 B=1 steps=127 nsave=16': 1.830e6,
-    'heom stage fp64': 3.363e9,
-    'heom stage mixed': 2.525e9,
+    'propagate prop mixed M=147 R=8 B=1 steps=127 nsave=16': 1.808e6,
+    'heom stage fp64': 3.525e9,
+    'heom stage mixed': 2.679e9,
 }
 
 
