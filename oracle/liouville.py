@@ -334,3 +334,72 @@ def heom_evolve_matrix(ham, hilbert_subspace, liouville_subspace, unit_convert,
     if heisenberg_picture:
         L = L.T
     return csr_matrix(L)
+
+
+class HEOMOperator(object):
+    """heom.py:14-58: a Liouville-space operator applied to every ADO block
+    of an ADO-major hierarchy vector; bra on ADO 0."""
+
+    def __init__(self, lspace_op, ado_count):
+        self.lspace_op = lspace_op
+        self.ado_count = ado_count
+
+    def commutator_matrix(self):
+        return np.kron(np.eye(self.ado_count), self.lspace_op.commutator_matrix())
+
+    def commutator(self, state):
+        return self.commutator_matrix().dot(state)
+
+    @property
+    def bra_vector(self):
+        b = self.lspace_op.bra_vector
+        out = np.zeros(b.size * self.ado_count, dtype=complex)
+        out[:b.size] = b
+        return out
+
+
+class HEOMOracleModel(object):
+    """heom.py:177-296 as a model for the response drivers: the generator
+    of heom_evolve_matrix, dipole operators mapped over ADOs, the thermal
+    state on ADO 0 (the Liouville-space part is the unitary model's)."""
+
+    def __init__(self, hamiltonian, rw_freq=None, hilbert_subspace='gef', unit_convert=1,
+                 level_cutoff=3, K=1, low_temp_corr=True, modified_HEOM=False):
+        self.lspace = LiouvilleModel(hamiltonian, rw_freq, hilbert_subspace, unit_convert,
+                                     kind='unitary')
+        self.hamiltonian = self.lspace.hamiltonian
+        self.rw_freq = self.lspace.rw_freq
+        self.hilbert_subspace = hilbert_subspace
+        self.unit_convert = unit_convert
+        self.level_cutoff, self.K = level_cutoff, K
+        self.low_temp_corr, self.modified_HEOM = low_temp_corr, modified_HEOM
+        self.ado_count = len(ado_mappings(self.hamiltonian.n_sites, K, level_cutoff)[0])
+
+    @property
+    def time_step(self):
+        return self.lspace.time_step
+
+    def evolve_matrix(self, subspace, heisenberg_picture=False):
+        return heom_evolve_matrix(self.hamiltonian, self.hilbert_subspace, subspace,
+                                  self.unit_convert, self.K, self.level_cutoff,
+                                  self.low_temp_corr, self.modified_HEOM,
+                                  heisenberg_picture).toarray()
+
+    def dipole_operator(self, liouv_subspace_map, polarization, transitions='-+'):
+        return HEOMOperator(self.lspace.dipole_operator(liouv_subspace_map, polarization,
+                                                        transitions), self.ado_count)
+
+    def thermal_state(self, liouville_subspace):
+        rho = self.lspace.thermal_state(liouville_subspace)
+        out = np.zeros(rho.size * self.ado_count, dtype=complex)
+        out[:rho.size] = rho
+        return out
+
+    def map_between_subspaces(self, state, from_subspace, to_subspace):
+        fs, ts = [self.lspace.liouville_subspace_index(s).size
+                  for s in (from_subspace, to_subspace)]
+        out = np.zeros(ts * self.ado_count, dtype=complex)
+        for n in range(self.ado_count):
+            out[n * ts:(n + 1) * ts] = self.lspace.map_between_subspaces(
+                state[n * fs:(n + 1) * fs], from_subspace, to_subspace)
+        return out

@@ -151,6 +151,24 @@ class DeviceHEOM(object):
         out = _native.heom_apply(self, yt)
         return out.cpu().numpy().reshape(y.shape)
 
+    def repeated(self, R):
+        """The same hierarchy with every sample repeated R times (sample-
+        major), so that R right-hand sides per sample propagate as batch
+        entries of one launch (the tables are shared; only the per-sample
+        operator block is copied)."""
+        new = DeviceHEOM(dict(self.__dict__))
+        new.Hop = self.Hop.repeat_interleave(int(R), dim=0).contiguous()
+        return new
+
+    def dense_generator(self):
+        """The hierarchy generator as dense matrices [B, S, S] (S = n_ado *
+        M), column j = the right-hand side applied to unit vector j, from
+        one application launch over all S unit vectors of every sample."""
+        B, S = self.batch, self.size
+        eye = torch.eye(S, dtype=torch.complex128, device=self.Hop.device)
+        cols = _native.heom_apply(self.repeated(S), eye.repeat(B, 1).contiguous())
+        return cols.reshape(B, S, S).transpose(1, 2).contiguous()
+
 
 class HEOMModel(DynamicalModel):
     """HEOM for an identical Drude bath on every site with K Matsubara terms,

@@ -242,11 +242,88 @@ def _third_order_response(dynamical_model, coherence_time_max, population_time_m
     return (p.t1, p.t2, p.t3), X.cpu().numpy()
 
 
+# hierarchies with at most this many elements (ADOs x Liouville block) are
+# propagated through their dense generator by the Liouville-space engine;
+# larger ones by the HEOM stage kernel with hierarchy states as batch entries
+HEOM_DENSE_MAX = 512
+
+
+def _hierarchy_maps(model, mats, what):
+    """Per-ADO dipole maps of a hierarchy model as hierarchy matrices:
+    commutators kron(I_ado, V) [B|1, n_ado*Mto, n_ado*Mfrom] (ADO-major
+    states), bras zero-padded beyond ADO 0 [B|1, n_ado*M]."""
+    n = model.ado_count
+    if what == 'commutator':
+        return np.stack([np.kron(np.eye(n), m) for m in mats])
+    out = np.zeros((mats.shape[0], n * mats.shape[-1]), dtype=complex)
+    out[:, :mats.shape[-1]] = mats
+    return out
+
+
+def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
+    """Third-order response of a hierarchy model (response.py:269-337 with
+    the HEOM state maps of heom.py:14-58): the whole hierarchy is carried
+    through t1 and t2 and mapped by the dipoles ADO by ADO; t3 runs in the
+    Heisenberg picture from the bra on ADO 0; the signal contracts over the
+    whole hierarchy.  Small hierarchies use the dense generator and the
+    Liouville-space engine, large ones the HEOM stage kernel."""
+    B = len(_members(model))
+    n_ado = model.ado_count
+    rho0 = model.thermal_state('gg')
+    dt, prec = kw.get('dt'), kw.get('precision', 'fp64')
+    trans = geometry + '-'
+    total = None
+    for path in paths:
+        s = path.split('->')
+        V = [_op_matrices(model, '%s->%s' % (s[i], s[i + 1]), polarization[i], trans[i],
+                          'commutator') for i in range(3)]
+        b3 = _op_matrices(model, '%s->%s' % (s[3], s[4]), polarization[3], trans[3], 'bra')
+        eoms = [model.equation_of_motion(s[1]), model.equation_of_motion(s[2]),
+                model.equation_of_motion(s[3], heisenberg_picture=True)]
+        if max(e.size for e in eoms) <= HEOM_DENSE_MAX:
+            from ..dynamics.liouville_space import DeviceEOM
+            dense = [DeviceEOM(e.dense_generator(), e.liouville_subspace, e.heisenberg_picture)
+                     for e in eoms]
+            Vh = [_hierarchy_maps(model, v, 'commutator') for v in V]
+            bh = _hierarchy_maps(model, b3, 'bra')
+            V_rho0 = _expand(_dev(np.einsum('bij,j->bi', Vh[0], rho0)[:, None, :]), B)
+            V_rho1 = propagate(dense[0], V_rho0, t1, proj=_dev(Vh[1]), **kw)
+            V_rho1 = V_rho1.reshape(B, len(t1), -1)
+            V_rho2 = propagate(dense[1], V_rho1, t2, t0=0, proj=_dev(Vh[2]), **kw)
+            G3 = propagate(dense[2], _expand(_dev(bh[:, None, :]), B), t3, **kw)[:, 0]
+        else:
+            def per_ado(x, Vm):          # x [B, ..., n_ado*M] -> [B, ..., n_ado*Mto]
+                Vt = _expand(_dev(Vm), B).transpose(1, 2).contiguous()    # [B, M, Mto]
+                lead = x.shape[1:-1]
+                xs = x.reshape(B, -1, Vm.shape[-1])
+                return torch.matmul(xs, Vt).reshape((B,) + lead + (-1,))
+            M0 = V[0].shape[-1]
+            y0 = torch.zeros((B, eoms[0].size), dtype=torch.complex128, device=_native.device())
+            y0[:, :V[0].shape[1]] = _expand(_dev(np.einsum('bij,j->bi', V[0], rho0[:M0])), B)
+            st1 = propagate_heom(eoms[0], y0, t1, dt=dt, precision=prec, full_states=True)
+            X1 = per_ado(st1, V[1]).reshape(B * len(t1), -1)
+            st2 = propagate_heom(eoms[1].repeated(len(t1)), X1, t2, t0=0, dt=dt, precision=prec,
+                                 full_states=True)
+            V_rho2 = per_ado(st2.reshape(B, len(t1), len(t2), -1), V[2])
+            g0 = torch.zeros((B, eoms[2].size), dtype=torch.complex128, device=_native.device())
+            g0[:, :b3.shape[-1]] = _expand(_dev(b3), B)
+            G3 = propagate_heom(eoms[2], g0, t3, dt=dt, precision=prec, full_states=True)
+        S3 = G3.shape[-1]
+        prod = torch.matmul(V_rho2.reshape(B, -1, S3), G3.transpose(1, 2).contiguous())
+        prod = prod.reshape(B, len(t1), len(t2), len(t3))
+        total = prod if total is None else total + prod
+    return (t1, t2, t3), total.cpu().numpy()
+
+
 def _pathway_by_pathway(model, coherence_time_max, population_time_max, population_times,
                         geometry, polarization, include_signal, kw):
     """Each pathway evaluated on its own, in the reference's order."""
     t1, t2, t3 = third_order_grids(model, coherence_time_max, population_time_max,
                                    population_times)
+    if hasattr(model, 'ado_count'):
+        return _heom_pathways(model, t1, t2, t3,
+                              _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal),
+                              geometry, polarization, kw)
     B = len(_members(model))
     rho0 = model.thermal_state('gg')
     total = None

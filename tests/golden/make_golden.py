@@ -4,6 +4,7 @@ Run only in the build container, where the reference sources are present:
 
     python tests/golden/make_golden.py /root/reference            # every file
     python tests/golden/make_golden.py /root/reference fields     # fields.npz only
+    python tests/golden/make_golden.py /root/reference heom_spectra
 
 It imports qspectra from the given path and stores plain arrays (inputs and
 outputs) as .npz files.  Nothing here is imported by the tests or by the
@@ -14,6 +15,7 @@ known-answer test pins are written:
 * HEOM sparse Liouvillians (ADO order, low-temperature correction),
 * zvode-integrated trajectories and end-to-end spectra (absorption,
   third-order response, 2-D spectra, isotropic and ensemble averages),
+* linear and third-order responses of HEOM models (dimer, 'gef'),
 * field-driven trajectories (simulate_with_fields, simulate_pump: Gaussian
   pulses, complex amplitudes, three field terms, unitary / 7-site /
   ensemble models) and the impulsive probe of a pumped state,
@@ -93,6 +95,30 @@ def write_fields(qs, ham, ham7):
     np.savez_compressed(os.path.join(HERE, 'fields.npz'), **out)
 
 
+def write_heom_spectra(qs, HEOMModel, dim, extra):
+    """heom_spectra.npz: linear and third-order responses of HEOM models
+    (the hierarchy states carried through every interval)."""
+    out = {}
+    ham = qs.ElectronicHamiltonian(dim['H1'], bath=qs.DebyeBath(dim['T'], dim['lam'],
+                                                              dim['gamma']),
+                                   dipoles=dim['dipoles'], energy_spread_extra=extra)
+    for name, kw in [('lt', dict(level_cutoff=3, K=1)),
+                     ('nolt', dict(level_cutoff=3, K=1, low_temp_corr=False))]:
+        hm = HEOMModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS, **kw)
+        t, S = qs.linear_response(hm, 'gg->eg->gg', 150.5, **TIGHT)
+        out[name + '_lin_t'], out[name + '_lin_S'] = t, S
+        f, X = qs.absorption_spectra(hm, 150.5, **TIGHT)
+        out[name + '_abs_f'], out[name + '_abs_X'] = f, X
+        for geom in ('-++', '++-'):
+            (t1, t2, t3), R = qs.third_order_response(hm, 8.5, population_times=[0., 3., 7.],
+                                                      geometry=geom, **TIGHT)
+            g = geom.replace('-', 'm').replace('+', 'p')
+            out['%s_R3_%s' % (name, g)] = R
+            out['t1'], out['t2'], out['t3'] = t1, t2, t3
+    out['time_step'] = hm.time_step
+    np.savez_compressed(os.path.join(HERE, 'heom_spectra.npz'), **out)
+
+
 def write_manifest(ref_path):
     with open(os.path.join(HERE, 'MANIFEST.txt'), 'w') as fh:
         fh.write('written by tests/golden/make_golden.py from the reference at %s\n' % ref_path)
@@ -118,6 +144,12 @@ def main(ref_path, only=None):
             dipoles=inp['dipoles'], disorder=disorder, random_seed=seed,
             energy_spread_extra=extra)
 
+    if only == 'heom_spectra':
+        from qspectra.dynamics.heom import HEOMModel
+        dim = dimer_inputs()
+        write_heom_spectra(qs, HEOMModel, dim, extra_for_dt(qs, dim['H1'], 1.0))
+        write_manifest(ref_path)
+        return
     if only == 'fields':
         dim, agg = dimer_inputs(), aggregate7_inputs()
         ham = ham_of(dim, extra_for_dt(qs, dim['H1'], 1.0))
@@ -270,6 +302,7 @@ def main(ref_path, only=None):
     np.savez_compressed(os.path.join(HERE, 'dynamics.npz'), **out)
 
     write_fields(qs, ham, ham7)
+    write_heom_spectra(qs, HEOMModel, dim, extra)
     write_manifest(ref_path)
 
 

@@ -105,3 +105,50 @@ def test_heom_ensemble_batch_one_launch(gpu):
         assert rel_err(got[n], ref) < 1e-13, n
     _, avg = qa.simulate_dynamics(m, rho0, times=t, ensemble_size=8, dt=0.5)
     assert rel_err(avg, got[:, :, :49].mean(axis=0).reshape(-1, 7, 7, order='F')) < 1e-12
+
+
+def _spectra_models(**kw):
+    from oracle import system as S, liouville as Lv
+    g = golden('redfield_dimer')
+    hp = qa.ElectronicHamiltonian(g['in_H1'], bath=qa.DebyeBath(qa.CM_K * 77, 35, 106),
+                                  dipoles=g['in_dipoles'], energy_spread_extra=float(g['extra']))
+    ho = S.ElectronicHamiltonian(g['in_H1'], bath=S.DebyeBath(S.CM_K * 77, 35, 106),
+                                 dipoles=g['in_dipoles'], energy_spread_extra=float(g['extra']))
+    return (qa.HEOMModel(hp, hilbert_subspace='gef', unit_convert=qa.CM_FS, **kw),
+            Lv.HEOMOracleModel(ho, hilbert_subspace='gef', unit_convert=S.CM_FS, **kw))
+
+
+@pytest.mark.parametrize('name,kw', [('lt', dict(level_cutoff=3, K=1)),
+                                     ('nolt', dict(level_cutoff=3, K=1, low_temp_corr=False))])
+def test_heom_third_order_matches_oracle_and_reference(gpu, name, kw, monkeypatch):
+    """HEOM third-order responses (the hierarchy through t1 and t2, per-ADO
+    dipoles, Heisenberg t3): the dense-generator path and the stage-kernel
+    path against the oracle's RK4 on the same schedule (FP64 1e-10) and the
+    reference's tight zvode (tests/golden/heom_spectra.npz, 1e-7 at 1/8 of
+    the model step); mixed 1e-6."""
+    from qspectra_amd.simulate import response
+    g = golden('heom_spectra')
+    mp, mo = _spectra_models(**kw)
+    dt = float(g['time_step']) / 8
+    for geom in ('-++', '++-'):
+        key = '%s_R3_%s' % (name, geom.replace('-', 'm').replace('+', 'p'))
+        (t1, t2, t3), R = qa.third_order_response(mp, 8.5, population_times=[0., 3., 7.],
+                                                  geometry=geom, dt=dt)
+        _, Ro = P.third_order_response(mo, 8.5, population_times=[0., 3., 7.], geometry=geom,
+                                       dt=dt)
+        np.testing.assert_allclose(t1, g['t1'])
+        assert rel_err(R, Ro) < 1e-10, key
+        assert rel_err(R, g[key]) < 1e-7, key
+        _, Rm = qa.third_order_response(mp, 8.5, population_times=[0., 3., 7.], geometry=geom,
+                                        dt=dt, precision='mixed')
+        assert rel_err(Rm, Ro) < 1e-6, key
+        monkeypatch.setattr(response, 'HEOM_DENSE_MAX', 0)       # stage-kernel path
+        _, Rk = qa.third_order_response(mp, 8.5, population_times=[0., 3., 7.], geometry=geom,
+                                        dt=dt)
+        monkeypatch.undo()
+        assert rel_err(Rk, Ro) < 1e-10, key
+    f, X = qa.absorption_spectra(mp, 150.5, dt=dt)
+    np.testing.assert_allclose(f, g[name + '_abs_f'])
+    assert rel_err(X, g[name + '_abs_X']) < 1e-7
+    (f1, t2, f3), X2 = qa.two_dimensional_spectra(mp, 8.5, population_times=[0., 3.], dt=dt)
+    assert X2.shape == (len(f1), 2, len(f3)) and np.isfinite(X2).all()

@@ -274,3 +274,26 @@ def test_fields_match_reference():
     f, X = P.impulsive_probe(md, g['probe_states'][0], 300.5, polarization='xy',
                              include_signal='ESE,ESA', integrator='zvode', **TIGHT)
     assert rel_err(X, g['probe1_X']) < 1e-9
+
+
+def test_heom_spectra_match_reference():
+    """Linear and third-order responses of HEOM models (the hierarchy
+    carried through every interval, dipoles per ADO, bra on ADO 0) against
+    the reference's tight zvode outputs (tests/golden/heom_spectra.npz)."""
+    g = golden('heom_spectra')
+    for name, kw in [('lt', dict(level_cutoff=3, K=1)),
+                     ('nolt', dict(level_cutoff=3, K=1, low_temp_corr=False))]:
+        hm = Lv.HEOMOracleModel(_dimer(), hilbert_subspace='gef', unit_convert=S.CM_FS, **kw)
+        t, Sg = P.linear_response(hm, 'gg->eg->gg', 150.5, integrator='zvode', **TIGHT)
+        assert_allclose(t, g[name + '_lin_t'])
+        assert rel_err(Sg, g[name + '_lin_S']) < 1e-9
+        f, X = P.absorption_spectra(hm, 150.5, integrator='zvode', **TIGHT)
+        assert rel_err(X, g[name + '_abs_X']) < 1e-9
+        for geom in ('-++', '++-'):
+            (t1, t2, t3), R = P.third_order_response(hm, 8.5, population_times=[0., 3., 7.],
+                                                     geometry=geom, integrator='zvode', **TIGHT)
+            key = '%s_R3_%s' % (name, geom.replace('-', 'm').replace('+', 'p'))
+            assert rel_err(R, g[key]) < 1e-9, key
+            (t1, t2, t3), R = P.third_order_response(hm, 8.5, population_times=[0., 3., 7.],
+                                                     geometry=geom, dt=g['time_step'] / 8)
+            assert rel_err(R, g[key]) < 1e-7, key
