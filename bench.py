@@ -42,7 +42,8 @@ def parse_args(argv=None):
                     help='seconds of CPU work allowed for the baseline sample')
     ap.add_argument('--workloads', default='all',
                     help="'all', 'none' or a comma list of ensemble_absorption, "
-                         "adjoint_gradient, heom, pump: the further workloads to measure")
+                         "adjoint_gradient, heom, pump, example_dimer_2d: the further "
+                         "workloads to measure")
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
                          'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
@@ -346,8 +347,32 @@ def pump_leg(torch):
     return out
 
 
+def example_dimer_2d_leg(torch):
+    """two_dimensional_spectra of the reference's example dimer (Redfield and
+    HEOM, g/e/f, 1 ps coherence times, 50 population times to 1 ps, GSB +
+    ESE + ESA) at the default step; FP64 and mixed; end to end from the
+    models."""
+    import qspectra_amd as qa
+    from qspectra_amd import workloads as W
+    red, heom = W.example_dimer_models()
+    out = {'workload': "2-D spectra of the reference's HEOM / Redfield example dimer: "
+                       "g/e/f, t1 = t3 = 1 ps at the model step, 50 population times to 1 ps, "
+                       "GSB+ESE+ESA; HEOM level_cutoff 3, K 1"}
+    for name, model in (('redfield', red), ('heom', heom)):
+        res = {}
+        for prec in ('fp64', 'mixed'):
+            qa.two_dimensional_spectra(model, precision=prec, **W.EXAMPLE_DIMER_2D)   # warm-up
+            res[prec] = _sync_time(torch, lambda: qa.two_dimensional_spectra(
+                model, precision=prec, **W.EXAMPLE_DIMER_2D))
+        X64, X32 = res['fp64'][1][1], res['mixed'][1][1]
+        out[name] = {'seconds_fp64': res['fp64'][0], 'seconds_mixed': res['mixed'][0],
+                     'shape': list(X64.shape),
+                     'rel_err_mixed_vs_fp64': float(np.abs(X32 - X64).max() / np.abs(X64).max())}
+    return out
+
+
 LEGS = (('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', gradient_leg),
-        ('heom', heom_leg), ('pump', pump_leg))
+        ('heom', heom_leg), ('pump', pump_leg), ('example_dimer_2d', example_dimer_2d_leg))
 
 
 def main(argv=None):

@@ -130,3 +130,26 @@ def heom_aggregate7_model(kelvin=300.0, level_cutoff=5):
                                 disorder=100.0 / GAUSSIAN_SD_FWHM)
     return HEOMModel(ham, hilbert_subspace='ge', unit_convert=CM_FS, level_cutoff=level_cutoff,
                      K=1)
+
+
+def example_dimer_models():
+    """The electronic dimer of the reference's example comparing HEOM,
+    Redfield and ZOFE dynamics: site energies 12881 / 12719 cm^-1, coupling
+    120 cm^-1, dipoles (1, 0, 0) and (2 cos 0.3, 2 sin 0.3, 0), Drude bath
+    (35 cm^-1, 106 cm^-1, 77 K); Redfield (imaginary part of C discarded)
+    and HEOM (level_cutoff 3, K = 1, no low-temperature correction) models
+    on the g/e/f space."""
+    from .dynamics.heom import HEOMModel
+    from .dynamics.redfield import RedfieldModel
+    from .hamiltonian import ElectronicHamiltonian
+    ham = ElectronicHamiltonian(np.array([[12881., 120.], [120., 12719.]]), bath=drude_bath(77.0),
+                                dipoles=[[1., 0., 0.], [2. * np.cos(.3), 2. * np.sin(.3), 0.]])
+    red = RedfieldModel(ham, hilbert_subspace='gef', discard_imag_corr=True, unit_convert=CM_FS)
+    heom = HEOMModel(ham, hilbert_subspace='gef', unit_convert=CM_FS, level_cutoff=3,
+                     low_temp_corr=False)
+    return red, heom
+
+
+# the example's 2-D spectra: 1 ps coherence times, 50 population times to 1 ps
+EXAMPLE_DIMER_2D = dict(coherence_time_max=1000, population_times=np.linspace(0, 1000, 50),
+                        geometry='-++', polarization='xxxx', include_signal='GSB,ESE,ESA')
