@@ -104,6 +104,44 @@ def integrate_rk4(L, y0, t, t0=None, save_matrix=None, dt=None):
     return np.stack(out, axis=-2)
 
 
+def integrate_rk4_f(f, y0, t, t0=None, dt=None, save_func=None):
+    """Classical RK4 on y' = f(t, y) (any right-hand side, e.g. the
+    nonlinear ZOFE equation) on the same schedule as integrate_rk4; rows
+    of a multi-dimensional y0 are integrated one by one, as
+    simulate/utils.py:103-109 does.  Returns [..., len(t), *save_shape]."""
+    y0 = np.asarray(y0)
+    if y0.ndim > 1:
+        return np.array([integrate_rk4_f(f, yi, t, t0, dt, save_func) for yi in y0])
+    t = np.asarray(t, float)
+    if t0 is None:
+        t0 = t[0]
+    n, h = rk4_schedule(t, t0, dt)
+    starts = np.concatenate([[t0], t[:-1]])
+    save = save_func if save_func is not None else (lambda v: v)
+    y = np.array(y0, dtype=complex)
+    out = []
+    for s, (ns, hs) in enumerate(zip(n, h)):
+        for k in range(ns):
+            tk = starts[s] + k * hs
+            k1 = f(tk, y)
+            k2 = f(tk + hs / 2, y + (hs / 2) * k1)
+            k3 = f(tk + hs / 2, y + (hs / 2) * k2)
+            k4 = f(tk + hs, y + hs * k3)
+            y = y + (hs / 6) * (k1 + 2 * k2 + 2 * k3 + k4)
+        out.append(save(y))
+    return np.array(out)
+
+
+def _integrate_f(kind, dt, zvode_kw):
+    """integrate(f, y0, t, t0, save_func) for a right-hand-side function:
+    zvode (simulate/utils.py:16-109, rows looped) or the fixed-step RK4."""
+    def run(f, y0, t, t0=None, save_func=None):
+        if kind == 'rk4':
+            return integrate_rk4_f(f, y0, t, t0, dt, save_func)
+        return integrate_zvode(f, y0, t, t0, save_func, **zvode_kw)
+    return run
+
+
 def integrate_rk4_driven(L, fields, y0, t, t0=None, dt=None):
     """RK4 on y' = L y + sum_i f_i(t) (C_i y), fields = [(f_i, C_i)] with
     f_i a function of time; step k of interval s starts at t_{s-1} + k h_s
@@ -206,6 +244,13 @@ def linear_response(model, path, time_max, polarization='xx', initial_state=None
         V = [model.dipole_operator('%s->%s' % (a, b), p, tr)
              for a, b, p, tr in zip(subs[:-1], subs[1:], polarization, '+-')]
         V_rho0 = np.apply_along_axis(V[0].commutator, -1, initial_state)
+        if not hasattr(model, 'evolve_matrix'):
+            # no Heisenberg picture (ZOFE): Schroedinger picture with the
+            # expectation value as save function (response.py:33-40)
+            runf = _integrate_f(integrator, dt, zvode_kw)
+            signal = signal - runf(model.equation_of_motion(sim), V_rho0, t,
+                                   save_func=V[1].expectation_value)
+            continue
         L = model.evolve_matrix(sim, heisenberg_picture=True)
         V_Gt = run(L, -V[1].bra_vector, t)
         signal = signal + np.tensordot(V_rho0, V_Gt, (-1, -1))
@@ -278,6 +323,14 @@ def third_order_response(model, coherence_time_max, population_time_max=None,
         subs = path.split('->')
         V = [model.dipole_operator('%s->%s' % (a, b), p, tr)
              for a, b, p, tr in zip(subs[:-1], subs[1:], polarization, geometry + '-')]
+        if not hasattr(model, 'evolve_matrix'):
+            # Schroedinger picture throughout (response.py:315-332 fallback)
+            runf = _integrate_f(integrator, dt, zvode_kw)
+            eom = [model.equation_of_motion(sub) for sub in subs[1:-1]]
+            V_rho1 = runf(eom[0], V[0].commutator(rho0), t1, save_func=V[1].commutator)
+            V_rho2 = runf(eom[1], V_rho1, t2, t0=0, save_func=V[2].commutator)
+            total = total + runf(eom[2], V_rho2, t3, save_func=V[3].expectation_value)
+            continue
         L = [model.evolve_matrix(s) for s in subs[1:-1]]
         V_rho0 = V[0].commutator(rho0)
         V_rho1 = run(L[0], V_rho0, t1, save_matrix=V[1].commutator_matrix())
@@ -305,14 +358,17 @@ def two_dimensional_spectra(model, coherence_time_max, population_time_max=None,
 def simulate_dynamics(model, initial_state, duration=None, times=None,
                       liouville_subspace='ee', integrator='rk4', dt=None, **zvode_kw):
     """simulate/eom.py:11-26 (psi -> outer(conj psi, psi), then reshape 'F')."""
-    run = _integrator(integrator, dt, zvode_kw)
-    L = model.evolve_matrix(liouville_subspace)
     rho = np.asarray(initial_state)
     if rho.ndim == 1:
         rho = np.outer(rho.conj(), rho)
     y0 = model.density_matrix_to_state_vector(rho, liouville_subspace)
     t = np.arange(0, duration, model.time_step) if times is None else times
-    states = run(L, y0, t)
+    if not hasattr(model, 'evolve_matrix'):
+        runf = _integrate_f(integrator, dt, zvode_kw)
+        states = runf(model.equation_of_motion(liouville_subspace), y0, t)
+        return t, model.state_vector_to_density_matrix(states)
+    run = _integrator(integrator, dt, zvode_kw)
+    states = run(model.evolve_matrix(liouville_subspace), y0, t)
     N = int(np.sqrt(states.shape[-1]))
     return t, states.reshape(-1, N, N, order='F')
 
@@ -323,7 +379,6 @@ def simulate_with_fields(model, pulses, geometry='-+', polarization='xx', time_e
     """simulate/eom.py:79-103: thermal state at t0 = min t_init, derivative
     L y + sum (-1j E (conj for '+'))[V, y], outputs on arange(t0, tf +
     time_extra, time_step) or tf + times; states (or save_func of them)."""
-    L = model.evolve_matrix(liouville_subspace)
     V = [model.dipole_operator(liouville_subspace, p, tr)
          for p, tr in zip(polarization, geometry)]
     info = list(zip(pulses, geometry, V))
@@ -333,12 +388,25 @@ def simulate_with_fields(model, pulses, geometry='-+', polarization='xx', time_e
             E = pulse(tt, model.rw_freq)
             return -1j * (np.conj(E) if trans == '+' else E)
         return fi
-    fields = [(coef(p, tr), Vi.commutator_matrix()) for p, tr, Vi in info]
     rho0 = model.thermal_state(liouville_subspace)
     t0 = min(p.t_init for p in pulses)
     tf = max(p.t_final for p in pulses)
     t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
          else tf + np.asarray(times))
+    if not hasattr(model, 'evolve_matrix'):
+        # a right-hand-side function (ZOFE): eom.py:86-93 literally
+        eom = model.equation_of_motion(liouville_subspace)
+        terms = [(coef(p, tr), Vi) for p, tr, Vi in info]
+
+        def f_fields(tt, y):
+            d = eom(tt, y)
+            for fi, Vi in terms:
+                d = d + fi(tt) * Vi.commutator(y)
+            return d
+        runf = _integrate_f(integrator, dt, zvode_kw)
+        return t, runf(f_fields, rho0, t, t0, save_func)
+    L = model.evolve_matrix(liouville_subspace)
+    fields = [(coef(p, tr), Vi.commutator_matrix()) for p, tr, Vi in info]
     if integrator == 'rk4':
         states = integrate_rk4_driven(L, fields, rho0, t, t0, dt)
         if save_func is not None:

@@ -5,6 +5,7 @@ Run only in the build container, where the reference sources are present:
     python tests/golden/make_golden.py /root/reference            # every file
     python tests/golden/make_golden.py /root/reference fields     # fields.npz only
     python tests/golden/make_golden.py /root/reference heom_spectra
+    python tests/golden/make_golden.py /root/reference zofe
 
 It imports qspectra from the given path and stores plain arrays (inputs and
 outputs) as .npz files.  Nothing here is imported by the tests or by the
@@ -16,6 +17,7 @@ known-answer test pins are written:
 * zvode-integrated trajectories and end-to-end spectra (absorption,
   third-order response, 2-D spectra, isotropic and ensemble averages),
 * linear and third-order responses of HEOM models (dimer, 'gef'),
+* ZOFE (pseudomode) dynamics, absorption, pump and third-order outputs,
 * field-driven trajectories (simulate_with_fields, simulate_pump: Gaussian
   pulses, complex amplitudes, three field terms, unitary / 7-site /
   ensemble models) and the impulsive probe of a pumped state,
@@ -119,6 +121,67 @@ def write_heom_spectra(qs, HEOMModel, dim, extra):
     np.savez_compressed(os.path.join(HERE, 'heom_spectra.npz'), **out)
 
 
+PM_OMEGA = [-500., -200., -90., 1., 21., 60., 80., 130., 200., 300., 400., 500., 600., 800.,
+            1100., 1500.]
+PM_HUANG = [-2.5133e-03, -7.5398e-03, -2.5133e-02, 5.0265e+01, 2.2619e+00, 4.5239e-02,
+            2.7646e-01, 9.2991e-03, 2.2619e-02, 1.5080e-02, 3.0159e-03, 3.5186e-03, 2.8274e-04,
+            1.7593e-03, 4.3982e-04, 4.3982e-04]
+PM_GAMMA = [500., 100., 50., 50., 50., 50., 80., 40., 80., 150., 200., 200., 80., 250., 200.,
+            300.]
+
+
+def pseudomode_arrays(n_sites):
+    """The 16-pseudomode fit of the reference's ZOFE examples, per site."""
+    on = np.ones(n_sites, complex)
+    return [np.array([v[pm] * on for pm in range(len(v))])
+            for v in (PM_OMEGA, PM_GAMMA, PM_HUANG)]
+
+
+def write_zofe(qs, agg):
+    """zofe.npz: ZOFE dynamics, absorption, pump and third-order outputs."""
+    from qspectra.dynamics.zofe import ZOFEModel
+    out = {'pm_Omega': np.array(PM_OMEGA), 'pm_gamma': np.array(PM_GAMMA),
+           'pm_huang': np.array(PM_HUANG)}
+    H1 = np.array([[12881., 120.], [120., 12719.]])
+    d2 = [[1., 0., 0.], [2. * np.cos(.3), 2. * np.sin(.3), 0.]]
+    out['in_H1'], out['in_dipoles'] = H1, np.array(d2)
+    out['extra_dimer'] = extra_for_dt(qs, H1, 1.0)
+    out['extra_agg7'] = extra_for_dt(qs, agg['H1'], 2.0)
+    Om, ga, hu = pseudomode_arrays(2)
+    ham = qs.ElectronicHamiltonian(H1, bath=qs.PseudomodeBath(16, Om, ga, hu), dipoles=d2,
+                                   energy_spread_extra=extra_for_dt(qs, H1, 1.0))
+    m = ZOFEModel(ham, hilbert_subspace='e', unit_convert=qs.CM_FS)
+    t, rho = qs.simulate_dynamics(m, [1, 0], 100, **TIGHT)
+    out['dyn_e_t'], out['dyn_e_rho'] = t, rho
+    out['time_step'] = m.time_step
+    for flags in [(True, False), (False, True), (True, True)]:
+        mf = ZOFEModel(ham, hilbert_subspace='e', unit_convert=qs.CM_FS, ham_hermit=flags[0],
+                       rho_hermit=flags[1])
+        t, rho = qs.simulate_dynamics(mf, [1, 0], 50, **TIGHT)
+        out['dyn_e_flags%d%d_rho' % flags] = rho
+    mg = ZOFEModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS)
+    psi = np.array([0.6, 0.8j, 0., 0.])
+    t, rho = qs.simulate_dynamics(mg, psi, 60, **TIGHT)
+    out['dyn_gef_t'], out['dyn_gef_rho'] = t, rho
+    mge = ZOFEModel(ham, hilbert_subspace='ge', unit_convert=qs.CM_FS)
+    f, X = qs.absorption_spectra(mge, 300.5, **TIGHT)
+    out['abs_f'], out['abs_X'] = f, X
+    pump = qs.GaussianPulse(12800., 30., scale=2e-3, freq_convert=qs.CM_FS)
+    t, y = qs.simulate_pump(mge, pump, 'x', time_extra=40., **TIGHT)
+    out['pump_t'], out['pump_y'] = t, y
+    (t1, t2, t3), R = qs.third_order_response(mg, 6.5, population_times=[0., 4.], **TIGHT)
+    out['R3_t'] = np.array([len(t1), len(t2), len(t3)])
+    out['R3'], out['t1'], out['t2'], out['t3'] = R, t1, t2, t3
+    Om7, ga7, hu7 = pseudomode_arrays(7)
+    ham7 = qs.ElectronicHamiltonian(agg['H1'], bath=qs.PseudomodeBath(16, Om7, ga7, hu7),
+                                    dipoles=agg['dipoles'],
+                                    energy_spread_extra=extra_for_dt(qs, agg['H1'], 2.0))
+    m7 = ZOFEModel(ham7, hilbert_subspace='e', unit_convert=qs.CM_FS)
+    t, rho = qs.simulate_dynamics(m7, [1, 0, 0, 0, 0, 0, 0], 40, **TIGHT)
+    out['agg7_t'], out['agg7_rho'] = t, rho
+    np.savez_compressed(os.path.join(HERE, 'zofe.npz'), **out)
+
+
 def write_manifest(ref_path):
     with open(os.path.join(HERE, 'MANIFEST.txt'), 'w') as fh:
         fh.write('written by tests/golden/make_golden.py from the reference at %s\n' % ref_path)
@@ -144,6 +207,10 @@ def main(ref_path, only=None):
             dipoles=inp['dipoles'], disorder=disorder, random_seed=seed,
             energy_spread_extra=extra)
 
+    if only == 'zofe':
+        write_zofe(qs, aggregate7_inputs())
+        write_manifest(ref_path)
+        return
     if only == 'heom_spectra':
         from qspectra.dynamics.heom import HEOMModel
         dim = dimer_inputs()
@@ -303,6 +370,7 @@ def main(ref_path, only=None):
 
     write_fields(qs, ham, ham7)
     write_heom_spectra(qs, HEOMModel, dim, extra)
+    write_zofe(qs, agg)
     write_manifest(ref_path)
 
 

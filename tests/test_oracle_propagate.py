@@ -297,3 +297,58 @@ def test_heom_spectra_match_reference():
             (t1, t2, t3), R = P.third_order_response(hm, 8.5, population_times=[0., 3., 7.],
                                                      geometry=geom, dt=g['time_step'] / 8)
             assert rel_err(R, g[key]) < 1e-7, key
+
+
+def _pm_arrays(g, n_sites):
+    """The 16 pseudomodes of the reference's ZOFE examples, per site."""
+    on = np.ones(n_sites, complex)
+    return [np.outer(g[k], on) for k in ('pm_Omega', 'pm_gamma', 'pm_huang')]
+
+
+def test_zofe_matches_reference():
+    """ZOFE dynamics (three Hilbert spaces, the ham_hermit / rho_hermit
+    forms), absorption (Schroedinger picture with the expectation value),
+    pump and third-order response against the reference's tight zvode
+    outputs (tests/golden/zofe.npz), by zvode and by the fixed-step RK4."""
+    from oracle import zofe as Z
+    g = golden('zofe')
+    H1, d2 = g['in_H1'], g['in_dipoles']
+
+    def ham(extra, H=H1, d=d2, n=2):
+        return S.ElectronicHamiltonian(H, bath=Z.PseudomodeBath(16, *_pm_arrays(g, n)),
+                                       dipoles=d, energy_spread_extra=extra)
+    extra = float(g['extra_dimer'])
+    m = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='e', unit_convert=S.CM_FS)
+    assert abs(m.time_step - float(g['time_step'])) < 1e-12
+    t, rho = P.simulate_dynamics(m, [1, 0], 100, integrator='zvode', **TIGHT)
+    assert_allclose(t, g['dyn_e_t'])
+    assert rel_err(rho, g['dyn_e_rho']) < 1e-9
+    t, rho = P.simulate_dynamics(m, [1, 0], 100, dt=0.125)
+    assert rel_err(rho, g['dyn_e_rho']) < 1e-8
+    for hh, rh in [(True, False), (False, True), (True, True)]:
+        mf = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='e', unit_convert=S.CM_FS,
+                               ham_hermit=hh, rho_hermit=rh)
+        t, rho = P.simulate_dynamics(mf, [1, 0], 50, integrator='zvode', **TIGHT)
+        assert rel_err(rho, g['dyn_e_flags%d%d_rho' % (hh, rh)]) < 1e-9
+    mg = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='gef', unit_convert=S.CM_FS)
+    t, rho = P.simulate_dynamics(mg, np.array([0.6, 0.8j, 0., 0.]), 60, integrator='zvode',
+                                 **TIGHT)
+    assert rel_err(rho, g['dyn_gef_rho']) < 1e-9
+    mge = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='ge', unit_convert=S.CM_FS)
+    f, X = P.absorption_spectra(mge, 300.5, integrator='zvode', **TIGHT)
+    assert_allclose(f, g['abs_f'])
+    assert rel_err(X, g['abs_X']) < 1e-9
+    pump = S.GaussianPulse(12800., 30., scale=2e-3, freq_convert=S.CM_FS)
+    t, y = P.simulate_pump(mge, pump, 'x', time_extra=40., integrator='zvode', **TIGHT)
+    assert_allclose(t, g['pump_t'])
+    assert rel_err(y, g['pump_y']) < 1e-9
+    (t1, t2, t3), R = P.third_order_response(mg, 6.5, population_times=[0., 4.],
+                                             integrator='zvode', **TIGHT)
+    assert rel_err(R, g['R3']) < 1e-9
+    (t1, t2, t3), R = P.third_order_response(mg, 6.5, population_times=[0., 4.], dt=0.125)
+    assert rel_err(R, g['R3']) < 1e-7
+    m7 = Z.ZOFEOracleModel(ham(float(g['extra_agg7']), golden('redfield_7site')['in_H1'],
+                               golden('redfield_7site')['in_dipoles'], 7),
+                           hilbert_subspace='e', unit_convert=S.CM_FS)
+    t, rho = P.simulate_dynamics(m7, np.eye(7)[0], 40, integrator='zvode', **TIGHT)
+    assert rel_err(rho, g['agg7_rho']) < 1e-9
