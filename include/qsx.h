@@ -271,6 +271,47 @@ int32_t qsx_propagate_driven(int32_t precision, const void* A, int64_t strideA,
                              int32_t B, const int32_t* nsub, const double* h, int32_t nsave,
                              void* out, int32_t* status, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * ZOFE master equation (qspectra/dynamics/zofe.py:121-225): fixed-step RK4
+ * of the density matrix rho (N x N) and the auxiliary operators O[p, s]
+ * (P pseudomodes, S sites), optionally driven by classical fields as
+ * qsx_propagate_driven (the commutator acting on rho and on every O[p, s],
+ * zofe.py:24-34).  One work-group per right-hand side; the state must fit
+ * the LDS (N^2 (1 + P S) <= 8192 elements).
+ *
+ *   H       [B|1][N][N] rotating-frame Hamiltonian of the Hilbert subspace
+ *           (complex128, not multiplied by unit_convert; strideH)
+ *   L       [B|1][S][N][N] coupling operators (the negated system-bath
+ *           couplings, zofe.py:214-215; strideL)
+ *   Gamma, w  [B|1][P][S] Omega^2 huang and i Omega + gamma (strideG)
+ *   flags   bit 0 ham_hermit, bit 1 rho_hermit (zofe.py:170-186)
+ *   nfield, V [B|1][nfield][N][N], F [B|1][nF][3][nfield]: field terms
+ *   y0      [B][R][N^2 (1 + P S)] states in the reference's layout (vec of
+ *           rho, then vec of O with shape (P, S, N, N), both column-major)
+ *   nsub, h, nsave  schedule as qsx_propagate
+ *   K, proj [B|1][K][N^2] rows applied to vec(rho) at every save (for
+ *           expectation values, zofe.py:36-39), or K = 0: full states
+ *   out     [B][R][nsave][K or size]
+ *   workspace  qsx_zofe_workspace_bytes(B, R, N, S, P) bytes
+ *   status  1 non-finite state, 2 fewer field samples than steps.
+ * Replaces the zvode integration of ZOFEModel.equation_of_motion
+ * (simulate/utils.py:16-109 with zofe.py:204-225). */
+int64_t qsx_zofe_workspace_bytes(int32_t B, int32_t R, int32_t N, int32_t S, int32_t P);
+int32_t qsx_zofe_propagate(int32_t B, int32_t R, int32_t N, int32_t S, int32_t P, const void* H,
+                           int64_t strideH, const void* L, int64_t strideL, const void* Gamma,
+                           const void* w, int64_t strideG, int32_t flags, double unit_convert,
+                           int32_t nfield, const void* V, int64_t strideV, const void* F,
+                           int64_t strideF, int64_t nF, const void* y0, const int32_t* nsub,
+                           const double* h, int32_t nsave, int32_t K, const void* proj,
+                           int64_t strideP, void* out, void* workspace, int32_t* status,
+                           void* stream);
+/* eom(t, y) of the ZOFE model (zofe.py:211-225) for R states per sample:
+ * out [B][R][size] in the reference's layout. */
+int32_t qsx_zofe_apply(int32_t B, int32_t R, int32_t N, int32_t S, int32_t P, const void* H,
+                       int64_t strideH, const void* L, int64_t strideL, const void* Gamma,
+                       const void* w, int64_t strideG, int32_t flags, double unit_convert,
+                       const void* y, void* out, void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

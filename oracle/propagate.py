@@ -312,11 +312,12 @@ def third_order_response(model, coherence_time_max, population_time_max=None,
                          include_signal=None, integrator='rk4', dt=None, **zvode_kw):
     """response.py:269-337 (Schroedinger t1,t2 + Heisenberg t3)."""
     run = _integrator(integrator, dt, zvode_kw)
-    dt = model.time_step
-    t1 = np.arange(0, coherence_time_max, dt)
-    t2 = (np.arange(0, population_time_max, dt) if population_times is None
+    runf = _integrate_f(integrator, dt, zvode_kw)
+    grid = model.time_step
+    t1 = np.arange(0, coherence_time_max, grid)
+    t2 = (np.arange(0, population_time_max, grid) if population_times is None
           else np.asarray(population_times))
-    t3 = np.arange(0, coherence_time_max, dt)
+    t3 = np.arange(0, coherence_time_max, grid)
     rho0 = model.thermal_state('gg')
     total = 0
     for path in parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal):
@@ -325,7 +326,6 @@ def third_order_response(model, coherence_time_max, population_time_max=None,
              for a, b, p, tr in zip(subs[:-1], subs[1:], polarization, geometry + '-')]
         if not hasattr(model, 'evolve_matrix'):
             # Schroedinger picture throughout (response.py:315-332 fallback)
-            runf = _integrate_f(integrator, dt, zvode_kw)
             eom = [model.equation_of_motion(sub) for sub in subs[1:-1]]
             V_rho1 = runf(eom[0], V[0].commutator(rho0), t1, save_func=V[1].commutator)
             V_rho2 = runf(eom[1], V_rho1, t2, t0=0, save_func=V[2].commutator)

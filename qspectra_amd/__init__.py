@@ -9,13 +9,14 @@ from ._native import (NativeUnavailable, NativeError, require_native, load_libra
                       QSX_FP64, QSX_MIXED)
 
 __version__ = '0.1.0'
-from .bath import DebyeBath, ArbitraryBath, UncoupledBath
+from .bath import DebyeBath, ArbitraryBath, UncoupledBath, PseudomodeBath
 from .constants import CM_FS, CM_K, GAUSSIAN_SD_FWHM
 from .dynamics.liouville_space import (n_excitations, matrix_to_ket_vec, ket_vec_to_matrix,
                                        matrix_to_bra_vec)
 from .dynamics.redfield import RedfieldModel
 from .dynamics.unitary import UnitaryModel
 from .dynamics.heom import HEOMModel
+from .dynamics.zofe import ZOFEModel
 from .hamiltonian import Hamiltonian, ElectronicHamiltonian
 from .operator_tools import (unit_vec, basis_transform_operator, basis_transform_vector,
                              all_states)
@@ -29,7 +30,7 @@ from .simulate.response import (linear_response, absorption_spectra, third_order
                                 THIRD_ORDER_PATHWAYS)
 from .simulate.utils import fourier_transform, integrate, bound_signal, IntegratorError
 
-__all__ = ['DebyeBath', 'ArbitraryBath', 'UncoupledBath', 'CM_FS', 'CM_K', 'GAUSSIAN_SD_FWHM',
+__all__ = ['DebyeBath', 'ArbitraryBath', 'UncoupledBath', 'PseudomodeBath', 'ZOFEModel', 'CM_FS', 'CM_K', 'GAUSSIAN_SD_FWHM',
            'Hamiltonian', 'ElectronicHamiltonian', 'n_excitations', 'matrix_to_ket_vec',
            'ket_vec_to_matrix', 'matrix_to_bra_vec', 'unit_vec', 'basis_transform_operator',
            'basis_transform_vector', 'all_states', 'polarization_vector',

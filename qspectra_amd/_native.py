@@ -39,7 +39,7 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
 
 # name -> argtypes (restype int32 unless listed in RESTYPES)
 RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64,
-            'qsx_adjoint_workspace_bytes': c_i64}
+            'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -67,6 +67,12 @@ SIGNATURES = {
     'qsx_propagate_driven': [c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp, c_i64, c_i32,
                              c_vp, c_i64, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_vp, c_vp, c_vp],
+    'qsx_zofe_workspace_bytes': [c_i32, c_i32, c_i32, c_i32, c_i32],
+    'qsx_zofe_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp,
+                           c_vp, c_i64, c_i32, c_dbl, c_i32, c_vp, c_i64, c_vp, c_i64, c_i64,
+                           c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    'qsx_zofe_apply': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp,
+                       c_i64, c_i32, c_dbl, c_vp, c_vp, c_vp, c_vp],
 }
 
 
@@ -126,7 +132,10 @@ def to_device(x, dtype=torch.complex128, dev=None):
     dev = device() if dev is None else dev
     if isinstance(x, torch.Tensor):
         return x.to(device=dev, dtype=dtype).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(x)).to(device=dev, dtype=dtype).contiguous()
+    x = np.ascontiguousarray(x)
+    if not x.flags.writeable:              # broadcast views: torch wants writable memory
+        x = x.copy()
+    return torch.as_tensor(x).to(device=dev, dtype=dtype).contiguous()
 
 
 # --------------------------------------------------------------- wrappers
@@ -496,3 +505,71 @@ def heom_propagate(eom, y0, nsub, h, precision=QSX_FP64, want_final=False, strea
             stream_ptr(stream))
     check(st, 'qsx_heom_propagate')
     return out, yfin, status
+
+
+def _zofe_ws(lib, B, R, eom, dev):
+    nbytes = lib.qsx_zofe_workspace_bytes(B, R, eom.N, eom.S, eom.P)
+    if nbytes < 0:
+        raise NativeError('invalid ZOFE workspace request')
+    return torch.empty((nbytes + 15) // 16, dtype=torch.complex128, device=dev)
+
+
+def _zofe_common(eom, B):
+    strideH = 0 if eom.H.shape[0] == 1 else eom.N * eom.N
+    strideL = 0 if eom.L.shape[0] == 1 else eom.S * eom.N * eom.N
+    strideG = 0 if eom.Gamma.shape[0] == 1 else eom.P * eom.S
+    return (ptr(eom.H), strideH, ptr(eom.L), strideL, ptr(eom.Gamma), ptr(eom.w), strideG,
+            eom.flags, float(eom.unit_convert))
+
+
+def zofe_apply(eom, y, stream=None):
+    """qsx_zofe_apply: eom(t, y) for y [B, R, size] complex128 (reference
+    layout); returns the derivatives in the same layout."""
+    lib = require_native()
+    B, R, size = y.shape
+    y = y.contiguous()
+    out = torch.empty_like(y)
+    ws = _zofe_ws(lib, B, R, eom, y.device)
+    st = lib.qsx_zofe_apply(B, R, eom.N, eom.S, eom.P, *_zofe_common(eom, B), ptr(y), ptr(out),
+                            ptr(ws), stream_ptr(stream))
+    check(st, 'qsx_zofe_apply')
+    return out
+
+
+def zofe_propagate(eom, y0, nsub, h, proj=None, V=None, F=None, stream=None):
+    """qsx_zofe_propagate: y0 [B, R, size] (reference layout), device
+    schedule (nsub int32, h float64); proj [B|1, K, N*N] rows on vec(rho)
+    or None (full states); field terms V [B|1, P, N, N] with samples
+    F [B|1, nF, 3, P].  Returns (out [B, R, nsave, K or size], status)."""
+    lib = require_native()
+    B, R, size = y0.shape
+    if size != eom.size:
+        raise ValueError('ZOFE states have %d elements, the model %d' % (eom.size, size))
+    nsave = int(nsub.shape[0])
+    y0 = y0.contiguous()
+    K, strideP = 0, 0
+    if proj is not None:
+        proj = proj.contiguous()
+        K = int(proj.shape[1])
+        strideP = 0 if proj.shape[0] == 1 else K * eom.N * eom.N
+    nfield, strideV, strideF, nF = 0, 0, 0, 0
+    if V is not None:
+        V, F = V.contiguous(), F.contiguous()
+        nfield = int(V.shape[1])
+        strideV = 0 if V.shape[0] == 1 else nfield * eom.N * eom.N
+        nF = int(F.shape[1])
+        strideF = 0 if F.shape[0] == 1 else nF * 3 * nfield
+    out = torch.empty((B, R, nsave, K if K else size), dtype=torch.complex128, device=y0.device)
+    ws = _zofe_ws(lib, B, R, eom, y0.device)
+    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    label = 'zofe N=%d S=%d P=%d R=%d B=%d fields=%d nsave=%d' % (eom.N, eom.S, eom.P, R, B,
+                                                                   nfield, nsave)
+    def go():
+        return lib.qsx_zofe_propagate(B, R, eom.N, eom.S, eom.P, *_zofe_common(eom, B), nfield,
+                                      ptr(V), strideV, ptr(F), strideF, nF, ptr(y0), ptr(nsub),
+                                      ptr(h), nsave, K, ptr(proj), strideP, ptr(out), ptr(ws),
+                                      ptr(status), stream_ptr(stream))
+    with timing.launch(label, stream, go):
+        st = go()
+    check(st, 'qsx_zofe_propagate')
+    return out, status

@@ -99,3 +99,17 @@ def debye_parameters(bath):
         return 1.0, 0.0, 1.0
     raise NotImplementedError('the GPU Redfield builder supports DebyeBath and UncoupledBath; '
                               'got %s' % type(bath).__name__)
+
+
+class PseudomodeBath(Bath):
+    """Bath given by pseudomodes: Lorentzians of the bath correlation
+    spectrum centred at Omega with widths gamma and weights huang
+    (qspectra/bath.py:105-145); arrays of shape (numb_pm, n_sites) when the
+    sites have separate baths.  It defines no temperature (thermal states
+    are ground states), and is the bath of the ZOFE model."""
+
+    def __init__(self, numb_pm, Omega, gamma, huang):
+        self.numb_pm = numb_pm
+        self.Omega = Omega
+        self.gamma = gamma
+        self.huang = huang

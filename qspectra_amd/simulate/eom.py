@@ -9,9 +9,11 @@ from ..engine import rk4_schedule, schedule_tensors
 from ..lti import split_diagonal
 from ..pulse import sample_field, field_rate
 from .decorators import optional_ensemble_average, optional_2nd_order_isotropic_average
-from .utils import propagate, propagate_heom, IntegratorError, ZVODE_OPTIONS
+from .utils import (propagate, propagate_heom, propagate_zofe, zofe_projection, IntegratorError,
+                    ZVODE_OPTIONS)
 from ..dynamics.liouville_space import LinearMap
 from ..dynamics.heom import DeviceHEOM
+from ..dynamics.zofe import DeviceZOFE
 
 
 @optional_ensemble_average
@@ -28,6 +30,19 @@ def _simulate_dynamics(dynamical_model, initial_state, duration, times, liouvill
     t = np.arange(0, duration, model.time_step) if times is None else np.asarray(times)
     kw = {k: integrate_kwargs[k] for k in ('dt', 'precision', 'rtol', 'schedule')
           if k in integrate_kwargs}
+    if isinstance(eom, DeviceZOFE):
+        # rho and the auxiliary operators; density matrices from rho
+        B = eom.batch
+        y = _native.to_device(np.broadcast_to(y0, (B, 1, y0.shape[-1])).copy())
+        proj, scalar = (zofe_projection(eom, save_func) if isinstance(save_func, LinearMap)
+                        else (None, False))
+        out = propagate_zofe(eom, y, t, dt=kw.get('dt'), proj=proj,
+                             rtol=kw.get('rtol', 1e-6))[:, 0].cpu().numpy()   # [B, nt, *]
+        if proj is not None:
+            return t, out[..., 0] if scalar else out
+        if save_func is not None:
+            return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in out])
+        return t, np.stack([model.state_vector_to_density_matrix(ob) for ob in out])
     if isinstance(eom, DeviceHEOM):
         # hierarchy: the reduced density matrix is ADO 0 of the state
         B = eom.batch
@@ -117,7 +132,10 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     eom = model.equation_of_motion(liouville_subspace)
     if isinstance(eom, DeviceHEOM):
         raise NotImplementedError('field-driven propagation of hierarchy (HEOM) models is not '
-                                  'implemented; use a Redfield or unitary model')
+                                  'implemented; use a Redfield, unitary or ZOFE model')
+    if isinstance(eom, DeviceZOFE):
+        return _zofe_with_fields(model, eom, pulses, geometry, polarization, time_extra, times,
+                                 save_func, dt, rtol)
     pairs = list(zip(polarization, geometry))
     terms = list(zip(pulses, geometry))[:len(pairs)]
     pairs = pairs[:len(terms)]
@@ -186,6 +204,65 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
         states = states.cpu().numpy()
         return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in states])
     return t, states                       # device tensor: ensembles average on the GPU
+
+
+def _zofe_with_fields(model, eom, pulses, geometry, polarization, time_extra, times,
+                      save_func, dt, rtol):
+    """simulate_with_fields of a ZOFE model: the field terms act through
+    the ZOFE commutator (on rho and on every auxiliary operator) inside the
+    ZOFE kernel."""
+    from .response import _members
+    pairs = list(zip(polarization, geometry))
+    terms = list(zip(pulses, geometry))[:len(pairs)]
+    pairs = pairs[:len(terms)]
+    if not terms:
+        raise ValueError('at least one pulse is needed')
+    if len(terms) > _native.MAX_FIELD_TERMS:
+        raise ValueError('at most %d field terms are supported' % _native.MAX_FIELD_TERMS)
+    hams = _members(model)
+    V = np.stack([np.stack([np.asarray(h.dipole_operator(model.hilbert_subspace, pol, trans),
+                                       dtype=complex) for pol, trans in pairs])
+                  for h in hams])                                      # [B, P, N, N]
+    if all(np.array_equal(V[0], v) for v in V[1:]):
+        V = V[:1]
+    rho0 = np.stack([model.density_matrix_to_state_vector(
+        h.thermal_state(model.hilbert_subspace)) for h in hams])
+    t0 = min(p.t_init for p in pulses)
+    tf = max(p.t_final for p in pulses)
+    t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
+         else tf + np.asarray(times, dtype=float))
+    t = np.atleast_1d(np.asarray(t, dtype=float))
+    if np.any(t < t0):
+        raise ValueError('output times must not precede the first pulse (t=%g)' % t0)
+    rw_freq = model.rw_freq
+    rate = 0.0
+    for i, (pulse, _) in enumerate(terms):
+        amp, r = field_rate(pulse, rw_freq)
+        rate += amp * float(np.abs(V[:, i]).sum(axis=-1).max()) * 2 + r
+    if dt is None:
+        from .utils import zofe_step
+        dt = zofe_step(eom, float(t[-1]) - float(t0), rtol, rate)
+        gaps = np.diff(np.concatenate([[t0], t]))
+        gaps = gaps[gaps > 0]
+        if gaps.size:
+            dt = gaps.min() if dt is None else min(dt, float(gaps.min()))
+    n, h = rk4_schedule(t, t0, dt)
+    tg, hg = stage_times(t, t0, n, h)
+    F = field_samples(terms, rw_freq, tg, hg)
+    if F.shape[0] == 0:
+        F = np.zeros((1, 3, len(terms)), dtype=complex)
+    dev = eom.H.device
+    proj, scalar = (zofe_projection(eom, save_func) if isinstance(save_func, LinearMap)
+                    else (None, False))
+    out = propagate_zofe(eom, _native.to_device(rho0[:, None, :], dev=dev), t, t0, dt,
+                         proj=proj, V=_native.to_device(V, dev=dev),
+                         F=_native.to_device(F[None], dev=dev))[:, 0]      # [B, nt, *]
+    if proj is not None:
+        return t, out[..., 0] if scalar else out
+    if save_func is not None:
+        states = out.cpu().numpy()
+        return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in states])
+    return t, out
 
 
 def simulate_with_fields(dynamical_model, pulses, geometry='-+', polarization='xx',

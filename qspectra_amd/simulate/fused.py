@@ -71,8 +71,9 @@ class ThirdOrderPlan(object):
         if schedule == 'rk4':
             raise PlanNotApplicable("the fused plan needs the RK4 propagator; use "
                                     "plan='pathways' for schedule='rk4'")
-        if hasattr(model, 'ado_count'):
-            raise PlanNotApplicable('hierarchy (HEOM) models are evaluated pathway by pathway')
+        if hasattr(model, 'ado_count') or hasattr(model, 'oop_shape'):
+            raise PlanNotApplicable('hierarchy (HEOM) and ZOFE models are evaluated pathway '
+                                    'by pathway')
         self.model = model
         self.t1, self.t2, self.t3 = third_order_grids(model, coherence_time_max,
                                                       population_time_max, population_times)

@@ -23,7 +23,7 @@ import torch
 from .. import _native
 from .decorators import (optional_ensemble_average, optional_2nd_order_isotropic_average,
                          optional_4th_order_isotropic_average)
-from .utils import propagate, propagate_heom, fourier_transform
+from .utils import propagate, propagate_heom, propagate_zofe, fourier_transform
 from ..dynamics.heom import DeviceHEOM
 from ..utils import ZeroArray
 
@@ -84,10 +84,71 @@ def _expand(x, B):
     return x if x.shape[0] == B else x.expand((B,) + tuple(x.shape[1:])).contiguous()
 
 
+def _zofe_ops(model, polarization, transition):
+    """Dipole operator matrices of every member of a ZOFE model, [B|1, N, N]
+    (one shared copy when the members' dipoles agree)."""
+    hams = _members(model)
+    ops = np.stack([np.asarray(h.dipole_operator(model.hilbert_subspace, polarization,
+                                                 transition), dtype=complex) for h in hams])
+    return ops[:1] if all(np.array_equal(ops[0], o) for o in ops[1:]) else ops
+
+
+def _zofe_commutator(model, states, V):
+    """The ZOFE commutator [V, X] on rho and on every O[p, s] of device
+    states [B, ..., size] (reference layout), V [B|1, N, N] on the device:
+    batched matrix products on the GPU."""
+    P, S, N, _ = model.oop_shape
+    B = states.shape[0]
+    lead = states.shape[1:-1]
+    x = states.reshape(B, -1, states.shape[-1])
+    n2 = N * N
+    # column-major blocks: rho = vec^T, O[p, s, i, j] = v[p + P (s + S (i + N j))]
+    rho = x[..., :n2].reshape(B, -1, N, N).transpose(-1, -2)
+    oop = x[..., n2:].reshape(B, -1, N, N, S, P).permute(0, 1, 5, 4, 3, 2)   # [B, R, P, S, i, j]
+    Vb = V.reshape(V.shape[0], 1, N, N)
+    crho = torch.matmul(Vb, rho) - torch.matmul(rho, Vb)
+    Vo = V.reshape(V.shape[0], 1, 1, 1, N, N)
+    coop = torch.matmul(Vo, oop) - torch.matmul(oop, Vo)
+    out = torch.cat([crho.transpose(-1, -2).reshape(B, x.shape[1], n2),
+                     coop.permute(0, 1, 5, 4, 3, 2).reshape(B, x.shape[1], -1)], dim=-1)
+    return out.reshape((B,) + tuple(lead) + (states.shape[-1],))
+
+
+def _zofe_expectation_rows(V):
+    """tr(V rho) as rows on vec(rho): V[i, j] at index j + N i, [B|1, 1, N*N]."""
+    return _dev(V.reshape(V.shape[0], 1, -1))
+
+
+def _zofe_linear_response(model, liouv_space_path, time_max, initial_state, polarization, kw):
+    """Linear response of a ZOFE model in the Schroedinger picture with the
+    expectation value as save function (response.py:33-40): (t, S [B, ..., nt])."""
+    subspaces = liouv_space_path.split('->')
+    B = len(_members(model))
+    if initial_state is None:
+        initial_state = np.stack([model.density_matrix_to_state_vector(
+            h.thermal_state(model.hilbert_subspace)) for h in _members(model)])
+    else:
+        initial_state = np.broadcast_to(np.asarray(initial_state),
+                                        (B,) + np.shape(initial_state))
+    t = np.arange(0, time_max, model.time_step)
+    V0 = _dev(_zofe_ops(model, polarization[0], '+'))
+    V1 = _zofe_ops(model, polarization[1], '-')
+    init = _dev(np.ascontiguousarray(initial_state))
+    lead = init.shape[1:-1]
+    y0 = _zofe_commutator(model, init, V0).reshape(B, -1, model.state_size).contiguous()
+    eom = model.equation_of_motion(subspaces[1])
+    out = propagate_zofe(eom, y0, t, dt=kw.get('dt'), proj=_zofe_expectation_rows(V1),
+                         rtol=kw.get('rtol', 1e-6))                      # [B, R, nt, 1]
+    return t, (-out[..., 0]).reshape((B,) + tuple(lead) + (len(t),)).cpu().numpy()
+
+
 def _batched_linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,
                              polarization='xx', **integrate_kwargs):
     """Linear response of a (possibly batched) model: (t, S [B, ..., nt])."""
     model = dynamical_model
+    if hasattr(model, 'oop_shape'):
+        return _zofe_linear_response(model, liouv_space_path, time_max, initial_state,
+                                     polarization, _propagate_kw(integrate_kwargs))
     kw = _propagate_kw(integrate_kwargs)
     subspaces = liouv_space_path.split('->')
     B = len(_members(model))
@@ -315,6 +376,35 @@ def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
     return (t1, t2, t3), total.cpu().numpy()
 
 
+def _zofe_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
+    """Third-order response of a ZOFE model: the Schroedinger picture in
+    every interval (no Heisenberg picture, response.py:315-332), the
+    commutators applied on the GPU between intervals, all (t1, t2) states
+    right-hand sides of the t3 launch with the expectation value saved."""
+    B = len(_members(model))
+    dt, rtol = kw.get('dt'), kw.get('rtol', 1e-6)
+    rho0 = np.stack([model.density_matrix_to_state_vector(
+        h.thermal_state(model.hilbert_subspace)) for h in _members(model)])
+    trans = geometry + '-'
+    size = model.state_size
+    total = None
+    for path in paths:
+        s = path.split('->')
+        V = [_dev(_zofe_ops(model, polarization[i], trans[i])) for i in range(3)]
+        V3 = _zofe_ops(model, polarization[3], trans[3])
+        eoms = [model.equation_of_motion(x) for x in s[1:4]]
+        y0 = _zofe_commutator(model, _dev(rho0[:, None, :]), V[0])           # [B, 1, size]
+        st1 = propagate_zofe(eoms[0], y0.contiguous(), t1, dt=dt, rtol=rtol)   # [B, 1, n1, size]
+        X1 = _zofe_commutator(model, st1[:, 0], V[1]).contiguous()           # [B, n1, size]
+        st2 = propagate_zofe(eoms[1], X1, t2, t0=0, dt=dt, rtol=rtol)       # [B, n1, n2, size]
+        X2 = _zofe_commutator(model, st2, V[2]).reshape(B, -1, size).contiguous()
+        sig = propagate_zofe(eoms[2], X2, t3, dt=dt, proj=_zofe_expectation_rows(V3),
+                             rtol=rtol)[..., 0]                              # [B, n1*n2, n3]
+        sig = sig.reshape(B, len(t1), len(t2), len(t3))
+        total = sig if total is None else total + sig
+    return (t1, t2, t3), total.cpu().numpy()
+
+
 def _pathway_by_pathway(model, coherence_time_max, population_time_max, population_times,
                         geometry, polarization, include_signal, kw):
     """Each pathway evaluated on its own, in the reference's order."""
@@ -322,6 +412,10 @@ def _pathway_by_pathway(model, coherence_time_max, population_time_max, populati
                                    population_times)
     if hasattr(model, 'ado_count'):
         return _heom_pathways(model, t1, t2, t3,
+                              _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal),
+                              geometry, polarization, kw)
+    if hasattr(model, 'oop_shape'):
+        return _zofe_pathways(model, t1, t2, t3,
                               _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal),
                               geometry, polarization, kw)
     B = len(_members(model))

@@ -25,6 +25,7 @@ from ..lti import split_diagonal
 from ..schedules import Stage, stepper_operator, _mode
 from ..dynamics.liouville_space import DeviceEOM, LinearMap
 from ..dynamics.heom import DeviceHEOM
+from ..dynamics.zofe import DeviceZOFE
 
 ZVODE_OPTIONS = ('rtol', 'atol', 'nsteps', 'method', 'with_jacobian', 'lband', 'uband',
                  'order', 'first_step', 'min_step', 'max_step')
@@ -136,6 +137,58 @@ def propagate_heom(eom, y0, t, t0=None, dt=None, precision='fp64', full_states=F
     return torch.stack(states, dim=1)
 
 
+def zofe_step(eom, span, rtol=1e-6, field_rate=0.0):
+    """RK4 step for the ZOFE equation with estimated global error <= rtol
+    over ``span``: rate bound unit * (2 max row sum |H| + max |w| + 2 S
+    sum_p max_s |Gamma / w|) (+ the fields' rate), as step_size does for a
+    Liouvillian."""
+    H = eom.H.abs().sum(dim=-1).max().item()
+    w = eom.w.abs()
+    rate = abs(eom.unit_convert) * (2 * H + float(w.max().item())
+                                    + 2 * eom.S * float((eom.Gamma.abs() / w).max(dim=-1)
+                                                        .values.sum().item()))
+    rate += field_rate
+    if rate == 0 or span <= 0:
+        return None
+    return (120.0 * rtol / (rate * span)) ** 0.25 / rate
+
+
+def propagate_zofe(eom, y0, t, t0=None, dt=None, proj=None, V=None, F=None, rtol=1e-6,
+                   field_rate=0.0):
+    """ZOFE propagation: y0 [B, R, size] complex128 device tensor (the
+    reference's state layout); proj [B|1, K, N*N] rows on vec(rho) or None
+    (full states); field terms V / F as _native.zofe_propagate.  Returns
+    out [B, R, len(t), K or size]."""
+    t = np.asarray(t, dtype=float).reshape(-1)
+    if t0 is None:
+        t0 = float(t[0])
+    if dt is None:
+        dt = zofe_step(eom, float(t[-1]) - float(t0), rtol, field_rate)
+        gaps = np.diff(np.concatenate([[t0], t]))
+        gaps = gaps[gaps > 0]
+        if gaps.size:
+            dt = gaps.min() if dt is None else min(dt, float(gaps.min()))
+    nsub, h = schedule_tensors(t, t0, dt, y0.device)
+    out, status = _native.zofe_propagate(eom, y0, nsub, h, proj=proj, V=V, F=F)
+    if int(status.item()) != 0:
+        raise IntegratorError('integration failed: non-finite state between t=%g and t=%g'
+                              % (t0, t[-1]))
+    return out
+
+
+def zofe_projection(eom, save_func):
+    """The rows of a LinearMap save function on a ZOFE state, restricted to
+    vec(rho), as [1, K, N*N] (None when the map also reads the auxiliary
+    operators), and whether it returns a scalar."""
+    mat = np.asarray(save_func.matrix)
+    scalar = mat.ndim == 1
+    mat = np.atleast_2d(mat)
+    n2 = eom.N * eom.N
+    if np.any(mat[:, n2:] != 0):
+        return None, scalar
+    return _native.to_device(mat[None, :, :n2].copy()), scalar
+
+
 def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=None, **kwargs):
     """Solves dy/dt = f(t, y) from y(t0) = y0 and returns save_func(y(t_i))
     for every t_i: shape [..., len(t), *save_shape] for y0 of shape [..., M]
@@ -145,6 +198,8 @@ def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=N
     (the GPU needs the operator, not a Python callback)."""
     if isinstance(f, DeviceHEOM):
         return _integrate_heom(f, y0, t, t0, save_func, kwargs)
+    if isinstance(f, DeviceZOFE):
+        return _integrate_zofe(f, y0, t, t0, save_func, kwargs)
     if not isinstance(f, DeviceEOM):
         raise TypeError('integrate needs an equation of motion from a qspectra_amd model '
                         '(DynamicalModel.equation_of_motion); arbitrary Python callables '
@@ -188,6 +243,39 @@ def integrate(f, y0, t, t0=None, method_name='zvode', f_params=None, save_func=N
     if host_save is not None:
         saved = [host_save(v) for v in result.reshape(-1, M)]
         saved = np.asarray(saved)
+        result = saved.reshape(lead + (nt,) + saved.shape[1:])
+    return result
+
+
+def _integrate_zofe(f, y0, t, t0, save_func, kwargs):
+    dt = kwargs.pop('dt', None)
+    rtol = kwargs.pop('rtol', 1e-6)
+    kwargs.pop('precision', None)
+    kwargs.pop('schedule', None)
+    unknown = set(kwargs) - set(ZVODE_OPTIONS)
+    if unknown:
+        raise TypeError('unexpected integrate options: %s' % sorted(unknown))
+    y0 = np.asarray(y0)
+    if y0.shape[-1] != f.size:
+        raise ValueError('y0 has %d components, the ZOFE state %d' % (y0.shape[-1], f.size))
+    lead = y0.shape[:-1]
+    B = f.batch
+    y = y0.reshape(B, -1, f.size) if B > 1 else y0.reshape(1, -1, f.size)
+    proj, scalar, host_save = None, False, None
+    if isinstance(save_func, LinearMap):
+        proj, scalar = zofe_projection(f, save_func)
+        if proj is None:
+            host_save = save_func
+    elif save_func is not None:
+        host_save = save_func
+    out = propagate_zofe(f, _native.to_device(y), t, t0, dt, proj=proj, rtol=rtol)
+    out = out.cpu().numpy()                                     # [B, R, nt, K or size]
+    nt = out.shape[2]
+    if scalar and proj is not None:
+        out = out[..., 0]
+    result = out.reshape(lead + (nt,) + out.shape[3:])
+    if host_save is not None:
+        saved = np.asarray([host_save(v) for v in result.reshape(-1, f.size)])
         result = saved.reshape(lead + (nt,) + saved.shape[1:])
     return result
 

@@ -151,17 +151,17 @@ def write_zofe(qs, agg):
     ham = qs.ElectronicHamiltonian(H1, bath=qs.PseudomodeBath(16, Om, ga, hu), dipoles=d2,
                                    energy_spread_extra=extra_for_dt(qs, H1, 1.0))
     m = ZOFEModel(ham, hilbert_subspace='e', unit_convert=qs.CM_FS)
-    t, rho = qs.simulate_dynamics(m, [1, 0], 100, **TIGHT)
+    t, rho = qs.simulate_dynamics(m, [1, 0], 100.5, **TIGHT)
     out['dyn_e_t'], out['dyn_e_rho'] = t, rho
     out['time_step'] = m.time_step
     for flags in [(True, False), (False, True), (True, True)]:
         mf = ZOFEModel(ham, hilbert_subspace='e', unit_convert=qs.CM_FS, ham_hermit=flags[0],
                        rho_hermit=flags[1])
-        t, rho = qs.simulate_dynamics(mf, [1, 0], 50, **TIGHT)
+        t, rho = qs.simulate_dynamics(mf, [1, 0], 50.5, **TIGHT)
         out['dyn_e_flags%d%d_rho' % flags] = rho
     mg = ZOFEModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS)
     psi = np.array([0.6, 0.8j, 0., 0.])
-    t, rho = qs.simulate_dynamics(mg, psi, 60, **TIGHT)
+    t, rho = qs.simulate_dynamics(mg, psi, 60.5, **TIGHT)
     out['dyn_gef_t'], out['dyn_gef_rho'] = t, rho
     mge = ZOFEModel(ham, hilbert_subspace='ge', unit_convert=qs.CM_FS)
     f, X = qs.absorption_spectra(mge, 300.5, **TIGHT)
@@ -177,7 +177,7 @@ def write_zofe(qs, agg):
                                     dipoles=agg['dipoles'],
                                     energy_spread_extra=extra_for_dt(qs, agg['H1'], 2.0))
     m7 = ZOFEModel(ham7, hilbert_subspace='e', unit_convert=qs.CM_FS)
-    t, rho = qs.simulate_dynamics(m7, [1, 0, 0, 0, 0, 0, 0], 40, **TIGHT)
+    t, rho = qs.simulate_dynamics(m7, [1, 0, 0, 0, 0, 0, 0], 41., **TIGHT)
     out['agg7_t'], out['agg7_rho'] = t, rho
     np.savez_compressed(os.path.join(HERE, 'zofe.npz'), **out)
 

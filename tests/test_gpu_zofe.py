@@ -1,0 +1,109 @@
+"""ZOFE master equation (qspectra/dynamics/zofe.py) on the GPU: the ZOFE
+kernel's right-hand side against the oracle's restatement (1e-12), RK4
+propagation against the oracle's RK4 on the same schedule (FP64 1e-10), and
+dynamics, absorption, pump and third-order outputs against the reference's
+tight zvode runs (tests/golden/zofe.npz) at 1/8 of the model step (1e-7).
+"""
+import numpy as np
+import pytest
+import torch
+
+import qspectra_amd as qa
+from qspectra_amd import _native
+from oracle import system as S
+from oracle import zofe as Z
+from oracle import propagate as P
+from tests.conftest import golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _pm(g, n):
+    on = np.ones(n, complex)
+    return [np.outer(g[k], on) for k in ('pm_Omega', 'pm_gamma', 'pm_huang')]
+
+
+def _models(subspace='gef', agg7=False, **kw):
+    g = golden('zofe')
+    if agg7:
+        g7 = golden('redfield_7site')
+        H1, d, extra, n = g7['in_H1'], g7['in_dipoles'], float(g['extra_agg7']), 7
+    else:
+        H1, d, extra, n = g['in_H1'], g['in_dipoles'], float(g['extra_dimer']), 2
+    hp = qa.ElectronicHamiltonian(H1, bath=qa.PseudomodeBath(16, *_pm(g, n)), dipoles=d,
+                                  energy_spread_extra=extra)
+    ho = S.ElectronicHamiltonian(H1, bath=Z.PseudomodeBath(16, *_pm(g, n)), dipoles=d,
+                                 energy_spread_extra=extra)
+    return (qa.ZOFEModel(hp, hilbert_subspace=subspace, unit_convert=qa.CM_FS, **kw),
+            Z.ZOFEOracleModel(ho, hilbert_subspace=subspace, unit_convert=S.CM_FS, **kw))
+
+
+@pytest.mark.parametrize('subspace,agg7', [('gef', False), ('ge', False), ('e', True)])
+def test_zofe_rhs_and_commutator(gpu, subspace, agg7):
+    rs = np.random.RandomState(5)
+    for flags in [(False, False), (True, False), (False, True), (True, True)]:
+        mp, mo = _models(subspace, agg7, ham_hermit=flags[0], rho_hermit=flags[1])
+        eom = mp.equation_of_motion(subspace)
+        y = (rs.randn(3, mp.state_size) + 1j * rs.randn(3, mp.state_size)) * 0.3
+        got = _native.zofe_apply(eom, _native.to_device(y[None])).cpu().numpy()[0]
+        ref = np.array([mo.rhs(v) for v in y])
+        assert rel_err(got, ref) < 1e-12, flags
+    from qspectra_amd.simulate.response import _zofe_commutator, _zofe_ops
+    V = _zofe_ops(mp, 'x', '-+')
+    op = mp.dipole_operator(subspace, 'x', '-+')
+    got = _zofe_commutator(mp, _native.to_device(y[None]), _native.to_device(V)).cpu().numpy()[0]
+    assert rel_err(got, np.array([op.commutator(v) for v in y])) < 1e-13
+    assert rel_err(got, np.array([mo.dipole_operator(subspace, 'x', '-+').commutator(v)
+                                  for v in y])) < 1e-13
+
+
+def test_zofe_dynamics_match_oracle_and_reference(gpu):
+    g = golden('zofe')
+    dt = float(g['time_step']) / 8
+    mp, mo = _models('e')
+    t, rho = qa.simulate_dynamics(mp, [1, 0], 100.5, dt=dt)
+    _, rho_o = P.simulate_dynamics(mo, [1, 0], 100.5, dt=dt)
+    np.testing.assert_allclose(t, g['dyn_e_t'])
+    assert rel_err(rho, rho_o) < 1e-10
+    assert rel_err(rho, g['dyn_e_rho']) < 1e-7
+    for hh, rh in [(True, False), (False, True), (True, True)]:
+        mf, _ = _models('e', ham_hermit=hh, rho_hermit=rh)
+        _, rho = qa.simulate_dynamics(mf, [1, 0], 50.5, dt=dt)
+        assert rel_err(rho, g['dyn_e_flags%d%d_rho' % (hh, rh)]) < 1e-7
+    mg, mgo = _models('gef')
+    psi = np.array([0.6, 0.8j, 0., 0.])
+    _, rho = qa.simulate_dynamics(mg, psi, 60.5, dt=dt)
+    _, rho_o = P.simulate_dynamics(mgo, psi, 60.5, dt=dt)
+    assert rel_err(rho, rho_o) < 1e-10
+    assert rel_err(rho, g['dyn_gef_rho']) < 1e-7
+    _, rho = qa.simulate_dynamics(mg, psi, 60.5)                 # default step
+    assert rel_err(rho, g['dyn_gef_rho']) < 1e-5
+    m7, m7o = _models('e', agg7=True)
+    _, rho = qa.simulate_dynamics(m7, np.eye(7)[0], 41., dt=0.25)
+    _, rho_o = P.simulate_dynamics(m7o, np.eye(7)[0], 41., dt=0.25)
+    assert rel_err(rho, rho_o) < 1e-10
+    _, rho = qa.simulate_dynamics(m7, np.eye(7)[0], 41., dt=0.125)
+    assert rel_err(rho, g['agg7_rho']) < 1e-7
+
+
+def test_zofe_spectra_and_pump(gpu):
+    g = golden('zofe')
+    dt = float(g['time_step']) / 8
+    mge, mgeo = _models('ge')
+    f, X = qa.absorption_spectra(mge, 300.5, dt=dt)
+    fo, Xo = P.absorption_spectra(mgeo, 300.5, dt=dt)
+    np.testing.assert_allclose(f, g['abs_f'])
+    assert rel_err(X, Xo) < 1e-10
+    assert rel_err(X, g['abs_X']) < 1e-7
+    pp = qa.GaussianPulse(12800., 30., scale=2e-3, freq_convert=qa.CM_FS)
+    po = S.GaussianPulse(12800., 30., scale=2e-3, freq_convert=S.CM_FS)
+    t, y = qa.simulate_pump(mge, pp, 'x', time_extra=40., dt=dt)
+    _, yo = P.simulate_pump(mgeo, po, 'x', time_extra=40., dt=dt)
+    np.testing.assert_allclose(t, g['pump_t'])
+    assert rel_err(y, yo) < 1e-10
+    assert rel_err(y, g['pump_y']) < 1e-7
+    mg, mgo = _models('gef')
+    (t1, t2, t3), R = qa.third_order_response(mg, 6.5, population_times=[0., 4.], dt=dt)
+    _, Ro = P.third_order_response(mgo, 6.5, population_times=[0., 4.], dt=dt)
+    assert rel_err(R, Ro) < 1e-10
+    assert rel_err(R, g['R3']) < 1e-7

@@ -320,18 +320,18 @@ def test_zofe_matches_reference():
     extra = float(g['extra_dimer'])
     m = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='e', unit_convert=S.CM_FS)
     assert abs(m.time_step - float(g['time_step'])) < 1e-12
-    t, rho = P.simulate_dynamics(m, [1, 0], 100, integrator='zvode', **TIGHT)
+    t, rho = P.simulate_dynamics(m, [1, 0], 100.5, integrator='zvode', **TIGHT)
     assert_allclose(t, g['dyn_e_t'])
     assert rel_err(rho, g['dyn_e_rho']) < 1e-9
-    t, rho = P.simulate_dynamics(m, [1, 0], 100, dt=0.125)
+    t, rho = P.simulate_dynamics(m, [1, 0], 100.5, dt=0.125)
     assert rel_err(rho, g['dyn_e_rho']) < 1e-8
     for hh, rh in [(True, False), (False, True), (True, True)]:
         mf = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='e', unit_convert=S.CM_FS,
                                ham_hermit=hh, rho_hermit=rh)
-        t, rho = P.simulate_dynamics(mf, [1, 0], 50, integrator='zvode', **TIGHT)
+        t, rho = P.simulate_dynamics(mf, [1, 0], 50.5, integrator='zvode', **TIGHT)
         assert rel_err(rho, g['dyn_e_flags%d%d_rho' % (hh, rh)]) < 1e-9
     mg = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='gef', unit_convert=S.CM_FS)
-    t, rho = P.simulate_dynamics(mg, np.array([0.6, 0.8j, 0., 0.]), 60, integrator='zvode',
+    t, rho = P.simulate_dynamics(mg, np.array([0.6, 0.8j, 0., 0.]), 60.5, integrator='zvode',
                                  **TIGHT)
     assert rel_err(rho, g['dyn_gef_rho']) < 1e-9
     mge = Z.ZOFEOracleModel(ham(extra), hilbert_subspace='ge', unit_convert=S.CM_FS)
@@ -350,5 +350,5 @@ def test_zofe_matches_reference():
     m7 = Z.ZOFEOracleModel(ham(float(g['extra_agg7']), golden('redfield_7site')['in_H1'],
                                golden('redfield_7site')['in_dipoles'], 7),
                            hilbert_subspace='e', unit_convert=S.CM_FS)
-    t, rho = P.simulate_dynamics(m7, np.eye(7)[0], 40, integrator='zvode', **TIGHT)
+    t, rho = P.simulate_dynamics(m7, np.eye(7)[0], 41., integrator='zvode', **TIGHT)
     assert rel_err(rho, g['agg7_rho']) < 1e-9
