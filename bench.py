@@ -42,8 +42,8 @@ def parse_args(argv=None):
                     help='seconds of CPU work allowed for the baseline sample')
     ap.add_argument('--workloads', default='all',
                     help="'all', 'none' or a comma list of ensemble_absorption, "
-                         "adjoint_gradient, heom, pump, example_dimer_2d: the further "
-                         "workloads to measure")
+                         "adjoint_gradient, heom, pump, example_dimer_2d, zofe_fmo: the "
+                         "further workloads to measure")
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
                          'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
@@ -371,8 +371,30 @@ def example_dimer_2d_leg(torch):
     return out
 
 
+def zofe_fmo_leg(torch):
+    """The reference's FMO ZOFE example: dynamics in the one-exciton space
+    from site 1 for 1 ps, a Gaussian pump (FWHM 50 fs) in g + e, and the
+    absorption spectrum over 2 ps, at the default step (FP64)."""
+    import qspectra_amd as qa
+    from qspectra_amd import workloads as W
+    me, mge = W.fmo_zofe_model('e'), W.fmo_zofe_model('ge')
+    pump = qa.GaussianPulse(12500, 50, scale=1e-3, freq_convert=qa.CM_FS)
+    calls = {
+        'dynamics_e_1ps': lambda: qa.simulate_dynamics(me, [1, 0, 0, 0, 0, 0, 0], 1000),
+        'pump_ge': lambda: qa.simulate_pump(mge, pump, 'x', 1000),
+        'absorption_ge_2ps': lambda: qa.absorption_spectra(mge, 2000),
+    }
+    out = {'workload': "the reference's FMO ZOFE example (7 sites, 16 pseudomodes): "
+                       "dynamics 1 ps ('e'), pump ('ge'), absorption 2 ps ('ge'); default step"}
+    for name, fn in calls.items():
+        fn()                                                  # warm-up
+        out[name + '_seconds'] = _sync_time(torch, fn)[0]
+    return out
+
+
 LEGS = (('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', gradient_leg),
-        ('heom', heom_leg), ('pump', pump_leg), ('example_dimer_2d', example_dimer_2d_leg))
+        ('heom', heom_leg), ('pump', pump_leg), ('example_dimer_2d', example_dimer_2d_leg),
+        ('zofe_fmo', zofe_fmo_leg))
 
 
 def main(argv=None):

@@ -16,13 +16,17 @@
 // conjugated for '+', qspectra/simulate/eom.py:86-93).
 //
 // One work-group per right-hand side.  The stage vector (rho and all O) and
-// the intermediates Osum_s, L_s rho, Osum_s rho, a live in LDS; thread t
+// the per-site intermediates (Osum_s, L_s rho, Osum_s rho, L_s^+ Osum_s,
+// L_s rho Osum_s^+, Osum_s rho L_s^+, b and i H - a^+) live in LDS (the
+// operators are read through the cache); thread t
 // owns the elements t, t + NT, ...; their state y, RK4 accumulator and next
 // stage value sit in a workspace slice of the right-hand side, read and
-// written by the owner only (L1/L2 resident).  Per stage: (1) Osum; (2) a,
-// L_s rho, Osum_s rho; (3) the derivative of every owned element (b built
-// on the fly from a and H) and its RK4 update into the workspace; (4) after
-// a barrier, the next stage vector is copied into LDS.  The host state
+// written by the owner only (L1/L2 resident).  Per stage, in phases
+// separated by barriers and split into per-site items so that no item loops
+// longer than max(N, S, P): (1) Osum; (2) the three per-site products;
+// (3) the two per-site triple products, and a summed over sites (stored as
+// b and i H - a^+); (4) the derivative of every owned element and its RK4
+// update into the workspace; (5) the next stage vector copied into LDS.  The host state
 // layout is the reference's (vec(rho) then vec(O) of shape (P, S, N, N),
 // both column-major); in LDS rho and each O[p, s] are row-major N x N
 // blocks, O ordered p-major.
@@ -73,94 +77,6 @@ __device__ inline int zofe_ref_index(int e, int N, int S, int P) {
   return N2 + p + P * (s + S * (i + N * j));
 }
 
-struct ZofeStage {
-  const double2 *rho, *O, *Os, *T1, *T2, *av, *H, *L, *G, *W, *V;
-  const double2* fs;
-  int N, S, nfld;
-  bool ham_h, rho_h;
-  double unit;
-};
-
-// b_ij = -i H_ij - a_ij
-__device__ inline double2 zofe_b(const ZofeStage& c, int i, int j) {
-  const double2 h = c.H[i * c.N + j], x = c.av[i * c.N + j];
-  return make_double2(h.y - x.x, -h.x - x.y);
-}
-
-// d_ij = (b rho)_ij + c_ij, c_ij = sum_s sum_k (L_s rho)_ik conj(Osum_s)_jk
-__device__ inline double2 zofe_d(const ZofeStage& c, int i, int j) {
-  const int N = c.N, N2 = N * N;
-  double2 v = make_double2(0.0, 0.0);
-  for (int k = 0; k < N; ++k) cfmad(v, zofe_b(c, i, k), c.rho[k * N + j]);
-  for (int s = 0; s < c.S; ++s)
-    for (int k = 0; k < N; ++k) cfmad(v, c.T1[s * N2 + i * N + k], cconj(c.Os[s * N2 + j * N + k]));
-  return v;
-}
-
-// time derivative of internal element e
-__device__ __forceinline__ double2 zofe_element(const ZofeStage& c, int e) {
-  const int N = c.N, N2 = N * N, S = c.S;
-  const double2* X;
-  int i, j;
-  double2 v;
-  if (e < N2) {
-    i = e / N;
-    j = e % N;
-    X = c.rho;
-    double2 f = make_double2(0.0, 0.0);
-    if (c.rho_h && c.ham_h) {
-      f = cconj(zofe_d(c, j, i));                          // d^+
-    } else if (c.ham_h) {                                  // rho b^+
-      for (int k = 0; k < N; ++k) cfmad(f, c.rho[i * N + k], cconj(zofe_b(c, j, k)));
-    } else {                                               // rho (i H - a^+)
-      for (int k = 0; k < N; ++k) {
-        const double2 h = c.H[k * N + j], x = c.av[j * N + k];
-        cfmad(f, c.rho[i * N + k], make_double2(-h.y - x.x, h.x + x.y));
-      }
-    }
-    if (!c.rho_h) {                                        // + sum_s Osum_s rho L_s^+
-      for (int s = 0; s < S; ++s)
-        for (int k = 0; k < N; ++k)
-          cfmad(f, c.T2[s * N2 + i * N + k], cconj(c.L[(s * N + j) * N + k]));
-    } else if (!c.ham_h) {                                 // + c^+, c^+_ij = conj(c_ji)
-      double2 cc = make_double2(0.0, 0.0);
-      for (int s = 0; s < S; ++s)
-        for (int k = 0; k < N; ++k)
-          cfmad(cc, c.T1[s * N2 + j * N + k], cconj(c.Os[s * N2 + i * N + k]));
-      f = cadd(f, cconj(cc));
-    }
-    v = cadd(zofe_d(c, i, j), f);
-  } else {
-    const int rr = e - N2, ps = rr / N2, ij = rr % N2, s = ps % S;
-    i = ij / N;
-    j = ij % N;
-    X = c.O + ps * N2;
-    v = cmul(c.G[ps], c.L[s * N2 + ij]);
-    const double2 wv = c.W[ps], xo = X[ij];
-    v.x -= wv.x * xo.x - wv.y * xo.y;
-    v.y -= wv.x * xo.y + wv.y * xo.x;
-    for (int k = 0; k < N; ++k) {
-      cfmad(v, zofe_b(c, i, k), X[k * N + j]);
-      const double2 t = cmul(X[i * N + k], zofe_b(c, k, j));
-      v.x -= t.x;
-      v.y -= t.y;
-    }
-  }
-  v = make_double2(c.unit * v.x, c.unit * v.y);
-  for (int fi = 0; fi < c.nfld; ++fi) {                    // f_i [V_i, X]
-    const double2* Vi = c.V + fi * N2;
-    double2 cm = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k) {
-      cfmad(cm, Vi[i * N + k], X[k * N + j]);
-      const double2 t = cmul(X[i * N + k], Vi[k * N + j]);
-      cm.x -= t.x;
-      cm.y -= t.y;
-    }
-    v = cadd(v, cmul(c.fs[fi], cm));
-  }
-  return v;
-}
-
 template <int NT>
 __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __restrict__ y0,
                                                   const int* __restrict__ nsub,
@@ -169,13 +85,17 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
                                                   double2* __restrict__ work,
                                                   int* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = a.N, S = a.S, P = a.P, N2 = N * N, size = a.size;
+  const int N = a.N, S = a.S, P = a.P, N2 = N * N, SN2 = S * N * N, size = a.size;
   double2* z = reinterpret_cast<double2*>(smem);      // [size] stage vector
   double2* Os = z + size;                             // [S][N2] sum over pseudomodes
-  double2* T1 = Os + S * N2;                          // [S][N2] L_s rho
-  double2* T2 = T1 + S * N2;                          // [S][N2] Osum_s rho
-  double2* av = T2 + S * N2;                          // [N2]    a
-  double2* fb = av + N2;                              // [2][3 * kZofeMaxFields]
+  double2* T1 = Os + SN2;                             // [S][N2] L_s rho
+  double2* T2 = T1 + SN2;                             // [S][N2] Osum_s rho
+  double2* U = T2 + SN2;                              // [S][N2] L_s^+ Osum_s
+  double2* X1 = U + SN2;                              // [S][N2] L_s rho Osum_s^+
+  double2* X2 = X1 + SN2;                             // [S][N2] Osum_s rho L_s^+
+  double2* bm = X2 + SN2;                             // [N2]    b = -i H - a
+  double2* gm = bm + N2;                              // [N2]    i H - a^+
+  double2* fb = gm + N2;                              // [2][3 * kZofeMaxFields]
   const int tid = threadIdx.x;
   const int r = blockIdx.x, b = blockIdx.y;
   const double2* Hb = a.H + (int64_t)b * a.strideH;
@@ -200,64 +120,128 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
   if (loader && a.nF > 0) fb[tid] = Fb[tid];
   __syncthreads();
 
-  // derivative of every owned element at stage input z (field samples fs)
+  // one RK4 stage at stage input z with field samples fs: the derivative of
+  // every owned element and its update into the workspace.  Every sum over
+  // sites or pseudomodes is split into per-site items, so no item has a
+  // loop longer than max(N, S, P).
   auto stage = [&](const double2* fs, int st, double h) {
     const double2* rho = z;
     const double2* O = z + N2;
-    for (int it = tid; it < S * N2; it += NT) {          // (1) Osum
+    for (int it = tid; it < SN2; it += NT) {             // (1) Osum
       const int s = it / N2, ij = it % N2;
       double2 v = make_double2(0.0, 0.0);
       for (int p = 0; p < P; ++p) v = cadd(v, O[(p * S + s) * N2 + ij]);
       Os[it] = v;
     }
     __syncthreads();
-    for (int it = tid; it < N2 + 2 * S * N2; it += NT) {  // (2) a, L rho, Osum rho
-      if (it < N2) {
-        const int i = it / N, k = it % N;
-        double2 v = make_double2(0.0, 0.0);
-        for (int s = 0; s < S; ++s)
-          for (int j = 0; j < N; ++j) cfmad(v, cconj(Lb[(s * N + j) * N + i]), Os[s * N2 + j * N + k]);
-        av[it] = v;
+    for (int it = tid; it < 3 * SN2; it += NT) {         // (2) L rho, Osum rho, L^+ Osum
+      const int which = it / SN2, w = it % SN2;
+      const int s = w / N2, i = (w % N2) / N, k = w % N;
+      const double2* Ls = Lb + s * N2;
+      const double2* Oss = Os + s * N2;
+      double2 v = make_double2(0.0, 0.0);
+      if (which == 0) {
+        for (int j = 0; j < N; ++j) cfmad(v, Ls[i * N + j], rho[j * N + k]);
+        T1[w] = v;
+      } else if (which == 1) {
+        for (int j = 0; j < N; ++j) cfmad(v, Oss[i * N + j], rho[j * N + k]);
+        T2[w] = v;
       } else {
-        const int u = it - N2, which = u / (S * N2), w = u % (S * N2);
-        const int s = w / N2, i = (w % N2) / N, k = w % N;
-        const double2* A = which == 0 ? Lb + s * N2 : Os + s * N2;
-        double2 v = make_double2(0.0, 0.0);
-        for (int j = 0; j < N; ++j) cfmad(v, A[i * N + j], rho[j * N + k]);
-        (which == 0 ? T1 : T2)[w] = v;
+        for (int j = 0; j < N; ++j) cfmad(v, cconj(Ls[j * N + i]), Oss[j * N + k]);
+        U[w] = v;
       }
     }
     __syncthreads();
-    ZofeStage c;
-    c.rho = rho;
-    c.O = O;
-    c.Os = Os;
-    c.T1 = T1;
-    c.T2 = T2;
-    c.av = av;
-    c.H = Hb;
-    c.L = Lb;
-    c.G = Gb;
-    c.W = Wb;
-    c.V = Vb;
-    c.fs = fs;
-    c.N = N;
-    c.S = S;
-    c.nfld = nfld;
-    c.ham_h = ham_h;
-    c.rho_h = rho_h;
-    c.unit = a.unit;
-    for (int e = tid; e < size; e += NT) {                 // (3) owned elements
-      const double2 k = zofe_element(c, e);
-      if (a.apply) {                                       // derivative only
-        zw[e] = k;
+    for (int it = tid; it < 2 * SN2 + N2; it += NT) {    // (3) X1, X2, b and i H - a^+
+      if (it < 2 * SN2) {
+        const int which = it / SN2, w = it % SN2;
+        const int s = w / N2, i = (w % N2) / N, j = w % N;
+        double2 v = make_double2(0.0, 0.0);
+        if (which == 0) {                                   // (L rho)(Osum)^+
+          for (int k = 0; k < N; ++k)
+            cfmad(v, T1[s * N2 + i * N + k], cconj(Os[s * N2 + j * N + k]));
+          X1[w] = v;
+        } else {                                            // (Osum rho) L^+
+          for (int k = 0; k < N; ++k)
+            cfmad(v, T2[s * N2 + i * N + k], cconj(Lb[(s * N + j) * N + k]));
+          X2[w] = v;
+        }
+      } else {
+        const int ik = it - 2 * SN2, i = ik / N, k = ik % N;
+        double2 v = make_double2(0.0, 0.0);
+        for (int s = 0; s < S; ++s) v = cadd(v, U[s * N2 + ik]);     // a_ik
+        const double2 hh = Hb[ik], hk = Hb[k * N + i];
+        bm[ik] = make_double2(hh.y - v.x, -hh.x - v.y);             // -i H_ik - a_ik
+        gm[k * N + i] = make_double2(-hk.y - v.x, hk.x + v.y);      // i H_ki - conj(a_ik)
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < size; e += NT) {                 // (4) owned elements
+      const double2* X;
+      int i, j;
+      double2 v;
+      if (e < N2) {
+        i = e / N;
+        j = e % N;
+        X = rho;
+        double2 d = make_double2(0.0, 0.0), f = make_double2(0.0, 0.0);
+        if (rho_h && ham_h) {                               // f = d^+
+          double2 dt = make_double2(0.0, 0.0);
+          for (int k = 0; k < N; ++k) cfmad(dt, bm[j * N + k], rho[k * N + i]);
+          for (int s = 0; s < S; ++s) dt = cadd(dt, X1[s * N2 + j * N + i]);
+          f = cconj(dt);
+        } else if (ham_h) {                                 // rho b^+
+          for (int k = 0; k < N; ++k) cfmad(f, rho[i * N + k], cconj(bm[j * N + k]));
+        } else {                                            // rho (i H - a^+)
+          for (int k = 0; k < N; ++k) cfmad(f, rho[i * N + k], gm[k * N + j]);
+        }
+        if (!rho_h) {                                       // + sum_s Osum rho L^+
+          for (int s = 0; s < S; ++s) f = cadd(f, X2[s * N2 + e]);
+        } else if (!ham_h) {                                // + c^+_ij = conj(c_ji)
+          double2 cc = make_double2(0.0, 0.0);
+          for (int s = 0; s < S; ++s) cc = cadd(cc, X1[s * N2 + j * N + i]);
+          f = cadd(f, cconj(cc));
+        }
+        for (int k = 0; k < N; ++k) cfmad(d, bm[i * N + k], rho[k * N + j]);
+        for (int s = 0; s < S; ++s) d = cadd(d, X1[s * N2 + e]);
+        v = cadd(d, f);
+      } else {
+        const int rr = e - N2, ps = rr / N2, ij = rr % N2, s = ps % S;
+        i = ij / N;
+        j = ij % N;
+        X = O + ps * N2;
+        v = cmul(Gb[ps], Lb[s * N2 + ij]);
+        const double2 wv = Wb[ps], xo = X[ij];
+        v.x -= wv.x * xo.x - wv.y * xo.y;
+        v.y -= wv.x * xo.y + wv.y * xo.x;
+        for (int k = 0; k < N; ++k) {
+          cfmad(v, bm[i * N + k], X[k * N + j]);
+          const double2 t = cmul(X[i * N + k], bm[k * N + j]);
+          v.x -= t.x;
+          v.y -= t.y;
+        }
+      }
+      v = make_double2(a.unit * v.x, a.unit * v.y);
+      for (int fi = 0; fi < nfld; ++fi) {                   // f_i [V_i, X]
+        const double2* Vi = Vb + fi * N2;
+        double2 cm = make_double2(0.0, 0.0);
+        for (int k = 0; k < N; ++k) {
+          cfmad(cm, Vi[i * N + k], X[k * N + j]);
+          const double2 t = cmul(X[i * N + k], Vi[k * N + j]);
+          cm.x -= t.x;
+          cm.y -= t.y;
+        }
+        v = cadd(v, cmul(fs[fi], cm));
+      }
+      if (a.apply) {                                        // derivative only
+        zw[e] = v;
         continue;
       }
       const double2 yv = yw[e];
-      const double2 an = caxpy(kZW[st] * h / 6.0, k, st == 0 ? yv : aw[e]);
+      const double2 an = caxpy(kZW[st] * h / 6.0, v, st == 0 ? yv : aw[e]);
       if (st < 3) {
         aw[e] = an;
-        zw[e] = caxpy(kZC[st] * h, k, yv);
+        zw[e] = caxpy(kZC[st] * h, v, yv);
       } else {
         yw[e] = an;
         zw[e] = an;
@@ -321,7 +305,8 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
 }
 
 inline size_t zofe_lds_bytes(int N, int S, int size) {
-  return ((size_t)size + 3 * (size_t)S * N * N + (size_t)N * N + 2 * 3 * kZofeMaxFields) *
+  const size_t N2 = (size_t)N * N;
+  return ((size_t)size + 6 * (size_t)S * N2 + 2 * N2 + 2 * 3 * kZofeMaxFields) *
          sizeof(double2);
 }
 
@@ -428,7 +413,7 @@ static int32_t zofe_launch(int apply,
     return QSX_OK;                                                                             \
   }
   QSX_ZOFE(256, 1024)
-  QSX_ZOFE(512, 8192)
+  QSX_ZOFE(1024, 8192)
 #undef QSX_ZOFE
   set_error("qsx_zofe_propagate: no kernel for %d elements", size);
   return QSX_EINVAL;

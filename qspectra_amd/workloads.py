@@ -153,3 +153,37 @@ def example_dimer_models():
 # the example's 2-D spectra: 1 ps coherence times, 50 population times to 1 ps
 EXAMPLE_DIMER_2D = dict(coherence_time_max=1000, population_times=np.linspace(0, 1000, 50),
                         geometry='-++', polarization='xxxx', include_signal='GSB,ESE,ESA')
+
+
+# the FMO complex of the reference's ZOFE example: one-exciton Hamiltonian
+# (cm^-1) and the 16-pseudomode fit of the 77 K Drude spectral density
+FMO_H1 = np.array([[12400, -87.7, 5.5, -5.9, 6.7, -13.7, -9.9],
+                   [-87.7, 12520, 30.8, 8.2, 0.7, 11.8, 4.3],
+                   [5.5, 30.8, 12200, -53.5, -2.2, -9.6, 6.],
+                   [-5.9, 8.2, -53.5, 12310, -70.7, -17., -63.3],
+                   [6.7, 0.7, -2.2, -70.7, 12470, 81.1, -1.3],
+                   [-13.7, 11.8, -9.6, -17., 81.1, 12620, 39.7],
+                   [-9.9, 4.3, 6., -63.3, -1.3, 39.7, 12430]])
+PM_OMEGA = [-500., -200., -90., 1., 21., 60., 80., 130., 200., 300., 400., 500., 600., 800.,
+            1100., 1500.]
+PM_HUANG = [-2.5133e-03, -7.5398e-03, -2.5133e-02, 5.0265e+01, 2.2619e+00, 4.5239e-02,
+            2.7646e-01, 9.2991e-03, 2.2619e-02, 1.5080e-02, 3.0159e-03, 3.5186e-03, 2.8274e-04,
+            1.7593e-03, 4.3982e-04, 4.3982e-04]
+PM_GAMMA = [500., 100., 50., 50., 50., 50., 80., 40., 80., 150., 200., 200., 80., 250., 200.,
+            300.]
+
+
+def pseudomode_bath(n_sites):
+    """The 16 pseudomodes, identical for every site (arrays [16, n_sites])."""
+    from .bath import PseudomodeBath
+    on = np.ones(n_sites, complex)
+    Om, ga, hu = [np.array([v[pm] * on for pm in range(len(v))])
+                  for v in (PM_OMEGA, PM_GAMMA, PM_HUANG)]
+    return PseudomodeBath(len(PM_OMEGA), Om, ga, hu)
+
+
+def fmo_zofe_model(hilbert_subspace='e'):
+    from .dynamics.zofe import ZOFEModel
+    from .hamiltonian import ElectronicHamiltonian
+    ham = ElectronicHamiltonian(FMO_H1, bath=pseudomode_bath(7), dipoles=FMO_DIPOLES)
+    return ZOFEModel(ham, hilbert_subspace=hilbert_subspace, unit_convert=CM_FS)
