@@ -312,6 +312,32 @@ int32_t qsx_zofe_apply(int32_t B, int32_t R, int32_t N, int32_t S, int32_t P, co
                        const void* w, int64_t strideG, int32_t flags, double unit_convert,
                        const void* y, void* out, void* workspace, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Dense-block Liouvillian builders, for Hamiltonians whose Hilbert blocks
+ * are not built from a one-exciton matrix (VibronicHamiltonian,
+ * qspectra/hamiltonian.py:633-807).  HX [B][nX][nX], HY [B][nY][nY]: the
+ * rotating-frame Hamiltonian restricted to Hilbert blocks X and Y (real
+ * symmetric, <= 64 states; pass the same pointers for X == Y); occX
+ * [n][nX], occY [n][nY]: the diagonals of the n system-bath coupling
+ * operators over the block bases.  Same output and flags as
+ * qsx_redfield_block / qsx_unitary_block (RedfieldModel / UnitaryModel
+ * evolution_super_operator, redfield.py:9-152, unitary.py:5-9).  The
+ * workspace holds qsx_redfield_dense_workspace_bytes(n, nX, nY, B) bytes. */
+int64_t qsx_redfield_dense_workspace_bytes(int32_t n, int32_t nX, int32_t nY, int32_t B);
+int32_t qsx_redfield_block_dense(const double* HX, const double* HY, int32_t nX, int32_t nY,
+                                 const double* occX, const double* occY, int32_t n, int32_t B,
+                                 double T, double lam, double gam, int32_t flags,
+                                 double unit_convert, void* L, void* workspace, void* stream);
+int32_t qsx_unitary_block_dense(const double* HX, const double* HY, int32_t nX, int32_t nY,
+                                int32_t B, double unit_convert, int32_t heisenberg, void* L,
+                                void* stream);
+/* Eigensystems of B dense real symmetric matrices H [B][nb][nb] (nb <= 64):
+ * E [B][nb] ascending, U [B][nb][nb] (columns, canonical signs) as
+ * qsx_block_eigh; workspace qsx_redfield_dense_workspace_bytes(1, nb, nb, B).
+ * Replaces Hamiltonian.eig (hamiltonian.py:310-328) for dense Hamiltonians. */
+int32_t qsx_dense_eigh(const double* H, int32_t nb, int32_t B, double* E, double* U,
+                       void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,7 +190,8 @@ class LiouvilleModel(object):
         return self.hamiltonian.time_step / self.unit_convert
 
     def liouville_subspace_index(self, s):
-        return liouville_subspace_index(s, self.hilbert_subspace, self.hamiltonian.n_sites)
+        return liouville_subspace_index(s, self.hilbert_subspace, self.hamiltonian.n_sites,
+                                        getattr(self.hamiltonian, 'n_vibrational_states', 1))
 
     @property
     def evolution_super_operator(self):

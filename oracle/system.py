@@ -69,9 +69,9 @@ def n_excitations(n_sites=1, n_vibrational_states=1):
     return np.array([1, n_sites, n_sites * (n_sites - 1) // 2]) * n_vibrational_states
 
 
-def hilbert_subspace_index(subspace, all_subspaces, n_sites):
+def hilbert_subspace_index(subspace, all_subspaces, n_sites, n_vibrational_states=1):
     """qspectra/operator_tools.py:311-335."""
-    n_exc = n_excitations(n_sites)
+    n_exc = n_excitations(n_sites, n_vibrational_states)
     breaks = [0] + list(np.cumsum(n_exc[['gef'.index(s) for s in all_subspaces]]))
     if subspace not in all_subspaces:
         raise ValueError('%s not in %s' % (subspace, all_subspaces))
@@ -316,6 +316,142 @@ class ElectronicHamiltonian(object):
 
     def thermal_state(self, subspace):
         """hamiltonian.py:163-175 on the non-rotating H."""
+        T = getattr(self.bath, 'temperature', 0) if self.bath is not None else 0
+        return thermal_state(self.H(subspace, rotating=False), T)
+
+
+class VibronicHamiltonian(object):
+    """qspectra/hamiltonian.py:633-807: an electronic Hamiltonian (this
+    module's) extended by harmonic modes, on the product basis |e>|v>
+    (electronic state outer, mode occupations inner, the last mode fastest,
+    the order of operator_tools.py:5-31 tensor / extend_vib_operator).
+
+    The matrix is written element by element over that basis:
+    <e v|H|e' v'> = H_el[e, e'] d(v, v') + d(e, e') d(v, v') sum_m w_m v_m
+                    + d(e, e') sum_{n in e, m} c_nm <v|b_m + b_m^+|v'>,
+    with <v|b_m + b_m^+|v'> = sqrt(max(v_m, v'_m)) when v and v' differ
+    only in mode m, by one quantum.  The rotating frame, ensemble samples
+    and bath are those of the electronic part (:741-751)."""
+
+    def __init__(self, electronic, n_vibrational_levels, vib_energies, elec_vib_couplings,
+                 energy_spread_extra=None):
+        self.electronic = electronic
+        self.bath = electronic.bath
+        self.levels = [int(x) for x in np.atleast_1d(n_vibrational_levels)]
+        self.vib_energies = np.asarray(vib_energies, float)
+        self.couplings = np.asarray(elec_vib_couplings, float)
+        self.energy_spread_extra = energy_spread_extra
+        self.rw_freq = electronic.rw_freq
+        self.parent = self
+
+    @property
+    def n_sites(self):
+        return self.electronic.n_sites
+
+    @property
+    def n_vibrational_states(self):
+        return int(np.prod(self.levels))
+
+    def vib_states(self):
+        return list(product(*[range(k) for k in self.levels]))
+
+    def n_states(self, subspace):
+        return self.electronic.n_states(subspace) * self.n_vibrational_states
+
+    def H(self, subspace, rotating=True):
+        Hel = self.electronic.H(subspace, rotating)
+        el = all_states(self.n_sites, subspace)
+        vib = self.vib_states()
+        basis = [(i, v) for i in range(len(el)) for v in vib]
+        H = np.zeros((len(basis), len(basis)))
+        for r, (i, v) in enumerate(basis):
+            for c, (j, u) in enumerate(basis):
+                h = Hel[i, j] if v == u else 0.0
+                if i == j:
+                    if v == u:
+                        h += float(np.dot(self.vib_energies, v))
+                    diff = [m for m in range(len(v)) if v[m] != u[m]]
+                    if len(diff) == 1 and abs(v[diff[0]] - u[diff[0]]) == 1:
+                        m = diff[0]
+                        q = np.sqrt(max(v[m], u[m]))
+                        h += sum(self.couplings[n, m] for n in el[i]) * q
+                H[r, c] = h
+        return H
+
+    def _extend(self, op):
+        """operator on the electronic states -> same operator x identity on
+        the modes (hamiltonian.py:763-768), element by element."""
+        op = np.asarray(op)
+        nv = self.n_vibrational_states
+        out = np.zeros((op.shape[0] * nv, op.shape[1] * nv), dtype=op.dtype)
+        for k in range(nv):
+            out[k::nv, k::nv] = op
+        return out
+
+    def in_rotating_frame(self, rw_freq=None):
+        """hamiltonian.py:177-209 with :741-745; default rw = mean one-exciton
+        eigenvalue of the full vibronic Hamiltonian of the original."""
+        if rw_freq is None:
+            rw_freq = float(np.mean(scipy.linalg.eigh(self.parent.H('e', rotating=False))[0]))
+        new = VibronicHamiltonian(self.electronic.in_rotating_frame(rw_freq), self.levels,
+                                  self.vib_energies, self.couplings, self.energy_spread_extra)
+        if self.parent is not self:
+            new.parent = self.parent.in_rotating_frame(rw_freq)
+        return new
+
+    def eig(self, subspace):
+        """hamiltonian.py:310-328 on the vibronic matrix."""
+        E, U = scipy.linalg.eigh(self.H(subspace, rotating=False))
+        nv = self.n_vibrational_states
+        if 'e' in subspace:
+            E[hilbert_subspace_index('e', subspace, self.n_sites, nv)] -= self.rw_freq
+        if 'f' in subspace:
+            E[hilbert_subspace_index('f', subspace, self.n_sites, nv)] -= 2 * self.rw_freq
+        return E, U
+
+    def E(self, subspace):
+        return self.eig(subspace)[0]
+
+    def U(self, subspace):
+        return self.eig(subspace)[1]
+
+    @property
+    def transition_energy(self):
+        """hamiltonian.py:373-380 on the original (non-rotating) vibronic H."""
+        return float(np.mean(scipy.linalg.eigh(self.parent.H('e', rotating=False))[0]))
+
+    @property
+    def freq_step(self):
+        """hamiltonian.py:382-399 (fixed under sampling)."""
+        E = self.parent.E('gef')
+        extra = (0.01 * self.transition_energy if self.energy_spread_extra is None
+                 else self.energy_spread_extra)
+        return 2 * (max(E.max(), -E.min()) + extra)
+
+    @property
+    def time_step(self):
+        return 1.0 / self.freq_step
+
+    def sample(self, n):
+        """hamiltonian.py:233-274 with :747-751: the electronic part sampled."""
+        base = self.parent
+        new = VibronicHamiltonian(self.electronic.sample(n), self.levels, self.vib_energies,
+                                  self.couplings, self.energy_spread_extra)
+        new.parent = base
+        return new
+
+    def dipole_operator(self, subspace='gef', polarization='x', transitions='-+'):
+        return self._extend(self.electronic.dipole_operator(subspace, polarization, transitions))
+
+    def number_operator(self, site, subspace='gef'):
+        return self._extend(self.electronic.number_operator(site, subspace))
+
+    def system_bath_couplings(self, subspace='gef'):
+        return np.array([self._extend(k)
+                         for k in self.electronic.system_bath_couplings(subspace)])
+
+    def thermal_state(self, subspace):
+        """hamiltonian.py:163-175 on the non-rotating vibronic H."""
         T = getattr(self.bath, 'temperature', 0) if self.bath is not None else 0
         return thermal_state(self.H(subspace, rotating=False), T)
 

@@ -17,7 +17,7 @@ from .dynamics.redfield import RedfieldModel
 from .dynamics.unitary import UnitaryModel
 from .dynamics.heom import HEOMModel
 from .dynamics.zofe import ZOFEModel
-from .hamiltonian import Hamiltonian, ElectronicHamiltonian
+from .hamiltonian import Hamiltonian, ElectronicHamiltonian, VibronicHamiltonian
 from .operator_tools import (unit_vec, basis_transform_operator, basis_transform_vector,
                              all_states)
 from .polarization import (polarization_vector, check_polarizations,
@@ -31,7 +31,7 @@ from .simulate.response import (linear_response, absorption_spectra, third_order
 from .simulate.utils import fourier_transform, integrate, bound_signal, IntegratorError
 
 __all__ = ['DebyeBath', 'ArbitraryBath', 'UncoupledBath', 'PseudomodeBath', 'ZOFEModel', 'CM_FS', 'CM_K', 'GAUSSIAN_SD_FWHM',
-           'Hamiltonian', 'ElectronicHamiltonian', 'n_excitations', 'matrix_to_ket_vec',
+           'Hamiltonian', 'ElectronicHamiltonian', 'VibronicHamiltonian', 'n_excitations', 'matrix_to_ket_vec',
            'ket_vec_to_matrix', 'matrix_to_bra_vec', 'unit_vec', 'basis_transform_operator',
            'basis_transform_vector', 'all_states', 'polarization_vector',
            'check_polarizations', 'invariant_weights_4th_order', 'invariant_polarizations',

@@ -39,7 +39,8 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
 
 # name -> argtypes (restype int32 unless listed in RESTYPES)
 RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64,
-            'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64}
+            'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64,
+            'qsx_redfield_dense_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -67,6 +68,11 @@ SIGNATURES = {
     'qsx_propagate_driven': [c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp, c_i64, c_i32,
                              c_vp, c_i64, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32,
                              c_vp, c_vp, c_vp],
+    'qsx_redfield_dense_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
+    'qsx_redfield_block_dense': [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_i32, c_dbl, c_dbl,
+                                 c_dbl, c_i32, c_dbl, c_vp, c_vp, c_vp],
+    'qsx_unitary_block_dense': [c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp],
+    'qsx_dense_eigh': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
     'qsx_zofe_workspace_bytes': [c_i32, c_i32, c_i32, c_i32, c_i32],
     'qsx_zofe_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp,
                            c_vp, c_i64, c_i32, c_dbl, c_i32, c_vp, c_i64, c_vp, c_i64, c_i64,
@@ -573,3 +579,50 @@ def zofe_propagate(eom, y0, nsub, h, proj=None, V=None, F=None, stream=None):
         st = go()
     check(st, 'qsx_zofe_propagate')
     return out, status
+
+
+def dense_eigh(H, stream=None):
+    """qsx_dense_eigh: E [B, nb], U [B, nb, nb] of real symmetric H [B, nb, nb]
+    (float64 device tensor)."""
+    lib = require_native()
+    H = H.to(torch.float64).contiguous()
+    B, nb, _ = H.shape
+    nbytes = lib.qsx_redfield_dense_workspace_bytes(1, nb, nb, B)
+    ws = torch.empty((max(nbytes, 8) + 7) // 8, dtype=torch.float64, device=H.device)
+    E = torch.empty((B, nb), dtype=torch.float64, device=H.device)
+    U = torch.empty((B, nb, nb), dtype=torch.float64, device=H.device)
+    check(lib.qsx_dense_eigh(ptr(H), nb, B, ptr(E), ptr(U), ptr(ws), stream_ptr(stream)),
+          'qsx_dense_eigh')
+    return E, U
+
+
+def redfield_block_dense(HX, HY, occX, occY, T, lam, gam, flags, unit_convert, stream=None):
+    """qsx_redfield_block_dense: L [B, nX*nY, nX*nY] complex128 from dense
+    rotating-frame blocks HX [B, nX, nX], HY [B, nY, nY] (float64; HY may be
+    HX) and coupling diagonals occX [n, nX], occY [n, nY]."""
+    lib = require_native()
+    B, nX, _ = HX.shape
+    nY = HY.shape[-1]
+    n = occX.shape[0]
+    M = nX * nY
+    nbytes = lib.qsx_redfield_dense_workspace_bytes(n, nX, nY, B)
+    ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=HX.device)
+    L = torch.empty((B, M, M), dtype=torch.complex128, device=HX.device)
+    check(lib.qsx_redfield_block_dense(ptr(HX), ptr(HY), nX, nY, ptr(occX), ptr(occY), n, B,
+                                       float(T), float(lam), float(gam), flags,
+                                       float(unit_convert), ptr(L), ptr(ws),
+                                       stream_ptr(stream)), 'qsx_redfield_block_dense')
+    return L
+
+
+def unitary_block_dense(HX, HY, unit_convert, heisenberg=False, stream=None):
+    """qsx_unitary_block_dense: -i unit [H, .] on the XY block."""
+    lib = require_native()
+    B, nX, _ = HX.shape
+    nY = HY.shape[-1]
+    M = nX * nY
+    L = torch.empty((B, M, M), dtype=torch.complex128, device=HX.device)
+    check(lib.qsx_unitary_block_dense(ptr(HX), ptr(HY), nX, nY, B, float(unit_convert),
+                                      int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
+          'qsx_unitary_block_dense')
+    return L

@@ -15,6 +15,13 @@
 //   5. liouv_rows:  L_site = W T^T (pass 2), T = kron(U_Y, U_X)
 // following qspectra/dynamics/redfield.py:9-104 (May & Kuhn 3.319/3.322)
 // and qspectra/bath.py:17-31, 84-102.
+//
+// Dense mode (qsx_redfield_block_dense, qsx_unitary_block_dense,
+// qsx_dense_eigh): the Hilbert blocks are given as dense rotating-frame
+// matrices and the system-bath couplings as diagonals over the block's basis
+// (the vibronic Hamiltonian of qspectra/hamiltonian.py:633-807: electronic
+// number operators extended by the identity on the vibrations); the
+// pipeline is the same from the eigensystem on.
 
 #include "qsx_common.h"
 
@@ -32,8 +39,10 @@ __host__ __device__ inline int block_size(int X, int n) {
 
 // per-(problem, Hilbert block) record, in doubles:
 //   E[nb] | U[nb*nb] | C[2*nb*nb] | Gs[2*nb*nb] | K[n*nb*nb]
+// rounded up to an even count so that every record (and its complex C, Gs,
+// at the even offset nb*(nb+1)) stays 16-byte aligned
 __host__ __device__ inline size_t rec_doubles(int nb, int n) {
-  return (size_t)nb + 5 * (size_t)nb * nb + (size_t)n * nb * nb;
+  return ((size_t)nb + 5 * (size_t)nb * nb + (size_t)n * nb * nb + 1) & ~(size_t)1;
 }
 struct Rec {
   double* E;
@@ -66,7 +75,9 @@ __device__ inline void pair_of(int s, int n, int& i, int& j) {
 // ------------------------------------------------------------------ 1. eigh
 __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __restrict__ H1, int n,
                                                               int X, const double* __restrict__ rw,
-                                                              double* __restrict__ ws, size_t rstride) {
+                                                              const double* __restrict__ Hd,
+                                                              int nbd, double* __restrict__ ws,
+                                                              size_t rstride) {
   __shared__ double A[kMaxBlock * kMaxBlock];
   __shared__ double V[kMaxBlock * kMaxBlock];
   __shared__ double rot[kMaxBlock / 2][2];
@@ -75,18 +86,20 @@ __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __re
   __shared__ double red[2][kThreads];
   __shared__ int perm[kMaxBlock];
   const int b = blockIdx.x, tid = threadIdx.x;
-  const int nb = block_size(X, n);
+  const int nb = Hd ? nbd : block_size(X, n);
   const int m = nb + (nb & 1);
-  const double* H = H1 + (size_t)b * n * n;
-  const double shift = X * (rw ? rw[b] : 0.0);
-  if (X == 2)
+  const double* H = Hd ? Hd + (size_t)b * nb * nb : H1 + (size_t)b * n * n;
+  const double shift = Hd ? 0.0 : X * (rw ? rw[b] : 0.0);
+  if (!Hd && X == 2)
     for (int s = tid; s < nb; s += kThreads) pair_of(s, n, pi_[s], pj_[s]);
   __syncthreads();
   for (int e = tid; e < m * m; e += kThreads) {
     const int r = e / m, c = e % m;
     double v = 0.0;
     if (r < nb && c < nb) {
-      if (X == 1) {
+      if (Hd) {
+        v = H[r * nb + c];
+      } else if (X == 1) {
         v = H[r * n + c];
       } else if (X == 2) {  // operator_1_to_2 (operator_tools.py:155-191)
         const int i = pi_[r], j = pj_[r], k = pi_[c], l = pj_[c];
@@ -231,12 +244,13 @@ __device__ double2 debye_corr(double x, double T, double lam, double gam, int re
   return make_double2(lam * gam * re, lam * gam * im);
 }
 
-__global__ __launch_bounds__(kThreads) void block_bath_kernel(int n, int X, double T, double lam,
+__global__ __launch_bounds__(kThreads) void block_bath_kernel(int n, int X, int nb,
+                                                              const double* __restrict__ occ,
+                                                              double T, double lam,
                                                               double gam, int real_corr,
                                                               double* __restrict__ ws,
                                                               size_t rstride) {
   const int b = blockIdx.x, tid = threadIdx.x;
-  const int nb = block_size(X, n);
   Rec rec = rec_at(ws + (size_t)b * rstride, nb, n);
   for (int e = tid; e < nb * nb; e += kThreads) {
     const int d = e / nb, c = e % nb;
@@ -247,7 +261,12 @@ __global__ __launch_bounds__(kThreads) void block_bath_kernel(int n, int X, doub
   for (int e = tid; e < nk; e += kThreads) {
     const int i = e / (nb * nb), a = (e / nb) % nb, m = e % nb;
     double v = 0.0;
-    if (X == 1) {
+    if (occ) {                       // diagonal coupling: occ[i][s] over the block basis
+      for (int s2 = 0; s2 < nb; ++s2) {
+        const double o = occ[(size_t)i * nb + s2];
+        if (o != 0.0) v += o * rec.U[s2 * nb + a] * rec.U[s2 * nb + m];
+      }
+    } else if (X == 1) {
       v = rec.U[i * nb + a] * rec.U[i * nb + m];
     } else if (X == 2) {
       for (int j = 0; j < n; ++j) {
@@ -262,10 +281,9 @@ __global__ __launch_bounds__(kThreads) void block_bath_kernel(int n, int X, doub
 }
 
 // --------------------------------------------------------------------- 3. Gs
-__global__ __launch_bounds__(kThreads) void block_gs_kernel(int n, int X, double* __restrict__ ws,
+__global__ __launch_bounds__(kThreads) void block_gs_kernel(int n, int nb, double* __restrict__ ws,
                                                             size_t rstride) {
   const int b = blockIdx.x, tid = threadIdx.x;
-  const int nb = block_size(X, n);
   Rec rec = rec_at(ws + (size_t)b * rstride, nb, n);
   for (int e = tid; e < nb * nb; e += kThreads) {
     const int a = e / nb, c = e % nb;
@@ -320,11 +338,12 @@ __device__ void sandwich(const double2* Z, double2* tmp, double2* outp, const do
 enum { kSecular = 1, kRealCorr = 2, kEigenBasis = 4, kHeisenberg = 8 };
 
 __global__ __launch_bounds__(kThreads) void liouv_cols_kernel(
-    int n, int X, int Y, const double* __restrict__ wsX, const double* __restrict__ wsY,
-    size_t rstride, int flags, double unit, double2* __restrict__ W, double2* __restrict__ L) {
+    int n, int nX, int nY, int same, const double* __restrict__ wsX,
+    const double* __restrict__ wsY, size_t rstride, int flags, double unit,
+    double2* __restrict__ W, double2* __restrict__ L) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int q = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const int nX = block_size(X, n), nY = block_size(Y, n), M = nX * nY;
+  const int M = nX * nY;
   Rec rx = rec_at(const_cast<double*>(wsX) + (size_t)b * rstride, nX, n);
   Rec ry = rec_at(const_cast<double*>(wsY) + (size_t)b * rstride, nY, n);
   double2* z = reinterpret_cast<double2*>(smem);
@@ -349,7 +368,7 @@ __global__ __launch_bounds__(kThreads) void liouv_cols_kernel(
       kk = fma(rx.K[((size_t)i * nX + a) * nX + c], ry.K[((size_t)i * nY + bb) * nY + d], kk);
     r.x -= kk * cs.x;
     r.y -= kk * cs.y;
-    const bool keep = (a == c && bb == d) || (X == Y && a == bb && c == d);
+    const bool keep = (a == c && bb == d) || (same && a == bb && c == d);
     if (secular && !keep) r = make_double2(0.0, 0.0);
     double2 v = make_double2(-r.x, -r.y);
     if (a == c && bb == d) v.y -= rx.E[a] - ry.E[bb];
@@ -373,11 +392,11 @@ __global__ __launch_bounds__(kThreads) void liouv_cols_kernel(
 
 // ----------------------------------------------------- 5. site rows, pass 2
 __global__ __launch_bounds__(kThreads) void liouv_rows_kernel(
-    int n, int X, int Y, const double* __restrict__ wsX, const double* __restrict__ wsY,
+    int n, int nX, int nY, const double* __restrict__ wsX, const double* __restrict__ wsY,
     size_t rstride, int flags, const double2* __restrict__ W, double2* __restrict__ L) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const int nX = block_size(X, n), nY = block_size(Y, n), M = nX * nY;
+  const int M = nX * nY;
   Rec rx = rec_at(const_cast<double*>(wsX) + (size_t)b * rstride, nX, n);
   Rec ry = rec_at(const_cast<double*>(wsY) + (size_t)b * rstride, nY, n);
   double2* z = reinterpret_cast<double2*>(smem);
@@ -439,6 +458,35 @@ __global__ __launch_bounds__(kThreads) void unitary_block_kernel(const double* _
   else Lb[e] = val;
 }
 
+// dense blocks: L[(i,j),(k,l)] = -i unit (HX[i][k] d_jl - d_ik HY[l][j])
+__global__ __launch_bounds__(kThreads) void unitary_dense_kernel(const double* __restrict__ HX,
+                                                                 const double* __restrict__ HY,
+                                                                 int nX, int nY, double unit,
+                                                                 int heisenberg,
+                                                                 double2* __restrict__ L) {
+  const int b = blockIdx.y;
+  const int M = nX * nY;
+  const int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= (int64_t)M * M) return;
+  const int p = (int)(e / M), q = (int)(e % M);
+  const int i = p % nX, j = p / nX, k = q % nX, l = q / nX;
+  const double* hx = HX + (size_t)b * nX * nX;
+  const double* hy = HY + (size_t)b * nY * nY;
+  double v = 0.0;
+  if (j == l) v += hx[i * nX + k];
+  if (i == k) v -= hy[l * nY + j];
+  double2* Lb = L + (size_t)b * M * M;
+  const double2 val = make_double2(0.0, -unit * v);
+  if (heisenberg) Lb[(size_t)q * M + p] = val;
+  else Lb[e] = val;
+}
+
+size_t ws_doubles_dense(int n, int nX, int nY, int B) {
+  const size_t r = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
+  const size_t M = (size_t)nX * nY;
+  return 2 * r * B + 2 * M * M * B;
+}
+
 size_t ws_doubles(int n, int X, int Y, int B) {
   const int nX = block_size(X, n), nY = block_size(Y, n);
   const size_t r = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
@@ -476,7 +524,8 @@ extern "C" int32_t qsx_block_eigh(const double* H1, int32_t n, int32_t B, int32_
   const size_t rs = rec_doubles(nb, n);
   double* ws = static_cast<double*>(workspace);
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, H1, n, X, rw, ws, rs);
+  hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, H1, n, X, rw,
+                     (const double*)nullptr, 0, ws, rs);
   QSX_CHECK_HIP(hipGetLastError());
   QSX_CHECK_HIP(hipMemcpy2DAsync(E, nb * sizeof(double), ws, rs * sizeof(double),
                                  nb * sizeof(double), B, hipMemcpyDeviceToDevice, s));
@@ -504,19 +553,21 @@ extern "C" int32_t qsx_redfield_block(const double* H1, int32_t n, int32_t B, co
   const int blocks[2] = {X, Y};
   double* recs[2] = {wsX, wsY};
   for (int k = 0; k < (X == Y ? 1 : 2); ++k) {
+    const int nb = block_size(blocks[k], n);
     hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, H1, n, blocks[k], rw,
+                       (const double*)nullptr, 0, recs[k], rs);
+    hipLaunchKernelGGL(block_bath_kernel, dim3(B), dim3(kThreads), 0, s, n, blocks[k], nb,
+                       (const double*)nullptr, T, lam, gam, (int)((flags & kRealCorr) != 0),
                        recs[k], rs);
-    hipLaunchKernelGGL(block_bath_kernel, dim3(B), dim3(kThreads), 0, s, n, blocks[k], T, lam,
-                       gam, (int)((flags & kRealCorr) != 0), recs[k], rs);
-    hipLaunchKernelGGL(block_gs_kernel, dim3(B), dim3(kThreads), 0, s, n, blocks[k], recs[k], rs);
+    hipLaunchKernelGGL(block_gs_kernel, dim3(B), dim3(kThreads), 0, s, n, nb, recs[k], rs);
   }
   const size_t shmem = 3 * (size_t)M * sizeof(double2) + (size_t)(nX * nX + nY * nY) * sizeof(double);
   QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_redfield_block: block too large (M=%d)", M);
-  hipLaunchKernelGGL(liouv_cols_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, X, Y, wsX, wsY,
-                     rs, flags, unit_convert, W, (double2*)L);
+  hipLaunchKernelGGL(liouv_cols_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY,
+                     (int)(X == Y), wsX, wsY, rs, flags, unit_convert, W, (double2*)L);
   if (!(flags & kEigenBasis))
-    hipLaunchKernelGGL(liouv_rows_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, X, Y, wsX, wsY,
-                       rs, flags, (const double2*)W, (double2*)L);
+    hipLaunchKernelGGL(liouv_rows_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY, wsX,
+                       wsY, rs, flags, (const double2*)W, (double2*)L);
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }
@@ -531,6 +582,86 @@ extern "C" int32_t qsx_unitary_block(const double* H1, int32_t n, int32_t B, con
   const int64_t blocks = ((int64_t)M * M + kThreads - 1) / kThreads;
   hipLaunchKernelGGL(unitary_block_kernel, dim3((unsigned)blocks, B), dim3(kThreads), 0,
                      as_stream(stream), H1, n, rw, unit_convert, X, Y, heisenberg, (double2*)L);
+  QSX_CHECK_HIP(hipGetLastError());
+  return QSX_OK;
+}
+
+extern "C" int64_t qsx_redfield_dense_workspace_bytes(int32_t n, int32_t nX, int32_t nY,
+                                                      int32_t B) {
+  if (n < 1 || nX < 1 || nY < 1 || B < 1) return -1;
+  return (int64_t)(ws_doubles_dense(n, nX, nY, B) * sizeof(double));
+}
+
+extern "C" int32_t qsx_dense_eigh(const double* H, int32_t nb, int32_t B, double* E, double* U,
+                                  void* workspace, void* stream) {
+  QSX_CHECK_ARG(H && E && U && workspace, "qsx_dense_eigh: null pointer");
+  QSX_CHECK_ARG(nb >= 1 && nb <= kMaxBlock && B >= 1 && B <= 65535,
+                "qsx_dense_eigh: bad sizes nb=%d B=%d", nb, B);
+  const size_t rs = rec_doubles(nb, 1);
+  double* ws = static_cast<double*>(workspace);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, (const double*)nullptr, 1,
+                     0, (const double*)nullptr, H, nb, ws, rs);
+  QSX_CHECK_HIP(hipGetLastError());
+  QSX_CHECK_HIP(hipMemcpy2DAsync(E, nb * sizeof(double), ws, rs * sizeof(double),
+                                 nb * sizeof(double), B, hipMemcpyDeviceToDevice, s));
+  QSX_CHECK_HIP(hipMemcpy2DAsync(U, (size_t)nb * nb * sizeof(double), ws + nb, rs * sizeof(double),
+                                 (size_t)nb * nb * sizeof(double), B, hipMemcpyDeviceToDevice, s));
+  return QSX_OK;
+}
+
+extern "C" int32_t qsx_redfield_block_dense(const double* HX, const double* HY, int32_t nX,
+                                            int32_t nY, const double* occX, const double* occY,
+                                            int32_t n, int32_t B, double T, double lam,
+                                            double gam, int32_t flags, double unit_convert,
+                                            void* L, void* workspace, void* stream) {
+  QSX_CHECK_ARG(HX && HY && occX && occY && L && workspace,
+                "qsx_redfield_block_dense: null pointer");
+  QSX_CHECK_ARG(n >= 1 && B >= 1 && B <= 65535 && nX >= 1 && nY >= 1 && nX <= kMaxBlock &&
+                    nY <= kMaxBlock,
+                "qsx_redfield_block_dense: bad sizes n=%d nX=%d nY=%d B=%d", n, nX, nY, B);
+  QSX_CHECK_ARG((flags & kRealCorr) ? T > 0 : (T > 0 && gam > 0),
+                "qsx_redfield_block_dense: bath needs T > 0 and cutoff > 0");
+  QSX_CHECK_ARG(gam > 0, "qsx_redfield_block_dense: cutoff frequency must be > 0");
+  const int M = nX * nY;
+  const bool same = HX == HY && occX == occY && nX == nY;
+  const size_t rs = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
+  double* wsX = static_cast<double*>(workspace);
+  double* wsY = same ? wsX : wsX + rs * B;
+  double2* W = reinterpret_cast<double2*>(wsX + 2 * rs * B);
+  hipStream_t s = as_stream(stream);
+  const double* Hs[2] = {HX, HY};
+  const double* occs[2] = {occX, occY};
+  const int nbs[2] = {nX, nY};
+  double* recs[2] = {wsX, wsY};
+  for (int k = 0; k < (same ? 1 : 2); ++k) {
+    hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, (const double*)nullptr,
+                       n, 0, (const double*)nullptr, Hs[k], nbs[k], recs[k], rs);
+    hipLaunchKernelGGL(block_bath_kernel, dim3(B), dim3(kThreads), 0, s, n, 0, nbs[k], occs[k],
+                       T, lam, gam, (int)((flags & kRealCorr) != 0), recs[k], rs);
+    hipLaunchKernelGGL(block_gs_kernel, dim3(B), dim3(kThreads), 0, s, n, nbs[k], recs[k], rs);
+  }
+  const size_t shmem = 3 * (size_t)M * sizeof(double2) + (size_t)(nX * nX + nY * nY) * sizeof(double);
+  QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_redfield_block_dense: block too large (M=%d)", M);
+  hipLaunchKernelGGL(liouv_cols_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY,
+                     (int)same, wsX, wsY, rs, flags, unit_convert, W, (double2*)L);
+  if (!(flags & kEigenBasis))
+    hipLaunchKernelGGL(liouv_rows_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY, wsX,
+                       wsY, rs, flags, (const double2*)W, (double2*)L);
+  QSX_CHECK_HIP(hipGetLastError());
+  return QSX_OK;
+}
+
+extern "C" int32_t qsx_unitary_block_dense(const double* HX, const double* HY, int32_t nX,
+                                           int32_t nY, int32_t B, double unit_convert,
+                                           int32_t heisenberg, void* L, void* stream) {
+  QSX_CHECK_ARG(HX && HY && L, "qsx_unitary_block_dense: null pointer");
+  QSX_CHECK_ARG(nX >= 1 && nY >= 1 && B >= 1 && B <= 65535,
+                "qsx_unitary_block_dense: bad sizes nX=%d nY=%d B=%d", nX, nY, B);
+  const int M = nX * nY;
+  const int64_t blocks = ((int64_t)M * M + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(unitary_dense_kernel, dim3((unsigned)blocks, B), dim3(kThreads), 0,
+                     as_stream(stream), HX, HY, nX, nY, unit_convert, heisenberg, (double2*)L);
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }

@@ -183,6 +183,9 @@ class HEOMModel(DynamicalModel):
         super(HEOMModel, self).__init__(hamiltonian, rw_freq, hilbert_subspace, unit_convert)
         if aki_temp_corr:
             raise NotImplementedError('aki_temp_corr is not supported')
+        if getattr(self.hamiltonian, 'n_vibrational_states', 1) != 1:
+            raise NotImplementedError('the HEOM path takes electronic Hamiltonians '
+                                      '(site occupations); vibronic ones are not supported')
         if modified_HEOM and not low_temp_corr:
             raise ValueError('modified_HEOM requires low_temp_corr')
         self.lspace_model = UnitaryModel(hamiltonian, rw_freq, hilbert_subspace, unit_convert)

@@ -85,12 +85,12 @@ def super_commutator_matrix(operator):
     return super_left_matrix(operator) - super_right_matrix(operator)
 
 
-def block_layout(liouville_subspace, hilbert_subspace, n_sites):
+def block_layout(liouville_subspace, hilbert_subspace, n_sites, n_vib=1):
     """For each Liouville block XY in the subspace: (X, Y, positions in the
     sorted subspace vector, local indices a + nX*b).  Blocks are the
     Hilbert-manifold pairs the subspace's indices fall into."""
-    starts, N = _block_starts(hilbert_subspace, n_sites)
-    idx = liouville_subspace_index(liouville_subspace, hilbert_subspace, n_sites)
+    starts, N = _block_starts(hilbert_subspace, n_sites, n_vib)
+    idx = liouville_subspace_index(liouville_subspace, hilbert_subspace, n_sites, n_vib)
     rows, cols = idx % N, idx // N
     layout = []
     for X, (x0, nx) in starts.items():
@@ -253,8 +253,8 @@ class LiouvilleSpaceModel(DynamicalModel):
         return self.system_operator(op, liouv_subspace_map, self)
 
     def thermal_state(self, liouville_subspace):
-        if liouville_subspace == 'gg':
-            return np.ones(1, dtype=complex)   # the ground manifold is one state
+        if liouville_subspace == 'gg' and self.hamiltonian.n_vibrational_states == 1:
+            return np.ones(1, dtype=complex)   # the electronic ground manifold is one state
         rho0 = self.hamiltonian.thermal_state(liouville_subspace)
         rho = self.map_between_subspaces(matrix_to_ket_vec(rho0),
                                          full_liouville_subspace(liouville_subspace),
@@ -272,7 +272,7 @@ class LiouvilleSpaceModel(DynamicalModel):
         hams = self._hamiltonians()
         if len(hams) == 1:
             return np.asarray(self.thermal_state(liouville_subspace))[None]
-        if liouville_subspace == 'gg':
+        if liouville_subspace == 'gg' and self.hamiltonian.n_vibrational_states == 1:
             return np.ones((len(hams), 1), dtype=complex)
         batched = (self.evolve_basis == 'site'
                    and all(type(h) is ElectronicHamiltonian for h in hams)
@@ -325,22 +325,60 @@ class LiouvilleSpaceModel(DynamicalModel):
     def _build_block(self, X, Y, heisenberg, H1, rw):
         raise NotImplementedError('subclass must build Liouvillian blocks')
 
+    def _build_dense_block(self, X, Y, heisenberg, dense):
+        raise NotImplementedError('%s has no dense-block builder' % type(self).__name__)
+
+    def _dense_inputs(self):
+        """For Hamiltonians without a one-exciton form (vibronic): per
+        Hilbert block X, the rotating-frame block of every member
+        (float64 [B, nX, nX]) and the diagonals of the system-bath
+        couplings over the block ([n, nX]; None without a bath)."""
+        hams = self._hamiltonians()
+        dev = _native.device()
+        out = {}
+        for X in 'gef':
+            if X not in self.hilbert_subspace or self.hamiltonian.n_states(X) == 0:
+                continue
+            HX = torch.as_tensor(np.stack([h.dense_block(X, self.hilbert_subspace)
+                                           for h in hams])).to(dev)
+            occ = None
+            if self.hamiltonian.bath is not None:
+                idx = self.hamiltonian.hilbert_subspace_index(X, self.hilbert_subspace)
+                K = np.asarray(self.hamiltonian.system_bath_couplings(self.hilbert_subspace))
+                sub = K[:, idx, idx]
+                if np.any(sub - np.einsum('kii->ki', sub)[:, :, None] * np.eye(sub.shape[-1])
+                          != 0):
+                    raise NotImplementedError('non-diagonal system-bath couplings')
+                occ = torch.as_tensor(np.ascontiguousarray(
+                    np.real(np.einsum('kii->ki', sub)))).to(dev)
+            out[X] = (HX, occ)
+        return out
+
     @imemoize
     def liouvillian(self, liouville_subspace, heisenberg_picture=False):
         """Device tensor [B, M, M] of unit_convert * L on the subspace (its
         transpose for the Heisenberg picture), B = batch size."""
         n = self.hamiltonian.n_sites
-        layout, idx = block_layout(liouville_subspace, self.hilbert_subspace, n)
-        H1, rw = self._device_inputs()
+        layout, idx = block_layout(liouville_subspace, self.hilbert_subspace, n,
+                                   self.hamiltonian.n_vibrational_states)
+        if hasattr(self.hamiltonian, 'dense_block'):
+            dense = self._dense_inputs()
+            build = lambda X, Y: self._build_dense_block(X, Y, heisenberg_picture, dense)
+            B = len(self._hamiltonians())
+            dev = _native.device()
+        else:
+            H1, rw = self._device_inputs()
+            build = lambda X, Y: self._build_block('gef'.index(X), 'gef'.index(Y),
+                                                   heisenberg_picture, H1, rw)
+            B, dev = H1.shape[0], H1.device
         if len(layout) == 1 and np.array_equal(layout[0][3], np.arange(idx.size)):
             X, Y = layout[0][:2]
-            return self._build_block('gef'.index(X), 'gef'.index(Y), heisenberg_picture, H1, rw)
-        L = torch.zeros((H1.shape[0], idx.size, idx.size), dtype=torch.complex128,
-                        device=H1.device)
+            return build(X, Y)
+        L = torch.zeros((B, idx.size, idx.size), dtype=torch.complex128, device=dev)
         for X, Y, sel, local in layout:
-            blk = self._build_block('gef'.index(X), 'gef'.index(Y), heisenberg_picture, H1, rw)
-            s = torch.as_tensor(sel, device=H1.device)
-            l = torch.as_tensor(local, device=H1.device)
+            blk = build(X, Y)
+            s = torch.as_tensor(sel, device=dev)
+            l = torch.as_tensor(local, device=dev)
             L[:, s[:, None], s[None, :]] = blk[:, l[:, None], l[None, :]]
         return L
 

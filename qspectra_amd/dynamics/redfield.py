@@ -35,6 +35,12 @@ class RedfieldModel(LiouvilleSpaceModel):
         return _native.redfield_block(H1, rw, T, lam, gam, X, Y, self._flags(heisenberg),
                                       self.unit_convert)
 
+    def _build_dense_block(self, X, Y, heisenberg, dense):
+        T, lam, gam = debye_parameters(self.hamiltonian.bath)
+        (HX, oX), (HY, oY) = dense[X], dense[Y]
+        return _native.redfield_block_dense(HX, HY, oX, oY, T, lam, gam,
+                                            self._flags(heisenberg), self.unit_convert)
+
 
 def redfield_evolve(hamiltonian, subspace='ge', evolve_basis='site', **kwargs):
     """Full Redfield Liouvillian on the Hilbert subspace (numpy), without

@@ -18,7 +18,8 @@ from . import _native
 from .constants import GAUSSIAN_SD_FWHM
 from .operator_tools import (operator_extend, transition_operator, unit_vec,
                              hilbert_subspace_index, basis_transform_vector,
-                             basis_transform_operator, SubspaceError)
+                             basis_transform_operator, SubspaceError, tensor, vib_annihilate,
+                             vib_create, extend_vib_operator)
 from .polarization import polarization_vector, random_rotation_matrix
 from .utils import imemoize, check_random_state, inspect_repr
 
@@ -288,8 +289,10 @@ class ElectronicHamiltonian(Hamiltonian):
         return len(self.H_1exc)
 
     def n_states(self, subspace):
+        """len(H(subspace)): each manifold named in ``subspace`` counts once
+        (operator_extend builds one block per manifold)."""
         sizes = (1, self.n_sites, self.n_sites * (self.n_sites - 1) // 2)
-        return sum(sizes['gef'.index(s)] for s in subspace)
+        return sum(size for s, size in zip('gef', sizes) if s in subspace)
 
     @imemoize
     def H(self, subspace):
@@ -383,3 +386,141 @@ class ElectronicHamiltonian(Hamiltonian):
         if 'g' in subspace:
             custom[0] = 'g'
         return custom
+
+
+class VibronicHamiltonian(Hamiltonian):
+    """An electronic Hamiltonian extended by explicit vibrations
+    (qspectra/hamiltonian.py:633-807): H = H_el x 1 + 1 x H_vib
+    + sum_{n,m} c_nm |n><n| x (b_m + b_m^+), on the product of the
+    electronic states and the Fock states of the modes.  Its Liouvillian
+    blocks are built on the GPU from the dense Hilbert blocks
+    (qsx_redfield_block_dense / qsx_unitary_block_dense), its eigensystems by
+    qsx_dense_eigh."""
+
+    _implements_eq = True
+
+    def __init__(self, electronic, n_vibrational_levels, vib_energies, elec_vib_couplings,
+                 energy_spread_extra=None, site_labels=None):
+        self.electronic = electronic
+        self.bath = electronic.bath
+        self.n_vibrational_levels = np.asarray(n_vibrational_levels)
+        self.vib_energies = np.asarray(vib_energies)
+        self.elec_vib_couplings = np.asarray(elec_vib_couplings)
+        super(VibronicHamiltonian, self).__init__(energy_spread_extra, site_labels)
+
+    def _eq(self, other, max_depth):
+        return (isinstance(other, VibronicHamiltonian) and self.electronic == other.electronic
+                and np.all(self.n_vibrational_levels == other.n_vibrational_levels)
+                and np.all(self.vib_energies == other.vib_energies)
+                and np.all(self.elec_vib_couplings == other.elec_vib_couplings)
+                and super(VibronicHamiltonian, self)._eq(other, max_depth))
+
+    __hash__ = object.__hash__
+
+    @property
+    def n_sites(self):
+        return self.electronic.n_sites
+
+    @property
+    def dipoles(self):
+        return self.electronic.dipoles
+
+    @property
+    def n_vibrational_states(self):
+        return int(np.prod(self.n_vibrational_levels))
+
+    @property
+    def H_vibrational(self):
+        H = np.zeros((self.n_vibrational_states,) * 2)
+        for m, (levels, energy) in enumerate(zip(self.n_vibrational_levels, self.vib_energies)):
+            H = H + energy * extend_vib_operator(self.n_vibrational_levels, m,
+                                                 np.diag(np.arange(levels)))
+        return H
+
+    def H_electronic_vibrational(self, subspace='gef'):
+        N = self.electronic.n_states(subspace) * self.n_vibrational_states
+        H = np.zeros((N, N))
+        for i in range(self.n_sites):
+            el = self.electronic.number_operator(i, subspace)
+            for m, levels in enumerate(self.n_vibrational_levels):
+                q = vib_annihilate(levels) + vib_create(levels)
+                H = H + self.elec_vib_couplings[i, m] * tensor(
+                    el, extend_vib_operator(self.n_vibrational_levels, m, q))
+        return H
+
+    def n_states(self, subspace):
+        return self.electronic.n_states(subspace) * self.n_vibrational_states
+
+    @imemoize
+    def H(self, subspace='gef'):
+        return (self.el_to_sys_operator(self.electronic.H(subspace))
+                + self.vib_to_sys_operator(self.H_vibrational, subspace)
+                + self.H_electronic_vibrational(subspace))
+
+    def _in_rotating_frame(self, rw_freq):
+        return type(self)(self.electronic.in_rotating_frame(rw_freq), self.n_vibrational_levels,
+                          self.vib_energies, self.elec_vib_couplings, self.energy_spread_extra,
+                          self.site_labels)
+
+    def _sample(self, n, random_orientations):
+        return type(self)(self.electronic.sample(n, random_orientations),
+                          self.n_vibrational_levels, self.vib_energies, self.elec_vib_couplings,
+                          self.energy_spread_extra, self.site_labels)
+
+    def el_to_sys_operator(self, el_operator):
+        return tensor(el_operator, np.eye(self.n_vibrational_states))
+
+    def vib_to_sys_operator(self, vib_operator, subspace='gef'):
+        return tensor(np.eye(self.electronic.n_states(subspace)), vib_operator)
+
+    def dipole_operator(self, *args, **kwargs):
+        return self.el_to_sys_operator(self.electronic.dipole_operator(*args, **kwargs))
+
+    def system_bath_couplings(self, *args, **kwargs):
+        return self.el_to_sys_operator(self.electronic.system_bath_couplings(*args, **kwargs))
+
+    def number_operator(self, site, subspace='gef'):
+        return self.el_to_sys_operator(self.electronic.number_operator(site, subspace))
+
+    def vib_basis_labels(self):
+        labels = np.zeros(self.n_vibrational_states)
+        n = len(self.n_vibrational_levels)
+        for m, levels in enumerate(self.n_vibrational_levels):
+            labels = labels + 10 ** (n - m - 1) * np.diag(
+                extend_vib_operator(self.n_vibrational_levels, m, np.diag(np.arange(levels))))
+        return [('{:0' + str(n) + '}').format(int(i)) for i in labels]
+
+    def basis_labels(self, subspace='gef', braket=False):
+        labels = [(e, v) for e in self.electronic.basis_labels(subspace)
+                  for v in self.vib_basis_labels()]
+        return add_braket(labels) if braket else labels
+
+    # ---------------------------------------------------------- device data
+    def dense_block(self, X, hilbert_subspace):
+        """This Hamiltonian (in its frame) restricted to Hilbert block X of
+        the given subspace, as a dense float64 matrix."""
+        idx = self.hilbert_subspace_index(X, hilbert_subspace)
+        H = np.asarray(self.H(hilbert_subspace))
+        if np.iscomplexobj(H) and np.any(np.imag(H) != 0):
+            raise NotImplementedError('complex Hermitian vibronic Hamiltonians: the GPU '
+                                      'eigensolver is real symmetric')
+        return np.ascontiguousarray(np.real(H)[idx, idx])
+
+    def _eig_blocks(self, subspace):
+        dev = _native.device()
+        Es, Us, xs = [], [], []
+        for s in 'gef':
+            if s not in subspace or self.electronic.n_states(s) == 0:
+                continue
+            Hb = torch.as_tensor(self.dense_block(s, subspace)[None]).to(dev)
+            E, U = _native.dense_eigh(Hb)
+            Es.append(E[0].cpu().numpy())
+            Us.append(U[0].cpu().numpy())
+            xs.append(np.full(len(Es[-1]), 'gef'.index(s)))
+        E = np.concatenate(Es)
+        U = np.zeros((len(E), len(E)))
+        k = 0
+        for u in Us:
+            U[k:k + len(u), k:k + len(u)] = u
+            k += len(u)
+        return E, U, np.concatenate(xs)

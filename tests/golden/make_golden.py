@@ -6,6 +6,7 @@ Run only in the build container, where the reference sources are present:
     python tests/golden/make_golden.py /root/reference fields     # fields.npz only
     python tests/golden/make_golden.py /root/reference heom_spectra
     python tests/golden/make_golden.py /root/reference zofe
+    python tests/golden/make_golden.py /root/reference vibronic
 
 It imports qspectra from the given path and stores plain arrays (inputs and
 outputs) as .npz files.  Nothing here is imported by the tests or by the
@@ -18,6 +19,8 @@ known-answer test pins are written:
   third-order response, 2-D spectra, isotropic and ensemble averages),
 * linear and third-order responses of HEOM models (dimer, 'gef'),
 * ZOFE (pseudomode) dynamics, absorption, pump and third-order outputs,
+* vibronic Hamiltonians: Redfield and unitary Liouvillian blocks,
+  dynamics, absorption and third-order GSB responses,
 * field-driven trajectories (simulate_with_fields, simulate_pump: Gaussian
   pulses, complex amplitudes, three field terms, unitary / 7-site /
   ensemble models) and the impulsive probe of a pumped state,
@@ -182,6 +185,83 @@ def write_zofe(qs, agg):
     np.savez_compressed(os.path.join(HERE, 'zofe.npz'), **out)
 
 
+JONAS = dict(H_el=np.array([[11500., 66.], [66., 11650.]]), levels=np.array([2, 2]),
+             vib=np.array([200., 200.]), couplings=-32. * np.eye(2),
+             dipoles=np.array([[1., 0., 0.], [0., 1., 0.]]), lam=1.5 * 35., gamma=106.)
+
+
+def frac_ok(duration, dt):
+    """time grids end away from a multiple of the step, so arange lengths
+    do not depend on the last bits of the step"""
+    f = (duration / dt) % 1.0
+    assert 0.02 < f < 0.98, (duration, dt, f)
+    return duration
+
+
+def pow2_grid(k, dt):
+    """a duration whose grid arange(0, T, dt) ends at index k = 2^m: then
+    t[-1] / dt == k exactly, and the zero-padding count of
+    fourier_transform (int(t[-1] / dt), utils.py:128-151) does not depend on
+    the last bits of dt"""
+    assert k & (k - 1) == 0
+    return frac_ok(float(np.ceil((k + 0.3) * dt)), dt)
+
+
+def write_vibronic(qs):
+    """vibronic.npz: vibronic Hamiltonians (hamiltonian.py:633-807) of the
+    reference's two vibronic examples: the Jonas dimer with Redfield
+    dynamics (real C(w)), its monomer (complex C(w)) and the unitary
+    five-level vibronic monomer."""
+    out = {'in_' + k: np.asarray(v) for k, v in JONAS.items()}
+    out['in_T'] = qs.CM_K * 77
+    bath = qs.DebyeBath(qs.CM_K * 77, JONAS['lam'], JONAS['gamma'])
+    el = qs.ElectronicHamiltonian(JONAS['H_el'], dipoles=JONAS['dipoles'], bath=bath)
+    ham = qs.VibronicHamiltonian(el, JONAS['levels'], JONAS['vib'], JONAS['couplings'])
+    dimer = qs.RedfieldModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS,
+                             discard_imag_corr=True)
+    dt = dimer.time_step
+    out['jonas_time_step'], out['jonas_rw'] = dt, dimer.rw_freq
+    out['jonas_H_gef'] = ham.H('gef')
+    out['jonas_E_gef'] = dimer.hamiltonian.E('gef')
+    out['jonas_thermal_gg'] = dimer.hamiltonian.thermal_state('g')
+    for s in ['gg', 'ee', 'eg', 'fe', 'ff']:
+        idx = dimer.liouville_subspace_index(s)
+        out['jonas_L_' + s] = dimer.evolution_super_operator[np.ix_(idx, idx)]
+    t, rho = qs.simulate_dynamics(dimer, qs.unit_vec(0, 8), frac_ok(1000., dt), **TIGHT)
+    out['jonas_dyn_t'], out['jonas_dyn_rho'] = t, rho
+    T = pow2_grid(2048, dt)
+    f, X = qs.absorption_spectra(dimer, T, **TIGHT)
+    out['jonas_abs_T'] = T
+    out['jonas_abs_f'], out['jonas_abs_X'] = f, X
+    (t1, t2, t3), R = qs.third_order_response(dimer, coherence_time_max=frac_ok(200.5, dt),
+                                              population_times=[0., 250., 500.],
+                                              geometry='-++', polarization='xxxx',
+                                              include_signal='GSB',
+                                              exact_isotropic_average=True, **TIGHT)
+    out['jonas_R3_t1'], out['jonas_R3_t2'], out['jonas_R3_t3'] = t1, t2, t3
+    out['jonas_R3'] = R
+    mono = qs.VibronicHamiltonian(
+        qs.ElectronicHamiltonian(JONAS['H_el'][:1, :1], dipoles=JONAS['dipoles'][:1],
+                                 bath=bath),
+        JONAS['levels'][:1], JONAS['vib'][:1], JONAS['couplings'][:1, :1])
+    mm = qs.RedfieldModel(mono, hilbert_subspace='ge', unit_convert=qs.CM_FS)
+    T = pow2_grid(1024, mm.time_step)
+    f, X = qs.absorption_spectra(mm, T, **TIGHT)
+    out['jmono_time_step'], out['jmono_abs_f'], out['jmono_abs_X'] = mm.time_step, f, X
+    out['jmono_abs_T'] = T
+    uham = qs.VibronicHamiltonian(
+        qs.ElectronicHamiltonian([[11500.]], dipoles=[[1., 0., 0.]], bath=None),
+        n_vibrational_levels=[5], vib_energies=[200.], elec_vib_couplings=[[-100.]])
+    um = qs.UnitaryModel(uham, hilbert_subspace='ge', unit_convert=qs.CM_FS)
+    out['umono_time_step'], out['umono_H_ge'] = um.time_step, uham.H('ge')
+    t, rho = qs.simulate_dynamics(um, qs.unit_vec(0, 5), frac_ok(1000., um.time_step), **TIGHT)
+    out['umono_dyn_t'], out['umono_dyn_rho'] = t, rho
+    T = pow2_grid(2048, um.time_step)
+    f, X = qs.absorption_spectra(um, T, correlation_decay_time=1000., **TIGHT)
+    out['umono_abs_f'], out['umono_abs_X'], out['umono_abs_T'] = f, X, T
+    np.savez_compressed(os.path.join(HERE, 'vibronic.npz'), **out)
+
+
 def write_manifest(ref_path):
     with open(os.path.join(HERE, 'MANIFEST.txt'), 'w') as fh:
         fh.write('written by tests/golden/make_golden.py from the reference at %s\n' % ref_path)
@@ -207,6 +287,10 @@ def main(ref_path, only=None):
             dipoles=inp['dipoles'], disorder=disorder, random_seed=seed,
             energy_spread_extra=extra)
 
+    if only == 'vibronic':
+        write_vibronic(qs)
+        write_manifest(ref_path)
+        return
     if only == 'zofe':
         write_zofe(qs, aggregate7_inputs())
         write_manifest(ref_path)
@@ -371,6 +455,7 @@ def main(ref_path, only=None):
     write_fields(qs, ham, ham7)
     write_heom_spectra(qs, HEOMModel, dim, extra)
     write_zofe(qs, agg)
+    write_vibronic(qs)
     write_manifest(ref_path)
 
 
