@@ -392,9 +392,48 @@ def zofe_fmo_leg(torch):
     return out
 
 
+def jonas_leg(torch):
+    """The reference's vibronic examples end to end at the default step:
+    Jonas dimer dynamics (1 ps from site 1 with no vibrational quanta),
+    absorption (10 ps), the third-order GSB response with the exact
+    isotropic average (t1 = t3 = 1 ps, 100 population times to 1 ps; FP64
+    and mixed), the monomer's absorption, and the unitary vibronic monomer
+    (dynamics 1 ps, absorption 10 ps with 1 ps decay)."""
+    import qspectra_amd as qa
+    from qspectra_amd import workloads as W
+    dimer, monomer, unitary = W.jonas_models()
+    e0 = np.zeros(8)
+    e0[0] = 1
+    u0 = np.zeros(5)
+    u0[0] = 1
+    calls = {
+        'dimer_dynamics_1ps': lambda: qa.simulate_dynamics(dimer, e0, 1000),
+        'dimer_absorption_10ps': lambda: qa.absorption_spectra(dimer, 10000),
+        'dimer_gsb_isotropic_fp64': lambda: qa.third_order_response(dimer, **W.JONAS_GSB),
+        'dimer_gsb_isotropic_mixed': lambda: qa.third_order_response(
+            dimer, precision='mixed', **W.JONAS_GSB),
+        'monomer_absorption_10ps': lambda: qa.absorption_spectra(monomer, 10000),
+        'unitary_dynamics_1ps': lambda: qa.simulate_dynamics(unitary, u0, 1000),
+        'unitary_absorption_10ps': lambda: qa.absorption_spectra(
+            unitary, 10000, correlation_decay_time=1000),
+    }
+    out = {'workload': "the reference's vibronic examples (Jonas dimer: 2 sites x 2 two-level "
+                       "modes, Redfield g/e/f; vibronic monomers), default step"}
+    res = {}
+    for name, fn in calls.items():
+        # first call: eigensystems, Liouvillian builds and plan set-up included
+        out[name + '_first_call_seconds'] = _sync_time(torch, fn)[0]
+        out[name + '_seconds'], res[name] = _sync_time(torch, fn)
+    X64 = res['dimer_gsb_isotropic_fp64'][1]
+    X32 = res['dimer_gsb_isotropic_mixed'][1]
+    out['gsb_shape'] = list(X64.shape)
+    out['gsb_rel_err_mixed_vs_fp64'] = float(np.abs(X32 - X64).max() / np.abs(X64).max())
+    return out
+
+
 LEGS = (('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', gradient_leg),
         ('heom', heom_leg), ('pump', pump_leg), ('example_dimer_2d', example_dimer_2d_leg),
-        ('zofe_fmo', zofe_fmo_leg))
+        ('zofe_fmo', zofe_fmo_leg), ('jonas_vibronic', jonas_leg))
 
 
 def main(argv=None):

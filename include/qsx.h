@@ -338,6 +338,20 @@ int32_t qsx_unitary_block_dense(const double* HX, const double* HY, int32_t nX, 
 int32_t qsx_dense_eigh(const double* H, int32_t nb, int32_t B, double* E, double* U,
                        void* workspace, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * fourier_transform (qspectra/simulate/utils.py:128-219): x [batch][n]
+ * complex128 sampled at dt, zero-padded to the grid of N points symmetric
+ * about t = 0 with the samples at lo .. lo+n-1, scaled by dt, ifftshift, DFT,
+ * fftshift, reversed for sign = +1; out [batch][N].  With adjoint != 0 the
+ * conjugate transpose: x [batch][N] -> out [batch][n] (the backward of the
+ * transform).  A direct sum (N*n multiply-adds per row, exact integer
+ * reduction of the exponent, a twiddle table of N entries in the workspace of
+ * qsx_fourier_workspace_bytes(N) bytes); nothing is compiled at run time. */
+int64_t qsx_fourier_workspace_bytes(int32_t N);
+int32_t qsx_fourier_transform(const void* x, int64_t batch, int32_t n, int32_t N, int32_t lo,
+                              int32_t sign, double dt, int32_t adjoint, void* workspace,
+                              void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,7 +40,7 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
 # name -> argtypes (restype int32 unless listed in RESTYPES)
 RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64,
             'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64,
-            'qsx_redfield_dense_workspace_bytes': c_i64}
+            'qsx_redfield_dense_workspace_bytes': c_i64, 'qsx_fourier_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -73,6 +73,9 @@ SIGNATURES = {
                                  c_dbl, c_i32, c_dbl, c_vp, c_vp, c_vp],
     'qsx_unitary_block_dense': [c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp],
     'qsx_dense_eigh': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    'qsx_fourier_workspace_bytes': [c_i32],
+    'qsx_fourier_transform': [c_vp, c_i64, c_i32, c_i32, c_i32, c_i32, ctypes.c_double, c_i32,
+                              c_vp, c_vp, c_vp],
     'qsx_zofe_workspace_bytes': [c_i32, c_i32, c_i32, c_i32, c_i32],
     'qsx_zofe_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp,
                            c_vp, c_i64, c_i32, c_dbl, c_i32, c_vp, c_i64, c_vp, c_i64, c_i64,
@@ -626,3 +629,33 @@ def unitary_block_dense(HX, HY, unit_convert, heisenberg=False, stream=None):
                                       int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
           'qsx_unitary_block_dense')
     return L
+
+
+def _fourier(x, n, N, lo, sign, dt, adjoint, stream):
+    lib = require_native()
+    x = x.to(torch.complex128).contiguous()
+    lead = tuple(x.shape[:-1])
+    if x.shape[-1] != (N if adjoint else n):
+        raise ValueError('fourier transform: last axis has %d entries, expected %d'
+                         % (x.shape[-1], N if adjoint else n))
+    out = torch.empty(lead + ((n if adjoint else N),), dtype=torch.complex128, device=x.device)
+    batch = int(np.prod(lead)) if lead else 1
+    if batch == 0:
+        return out
+    nbytes = lib.qsx_fourier_workspace_bytes(int(N))
+    ws = torch.empty((nbytes + 15) // 16, dtype=torch.complex128, device=x.device)
+    check(lib.qsx_fourier_transform(ptr(x), batch, int(n), int(N), int(lo), int(sign),
+                                    float(dt), int(adjoint), ptr(ws), ptr(out),
+                                    stream_ptr(stream)), 'qsx_fourier_transform')
+    return out
+
+
+def fourier_transform(x, N, lo, sign, dt, stream=None):
+    """qsx_fourier_transform along the last axis of a device tensor:
+    [..., n] samples -> [..., N] on the zero-padded symmetric grid."""
+    return _fourier(x, x.shape[-1], N, lo, sign, dt, False, stream)
+
+
+def fourier_transform_adjoint(g, n, lo, sign, dt, stream=None):
+    """The conjugate transpose of fourier_transform: [..., N] -> [..., n]."""
+    return _fourier(g, n, g.shape[-1], lo, sign, dt, True, stream)

@@ -335,10 +335,27 @@ def _symmetrize(t, x, axis=-1):
     return t_sym, x_sym
 
 
+class _SymmetricFourier(torch.autograd.Function):
+    """The transform of the last axis onto the zero-padded symmetric grid
+    (qsx_fourier_transform); backward: its conjugate transpose."""
+
+    @staticmethod
+    def forward(ctx, x, N, lo, sign, dt):
+        ctx.meta = (x.shape[-1], lo, sign, dt)
+        return _native.fourier_transform(x, N, lo, sign, dt)
+
+    @staticmethod
+    def backward(ctx, g):
+        n, lo, sign, dt = ctx.meta
+        return _native.fourier_transform_adjoint(g, n, lo, sign, dt), None, None, None, None
+
+
 def fourier_transform_device(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1,
                              convention='angular'):
     """fourier_transform on a complex128 device tensor x (result stays on the
-    device): zero-pad to the symmetric grid, ifftshift, FFT, fftshift."""
+    device; differentiable): the zero-padded symmetric grid, ifftshift, DFT,
+    fftshift of utils.py:128-219, evaluated as one direct sum by
+    qsx_fourier_transform."""
     t = np.asarray(t, dtype=float)
     if t.ndim != 1:
         raise ValueError('t must be one dimensional')
@@ -355,16 +372,11 @@ def fourier_transform_device(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1,
     lo = int(np.searchsorted(t_sym, t[0]))
     N = t_sym.size
     dt = t_sym[1] - t_sym[0]
-    shape = list(x.shape)
-    shape[ax] = N
-    xs = torch.zeros(shape, dtype=torch.complex128, device=x.device)
-    xs.narrow(ax, lo, t.size).copy_(x)
-    xs *= dt
-    X = torch.fft.fftshift(torch.fft.fft(torch.fft.ifftshift(xs, dim=ax), dim=ax), dim=ax)
+    xm = x.to(torch.complex128).movedim(ax, -1)
+    X = _SymmetricFourier.apply(xm, N, lo, sign, float(dt)).movedim(-1, ax)
     f = np.fft.fftshift(np.fft.fftfreq(N, dt * unit_convert))
     if sign == 1:
         f = -f[::-1]
-        X = torch.flip(X, dims=[ax])
     return f + rw_freq, X
 
 

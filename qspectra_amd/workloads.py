@@ -187,3 +187,33 @@ def fmo_zofe_model(hilbert_subspace='e'):
     from .hamiltonian import ElectronicHamiltonian
     ham = ElectronicHamiltonian(FMO_H1, bath=pseudomode_bath(7), dipoles=FMO_DIPOLES)
     return ZOFEModel(ham, hilbert_subspace=hilbert_subspace, unit_convert=CM_FS)
+
+
+def jonas_models():
+    """The vibronic examples of the reference: the Jonas dimer (two sites at
+    11500 / 11650 cm^-1, coupling 66, one two-level mode of 200 cm^-1 per
+    site with coupling -32, Drude bath 77 K, lambda 52.5, gamma 106;
+    Redfield, real C(w), g/e/f), its monomer (Redfield, complex C(w), g/e)
+    and the unitary five-level vibronic monomer (11500, mode 200,
+    coupling -100, g/e)."""
+    from . import ElectronicHamiltonian, RedfieldModel, UnitaryModel, VibronicHamiltonian
+    bath = DebyeBath(CM_K * 77, 1.5 * 35, 106)
+    H_el = np.array([[11500., 66.], [66., 11650.]])
+    dipoles = [[1., 0., 0.], [0., 1., 0.]]
+    ham = VibronicHamiltonian(ElectronicHamiltonian(H_el, dipoles=dipoles, bath=bath),
+                              [2, 2], [200., 200.], -32. * np.eye(2))
+    dimer = RedfieldModel(ham, hilbert_subspace='gef', unit_convert=CM_FS,
+                          discard_imag_corr=True)
+    mono = VibronicHamiltonian(ElectronicHamiltonian(H_el[:1, :1], dipoles=dipoles[:1],
+                                                     bath=bath),
+                               [2], [200.], [[-32.]])
+    monomer = RedfieldModel(mono, hilbert_subspace='ge', unit_convert=CM_FS)
+    uham = VibronicHamiltonian(ElectronicHamiltonian([[11500.]], dipoles=[[1., 0., 0.]]),
+                               [5], [200.], [[-100.]])
+    unitary = UnitaryModel(uham, hilbert_subspace='ge', unit_convert=CM_FS)
+    return dimer, monomer, unitary
+
+
+JONAS_GSB = dict(coherence_time_max=1000, population_times=np.linspace(0, 1000, num=100),
+                 geometry='-++', polarization='xxxx', include_signal='GSB',
+                 exact_isotropic_average=True)

@@ -1,0 +1,56 @@
+"""qsx_fourier_transform (the reference's zero-padded symmetric FFT,
+qspectra/simulate/utils.py:128-219, as a direct sum on the GPU) against the
+oracle's numpy FFT pipeline: grids starting at, before and after t = 0 (the
+zero padding on either side), both signs, batched rows along any axis, and
+the adjoint against the conjugate-transposed transform matrix and through
+autograd.  Tolerance 1e-13 relative (max-norm)."""
+import numpy as np
+import pytest
+import torch
+
+import qspectra_amd as qa
+from qspectra_amd import _native
+from qspectra_amd.simulate.utils import fourier_transform_device
+from oracle import propagate as P
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize('n,t0,dt', [(2, 0.0, 1.0), (7, 0.0, 0.7), (64, -20.0, 0.5),
+                                     (300, 11.0, 0.25), (1989, 0.0, 5.0297804513411934),
+                                     (4096, -4096 * 0.5, 0.5), (5000, 7.0, 2.0)])
+def test_matches_numpy_fft_pipeline(gpu, n, t0, dt):
+    rs = np.random.RandomState(n)
+    t = t0 + dt * np.arange(n)
+    x = rs.randn(3, n) + 1j * rs.randn(3, n)
+    for sign in (1, -1):
+        for conv in ('angular', 'linear'):
+            fo, Xo = P.fourier_transform(t, x, axis=-1, rw_freq=123.0, unit_convert=0.3,
+                                         sign=sign, convention=conv)
+            f, X = qa.fourier_transform(t, x, axis=-1, rw_freq=123.0, unit_convert=0.3,
+                                        sign=sign, convention=conv)
+            assert np.array_equal(f, fo)
+            assert rel_err(X, Xo) < 1e-13, (n, sign)
+
+
+def test_axes_and_adjoint(gpu):
+    rs = np.random.RandomState(1)
+    t1 = 0.5 * np.arange(40)
+    t3 = 0.5 * np.arange(33)
+    R = rs.randn(40, 5, 33) + 1j * rs.randn(40, 5, 33)
+    f1, A = P.fourier_transform(t1, R, axis=0, sign=-1)
+    f3, B = P.fourier_transform(t3, A, axis=2)
+    _, Ag = qa.fourier_transform(t1, R, axis=0, sign=-1)
+    _, Bg = qa.fourier_transform(t3, Ag, axis=2)
+    assert rel_err(Ag, A) < 1e-13 and rel_err(Bg, B) < 1e-13
+    # adjoint: <F x, g> = <x, F^H g>, and the autograd backward is F^H g
+    t = -3.0 + 0.25 * np.arange(21)
+    x = torch.as_tensor(rs.randn(21) + 1j * rs.randn(21)).cuda().requires_grad_(True)
+    _, X = fourier_transform_device(t, x, -1, 0.0, 1.0, sign=1)
+    g = torch.as_tensor(rs.randn(X.shape[-1]) + 1j * rs.randn(X.shape[-1])).cuda()
+    (gx,) = torch.autograd.grad((X * g.conj()).sum().real, x)
+    F = np.stack([P.fourier_transform(t, e, sign=1)[1] for e in np.eye(21)], axis=1)
+    assert rel_err(gx.cpu().numpy(), F.conj().T @ g.cpu().numpy()) < 1e-13
+    ga = _native.fourier_transform_adjoint(g[None], 21, 0, 1, 0.25)[0].cpu().numpy()
+    assert rel_err(ga, F.conj().T @ g.cpu().numpy()) < 1e-13
