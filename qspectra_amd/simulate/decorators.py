@@ -61,22 +61,27 @@ def optional_ensemble_average(func):
 def optional_4th_order_isotropic_average(func):
     """Adds ``exact_isotropic_average`` for four polarizations: the signal is
     summed over the 9-21 lab polarizations of the tensor invariants with
-    non-negligible weight (Hochstrasser), each evaluated once."""
+    non-negligible weight (Hochstrasser), each evaluated once; device
+    signals are summed on the device."""
     @wraps(func)
     def wrapper(*args, **kwargs):
         if not kwargs.pop('exact_isotropic_average', False):
             return func(*args, **kwargs)
         kwargs = _get_call_args(func, *args, **kwargs)
         weights = invariant_weights_4th_order(kwargs.pop('polarization'))
-        cache = {}
-        total = ZeroArray()
-        ticks = None
+        # each lab polarization once, with the summed weight of the
+        # invariants it belongs to (decorators.py:67-96 evaluates each once
+        # and adds it per invariant)
+        pol_weight = {}
         for invariant, w in zip(FOURTH_ORDER_INVARIANTS, weights):
             if w > 1e-8:
                 for p in invariant_polarizations(invariant):
-                    if p not in cache:
-                        ticks, cache[p] = func(polarization=p, **kwargs)
-                    total += w * cache[p]
+                    pol_weight[p] = pol_weight.get(p, 0.0) + w
+        total = ZeroArray()
+        ticks = None
+        for p, w in pol_weight.items():
+            ticks, signal = func(polarization=p, **kwargs)
+            total += w * signal
         return ticks, total
     return wrapper
 

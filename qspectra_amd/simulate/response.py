@@ -139,7 +139,7 @@ def _zofe_linear_response(model, liouv_space_path, time_max, initial_state, pola
     eom = model.equation_of_motion(subspaces[1])
     out = propagate_zofe(eom, y0, t, dt=kw.get('dt'), proj=_zofe_expectation_rows(V1),
                          rtol=kw.get('rtol', 1e-6))                      # [B, R, nt, 1]
-    return t, (-out[..., 0]).reshape((B,) + tuple(lead) + (len(t),)).cpu().numpy()
+    return t, (-out[..., 0]).reshape((B,) + tuple(lead) + (len(t),))
 
 
 def _batched_linear_response(dynamical_model, liouv_space_path, time_max, initial_state=None,
@@ -179,7 +179,7 @@ def _batched_linear_response(dynamical_model, liouv_space_path, time_max, initia
             out = propagate(eom, y0, t, proj=proj, **kw)                 # [B, 1, nt, K]
             s = out[:, 0].permute(0, 2, 1).reshape((B,) + lead + (len(t),))
         signal += s
-    return t, signal.cpu().numpy()
+    return t, signal
 
 
 _linear_response = optional_ensemble_average(
@@ -300,7 +300,7 @@ def _third_order_response(dynamical_model, coherence_time_max, population_time_m
                                    population_times, geometry, polarization, include_signal, kw)
     X = p.run()
     p.check_status()
-    return (p.t1, p.t2, p.t3), X.cpu().numpy()
+    return (p.t1, p.t2, p.t3), X
 
 
 # hierarchies with at most this many elements (ADOs x Liouville block) are
@@ -373,7 +373,7 @@ def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
         prod = torch.matmul(V_rho2.reshape(B, -1, S3), G3.transpose(1, 2).contiguous())
         prod = prod.reshape(B, len(t1), len(t2), len(t3))
         total = prod if total is None else total + prod
-    return (t1, t2, t3), total.cpu().numpy()
+    return (t1, t2, t3), total
 
 
 def _zofe_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
@@ -402,7 +402,7 @@ def _zofe_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
                              rtol=rtol)[..., 0]                              # [B, n1*n2, n3]
         sig = sig.reshape(B, len(t1), len(t2), len(t3))
         total = sig if total is None else total + sig
-    return (t1, t2, t3), total.cpu().numpy()
+    return (t1, t2, t3), total
 
 
 def _pathway_by_pathway(model, coherence_time_max, population_time_max, population_times,
@@ -442,7 +442,7 @@ def _pathway_by_pathway(model, coherence_time_max, population_time_max, populati
                             G3[:, 0].transpose(1, 2).contiguous())           # [B, n1*n2, n3]
         prod = prod.reshape(B, len(t1), len(t2), len(t3))
         total = prod if total is None else total + prod
-    return (t1, t2, t3), total.cpu().numpy()
+    return (t1, t2, t3), total
 
 
 def third_order_response(dynamical_model, coherence_time_max, population_time_max=None,
