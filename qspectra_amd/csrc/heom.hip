@@ -21,8 +21,9 @@
 // 7-site, K = 1 hierarchy on the 'ee' block.
 //
 // Lh is applied either as a dense M x M block (any subspace, M <= 64) or,
-// for a square subspace XX (M = n^2), as the commutator -i unit (H rho -
-// rho H) with the n x n block H_X (2n instead of n^2 products per element).
+// for a single Liouville block XY (M = nX nY), as the commutator
+// -i unit (H_X rho - rho H_Y) with the Hilbert blocks H_X (nX x nX) and H_Y
+// (nY x nY): nX + nY instead of M products per element.
 //
 // One RK4 stage is one launch over (sample, ADO tile); the four stages of
 // classical RK4 use buffers y (FP64 state), ynew (FP64 accumulated update)
@@ -48,8 +49,9 @@ constexpr int kXcds = 8;
 constexpr int kElemsPerThread = QSX_HEOM_ELEMS_PER_THREAD;
 
 struct HeomArgs {
-  int B, n_ado, M, n, dense, nbins, K1, nrel, tiles, ado_per_tile, xcd_map;
-  const double2* Hop;      // COMM: [B][n][n] = -i unit H_X (or its transpose); DENSE: [B][M][M]
+  int B, n_ado, M, nX, nY, dense, nbins, K1, nrel, tiles, ado_per_tile, xcd_map;
+  const double2* Hop;      // COMM: [B][nX*nX + nY*nY] = -i unit H_X, H_Y (or their
+                           // transposes); DENSE: [B][M][M]
   const double* shift;     // [n_ado]   -unit * sum m nu
   const double2* dvec;     // [M]       -unit * Delta * D_p
   const int* bsrc;         // [n_ado][nbins][2] neighbour ADO (-1 = none)
@@ -84,9 +86,9 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
     double2* __restrict__ y, double2* __restrict__ ynew, double h, int st) {
   using T2 = cplx_t<T>;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  const int opsz = a.dense ? a.M * a.M : a.nX * a.nX + a.nY * a.nY;
   const int nb2 = 2 * a.nbins;
-  double2* Hs = reinterpret_cast<double2*>(smem);                 // n*n (COMM) or M*M (DENSE)
+  double2* Hs = reinterpret_cast<double2*>(smem);                 // H_X, H_Y (COMM) or M*M
   T2* zt = reinterpret_cast<T2*>(Hs + opsz);                      // [ado_per_tile][M]
   size_t off = (size_t)opsz * sizeof(double2) + (size_t)a.ado_per_tile * a.M * sizeof(T2);
   off = (off + 15) & ~(size_t)15;
@@ -125,11 +127,12 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
     if (a.dense) {
       const double2* row = Hs + p * M;
       for (int q = 0; q < M; ++q) k = cadd(k, cmul(row[q], ld<T>(z + q)));
-    } else {   // (H rho - rho H)_{rc}, p = r + n c, column-major rho
-      const int n = a.n, r = p % n, c = p / n;
-      for (int q = 0; q < n; ++q) {
-        k = cadd(k, cmul(Hs[r * n + q], ld<T>(z + q + n * c)));
-        const double2 t = cmul(ld<T>(z + r + n * q), Hs[q * n + c]);
+    } else {   // (H_X rho - rho H_Y)_{rc}, p = r + nX c, column-major rho
+      const int nX = a.nX, nY = a.nY, r = p % nX, c = p / nX;
+      const double2* HY = Hs + nX * nX;
+      for (int q = 0; q < nX; ++q) k = cadd(k, cmul(Hs[r * nX + q], ld<T>(z + q + nX * c)));
+      for (int q = 0; q < nY; ++q) {
+        const double2 t = cmul(ld<T>(z + r + nX * q), HY[q * nY + c]);
         k.x -= t.x;
         k.y -= t.y;
       }
@@ -214,7 +217,7 @@ int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const d
   double2* ynew = y + total;
   T2* zA = reinterpret_cast<T2*>(ynew + total);
   T2* zB = zA + total;
-  const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  const int opsz = a.dense ? a.M * a.M : a.nX * a.nX + a.nY * a.nY;
   const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(T2));
   QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_heom_propagate: %zu bytes of LDS", shmem);
   if (shmem > 64 * 1024)
@@ -261,13 +264,14 @@ static int fill_args(qsx::HeomArgs& a, int B, int n_ado, int M, int n, const voi
   QSX_CHECK_ARG(K1 >= 1 && nbins >= K1 && nbins % K1 == 0 && nrel >= 1,
                 "qsx_heom: bad bin tables nbins=%d K1=%d nrel=%d", nbins, K1, nrel);
   const bool dense = n == 0;
-  QSX_CHECK_ARG(dense ? M <= 64 : (n >= 1 && n * n == M),
-                "qsx_heom: commutator form needs M = n^2, dense form M <= 64");
+  QSX_CHECK_ARG(dense ? M <= 64 : (n >= 1 && M % n == 0),
+                "qsx_heom: commutator form needs M = nX * nY with nX = n, dense form M <= 64");
   QSX_CHECK_ARG(M <= kThreads, "qsx_heom: M=%d exceeds one work-group", M);
   a.B = B;
   a.n_ado = n_ado;
   a.M = M;
-  a.n = n;
+  a.nX = dense ? 0 : n;
+  a.nY = dense ? 0 : M / n;
   a.dense = dense;
   a.nbins = nbins;
   a.K1 = K1;
@@ -300,7 +304,7 @@ extern "C" int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n
                      rel_site, rel_occ);
   if (st) return st;
   QSX_CHECK_ARG(y && out, "qsx_heom_apply: null pointer");
-  const int opsz = a.dense ? a.M * a.M : a.n * a.n;
+  const int opsz = a.dense ? a.M * a.M : a.nX * a.nX + a.nY * a.nY;
   const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(double2));
   QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_heom_apply: %zu bytes of LDS", shmem);
   const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;

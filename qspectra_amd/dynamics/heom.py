@@ -31,6 +31,12 @@ from .liouville_space import (LiouvilleSpaceOperator, super_commutator_matrix, _
 from .unitary import UnitaryModel
 
 
+# Liouville blocks of at most this many elements take the dense -i[H, .]
+# block in the stage kernel; single blocks XY above it take the commutator
+# form with H_X and H_Y (the square blocks XX always do)
+DENSE_BLOCK_MAX = 64
+
+
 def matsubara_frequencies(K, gamma, T):
     """nu_0 = gamma (Drude pole), nu_k = 2 pi k T (Matsubara), k = 1..K."""
     v = 2 * np.pi * T * np.arange(K + 1)
@@ -293,11 +299,12 @@ class HEOMModel(DynamicalModel):
                     bsrc=bsrc, bcoef=bcoef, ck=np.asarray(c[:K + 1], dtype=complex),
                     nrel=nrel, rel_site=rel_site, rel_occ=rel_occ)
 
-    def _square_block(self, liouville_subspace):
-        """Hilbert manifold X when the subspace is the single block 'XX'."""
+    def _single_block(self, liouville_subspace):
+        """Hilbert manifolds (X, Y) when the subspace is the single block
+        'XY'."""
         subs = liouville_subspace.split(',')
-        if len(subs) == 1 and subs[0][0] == subs[0][1]:
-            return subs[0][0]
+        if len(subs) == 1:
+            return subs[0][0], subs[0][1]
         return None
 
     def HEOM_tensor(self, liouville_subspace, heisenberg_picture=False):
@@ -350,20 +357,24 @@ class HEOMModel(DynamicalModel):
             return cache[key]
         tb = self.tables(liouville_subspace, heisenberg_picture)
         dev = _native.device()
-        X = self._square_block(liouville_subspace)
+        XY = self._single_block(liouville_subspace)
         hams = self._members()
         uc = self.unit_convert
-        if X is not None:
-            sl = self.hamiltonian.hilbert_subspace_index(X, self.hilbert_subspace)
-            Hs = np.stack([h.H(self.hilbert_subspace)[sl, sl] for h in hams])
-            if heisenberg_picture:
-                Hs = Hs.transpose(0, 2, 1)
-            Hop = _native.to_device(-1j * uc * Hs)
-            n_comm = Hs.shape[-1]
+        if XY is not None and (tb['M'] > DENSE_BLOCK_MAX or XY[0] == XY[1]):
+            # commutator form: H_X and H_Y blocks of every member
+            blocks = []
+            for Z in XY:
+                sl = self.hamiltonian.hilbert_subspace_index(Z, self.hilbert_subspace)
+                Hz = np.stack([h.H(self.hilbert_subspace)[sl, sl] for h in hams])
+                if heisenberg_picture:
+                    Hz = Hz.transpose(0, 2, 1)
+                blocks.append(Hz.reshape(len(hams), -1))
+            Hop = _native.to_device(-1j * uc * np.concatenate(blocks, axis=1))
+            n_comm = int(round(np.sqrt(blocks[0].shape[1])))
         else:
-            if tb['M'] > 64:
-                raise NotImplementedError('HEOM on a multi-block subspace of more than 64 '
-                                          'elements')
+            if tb['M'] > DENSE_BLOCK_MAX:
+                raise NotImplementedError('HEOM on a multi-block subspace of more than %d '
+                                          'elements' % DENSE_BLOCK_MAX)
             lm = self.lspace_model.with_hamiltonians(hams) if len(hams) > 1 else self.lspace_model
             Hop = lm.liouvillian(liouville_subspace, heisenberg_picture).contiguous()
             n_comm = 0

@@ -152,3 +152,62 @@ def test_heom_third_order_matches_oracle_and_reference(gpu, name, kw, monkeypatc
     assert rel_err(X, g[name + '_abs_X']) < 1e-7
     (f1, t2, f3), X2 = qa.two_dimensional_spectra(mp, 8.5, population_times=[0., 3.], dt=dt)
     assert X2.shape == (len(f1), 2, len(f3)) and np.isfinite(X2).all()
+
+
+def test_heom_rectangular_commutator_blocks(gpu, monkeypatch):
+    """Single Liouville blocks XY with X != Y through the commutator form
+    -i unit (H_X rho - rho H_Y): forced on the dimer's 'eg' block against
+    the reference's CSR generator, and the 7-site 'fe' block (M = 147, above
+    the dense-block limit) against the oracle's sparse generator, in both
+    pictures (RHS 1e-13)."""
+    from qspectra_amd.dynamics import heom as heom_mod
+    from oracle import system as S
+    from oracle import liouville as Lv
+    gh = golden('heom')
+    rs = np.random.RandomState(4)
+    monkeypatch.setattr(heom_mod, 'DENSE_BLOCK_MAX', 0)
+    m = _model()
+    L = golden_csr(gh, 'dimer_eg')
+    y = rs.randn(L.shape[0]) + 1j * rs.randn(L.shape[0])
+    for heis in (False, True):
+        eom = m.equation_of_motion('eg', heisenberg_picture=heis)
+        assert eom.n_comm == 2
+        assert rel_err(eom(0, y), (L.T if heis else L) @ y) < 1e-13, heis
+    monkeypatch.undo()
+    g7 = golden('redfield_7site')
+    hp = qa.ElectronicHamiltonian(g7['in_H1'], bath=qa.DebyeBath(qa.CM_K * 300, 35, 106),
+                                  dipoles=g7['in_dipoles'])
+    ho = S.ElectronicHamiltonian(g7['in_H1'], bath=S.DebyeBath(S.CM_K * 300, 35, 106),
+                                 dipoles=g7['in_dipoles']).in_rotating_frame(12400.0)
+    mp = qa.HEOMModel(hp, rw_freq=12400.0, hilbert_subspace='gef', unit_convert=qa.CM_FS,
+                      level_cutoff=2, K=1)
+    for heis in (False, True):
+        Lo = Lv.heom_evolve_matrix(ho, 'gef', 'fe', S.CM_FS, K=1, level_cutoff=2,
+                                   heisenberg_picture=heis)
+        eom = mp.equation_of_motion('fe', heisenberg_picture=heis)
+        assert eom.M == 147 and eom.n_comm == 21
+        y = rs.randn(Lo.shape[0]) + 1j * rs.randn(Lo.shape[0])
+        assert rel_err(eom(0, y), Lo @ y) < 1e-13, heis
+
+
+def test_heom_7site_esa_third_order_matches_oracle(gpu):
+    """ESA pathways of a 7-site HEOM model (Heisenberg t3 in the 147-element
+    'fe' block on the stage kernel) against the oracle's RK4 on the same
+    schedule (FP64 1e-10)."""
+    from oracle import system as S
+    from oracle import liouville as Lv
+    g7 = golden('redfield_7site')
+    extra = float(g7['extra'])
+    hp = qa.ElectronicHamiltonian(g7['in_H1'], bath=qa.DebyeBath(qa.CM_K * 300, 35, 106),
+                                  dipoles=g7['in_dipoles'], energy_spread_extra=extra)
+    ho = S.ElectronicHamiltonian(g7['in_H1'], bath=S.DebyeBath(S.CM_K * 300, 35, 106),
+                                 dipoles=g7['in_dipoles'], energy_spread_extra=extra)
+    mp = qa.HEOMModel(hp, hilbert_subspace='gef', unit_convert=qa.CM_FS, level_cutoff=2, K=1)
+    mo = Lv.HEOMOracleModel(ho, hilbert_subspace='gef', unit_convert=S.CM_FS, level_cutoff=2,
+                            K=1)
+    dt = mp.time_step / 4
+    kw = dict(population_times=[0., 3.], geometry='-++', include_signal='ESA')
+    (t1, t2, t3), R = qa.third_order_response(mp, 5.5, dt=dt, **kw)
+    _, Ro = P.third_order_response(mo, 5.5, dt=dt, **kw)
+    assert R.shape == (len(t1), 2, len(t3)) and len(t1) >= 2
+    assert rel_err(R, Ro) < 1e-10
