@@ -13,8 +13,9 @@
 // Only the n nonzero samples enter, so the direct sum costs N*n complex
 // multiply-adds, with no plan and nothing compiled at run time (a library FFT
 // of a new length is compiled on first use).  The exponent is reduced exactly
-// in integers, m = s_r * (lo + j - c) mod N, and the twiddle read from a table
-// exp(-2 pi i m / N) built once per call.  The adjoint (the backward of the
+// in integers, m = s_r * (lo + j - c) mod N; every 16th twiddle is read from
+// a table exp(-2 pi i m / N) built once per call, the ones in between are
+// advanced by one complex multiplication with the step twiddle.  The adjoint (the backward of the
 // transform, x -> conj(F)^T X) is the same sum over r with conjugated
 // twiddles.
 //
@@ -31,6 +32,7 @@ namespace {
 
 constexpr int kDftThreads = 256;
 constexpr int kDftTile = 1024;
+constexpr int kAnchor = 16;   // samples between exact twiddles (divides kDftTile)
 
 __global__ __launch_bounds__(kDftThreads) void dft_twiddle_kernel(int N, double2* __restrict__ T) {
   const int m = blockIdx.x * kDftThreads + threadIdx.x;
@@ -55,6 +57,7 @@ __global__ __launch_bounds__(kDftThreads) void dft_kernel(
   const int u = blockIdx.x * kDftThreads + threadIdx.x;
   const int64_t a = mod_n(a0 + (int64_t)da * u, N);
   const int64_t step = mod_n(a * dd, N);
+  const int64_t anchor_step = mod_n(step * kAnchor, N);
   for (int64_t b = blockIdx.y; b < batch; b += gridDim.y) {
     const double2* row = in + b * n_in;
     double2 acc = make_double2(0.0, 0.0);
@@ -64,16 +67,23 @@ __global__ __launch_bounds__(kDftThreads) void dft_kernel(
       for (int k = threadIdx.x; k < len; k += kDftThreads) xs[k] = row[v0 + k];
       __syncthreads();
       if (u < n_out) {
+        // exact twiddle from the table every kAnchor samples, in between
+        // multiplied by the step twiddle (error <= kAnchor rounding steps)
         int64_t m = mod_n(a * mod_n(d0 + (int64_t)dd * v0, N), N);
-        for (int k = 0; k < len; ++k) {
-          double2 t = T[m];
-          if (adj) t.y = -t.y;
-          const double2 x = xs[k];
-          acc.x = fma(x.x, t.x, acc.x);
-          acc.x = fma(-x.y, t.y, acc.x);
-          acc.y = fma(x.x, t.y, acc.y);
-          acc.y = fma(x.y, t.x, acc.y);
-          m += step;
+        const double sg = adj ? -1.0 : 1.0;
+        const double2 w = make_double2(T[step].x, sg * T[step].y);
+        for (int k0 = 0; k0 < len; k0 += kAnchor) {
+          double2 t = make_double2(T[m].x, sg * T[m].y);
+          const int kend = min(k0 + kAnchor, len);
+          for (int k = k0; k < kend; ++k) {
+            const double2 x = xs[k];
+            acc.x = fma(x.x, t.x, acc.x);
+            acc.x = fma(-x.y, t.y, acc.x);
+            acc.y = fma(x.x, t.y, acc.y);
+            acc.y = fma(x.y, t.x, acc.y);
+            t = make_double2(fma(t.x, w.x, -t.y * w.y), fma(t.x, w.y, t.y * w.x));
+          }
+          m += anchor_step;
           if (m >= N) m -= N;
         }
       }
