@@ -17,19 +17,21 @@
 //
 // One work-group per right-hand side.  The stage vector (rho and all O) and
 // the per-site intermediates (Osum_s, L_s rho, Osum_s rho, L_s^+ Osum_s,
-// L_s rho Osum_s^+, Osum_s rho L_s^+, b and i H - a^+) live in LDS (the
-// operators are read through the cache); thread t
-// owns the elements t, t + NT, ...; their state y, RK4 accumulator and next
-// stage value sit in a workspace slice of the right-hand side, read and
-// written by the owner only (L1/L2 resident).  Per stage, in phases
-// separated by barriers and split into per-site items so that no item loops
-// longer than max(N, S, P): (1) Osum; (2) the three per-site products;
-// (3) the two per-site triple products, and a summed over sites (stored as
-// b and i H - a^+); (4) the derivative of every owned element and its RK4
-// update into the workspace; (5) the next stage vector copied into LDS.  The host state
-// layout is the reference's (vec(rho) then vec(O) of shape (P, S, N, N),
-// both column-major); in LDS rho and each O[p, s] are row-major N x N
-// blocks, O ordered p-major.
+// L_s rho Osum_s^+, Osum_s rho L_s^+, b, i H - a^+ and be = unit b +
+// sum_i f_i V_i) live in LDS (the operators are read through the cache).
+// Per stage, in phases separated by barriers and split into per-site items
+// so that no item loops longer than max(N, S, P): (1) Osum; (2) the three
+// per-site products; (3) the two per-site triple products, and a summed
+// over sites (stored as b, i H - a^+ and be); (4) the derivative of every
+// owned element and its RK4 update into the workspace; (5) the next stage
+// vector copied into LDS.  The O derivative is unit (Gamma L - w O) +
+// be O - O be (the field commutators folded into be).  Thread t owns the
+// elements t, t + NT, ...; their state y, RK4 accumulator and next stage
+// value sit in a workspace slice of the right-hand side, read and written by
+// the owner only (L2 resident), the next element's entries loaded while the
+// current one is computed.  The host state layout is the reference's
+// (vec(rho) then vec(O) of shape (P, S, N, N), both column-major); in LDS
+// rho and each O[p, s] are row-major N x N blocks, O ordered p-major.
 
 #include "qsx_common.h"
 
@@ -77,6 +79,188 @@ __device__ inline int zofe_ref_index(int e, int N, int S, int P) {
   return N2 + p + P * (s + S * (i + N * j));
 }
 
+// LDS of a stage work-group (row-major N x N blocks)
+struct ZofeLds {
+  double2* z;    // [size] stage vector: rho, then O[p * S + s]
+  double2* Os;   // [S][N2] sum over pseudomodes
+  double2* T1;   // [S][N2] L_s rho
+  double2* T2;   // [S][N2] Osum_s rho
+  double2* U;    // [S][N2] L_s^+ Osum_s
+  double2* X1;   // [S][N2] L_s rho Osum_s^+
+  double2* X2;   // [S][N2] Osum_s rho L_s^+
+  double2* bm;   // [N2]    b = -i H - a
+  double2* gm;   // [N2]    i H - a^+
+  double2* be;   // [N2]    unit b + sum_i f_i V_i (the O blocks' commutator operator)
+  double2* fb;   // [2][3 * kZofeMaxFields] field values
+};
+
+__device__ inline ZofeLds zofe_lds(unsigned char* smem, int N, int S, int size) {
+  const int N2 = N * N, SN2 = S * N2;
+  ZofeLds l;
+  l.z = reinterpret_cast<double2*>(smem);
+  l.Os = l.z + size;
+  l.T1 = l.Os + SN2;
+  l.T2 = l.T1 + SN2;
+  l.U = l.T2 + SN2;
+  l.X1 = l.U + SN2;
+  l.X2 = l.X1 + SN2;
+  l.bm = l.X2 + SN2;
+  l.gm = l.bm + N2;
+  l.be = l.gm + N2;
+  l.fb = l.be + N2;
+  return l;
+}
+
+// Phases (1)-(3) of a stage at stage input l.z: Osum; the per-site products
+// L rho, Osum rho, L^+ Osum; the triple products, a = sum_s L_s^+ Osum_s and
+// b, i H - a^+ and be = unit b + sum_i f_i V_i.  Every sum over sites or pseudomodes is split
+// into per-site items, so no item loops longer than max(N, S, P).  Ends
+// after a barrier.
+template <int NT>
+__device__ __forceinline__ void zofe_prepare(const ZofeArgs& a, const ZofeLds& l,
+                                             const double2* Hb, const double2* Lb,
+                                             const double2* Vb, const double2* fs, int tid) {
+  const int N = a.N, S = a.S, P = a.P, N2 = N * N, SN2 = S * N2;
+  const double2* rho = l.z;
+  for (int it = tid; it < SN2; it += NT) {             // (1) Osum
+    const int s = it / N2, ij = it % N2;
+    double2 v = make_double2(0.0, 0.0);
+    for (int p = 0; p < P; ++p) v = cadd(v, l.z[N2 + (p * S + s) * N2 + ij]);
+    l.Os[it] = v;
+  }
+  __syncthreads();
+  for (int it = tid; it < 3 * SN2; it += NT) {         // (2) L rho, Osum rho, L^+ Osum
+    const int which = it / SN2, w = it % SN2;
+    const int s = w / N2, i = (w % N2) / N, k = w % N;
+    const double2* Ls = Lb + s * N2;
+    const double2* Oss = l.Os + s * N2;
+    double2 v = make_double2(0.0, 0.0);
+    if (which == 0) {
+      for (int j = 0; j < N; ++j) cfmad(v, Ls[i * N + j], rho[j * N + k]);
+      l.T1[w] = v;
+    } else if (which == 1) {
+      for (int j = 0; j < N; ++j) cfmad(v, Oss[i * N + j], rho[j * N + k]);
+      l.T2[w] = v;
+    } else {
+      for (int j = 0; j < N; ++j) cfmad(v, cconj(Ls[j * N + i]), Oss[j * N + k]);
+      l.U[w] = v;
+    }
+  }
+  __syncthreads();
+  for (int it = tid; it < 2 * SN2 + N2; it += NT) {    // (3) X1, X2, b, i H - a^+, be
+    if (it < 2 * SN2) {
+      const int which = it / SN2, w = it % SN2;
+      const int s = w / N2, i = (w % N2) / N, j = w % N;
+      double2 v = make_double2(0.0, 0.0);
+      if (which == 0) {                                   // (L rho)(Osum)^+
+        for (int k = 0; k < N; ++k)
+          cfmad(v, l.T1[s * N2 + i * N + k], cconj(l.Os[s * N2 + j * N + k]));
+        l.X1[w] = v;
+      } else {                                            // (Osum rho) L^+
+        for (int k = 0; k < N; ++k)
+          cfmad(v, l.T2[s * N2 + i * N + k], cconj(Lb[(s * N + j) * N + k]));
+        l.X2[w] = v;
+      }
+    } else {
+      const int ik = it - 2 * SN2, i = ik / N, k = ik % N;
+      double2 v = make_double2(0.0, 0.0);
+      for (int s = 0; s < S; ++s) v = cadd(v, l.U[s * N2 + ik]);     // a_ik
+      const double2 hh = Hb[ik], hk = Hb[k * N + i];
+      const double2 bik = make_double2(hh.y - v.x, -hh.x - v.y);     // -i H_ik - a_ik
+      l.bm[ik] = bik;
+      l.gm[k * N + i] = make_double2(-hk.y - v.x, hk.x + v.y);      // i H_ki - conj(a_ik)
+      double2 e = make_double2(a.unit * bik.x, a.unit * bik.y);
+      for (int fi = 0; fi < a.nfield; ++fi) e = cadd(e, cmul(fs[fi], Vb[fi * N2 + ik]));
+      l.be[ik] = e;
+    }
+  }
+  __syncthreads();
+}
+
+// d rho_ij / dt (element e = i N + j of the row-major rho), with the field
+// terms; needs the intermediates of zofe_prepare.
+__device__ __forceinline__ double2 zofe_rho_derivative(const ZofeArgs& a, const ZofeLds& l,
+                                                       const double2* Vb, const double2* fs,
+                                                       int e) {
+  const int N = a.N, S = a.S, N2 = N * N;
+  const bool ham_h = a.flags & 1, rho_h = a.flags & 2;
+  const double2* rho = l.z;
+  const int i = e / N, j = e % N;
+  double2 d = make_double2(0.0, 0.0), f = make_double2(0.0, 0.0);
+  if (rho_h && ham_h) {                                 // f = d^+
+    double2 dt = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k) cfmad(dt, l.bm[j * N + k], rho[k * N + i]);
+    for (int s = 0; s < S; ++s) dt = cadd(dt, l.X1[s * N2 + j * N + i]);
+    f = cconj(dt);
+  } else if (ham_h) {                                   // rho b^+
+    for (int k = 0; k < N; ++k) cfmad(f, rho[i * N + k], cconj(l.bm[j * N + k]));
+  } else {                                              // rho (i H - a^+)
+    for (int k = 0; k < N; ++k) cfmad(f, rho[i * N + k], l.gm[k * N + j]);
+  }
+  if (!rho_h) {                                         // + sum_s Osum rho L^+
+    for (int s = 0; s < S; ++s) f = cadd(f, l.X2[s * N2 + e]);
+  } else if (!ham_h) {                                  // + c^+_ij = conj(c_ji)
+    double2 cc = make_double2(0.0, 0.0);
+    for (int s = 0; s < S; ++s) cc = cadd(cc, l.X1[s * N2 + j * N + i]);
+    f = cadd(f, cconj(cc));
+  }
+  for (int k = 0; k < N; ++k) cfmad(d, l.bm[i * N + k], rho[k * N + j]);
+  for (int s = 0; s < S; ++s) d = cadd(d, l.X1[s * N2 + e]);
+  double2 v = cadd(d, f);
+  v = make_double2(a.unit * v.x, a.unit * v.y);
+  for (int fi = 0; fi < a.nfield; ++fi) {               // f_i [V_i, rho]
+    const double2* Vi = Vb + fi * N2;
+    double2 cm = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+      cfmad(cm, Vi[i * N + k], rho[k * N + j]);
+      const double2 t = cmul(rho[i * N + k], Vi[k * N + j]);
+      cm.x -= t.x;
+      cm.y -= t.y;
+    }
+    v = cadd(v, cmul(fs[fi], cm));
+  }
+  return v;
+}
+
+// RK4 bookkeeping of one element: returns the next stage value, updates
+// the accumulator an (stages 0-2) or the state yv (stage 3).
+__device__ __forceinline__ double2 zofe_rk4(double2 v, double2& yv, double2& an, int st,
+                                            double h) {
+  const double2 acc = caxpy(kZW[st] * h / 6.0, v, st == 0 ? yv : an);
+  if (st < 3) {
+    an = acc;
+    return caxpy(kZC[st] * h, v, yv);
+  }
+  yv = acc;
+  return acc;
+}
+
+// Saves of the state in l.z at save point s: the projections of vec(rho) or
+// the whole state (reference layout).
+template <int NT>
+__device__ __forceinline__ void zofe_save(const ZofeArgs& a, const ZofeLds& l,
+                                          const double2* Pb, double2* ob, int s, int tid,
+                                          bool& bad) {
+  const int N = a.N, N2 = N * N;
+  if (Pb) {
+    for (int k = tid; k < a.K; k += NT) {
+      double2 v = make_double2(0.0, 0.0);
+      for (int e = 0; e < N2; ++e) cfmad(v, Pb[(int64_t)k * N2 + (e % N) * N + e / N], l.z[e]);
+      ob[(int64_t)s * a.K + k] = v;
+      if (!cfinite(v)) bad = true;
+    }
+  } else {
+    for (int e = tid; e < a.size; e += NT) {
+      const double2 v = l.z[e];
+      ob[(int64_t)s * a.size + zofe_ref_index(e, N, a.S, a.P)] = v;
+      if (!cfinite(v)) bad = true;
+    }
+  }
+}
+
+// Per-element kernel (any N): thread t owns the elements t, t + NT, ...;
+// their state y, RK4 accumulator and next stage value sit in a workspace
+// slice of the right-hand side, read and written by the owner only.
 template <int NT>
 __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __restrict__ y0,
                                                   const int* __restrict__ nsub,
@@ -85,17 +269,8 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
                                                   double2* __restrict__ work,
                                                   int* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = a.N, S = a.S, P = a.P, N2 = N * N, SN2 = S * N * N, size = a.size;
-  double2* z = reinterpret_cast<double2*>(smem);      // [size] stage vector
-  double2* Os = z + size;                             // [S][N2] sum over pseudomodes
-  double2* T1 = Os + SN2;                             // [S][N2] L_s rho
-  double2* T2 = T1 + SN2;                             // [S][N2] Osum_s rho
-  double2* U = T2 + SN2;                              // [S][N2] L_s^+ Osum_s
-  double2* X1 = U + SN2;                              // [S][N2] L_s rho Osum_s^+
-  double2* X2 = X1 + SN2;                             // [S][N2] Osum_s rho L_s^+
-  double2* bm = X2 + SN2;                             // [N2]    b = -i H - a
-  double2* gm = bm + N2;                              // [N2]    i H - a^+
-  double2* fb = gm + N2;                              // [2][3 * kZofeMaxFields]
+  const int N = a.N, S = a.S, P = a.P, N2 = N * N, size = a.size;
+  const ZofeLds l = zofe_lds(smem, N, S, size);
   const int tid = threadIdx.x;
   const int r = blockIdx.x, b = blockIdx.y;
   const double2* Hb = a.H + (int64_t)b * a.strideH;
@@ -104,7 +279,6 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
   const double2* Wb = a.W + (int64_t)b * a.strideG;
   const double2* Vb = a.V ? a.V + (int64_t)b * a.strideV : nullptr;
   const double2* Fb = a.F ? a.F + (int64_t)b * a.strideF : nullptr;
-  const bool ham_h = a.flags & 1, rho_h = a.flags & 2;
   const int nfld = a.nfield;
   const bool loader = tid < 3 * nfld;
 
@@ -115,137 +289,49 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
   for (int e = tid; e < size; e += NT) {
     const double2 v = yb0[zofe_ref_index(e, N, S, P)];
     yw[e] = v;
-    z[e] = v;
+    l.z[e] = v;
   }
-  if (loader && a.nF > 0) fb[tid] = Fb[tid];
+  if (loader && a.nF > 0) l.fb[tid] = Fb[tid];
   __syncthreads();
 
-  // one RK4 stage at stage input z with field samples fs: the derivative of
-  // every owned element and its update into the workspace.  Every sum over
-  // sites or pseudomodes is split into per-site items, so no item has a
-  // loop longer than max(N, S, P).
   auto stage = [&](const double2* fs, int st, double h) {
-    const double2* rho = z;
-    const double2* O = z + N2;
-    for (int it = tid; it < SN2; it += NT) {             // (1) Osum
-      const int s = it / N2, ij = it % N2;
-      double2 v = make_double2(0.0, 0.0);
-      for (int p = 0; p < P; ++p) v = cadd(v, O[(p * S + s) * N2 + ij]);
-      Os[it] = v;
+    zofe_prepare<NT>(a, l, Hb, Lb, Vb, fs, tid);
+    // the workspace entries of the next owned element are loaded while the
+    // current one is computed
+    double2 y_next = make_double2(0.0, 0.0), a_next = make_double2(0.0, 0.0);
+    if (!a.apply && tid < size) {
+      y_next = yw[tid];
+      a_next = st == 0 ? y_next : aw[tid];
     }
-    __syncthreads();
-    for (int it = tid; it < 3 * SN2; it += NT) {         // (2) L rho, Osum rho, L^+ Osum
-      const int which = it / SN2, w = it % SN2;
-      const int s = w / N2, i = (w % N2) / N, k = w % N;
-      const double2* Ls = Lb + s * N2;
-      const double2* Oss = Os + s * N2;
-      double2 v = make_double2(0.0, 0.0);
-      if (which == 0) {
-        for (int j = 0; j < N; ++j) cfmad(v, Ls[i * N + j], rho[j * N + k]);
-        T1[w] = v;
-      } else if (which == 1) {
-        for (int j = 0; j < N; ++j) cfmad(v, Oss[i * N + j], rho[j * N + k]);
-        T2[w] = v;
-      } else {
-        for (int j = 0; j < N; ++j) cfmad(v, cconj(Ls[j * N + i]), Oss[j * N + k]);
-        U[w] = v;
-      }
-    }
-    __syncthreads();
-    for (int it = tid; it < 2 * SN2 + N2; it += NT) {    // (3) X1, X2, b and i H - a^+
-      if (it < 2 * SN2) {
-        const int which = it / SN2, w = it % SN2;
-        const int s = w / N2, i = (w % N2) / N, j = w % N;
-        double2 v = make_double2(0.0, 0.0);
-        if (which == 0) {                                   // (L rho)(Osum)^+
-          for (int k = 0; k < N; ++k)
-            cfmad(v, T1[s * N2 + i * N + k], cconj(Os[s * N2 + j * N + k]));
-          X1[w] = v;
-        } else {                                            // (Osum rho) L^+
-          for (int k = 0; k < N; ++k)
-            cfmad(v, T2[s * N2 + i * N + k], cconj(Lb[(s * N + j) * N + k]));
-          X2[w] = v;
-        }
-      } else {
-        const int ik = it - 2 * SN2, i = ik / N, k = ik % N;
-        double2 v = make_double2(0.0, 0.0);
-        for (int s = 0; s < S; ++s) v = cadd(v, U[s * N2 + ik]);     // a_ik
-        const double2 hh = Hb[ik], hk = Hb[k * N + i];
-        bm[ik] = make_double2(hh.y - v.x, -hh.x - v.y);             // -i H_ik - a_ik
-        gm[k * N + i] = make_double2(-hk.y - v.x, hk.x + v.y);      // i H_ki - conj(a_ik)
-      }
-    }
-    __syncthreads();
     for (int e = tid; e < size; e += NT) {                 // (4) owned elements
-      const double2* X;
-      int i, j;
+      double2 yv = y_next, an = a_next;
+      if (!a.apply && e + NT < size) {
+        y_next = yw[e + NT];
+        a_next = st == 0 ? y_next : aw[e + NT];
+      }
       double2 v;
       if (e < N2) {
-        i = e / N;
-        j = e % N;
-        X = rho;
-        double2 d = make_double2(0.0, 0.0), f = make_double2(0.0, 0.0);
-        if (rho_h && ham_h) {                               // f = d^+
-          double2 dt = make_double2(0.0, 0.0);
-          for (int k = 0; k < N; ++k) cfmad(dt, bm[j * N + k], rho[k * N + i]);
-          for (int s = 0; s < S; ++s) dt = cadd(dt, X1[s * N2 + j * N + i]);
-          f = cconj(dt);
-        } else if (ham_h) {                                 // rho b^+
-          for (int k = 0; k < N; ++k) cfmad(f, rho[i * N + k], cconj(bm[j * N + k]));
-        } else {                                            // rho (i H - a^+)
-          for (int k = 0; k < N; ++k) cfmad(f, rho[i * N + k], gm[k * N + j]);
-        }
-        if (!rho_h) {                                       // + sum_s Osum rho L^+
-          for (int s = 0; s < S; ++s) f = cadd(f, X2[s * N2 + e]);
-        } else if (!ham_h) {                                // + c^+_ij = conj(c_ji)
-          double2 cc = make_double2(0.0, 0.0);
-          for (int s = 0; s < S; ++s) cc = cadd(cc, X1[s * N2 + j * N + i]);
-          f = cadd(f, cconj(cc));
-        }
-        for (int k = 0; k < N; ++k) cfmad(d, bm[i * N + k], rho[k * N + j]);
-        for (int s = 0; s < S; ++s) d = cadd(d, X1[s * N2 + e]);
-        v = cadd(d, f);
+        v = zofe_rho_derivative(a, l, Vb, fs, e);
       } else {
         const int rr = e - N2, ps = rr / N2, ij = rr % N2, s = ps % S;
-        i = ij / N;
-        j = ij % N;
-        X = O + ps * N2;
-        v = cmul(Gb[ps], Lb[s * N2 + ij]);
-        const double2 wv = Wb[ps], xo = X[ij];
-        v.x -= wv.x * xo.x - wv.y * xo.y;
-        v.y -= wv.x * xo.y + wv.y * xo.x;
-        for (int k = 0; k < N; ++k) {
-          cfmad(v, bm[i * N + k], X[k * N + j]);
-          const double2 t = cmul(X[i * N + k], bm[k * N + j]);
+        const int i = ij / N, j = ij % N;
+        const double2* X = l.z + N2 + ps * N2;
+        const double2 g = cmul(Gb[ps], Lb[s * N2 + ij]), wx = cmul(Wb[ps], X[ij]);
+        v = make_double2(a.unit * (g.x - wx.x), a.unit * (g.y - wx.y));
+        for (int k = 0; k < N; ++k) {                       // be X - X be
+          cfmad(v, l.be[i * N + k], X[k * N + j]);
+          const double2 t = cmul(X[i * N + k], l.be[k * N + j]);
           v.x -= t.x;
           v.y -= t.y;
         }
-      }
-      v = make_double2(a.unit * v.x, a.unit * v.y);
-      for (int fi = 0; fi < nfld; ++fi) {                   // f_i [V_i, X]
-        const double2* Vi = Vb + fi * N2;
-        double2 cm = make_double2(0.0, 0.0);
-        for (int k = 0; k < N; ++k) {
-          cfmad(cm, Vi[i * N + k], X[k * N + j]);
-          const double2 t = cmul(X[i * N + k], Vi[k * N + j]);
-          cm.x -= t.x;
-          cm.y -= t.y;
-        }
-        v = cadd(v, cmul(fs[fi], cm));
       }
       if (a.apply) {                                        // derivative only
         zw[e] = v;
         continue;
       }
-      const double2 yv = yw[e];
-      const double2 an = caxpy(kZW[st] * h / 6.0, v, st == 0 ? yv : aw[e]);
-      if (st < 3) {
-        aw[e] = an;
-        zw[e] = caxpy(kZC[st] * h, v, yv);
-      } else {
-        yw[e] = an;
-        zw[e] = an;
-      }
+      zw[e] = zofe_rk4(v, yv, an, st, h);
+      if (st < 3) aw[e] = an;
+      else yw[e] = yv;
     }
     __syncthreads();                                       // all reads of z done
   };
@@ -253,7 +339,7 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
   int64_t g = 0;
   bool bad = false;
   if (a.apply) {                                           // out = F(y0), no fields
-    stage(fb, 0, 0.0);
+    stage(l.fb, 0, 0.0);
     double2* ob = out + ((int64_t)b * a.R + r) * size;
     for (int e = tid; e < size; e += NT) ob[zofe_ref_index(e, N, S, P)] = zw[e];
     return;
@@ -277,36 +363,23 @@ __global__ __launch_bounds__(NT) void zofe_kernel(ZofeArgs a, const double2* __r
       }
       double2 fnext = make_double2(0.0, 0.0);
       if (loader) fnext = Fb[(g + 1 < a.nF ? g + 1 : g) * 3 * nfld + tid];
-      const double2* fc = fb + (g & 1) * 3 * kZofeMaxFields;
+      const double2* fc = l.fb + (g & 1) * 3 * kZofeMaxFields;
       for (int st = 0; st < 4; ++st) {
         const int ti = st == 0 ? 0 : (st == 3 ? 2 : 1);
         stage(fc + ti * nfld, st, h);                      // field values in LDS
-        for (int e = tid; e < size; e += NT) z[e] = zw[e];
-        if (st == 3 && loader) fb[((g + 1) & 1) * 3 * kZofeMaxFields + tid] = fnext;
+        for (int e = tid; e < size; e += NT) l.z[e] = zw[e];
+        if (st == 3 && loader) l.fb[((g + 1) & 1) * 3 * kZofeMaxFields + tid] = fnext;
         __syncthreads();
       }
     }
-    if (Pb) {                                              // projections of vec(rho)
-      for (int k = tid; k < a.K; k += NT) {
-        double2 v = make_double2(0.0, 0.0);
-        for (int e = 0; e < N2; ++e) cfmad(v, Pb[(int64_t)k * N2 + (e % N) * N + e / N], z[e]);
-        ob[(int64_t)s * a.K + k] = v;
-        if (!cfinite(v)) bad = true;
-      }
-    } else {
-      for (int e = tid; e < size; e += NT) {
-        const double2 v = z[e];
-        ob[(int64_t)s * size + zofe_ref_index(e, N, S, P)] = v;
-        if (!cfinite(v)) bad = true;
-      }
-    }
+    zofe_save<NT>(a, l, Pb, ob, s, tid, bad);
   }
   if (bad) status[0] = 1;
 }
 
 inline size_t zofe_lds_bytes(int N, int S, int size) {
   const size_t N2 = (size_t)N * N;
-  return ((size_t)size + 6 * (size_t)S * N2 + 2 * N2 + 2 * 3 * kZofeMaxFields) *
+  return ((size_t)size + 6 * (size_t)S * N2 + 3 * N2 + 2 * 3 * kZofeMaxFields) *
          sizeof(double2);
 }
 
@@ -320,6 +393,7 @@ static int32_t zofe_launch(int apply,
     const void* F, int64_t strideF, int64_t nF, const void* y0, const int32_t* nsub,
     const double* h, int32_t nsave, int32_t K, const void* proj, int64_t strideP, void* out,
     void* workspace, int32_t* status, void* stream);
+
 
 extern "C" int64_t qsx_zofe_workspace_bytes(int32_t B, int32_t R, int32_t N, int32_t S,
                                             int32_t P) {

@@ -239,9 +239,19 @@ def _zofe_with_fields(model, eom, pulses, geometry, polarization, time_extra, ti
     for i, (pulse, _) in enumerate(terms):
         amp, r = field_rate(pulse, rw_freq)
         rate += amp * float(np.abs(V[:, i]).sum(axis=-1).max()) * 2 + r
+    dev = eom.H.device
+    Vd = _native.to_device(V, dev=dev)
+
+    def fields(t_out, t_start, step):
+        n_p, h_p = rk4_schedule(t_out, t_start, step)
+        F_p = field_samples(terms, rw_freq, *stage_times(t_out, t_start, n_p, h_p))
+        return Vd, _native.to_device(F_p[None], dev=dev)
+
+    y0 = _native.to_device(rho0[:, None, :], dev=dev)
     if dt is None:
         from .utils import zofe_step
-        dt = zofe_step(eom, float(t[-1]) - float(t0), rtol, rate)
+        dt = zofe_step(eom, float(t[-1]) - float(t0), rtol, rate, y0=y0, fields=fields,
+                       t0=float(t0))
         gaps = np.diff(np.concatenate([[t0], t]))
         gaps = gaps[gaps > 0]
         if gaps.size:
@@ -251,11 +261,9 @@ def _zofe_with_fields(model, eom, pulses, geometry, polarization, time_extra, ti
     F = field_samples(terms, rw_freq, tg, hg)
     if F.shape[0] == 0:
         F = np.zeros((1, 3, len(terms)), dtype=complex)
-    dev = eom.H.device
     proj, scalar = (zofe_projection(eom, save_func) if isinstance(save_func, LinearMap)
                     else (None, False))
-    out = propagate_zofe(eom, _native.to_device(rho0[:, None, :], dev=dev), t, t0, dt,
-                         proj=proj, V=_native.to_device(V, dev=dev),
+    out = propagate_zofe(eom, y0, t, t0, dt, proj=proj, V=Vd,
                          F=_native.to_device(F[None], dev=dev))[:, 0]      # [B, nt, *]
     if proj is not None:
         return t, out[..., 0] if scalar else out

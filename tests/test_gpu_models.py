@@ -315,3 +315,33 @@ def test_impulsive_probe_matches_oracle(gpu):
     _, Xg = qa.impulsive_probe(m7, states[0], 2 * 63.5, dt=2.0, include_signal='GSB')
     _, Xog = P.impulsive_probe(om, states[0], 2 * 63.5, dt=2.0, include_signal='GSB')
     assert rel_err(Xg, Xog) < 1e-10
+
+
+def test_reference_spectra_tests(gpu):
+    """tests/test_spectra.py of the reference: the Redfield monomer's
+    absorption peaks at its transition energy, rises monotonically to it and
+    falls after it, also in another rotating frame; the dimer's absorption is
+    the same in the site and eigen bases with and without sparse matrices."""
+    from qspectra_amd.simulate.utils import is_constant
+    ham = qa.ElectronicHamiltonian(np.array([[12500.]]), bath=qa.DebyeBath(qa.CM_K * 77, 35, 106),
+                                   dipoles=[[1, 0, 0]])
+    dyn = qa.RedfieldModel(ham, discard_imag_corr=True, unit_convert=qa.CM_FS)
+    f, X = qa.absorption_spectra(dyn, 10000)
+    assert abs(f[np.argmax(X)] - 12500) < 1e-7
+    assert is_constant(np.diff(f))
+    assert np.all(np.diff(X)[:int(len(X) / 2.0 - 1)] > 0)
+    assert np.all(np.diff(X)[-int(len(X) / 2.0 - 1):] < 0)
+    dyn2 = qa.RedfieldModel(ham, rw_freq=12400, discard_imag_corr=True, unit_convert=qa.CM_FS)
+    f2, X2 = qa.absorption_spectra(dyn2, 10000)
+    assert abs(f2[np.argmax(X2)] - 12500) < 0.5
+    hd = qa.ElectronicHamiltonian(np.array([[12881., 120.], [120., 12719.]]),
+                                  bath=qa.DebyeBath(qa.CM_K * 77, 35, 106),
+                                  dipoles=[[1, 0, 0], [2 * np.cos(.3), 2 * np.sin(.3), 0]])
+    spectra = []
+    for basis in ('site', 'eigen'):
+        for sp in (False, True, lambda mat: True):
+            m = qa.RedfieldModel(hd, hilbert_subspace='gef', discard_imag_corr=True,
+                                 unit_convert=qa.CM_FS, evolve_basis=basis, sparse_matrix=sp)
+            spectra.append(qa.absorption_spectra(m, 10000, dt=1.0)[1])   # one schedule
+    for X in spectra[:-1]:
+        assert rel_err(X, spectra[-1]) < 1e-9

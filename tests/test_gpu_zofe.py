@@ -107,3 +107,4 @@ def test_zofe_spectra_and_pump(gpu):
     _, Ro = P.third_order_response(mgo, 6.5, population_times=[0., 4.], dt=dt)
     assert rel_err(R, Ro) < 1e-10
     assert rel_err(R, g['R3']) < 1e-7
+
