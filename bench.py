@@ -42,8 +42,8 @@ def parse_args(argv=None):
                     help='seconds of CPU work allowed for the baseline sample')
     ap.add_argument('--workloads', default='all',
                     help="'all', 'none' or a comma list of ensemble_absorption, "
-                         "adjoint_gradient, heom, pump, example_dimer_2d, zofe_fmo: the "
-                         "further workloads to measure")
+                         "adjoint_gradient, heom, pump, example_dimer_2d, zofe_fmo, "
+                         "jonas_vibronic: the further workloads to measure")
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
                          'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
@@ -185,6 +185,10 @@ PMC_TRAFFIC = {
     'heom stage fp64': 3.525e9,
     'heom stage mixed': 2.679e9,
 }
+
+
+def _host(x):
+    return x.cpu().numpy() if hasattr(x, 'cpu') else np.asarray(x)
 
 
 def _sync_time(torch, fn):
@@ -329,15 +333,25 @@ def pump_leg(torch):
         with timing.kernel_timer() as kt:
             res[prec] = _sync_time(torch, lambda: qa.simulate_pump(m, pump, 'x', precision=prec,
                                                                    **kw))
-        (label, info), = [(k, v) for k, v in kt.summary().items() if k.startswith('driven')]
+        # the main launch: the driven launch with the most steps (the others
+        # are the step calibration's probe runs)
+        driven = [(k, v) for k, v in kt.summary().items() if k.startswith('driven')]
+        label, info = max(driven, key=lambda kv: int(re.search(r'steps=(\d+)', kv[0]).group(1)))
         kern[prec] = (label, info['mean_ms'])
     y64, y32 = res['fp64'][1][1], res['mixed'][1][1]
     err = float(np.abs(y32 - y64).max() / np.abs(y64).max())
     steps = int(re.search(r'steps=(\d+)', kern['fp64'][0]).group(1))
+    # accuracy of the default step: 16 members against a 0.05 fs step
+    small = dict(kw, ensemble_size=16)
+    _, yd = qa.simulate_pump(m, pump, 'x', **small)
+    _, yf = qa.simulate_pump(m, pump, 'x', dt=0.05, **small)
+    yd, yf = _host(yd), _host(yf)
     out = {'workload': 'simulate_pump, 1024 disorder members of the 7-site aggregate, '
                        "'gg,ge,eg,ee' (M = 64), Gaussian pump FWHM 20 fs, 1 ps after the pulse "
                        'at 2 fs output, default RK4 step',
-           'rk4_steps_per_member': steps, 'rel_err_mixed_vs_fp64': err}
+           'rk4_steps_per_member': steps, 'rel_err_mixed_vs_fp64': err,
+           'rel_err_default_step_vs_0.05fs_16_members':
+               float(np.abs(yd - yf).max() / np.abs(yf).max())}
     for prec in ('fp64', 'mixed'):
         sec = res[prec][0]
         out[prec] = {'seconds_end_to_end': sec, 'kernel': kern[prec][0],

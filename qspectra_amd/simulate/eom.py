@@ -100,18 +100,23 @@ def field_samples(terms, rw_freq, tg, hg):
     return F
 
 
-def _field_step(L, C, terms, rw_freq, span, rtol):
-    """RK4 step for the driven equation with estimated global error <= rtol
-    over ``span``: the rate bound is the largest absolute row sum of L plus,
-    for each field term, its peak amplitude times that of C_i, plus the
-    fastest relative rate of change of the fields."""
+def _field_rate_bound(L, C, terms, rw_freq):
+    """Rate bound of the driven equation: the largest absolute row sum of L
+    plus, for each field term, its peak amplitude times that of C_i, plus
+    the fastest relative rate of change of the fields."""
     rho = float(torch.max(torch.sum(torch.abs(L), dim=-1)).item())
     rate = 0.0
     for i, (pulse, _) in enumerate(terms):
         amp, r = field_rate(pulse, rw_freq)
         rho += amp * float(np.abs(C[:, i]).sum(axis=-1).max())
         rate = max(rate, r)
-    rho += rate
+    return rho + rate
+
+
+def _field_step(L, C, terms, rw_freq, span, rtol):
+    """RK4 step for the driven equation with estimated global error <= rtol
+    over ``span`` for any state, from the rate bound (as step_size)."""
+    rho = _field_rate_bound(L, C, terms, rw_freq)
     if rho == 0 or span <= 0:
         return None
     return (120.0 * rtol / (rho * span)) ** 0.25 / rho
@@ -164,8 +169,35 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     if np.any(t < t0):
         raise ValueError('output times must not precede the first pulse (t=%g)' % t0)
     rw_freq = model.rw_freq
+    dev = eom.L.device
+    y0 = _native.to_device(rho0[:, None, :], dev=dev)                  # [B, 1, M]
+    col, coef = _native.ell_pack(C)
+    col = torch.as_tensor(col, device=dev)
     if dt is None:
-        dt = _field_step(eom.L, C, terms, rw_freq, float(t[-1]) - float(t0), rtol)
+        span = float(t[-1]) - float(t0)
+        h_safe = _field_step(eom.L, C, terms, rw_freq, span, rtol)
+        coef64 = _native.to_device(coef, dev=dev)
+
+        def probe(H, n_p):
+            # n_p steps of H and 2 n_p of H / 2 from the thermal states, the
+            # fields sampled on each schedule (FP64)
+            ends = []
+            for hp in (H, H / 2):
+                tp = np.array([t0 + n_p * H])
+                ni, hi = rk4_schedule(tp, t0, hp)
+                Fp = field_samples(terms, rw_freq, *stage_times(tp, t0, ni, hi))
+                nsub_p, hs_p = schedule_tensors(tp, t0, hp, dev)
+                o, st = _native.propagate_driven(eom.L, col, coef64,
+                                                 _native.to_device(Fp[None], dev=dev), y0,
+                                                 nsub_p, hs_p, _native.QSX_FP64)
+                if int(st.item()) != 0:
+                    return None
+                ends.append(o[:, :, -1])
+            return ends
+        from .utils import calibrate_step
+        rho_rate = _field_rate_bound(eom.L, C, terms, rw_freq)
+        dt = calibrate_step(probe, h_safe, 2.0 / rho_rate if rho_rate > 0 else span, span,
+                            rtol, window=float(tf) - float(t0))
         gaps = np.diff(np.concatenate([[t0], t]))
         gaps = gaps[gaps > 0]
         if gaps.size:
@@ -175,12 +207,8 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     F = field_samples(terms, rw_freq, tg, hg)
     if F.shape[0] == 0:
         F = np.zeros((1, 3, len(terms)), dtype=complex)                # nothing to step
-    dev = eom.L.device
     nsub, hs = schedule_tensors(t, t0, dt, dev)
-    y0 = _native.to_device(rho0[:, None, :], dev=dev)                  # [B, 1, M]
     Ft = _native.to_device(F[None], dev=dev)
-    col, coef = _native.ell_pack(C)
-    col = torch.as_tensor(col, device=dev)
     if precision in ('fp64', 'double', None):
         out, status = _native.propagate_driven(eom.L, col, _native.to_device(coef, dev=dev), Ft,
                                                y0, nsub, hs, _native.QSX_FP64)
@@ -251,7 +279,7 @@ def _zofe_with_fields(model, eom, pulses, geometry, polarization, time_extra, ti
     if dt is None:
         from .utils import zofe_step
         dt = zofe_step(eom, float(t[-1]) - float(t0), rtol, rate, y0=y0, fields=fields,
-                       t0=float(t0))
+                       t0=float(t0), window=float(tf) - float(t0))
         gaps = np.diff(np.concatenate([[t0], t]))
         gaps = gaps[gaps > 0]
         if gaps.size:

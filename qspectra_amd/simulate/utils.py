@@ -148,43 +148,26 @@ def _zofe_rate(eom, field_rate=0.0):
     return rate + field_rate
 
 
-def zofe_step(eom, span, rtol=1e-6, field_rate=0.0, y0=None, fields=None, t0=0.0):
-    """RK4 step for the ZOFE equation with estimated global error <= rtol
-    over ``span``.  The rate bound (as step_size for a Liouvillian) gives a
-    step that is safe for any state, and far too small for the strongly
-    damped pseudomodes: with the initial states y0 [B, R, size] the step is
-    calibrated instead -- from the stable step 2 / rate down, pairs of probe
-    runs from y0 (n steps of H and 2n of H / 2, n H >= span / 20 with
-    16 <= n <= 256) estimate the error by Richardson extrapolation,
-    extrapolated linearly to the span; the first H whose estimate is below
-    rtol / 4 is taken (never below the rate-bound step).  ``fields(t_out, t0, dt)`` -> (V, F) gives
-    the field terms of a probe schedule (None: undriven)."""
-    rate = _zofe_rate(eom, field_rate)
-    if rate == 0 or span <= 0:
-        return None
-    h_safe = (120.0 * rtol / (rate * span)) ** 0.25 / rate
-    if y0 is None:
+def calibrate_step(probe, h_safe, h_max, span, rtol, window=0.0):
+    """The largest RK4 step, from h_max down and never below h_safe, whose
+    error estimate is below rtol / 4.  probe(H, n) returns the end states
+    of n steps of H and of 2 n steps of H / 2 from the initial states (None
+    when a run is not finite); their difference (x 16/15, Richardson) is the
+    error of the H run, extrapolated linearly from the probe's n H to the
+    span.  The probe covers max(span / 20, window) (16 to 256 steps)."""
+    if h_safe is None or span <= 0:
         return h_safe
-    H = min(2.0 / rate, span)
+    H = min(h_max, span)
     for _ in range(12):
         if H <= h_safe:
             return h_safe
-        # probe over at least 1/20 of the span (at most 256 steps), so the
-        # initial transient does not dominate the extrapolation
-        n = int(min(256, max(16, np.ceil(span / (20.0 * H)))))
-        ends = []
-        for hp, steps in ((H, n), (H / 2, 2 * n)):
-            tp = np.array([t0 + n * H])
-            nsub, hh = schedule_tensors(tp, t0, hp, y0.device)
-            V, F = fields(tp, t0, hp) if fields is not None else (None, None)
-            out, status = _native.zofe_propagate(eom, y0, nsub, hh, V=V, F=F)
-            ends.append(out[:, :, -1] if int(status.item()) == 0 else None)
-        if ends[0] is not None and ends[1] is not None:
+        n = int(min(256, max(16, np.ceil(max(span / 20.0, window) / H))))
+        ends = probe(H, n)
+        est = float('inf')
+        if ends is not None:
             scale = float(ends[1].abs().max().item())
             diff = float((ends[0] - ends[1]).abs().max().item())
             est = (16.0 / 15.0) * diff / max(scale, 1e-300) * max(span / (n * H), 1.0)
-        else:
-            est = float('inf')
         if np.isfinite(est) and est <= 0.25 * rtol:
             return H
         shrink = 0.5
@@ -192,6 +175,37 @@ def zofe_step(eom, span, rtol=1e-6, field_rate=0.0, y0=None, fields=None, t0=0.0
             shrink = min(0.5, 0.9 * (0.25 * rtol / est) ** 0.25)
         H *= shrink
     return h_safe
+
+
+def zofe_step(eom, span, rtol=1e-6, field_rate=0.0, y0=None, fields=None, t0=0.0,
+              window=0.0):
+    """RK4 step for the ZOFE equation with estimated global error <= rtol
+    over ``span``.  The rate bound (as step_size for a Liouvillian) gives a
+    step that is safe for any state, and far too small for the strongly
+    damped pseudomodes: with the initial states y0 [B, R, size] the step is
+    calibrated instead (calibrate_step, from the stable step 2 / rate).
+    ``fields(t_out, t0, dt)`` -> (V, F) gives the field terms of a probe
+    schedule (None: undriven); ``window`` the time the probe must cover (the
+    pulses)."""
+    rate = _zofe_rate(eom, field_rate)
+    if rate == 0 or span <= 0:
+        return None
+    h_safe = (120.0 * rtol / (rate * span)) ** 0.25 / rate
+    if y0 is None:
+        return h_safe
+
+    def probe(H, n):
+        ends = []
+        for hp in (H, H / 2):
+            tp = np.array([t0 + n * H])
+            nsub, hh = schedule_tensors(tp, t0, hp, y0.device)
+            V, F = fields(tp, t0, hp) if fields is not None else (None, None)
+            out, status = _native.zofe_propagate(eom, y0, nsub, hh, V=V, F=F)
+            if int(status.item()) != 0:
+                return None
+            ends.append(out[:, :, -1])
+        return ends
+    return calibrate_step(probe, h_safe, 2.0 / rate, span, rtol, window)
 
 
 def propagate_zofe(eom, y0, t, t0=None, dt=None, proj=None, V=None, F=None, rtol=1e-6,
