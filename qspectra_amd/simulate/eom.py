@@ -47,8 +47,8 @@ def _simulate_dynamics(dynamical_model, initial_state, duration, times, liouvill
         # hierarchy: the reduced density matrix is ADO 0 of the state
         B = eom.batch
         y = _native.to_device(np.broadcast_to(y0, (B, y0.shape[-1])).copy())
-        ado0 = propagate_heom(eom, y, t, dt=kw.get('dt'),
-                              precision=kw.get('precision', 'fp64')).cpu().numpy()
+        ado0 = propagate_heom(eom, y, t, dt=kw.get('dt'), precision=kw.get('precision', 'fp64'),
+                              rtol=kw.get('rtol', 1e-6)).cpu().numpy()
         if save_func is not None:
             return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in ado0])
         lm = model.lspace_model

@@ -171,7 +171,8 @@ def _batched_linear_response(dynamical_model, liouv_space_path, time_max, initia
             y0 = np.zeros((B, eom.size), dtype=complex)
             y0[:, :bra.shape[-1]] = -bra
             ado0 = propagate_heom(eom, _dev(y0), t, dt=kw.get('dt'),
-                                  precision=kw.get('precision', 'fp64'))   # [B, nt, M1]
+                                  precision=kw.get('precision', 'fp64'),
+                                  rtol=kw.get('rtol', 1e-6))               # [B, nt, M1]
             out = torch.matmul(ado0, _expand(proj, B).transpose(1, 2).contiguous())
             s = out.permute(0, 2, 1).reshape((B,) + lead + (len(t),))
         else:
@@ -331,7 +332,7 @@ def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
     B = len(_members(model))
     n_ado = model.ado_count
     rho0 = model.thermal_state('gg')
-    dt, prec = kw.get('dt'), kw.get('precision', 'fp64')
+    dt, prec, rtol = kw.get('dt'), kw.get('precision', 'fp64'), kw.get('rtol', 1e-6)
     trans = geometry + '-'
     total = None
     for path in paths:
@@ -361,14 +362,16 @@ def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
             M0 = V[0].shape[-1]
             y0 = torch.zeros((B, eoms[0].size), dtype=torch.complex128, device=_native.device())
             y0[:, :V[0].shape[1]] = _expand(_dev(np.einsum('bij,j->bi', V[0], rho0[:M0])), B)
-            st1 = propagate_heom(eoms[0], y0, t1, dt=dt, precision=prec, full_states=True)
+            st1 = propagate_heom(eoms[0], y0, t1, dt=dt, precision=prec, full_states=True,
+                                 rtol=rtol)
             X1 = per_ado(st1, V[1]).reshape(B * len(t1), -1)
             st2 = propagate_heom(eoms[1].repeated(len(t1)), X1, t2, t0=0, dt=dt, precision=prec,
-                                 full_states=True)
+                                 full_states=True, rtol=rtol)
             V_rho2 = per_ado(st2.reshape(B, len(t1), len(t2), -1), V[2])
             g0 = torch.zeros((B, eoms[2].size), dtype=torch.complex128, device=_native.device())
             g0[:, :b3.shape[-1]] = _expand(_dev(b3), B)
-            G3 = propagate_heom(eoms[2], g0, t3, dt=dt, precision=prec, full_states=True)
+            G3 = propagate_heom(eoms[2], g0, t3, dt=dt, precision=prec, full_states=True,
+                                rtol=rtol)
         S3 = G3.shape[-1]
         prod = torch.matmul(V_rho2.reshape(B, -1, S3), G3.transpose(1, 2).contiguous())
         prod = prod.reshape(B, len(t1), len(t2), len(t3))

@@ -113,13 +113,65 @@ def propagate(eom, y0, t, t0=None, proj=None, dt=None, precision='fp64', rtol=1e
     return (out, yfin) if want_final else out
 
 
-def propagate_heom(eom, y0, t, t0=None, dt=None, precision='fp64', full_states=False):
+def heom_rate(eom, iters=40):
+    """Spectral-radius estimate of a hierarchy generator (times 1.5): power
+    iteration with qsx_heom_apply from a fixed pseudo-random start, the
+    largest growth factor over the samples; cached on the equation of
+    motion."""
+    cached = getattr(eom, '_rate_estimate', None)
+    if cached is not None:
+        return cached
+    gen = torch.Generator().manual_seed(12345)
+    B, S = eom.batch, eom.size
+    v = torch.complex(torch.randn(B, S, generator=gen, dtype=torch.float64),
+                      torch.randn(B, S, generator=gen, dtype=torch.float64)).to(eom.Hop.device)
+    growth = torch.zeros(B, dtype=torch.float64, device=v.device)
+    for _ in range(iters):
+        v = v / v.abs().amax(dim=1, keepdim=True).clamp_min(1e-300)
+        w = _native.heom_apply(eom, v)
+        growth = w.abs().amax(dim=1)
+        v = w
+    rate = 1.5 * float(growth.max().item())
+    eom._rate_estimate = rate
+    return rate
+
+
+def heom_step(eom, y0, span, rtol=1e-6, t0=0.0):
+    """Default RK4 step of a hierarchy over ``span``: calibrate_step between
+    the stable step 2 / rate and the rate-bound step (rate from heom_rate),
+    probing from the initial hierarchy states y0 [B, n_ado * M]."""
+    rate = heom_rate(eom)
+    if rate == 0 or span <= 0:
+        return None
+    h_safe = (120.0 * rtol / (rate * span)) ** 0.25 / rate
+
+    def probe(H, n):
+        ends = []
+        for hp, steps in ((H, n), (H / 2, 2 * n)):
+            _, yf, status = _native.heom_propagate(eom, y0, [steps], [hp], _native.QSX_FP64,
+                                                   want_final=True)
+            if int(status.item()) != 0:
+                return None
+            ends.append(yf)
+        return ends
+    return calibrate_step(probe, h_safe, 2.0 / rate, span, rtol)
+
+
+def propagate_heom(eom, y0, t, t0=None, dt=None, precision='fp64', full_states=False,
+                   rtol=1e-6):
     """Hierarchy propagation: y0 [B, n_ado * M] complex128 tensor.  Returns
     ADO 0 at every output time, [B, len(t), M], or with full_states the
-    whole hierarchy, [B, len(t), n_ado * M] (one launch per output time)."""
+    whole hierarchy, [B, len(t), n_ado * M] (one launch per output time).
+    Without ``dt`` the step is calibrated to ``rtol`` (heom_step)."""
     t = np.asarray(t, dtype=float).reshape(-1)
     if t0 is None:
         t0 = float(t[0])
+    if dt is None:
+        dt = heom_step(eom, y0, float(t[-1]) - float(t0), rtol, float(t0))
+        gaps = np.diff(np.concatenate([[t0], t]))
+        gaps = gaps[gaps > 0]
+        if gaps.size:
+            dt = float(gaps.min()) if dt is None else min(dt, float(gaps.min()))
     n, h = rk4_schedule(t, t0, dt)
     mode = _mode(precision)
     if not full_states:
@@ -341,7 +393,7 @@ def _integrate_zofe(f, y0, t, t0, save_func, kwargs):
 def _integrate_heom(f, y0, t, t0, save_func, kwargs):
     dt = kwargs.pop('dt', None)
     precision = kwargs.pop('precision', 'fp64')
-    kwargs.pop('rtol', None)
+    rtol = kwargs.pop('rtol', 1e-6)
     kwargs.pop('schedule', None)
     unknown = set(kwargs) - set(ZVODE_OPTIONS)
     if unknown:
@@ -352,7 +404,8 @@ def _integrate_heom(f, y0, t, t0, save_func, kwargs):
     y = y0.reshape(B, f.size) if B > 1 else y0.reshape(-1, f.size)
     if y.shape[0] != B:
         raise ValueError('hierarchy states must have a leading axis of %d samples' % B)
-    out = propagate_heom(f, _native.to_device(y), t, t0, dt, precision, full_states=True)
+    out = propagate_heom(f, _native.to_device(y), t, t0, dt, precision, full_states=True,
+                         rtol=rtol)
     out = out.cpu().numpy().reshape(lead + (len(np.atleast_1d(t)), f.size))
     if save_func is not None:
         flat = out.reshape(-1, f.size)

@@ -70,6 +70,9 @@ def test_heom_dynamics_matches_reference_zvode(gpu):
     t, rho = qa.simulate_dynamics(m, rho0, times=gh['dyn_t'], dt=float(gh['dyn_time_step']) / 8)
     ref = gh['dyn_y'][:, :4].reshape(-1, 2, 2, order='F')
     assert rel_err(rho, ref) < 1e-6
+    # the default step (calibrated by probe runs to rtol = 1e-6 over the span)
+    t, rho = qa.simulate_dynamics(m, rho0, times=gh['dyn_t'])
+    assert rel_err(rho, ref) < 1e-5
     # 7-site hierarchy (level_cutoff 3): RK4 against the oracle on the golden CSR
     g7 = golden('redfield_7site')
     m7 = _model('redfield_7site', rw=float(gh['agg7_rw_freq']))
@@ -150,6 +153,15 @@ def test_heom_third_order_matches_oracle_and_reference(gpu, name, kw, monkeypatc
     f, X = qa.absorption_spectra(mp, 150.5, dt=dt)
     np.testing.assert_allclose(f, g[name + '_abs_f'])
     assert rel_err(X, g[name + '_abs_X']) < 1e-7
+    # default steps: the dense-generator path (rate-bound step) and the stage
+    # kernel (calibrated step) against the tight zvode outputs
+    key = '%s_R3_mpp' % name
+    _, Rd = qa.third_order_response(mp, 8.5, population_times=[0., 3., 7.])
+    assert rel_err(Rd, g[key]) < 1e-5
+    monkeypatch.setattr(response, 'HEOM_DENSE_MAX', 0)
+    _, Rk = qa.third_order_response(mp, 8.5, population_times=[0., 3., 7.])
+    monkeypatch.undo()
+    assert rel_err(Rk, g[key]) < 1e-5
     (f1, t2, f3), X2 = qa.two_dimensional_spectra(mp, 8.5, population_times=[0., 3.], dt=dt)
     assert X2.shape == (len(f1), 2, len(f3)) and np.isfinite(X2).all()
 
