@@ -93,13 +93,14 @@ class FusedSpectrum(object):
     transform matrices), the whole 2-D spectrum replayed from one HIP graph."""
 
     def __init__(self, model, work, precision='mixed'):
-        from qspectra_amd.simulate.fused import ThirdOrderPlan
+        from qspectra_amd.simulate.fused import ThirdOrderPlan, plan_chunk
         self.model = model
-        self.plan = ThirdOrderPlan(model, work['coherence_time_max'],
-                                   population_times=work['population_times'],
-                                   geometry=work['geometry'],
-                                   include_signal=work['include_signal'], dt=work['dt'],
-                                   precision=precision, fourier=True).capture()
+        kw = dict(population_times=work['population_times'], geometry=work['geometry'],
+                  include_signal=work['include_signal'], dt=work['dt'], precision=precision,
+                  fourier=True)
+        chunk = plan_chunk(model, work['coherence_time_max'], **kw)
+        self.plan = ThirdOrderPlan(model, work['coherence_time_max'], chunk=chunk,
+                                   **kw).capture()
 
     def __call__(self):
         return self.plan.run()[0]

@@ -5,22 +5,24 @@
 // (16 x 16 tiles per wave, 32 x 32 per 4-wave work-group), small ones on the
 // vector units.  MIXED: complex64 A and B with FP32 accumulation on the
 // vector units.  Vector path: 32 x 32 or 16 x 16 tiles per 256-thread
-// work-group, 2 x 2 or 1 x 1 block per thread, K in chunks of 16.  Used for the Horner construction of the RK4
-// propagator, its powers, save-point dipole maps and response contractions:
-// operands here are at most a few hundred wide, so the kernel favours many
-// small tiles over deep register blocking.
+// work-group, 2 x 2 or 1 x 1 block per thread, K in chunks of 16 or (for
+// k >= 48) 64.  Used for the Horner construction of the RK4 propagator, its
+// powers, save-point dipole maps and response contractions: operands here
+// are at most a few hundred wide and the products sit on sequential chains,
+// so the kernels favour many small tiles and few K chunks (each chunk's
+// operand loads are issued together: one memory latency per chunk) over
+// deep register blocking.
 
 #include "qsx_common.h"
 
 namespace qsx {
 namespace {
 
-constexpr int kTK = 16;
-
 // TM x TN output tile per 256-thread workgroup, each thread an (TM/16) x
-// (TN/16) block.  op(A) = A or A^T (TA), op(B) = B or B^T (TB); the LDS
-// staging loads walk the contiguous index of the stored operand.
-template <typename T, int TM, int TN, bool TA, bool TB>
+// (TN/16) block, K in chunks of TK.  op(A) = A or A^T (TA), op(B) = B or B^T
+// (TB); the LDS staging loads walk the contiguous index of the stored
+// operand and are unrolled, so a chunk's loads are all in flight together.
+template <typename T, int TM, int TN, int TK, bool TA, bool TB>
 __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2 alpha,
                                                     const cplx_t<T>* __restrict__ A, int64_t lda,
                                                     int64_t sA, const cplx_t<T>* __restrict__ B,
@@ -29,8 +31,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
                                                     int64_t sC, double2 gamma) {
   using T2 = cplx_t<T>;
   constexpr int RM = TM / 16, RN = TN / 16;
-  __shared__ T2 As[kTK][TM + 1];
-  __shared__ T2 Bs[kTK][TN + 1];
+  static_assert((TM * TK) % 256 == 0 && (TN * TK) % 256 == 0, "tile and chunk sizes");
+  __shared__ T2 As[TK][TM + 1];
+  __shared__ T2 Bs[TK][TN + 1];
   const int tid = threadIdx.x;
   const int tx = tid % 16, ty = tid / 16;
   const int bm = blockIdx.x * TM, bn = blockIdx.y * TN, bz = blockIdx.z;
@@ -42,26 +45,44 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
   for (int i = 0; i < RM; ++i)
 #pragma unroll
     for (int j = 0; j < RN; ++j) acc[i][j] = T2{0, 0};
-  for (int k0 = 0; k0 < k; k0 += kTK) {
-    for (int e = tid; e < TM * kTK; e += 256) {
+  for (int k0 = 0; k0 < k; k0 += TK) {
+    T2 av[TM * TK / 256], bv[TN * TK / 256];
+#pragma unroll
+    for (int u = 0; u < TM * TK / 256; ++u) {
+      const int e = tid + 256 * u;
       int mm, kk;
       if (TA) { kk = e / TM; mm = e % TM; }       // A stored k-major: walk m
-      else { mm = e / kTK; kk = e % kTK; }        // A stored m-major: walk k
+      else { mm = e / TK; kk = e % TK; }          // A stored m-major: walk k
       const int gi = bm + mm, gk = k0 + kk;
       const int64_t off = TA ? (int64_t)gk * lda + gi : (int64_t)gi * lda + gk;
-      As[kk][mm] = (gi < m && gk < k) ? Ab[off] : T2{0, 0};
+      av[u] = (gi < m && gk < k) ? Ab[off] : T2{0, 0};
     }
-    for (int e = tid; e < kTK * TN; e += 256) {
+#pragma unroll
+    for (int u = 0; u < TN * TK / 256; ++u) {
+      const int e = tid + 256 * u;
       int kk, nn;
-      if (TB) { nn = e / kTK; kk = e % kTK; }     // B stored n-major: walk k
+      if (TB) { nn = e / TK; kk = e % TK; }       // B stored n-major: walk k
       else { kk = e / TN; nn = e % TN; }          // B stored k-major: walk n
       const int gk = k0 + kk, gj = bn + nn;
       const int64_t off = TB ? (int64_t)gj * ldb + gk : (int64_t)gk * ldb + gj;
-      Bs[kk][nn] = (gk < k && gj < n) ? Bb[off] : T2{0, 0};
+      bv[u] = (gk < k && gj < n) ? Bb[off] : T2{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < TM * TK / 256; ++u) {
+      const int e = tid + 256 * u;
+      if (TA) As[e / TM][e % TM] = av[u];
+      else As[e % TK][e / TK] = av[u];
+    }
+#pragma unroll
+    for (int u = 0; u < TN * TK / 256; ++u) {
+      const int e = tid + 256 * u;
+      if (TB) Bs[e % TK][e / TK] = bv[u];
+      else Bs[e / TN][e % TN] = bv[u];
     }
     __syncthreads();
+    const int kend = min(TK, k - k0);
 #pragma unroll 4
-    for (int kk = 0; kk < kTK; ++kk) {
+    for (int kk = 0; kk < kend; ++kk) {
       T2 a[RM], b[RN];
 #pragma unroll
       for (int i = 0; i < RM; ++i) a[i] = As[kk][RM * ty + i];
@@ -95,8 +116,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
 // l = A[l % 16][k0 + l / 16], B lane l = B[k0 + l / 16][l % 16]; result lane
 // l, element i = D[l / 16 + 4 i][l % 16].  Complex product = four real MFMAs
 // (re re, -im im, re im, im re) on real/imaginary planes staged in LDS, K
-// in chunks of TK (64 for one-wave, 32 for four-wave work-groups) so that
-// the operand loads of a chunk are all in flight together.
+// in chunks of TK (64) whose operand loads are all in flight together.
 typedef double qsx_double4 __attribute__((ext_vector_type(4)));
 
 template <bool TA, bool TB, int WM, int WN, int TK>
@@ -116,26 +136,42 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_mfma_kernel(
   double2* Cb = C + (int64_t)bz * sC;
   qsx_double4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
   const int r16 = lane % 16, q4 = lane / 16;
+  static_assert((TM * TK) % NT == 0 && (TN * TK) % NT == 0, "tile and chunk sizes");
   for (int k0 = 0; k0 < k; k0 += TK) {
-    for (int e = tid; e < TM * TK; e += NT) {
+    double2 av[TM * TK / NT], bv[TN * TK / NT];   // a chunk's loads, all in flight
+#pragma unroll
+    for (int u = 0; u < TM * TK / NT; ++u) {
+      const int e = tid + NT * u;
       int mm, kk;
       if (TA) { kk = e / TM; mm = e % TM; }
       else { mm = e / TK; kk = e % TK; }
       const int gi = bm + mm, gk = k0 + kk;
       const int64_t off = TA ? (int64_t)gk * lda + gi : (int64_t)gi * lda + gk;
-      const double2 v = (gi < m && gk < k) ? Ab[off] : make_double2(0.0, 0.0);
-      Ar[kk][mm] = v.x;
-      Ai[kk][mm] = v.y;
+      av[u] = (gi < m && gk < k) ? Ab[off] : make_double2(0.0, 0.0);
     }
-    for (int e = tid; e < TK * TN; e += NT) {
+#pragma unroll
+    for (int u = 0; u < TN * TK / NT; ++u) {
+      const int e = tid + NT * u;
       int kk, nn;
       if (TB) { nn = e / TK; kk = e % TK; }
       else { kk = e / TN; nn = e % TN; }
       const int gk = k0 + kk, gj = bn + nn;
       const int64_t off = TB ? (int64_t)gj * ldb + gk : (int64_t)gk * ldb + gj;
-      const double2 v = (gk < k && gj < n) ? Bb[off] : make_double2(0.0, 0.0);
-      Br[kk][nn] = v.x;
-      Bi[kk][nn] = v.y;
+      bv[u] = (gk < k && gj < n) ? Bb[off] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < TM * TK / NT; ++u) {
+      const int e = tid + NT * u;
+      const int kk = TA ? e / TM : e % TK, mm = TA ? e % TM : e / TK;
+      Ar[kk][mm] = av[u].x;
+      Ai[kk][mm] = av[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < TN * TK / NT; ++u) {
+      const int e = tid + NT * u;
+      const int kk = TB ? e % TK : e / TN, nn = TB ? e / TK : e % TN;
+      Br[kk][nn] = bv[u].x;
+      Bi[kk][nn] = bv[u].y;
     }
     __syncthreads();
     const int kend = min(TK, k - k0);
@@ -179,14 +215,15 @@ void launch_zgemm_mfma(int trans, hipStream_t s, int m, int n, int k, double2 al
 #undef QSX_ZM
 }
 
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, int TK>
 void launch_zgemm(int trans, dim3 grid, hipStream_t s, int m, int n, int k, double2 al,
                   const void* A, int64_t lda, int64_t sA, const void* B, int64_t ldb, int64_t sB,
                   double2 be, void* C, int64_t ldc, int64_t sC, double2 ga) {
   using T2 = cplx_t<T>;
 #define QSX_ZG(TA_, TB_)                                                                      \
-  hipLaunchKernelGGL((zgemm_kernel<T, TM, TN, TA_, TB_>), grid, dim3(256), 0, s, m, n, k, al, \
-                     (const T2*)A, lda, sA, (const T2*)B, ldb, sB, be, (double2*)C, ldc, sC, ga)
+  hipLaunchKernelGGL((zgemm_kernel<T, TM, TN, TK, TA_, TB_>), grid, dim3(256), 0, s, m, n, k, \
+                     al, (const T2*)A, lda, sA, (const T2*)B, ldb, sB, be, (double2*)C, ldc, sC, \
+                     ga)
   switch (trans & 3) {
     case 0: QSX_ZG(false, false); break;
     case 1: QSX_ZG(true, false); break;
@@ -225,28 +262,27 @@ extern "C" int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_
   // (measured on MI355X: 255 x 255 x 154 x 4 16 us vs 21 us; 255 x 147 x 128
   // 16 us vs 9 us)
   if (precision == QSX_FP64 && tiles32 >= 192) {
-    launch_zgemm_mfma<2, 2, 32>(trans, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC,
+    launch_zgemm_mfma<2, 2, 64>(trans, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC,
                                 ga, batch);
     QSX_CHECK_HIP(hipGetLastError());
     return QSX_OK;
   }
+  const bool deep = k >= 48;                 // few, deep K chunks
+#define QSX_ZL(T_, TM_, TN_)                                                                  \
+  (deep ? launch_zgemm<T_, TM_, TN_, 64>(trans, grid, s, m, n, k, al, A, lda, sA, B, ldb, sB, \
+                                         be, C, ldc, sC, ga)                                  \
+        : launch_zgemm<T_, TM_, TN_, 16>(trans, grid, s, m, n, k, al, A, lda, sA, B, ldb, sB, \
+                                         be, C, ldc, sC, ga))
   if (tiles32 < 192) {
     dim3 grid((m + 15) / 16, (n + 15) / 16, batch);
-    if (precision == QSX_FP64)
-      launch_zgemm<double, 16, 16>(trans, grid, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C,
-                                   ldc, sC, ga);
-    else
-      launch_zgemm<float, 16, 16>(trans, grid, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C,
-                                  ldc, sC, ga);
+    if (precision == QSX_FP64) QSX_ZL(double, 16, 16);
+    else QSX_ZL(float, 16, 16);
   } else {
     dim3 grid((m + 31) / 32, (n + 31) / 32, batch);
-    if (precision == QSX_FP64)
-      launch_zgemm<double, 32, 32>(trans, grid, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C,
-                                   ldc, sC, ga);
-    else
-      launch_zgemm<float, 32, 32>(trans, grid, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C,
-                                  ldc, sC, ga);
+    if (precision == QSX_FP64) QSX_ZL(double, 32, 32);
+    else QSX_ZL(float, 32, 32);
   }
+#undef QSX_ZL
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }

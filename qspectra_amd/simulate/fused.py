@@ -67,7 +67,7 @@ class ThirdOrderPlan(object):
     def __init__(self, model, coherence_time_max, population_time_max=None,
                  population_times=None, geometry='-++', polarization='xxxx',
                  include_signal=None, dt=None, precision='fp64', schedule='auto',
-                 fourier=False):
+                 fourier=False, chunk=None):
         if schedule == 'rk4':
             raise PlanNotApplicable("the fused plan needs the RK4 propagator; use "
                                     "plan='pathways' for schedule='rk4'")
@@ -118,7 +118,7 @@ class ThirdOrderPlan(object):
                 y1 = np.einsum('bij,j->bi', V0, rho0)[:, None, :]
                 y1 = np.broadcast_to(y1, (B,) + y1.shape[1:])
                 stage = _Stage(model.equation_of_motion(s[1]), self.t1, None, dt, precision,
-                               schedule, 1, graph=True)
+                               schedule, 1, chunk=chunk, graph=True)
                 self.t1_stages[k1] = (stage, dev_t(y1))
             k12 = tuple(s[:3])
             if k12 not in self.groups:
@@ -144,7 +144,8 @@ class ThirdOrderPlan(object):
                                   'bra')[:, None, :]
                 b3 = np.broadcast_to(b3, (B,) + b3.shape[1:])
                 stage = _Stage(model.equation_of_motion(s[3], heisenberg_picture=True),
-                               self.t3, None, dt, precision, schedule, 1, graph=True)
+                               self.t3, None, dt, precision, schedule, 1, chunk=chunk,
+                               graph=True)
                 self.t3_stages[k3] = dict(stage=stage, bra=dev_t(b3), col=None)
             V2 = _op_matrices(model, '%s->%s' % (s[2], s[3]), polarization[2], trans[2],
                               'commutator')                            # [B|1, M3, M2]
@@ -278,3 +279,50 @@ class ThirdOrderPlan(object):
         for st in self._statuses:
             if int(st.item()) != 0:
                 raise IntegratorError('integration failed: non-finite state in a response stage')
+
+
+def plan_chunk(model, coherence_time_max, population_time_max=None, population_times=None,
+               geometry='-++', polarization='xxxx', include_signal=None, dt=None,
+               precision='fp64', schedule='auto', fourier=False):
+    """The chunk length of the plan's time-chunked trajectory stages, chosen
+    for the plan as a whole: the stages run concurrently on their streams,
+    so the fastest schedule of each stage on its own is not the fastest
+    plan.  Candidates (8, 12, 16 and sqrt(n1)) are timed as replays of the
+    captured plan; the decision is cached like the stage tuner's (per
+    device and plan shape).  None when tuning is off or the schedule is not
+    'auto' (each stage then takes its own default)."""
+    from ..schedules import autotune, autotune_enabled
+    if schedule != 'auto' or not autotune_enabled():
+        return None
+    if hasattr(model, 'ado_count') or hasattr(model, 'oop_shape'):
+        return None                        # served pathway by pathway, not by a plan
+    t1, t2, t3 = third_order_grids(model, coherence_time_max, population_time_max,
+                                   population_times)
+    members = _members(model)
+    sizes = []
+    paths = _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal)
+    for path in paths:
+        s = path.split('->')
+        sizes.append(model.equation_of_motion(s[1]).L.shape[-1])
+        sizes.append(model.equation_of_motion(s[3], heisenberg_picture=True).L.shape[-1])
+    key = 'plan M=%s B=%d n1=%d n2=%d n3=%d %s %s %s' % (
+        ','.join(map(str, sizes)), len(members), len(t1), len(t2), len(t3), geometry,
+        precision, 'fourier' if fourier else 'time')
+    cands = sorted(set([8, 12, 16, max(4, int(np.ceil(np.sqrt(len(t1)))))]))
+    cands = [('chunks', c) for c in cands if 2 * c < len(t1)]
+    if not cands:
+        return None
+    built = {}
+
+    def trial(cand):
+        p = built.get(cand)
+        if p is None:
+            p = ThirdOrderPlan(model, coherence_time_max, population_time_max, population_times,
+                               geometry, polarization, include_signal, dt, precision, schedule,
+                               fourier, chunk=cand[1]).capture()
+            built[cand] = p
+        return p.run
+    try:
+        return autotune(key, cands, trial, reps=5)[1]
+    except PlanNotApplicable:
+        return None

@@ -288,12 +288,12 @@ def _third_order_response(dynamical_model, coherence_time_max, population_time_m
         raise ValueError("plan must be 'fused' or 'pathways'")
     p = None
     if plan == 'fused':
-        from .fused import ThirdOrderPlan, PlanNotApplicable
+        from .fused import ThirdOrderPlan, PlanNotApplicable, plan_chunk
+        args = (model, coherence_time_max, population_time_max, population_times, geometry,
+                polarization, include_signal, kw.get('dt'), kw.get('precision', 'fp64'),
+                kw.get('schedule', 'auto'))
         try:
-            p = ThirdOrderPlan(model, coherence_time_max, population_time_max, population_times,
-                               geometry, polarization, include_signal, dt=kw.get('dt'),
-                               precision=kw.get('precision', 'fp64'),
-                               schedule=kw.get('schedule', 'auto'))
+            p = ThirdOrderPlan(*args, chunk=plan_chunk(*args))
         except PlanNotApplicable:
             p = None
     if p is None:
