@@ -16,7 +16,7 @@ import torch
 
 from . import _native
 from .constants import GAUSSIAN_SD_FWHM
-from .operator_tools import (operator_extend, transition_operator, unit_vec,
+from .operator_tools import (operator_extend, transition_operators, unit_vec,
                              hilbert_subspace_index, basis_transform_vector,
                              basis_transform_operator, SubspaceError, tensor, vib_annihilate,
                              vib_create, extend_vib_operator)
@@ -361,8 +361,8 @@ class ElectronicHamiltonian(Hamiltonian):
         if self.dipoles is None:
             raise HamiltonianError('transition dipole moments undefined')
         mu = self.dipoles @ polarization_vector(polarization)
-        return sum(mu[n] * transition_operator(n, self.n_sites, subspace, transitions)
-                   for n in range(self.n_sites))
+        return np.tensordot(mu, transition_operators(self.n_sites, subspace, transitions),
+                            axes=1)
 
     def number_operator(self, site, subspace='gef'):
         return operator_extend(np.diag(unit_vec(site, self.n_sites, dtype=float)), subspace)

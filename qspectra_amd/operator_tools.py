@@ -7,6 +7,8 @@ Everything here is host-side bookkeeping on small matrices.
 """
 from functools import reduce
 
+import functools
+
 import numpy as np
 
 
@@ -121,19 +123,28 @@ def _state_masks(n_sites, subspace):
                     dtype=np.int64)
 
 
+@functools.lru_cache(maxsize=256)
+def transition_operators(n_sites, subspace='gef', include_transitions='-+'):
+    """transition_operator of every site, [n_sites, N, N] (read-only, cached
+    per argument set: the dipole operators of an ensemble share them)."""
+    masks = _state_masks(n_sites, subspace)
+    bits = np.int64(1) << np.arange(n_sites, dtype=np.int64)
+    Mi, Mj = masks[None, :, None], masks[None, None, :]
+    b = bits[:, None, None]
+    out = np.zeros((n_sites, len(masks), len(masks)))
+    if '+' in include_transitions:
+        out[(Mi == (Mj | b)) & ((Mj & b) == 0)] = 1
+    if '-' in include_transitions:
+        out[(Mj == (Mi | b)) & ((Mi & b) == 0)] = 1
+    out.setflags(write=False)
+    return out
+
+
 def transition_operator(n, n_sites, subspace='gef', include_transitions='-+'):
     """Matrix of the excitation creation ('+') and/or annihilation ('-') at
     site n; element (i, j) is 1 when state i = state j with site n added
     ('+') or state j = state i with site n added ('-')."""
-    masks = _state_masks(n_sites, subspace)
-    bit = np.int64(1) << n
-    Mi, Mj = masks[:, None], masks[None, :]
-    out = np.zeros((len(masks), len(masks)))
-    if '+' in include_transitions:
-        out[(Mi == (Mj | bit)) & ((Mj & bit) == 0)] = 1
-    if '-' in include_transitions:
-        out[(Mj == (Mi | bit)) & ((Mi & bit) == 0)] = 1
-    return out
+    return transition_operators(n_sites, subspace, include_transitions)[n].copy()
 
 
 def n_excitations(n_sites=1, n_vibrational_states=1):
