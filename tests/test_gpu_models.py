@@ -345,3 +345,39 @@ def test_reference_spectra_tests(gpu):
             spectra.append(qa.absorption_spectra(m, 10000, dt=1.0)[1])   # one schedule
     for X in spectra[:-1]:
         assert rel_err(X, spectra[-1]) < 1e-9
+
+
+def test_edge_shapes_match_oracle(gpu):
+    """Shapes at the edges of the pathway machinery against the oracle on
+    the same schedule: a monomer in g/e/f (an empty two-exciton block), a
+    single population time, coherence grids of one point, and an ensemble of
+    one member."""
+    mono_p = qa.ElectronicHamiltonian(np.array([[12500.]]), bath=qa.DebyeBath(qa.CM_K * 77, 35, 106),
+                                      dipoles=[[1., 0, 0]])
+    mono_o = S.ElectronicHamiltonian(np.array([[12500.]]), bath=S.DebyeBath(S.CM_K * 77, 35, 106),
+                                     dipoles=[[1., 0, 0]])
+    mp = qa.RedfieldModel(mono_p, hilbert_subspace='gef', unit_convert=qa.CM_FS)
+    mo = Lv.LiouvilleModel(mono_o, hilbert_subspace='gef', unit_convert=S.CM_FS)
+    dt = mp.time_step / 4
+    for geom in ('-++', '+-+'):          # ESA needs the (empty) 'fe' block
+        kw = dict(population_times=[0., 10.], geometry=geom, include_signal='GSB,ESE', dt=dt)
+        (t1, t2, t3), R = qa.third_order_response(mp, 100.5, **kw)
+        _, Ro = P.third_order_response(mo, 100.5, **kw)
+        assert R.shape == Ro.shape
+        assert np.allclose(R, Ro, rtol=0, atol=1e-10 * max(1.0, np.abs(Ro).max()))
+    _, inp = _inputs('redfield_dimer')
+    md = qa.RedfieldModel(_product_ham(inp), hilbert_subspace='gef', unit_convert=qa.CM_FS)
+    od = Lv.LiouvilleModel(_oracle_ham(inp), hilbert_subspace='gef', unit_convert=S.CM_FS)
+    for kw in (dict(coherence_time_max=30.5, population_times=[7.]),
+               dict(coherence_time_max=0.5, population_times=[0., 3.])):
+        (t1, t2, t3), R = qa.third_order_response(md, dt=0.25, **kw)
+        _, Ro = P.third_order_response(od, dt=0.25, **kw)
+        assert R.shape == Ro.shape == (len(t1), len(t2), len(t3))
+        assert rel_err(R, Ro) < 1e-10
+    hd = _product_ham(inp, disorder=50.0)
+    ms = qa.RedfieldModel(hd, hilbert_subspace='ge', unit_convert=qa.CM_FS)
+    f, X = qa.absorption_spectra(ms, 100.5, dt=0.25, ensemble_size=1)
+    os_ = Lv.LiouvilleModel(_oracle_ham(inp, disorder=50.0), hilbert_subspace='ge',
+                            unit_convert=S.CM_FS)
+    fo, Xo = P.absorption_spectra(os_.sample(0), 100.5, dt=0.25)
+    assert rel_err(X, Xo) < 1e-10
