@@ -47,7 +47,8 @@ def parse_args(argv=None):
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
                          'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
-                         'launch sequence, two HEOM RK4 steps of 256 members)')
+                         'launch sequence with chunk lengths 12 and 16, two HEOM RK4 steps '
+                         'of 256 members)')
     return ap.parse_args(argv)
 
 
@@ -61,10 +62,16 @@ def counter_sequence(torch, qa):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    fused = FusedSpectrum(W.aggregate7_model(secular=False), W.AGG7_2D, 'mixed')
-    for _ in range(3):
-        fused.plan._launch()
-    torch.cuda.synchronize()
+    from qspectra_amd.simulate.fused import ThirdOrderPlan
+    work = W.AGG7_2D
+    for chunk in (12, 16):                 # the two chunk lengths the plan tuner picks
+        plan = ThirdOrderPlan(W.aggregate7_model(secular=False), work['coherence_time_max'],
+                              population_times=work['population_times'],
+                              geometry=work['geometry'], include_signal=work['include_signal'],
+                              dt=work['dt'], precision='mixed', fourier=True, chunk=chunk)
+        for _ in range(3):
+            plan._launch()
+        torch.cuda.synchronize()
     m = W.heom_aggregate7_model()
     eom = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(256)]) \
         .equation_of_motion('ee')
@@ -182,10 +189,27 @@ HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
 # scaled by 2.0 and WRITE by 1.0 (the factors that make a 256 MiB device copy
 # in the same runs read and write 256 MiB).  Keyed by launch label.
 PMC_TRAFFIC = {
-    'propagate prop mixed M=147 R=8 B=1 steps=127 nsave=16': 1.808e6,
-    'heom stage fp64': 3.525e9,
-    'heom stage mixed': 2.679e9,
+    'heom stage fp64': 3.839e9,
+    'heom stage mixed': 2.988e9,
 }
+# HBM bytes per launch of the 2-D plan's M = 147 trajectory launches, by
+# (precision, M, right-hand sides R, saves): rocprofv3 FETCH_SIZE x 2 +
+# WRITE_SIZE, the copy-calibrated factors (profiles/hbm_traffic_counters.md);
+# the plan's chunk length (12 or 16, tuned per box) selects the shapes
+PMC_PROPAGATE = {
+    ('mixed', 147, 8, 16): 1.807e6,     # 8 chunks x 16 saved steps
+    ('mixed', 147, 11, 12): 1.825e6,    # 11 chunks x 12 saved steps
+    ('fp64', 147, 1, 8): 3.80e5,        # chunk starts, P^16 chain
+    ('fp64', 147, 1, 11): 3.87e5,       # chunk starts, P^12 chain
+}
+
+
+def launch_traffic(label):
+    """Measured HBM bytes of a propagate launch (None if not profiled)."""
+    m = re.match(r'propagate \w+ (\w+) M=(\d+) R=(\d+) B=\d+ steps=\S+ nsave=(\d+)', label)
+    if not m:
+        return PMC_TRAFFIC.get(label)
+    return PMC_PROPAGATE.get((m.group(1), int(m.group(2)), int(m.group(3)), int(m.group(4))))
 
 
 def _host(x):
@@ -486,6 +510,7 @@ def main(argv=None):
     summary = kt.summary()
     label, info, launch_ms, flops = dominant_kernel(summary, kt.relaunch, timing)
     achieved = flops / (launch_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS['fp64' if ' fp64 ' in label else 'mixed']
     value = steps_per_spectrum * args.steps / t_mixed
     plain = steps_per_spectrum * args.steps / t_plain
 
@@ -503,9 +528,8 @@ def main(argv=None):
         'speedup_vs_plain': value / plain,
         'rel_err_vs_fp64': err,
         'roofline': {'bound': 'mfma', 'kernel': label, 'achieved': achieved,
-                     'peak': PEAK_TFLOPS['mixed'], 'unit': 'TFLOP/s',
-                     'frac': achieved / PEAK_TFLOPS['mixed'],
-                     'traffic': PMC_TRAFFIC.get(label),
+                     'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
+                     'traffic': launch_traffic(label),
                      'launch_ms': launch_ms, 'launch_ms_in_sequence': info['mean_ms'],
                      'algorithmic_flop': flops},
         'kernels': {k: {'count': v['count'], 'mean_ms': round(v['mean_ms'], 5)}

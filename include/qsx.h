@@ -200,9 +200,10 @@ int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_t n, int32_
  * HEOM for a Drude bath (qspectra/dynamics/heom.py:177-443), batched over B
  * samples that share the hierarchy structure.  State per sample: ADO-major
  * [n_ado][M] complex128.  Operator tables (device pointers):
- *   Hop    n > 0: [B][nX*nX + nY*nY] = -i*unit*H_X followed by -i*unit*H_Y
- *          (their transposes for the Heisenberg picture), the subspace
- *          being the single block XY, nX = n, nY = M / n (M <= 256);
+ *   Hop    n > 0: the subspace is the single block XY, nX = n, nY = M / n
+ *          (M <= 256): [B][nX*nX + nY*nY] = -i*unit*H_X followed by
+ *          -i*unit*H_Y, or for a square block (nX = nY) [B][n][n] = H_X
+ *          alone (their transposes for the Heisenberg picture);
  *          n = 0: [B][M][M] dense -i*unit*[H, .] block (M <= 64)
  *   shift  [n_ado] double, -unit * sum_jk m_jk nu_k
  *   dvec   [M] complex, -unit * Delta * sum_j (n_j(r) + n_j(c) - 2 n_j(r) n_j(c))

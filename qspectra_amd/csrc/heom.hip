@@ -64,6 +64,12 @@ struct HeomArgs {
 template <typename T>
 __device__ inline double2 ld(const cplx_t<T>* p) { return to_d2(*p); }
 
+// operator entries per sample: the dense block, or H_X and H_Y (one block
+// for a square subspace XX, where H_Y = H_X)
+__host__ __device__ inline int heom_opsz(const HeomArgs& a) {
+  return a.dense ? a.M * a.M : (a.nX == a.nY ? a.nX * a.nX : a.nX * a.nX + a.nY * a.nY);
+}
+
 // Stage st of classical RK4: z_in = input stage vector, writes z_out and
 // updates ynew (st 0..2) or y (st 3); st = 4 only writes z_out = F(z_in).
 // LDS layout of a stage work-group: operator block, the tile's stage
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
     double2* __restrict__ y, double2* __restrict__ ynew, double h, int st) {
   using T2 = cplx_t<T>;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int opsz = a.dense ? a.M * a.M : a.nX * a.nX + a.nY * a.nY;
+  const int opsz = heom_opsz(a);
   const int nb2 = 2 * a.nbins;
   double2* Hs = reinterpret_cast<double2*>(smem);                 // H_X, H_Y (COMM) or M*M
   T2* zt = reinterpret_cast<T2*>(Hs + opsz);                      // [ado_per_tile][M]
@@ -129,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
       for (int q = 0; q < M; ++q) k = cadd(k, cmul(row[q], ld<T>(z + q)));
     } else {   // (H_X rho - rho H_Y)_{rc}, p = r + nX c, column-major rho
       const int nX = a.nX, nY = a.nY, r = p % nX, c = p / nX;
-      const double2* HY = Hs + nX * nX;
+      const double2* HY = nX == nY ? Hs : Hs + nX * nX;     // square blocks: H_Y = H_X
       for (int q = 0; q < nX; ++q) k = cadd(k, cmul(Hs[r * nX + q], ld<T>(z + q + nX * c)));
       for (int q = 0; q < nY; ++q) {
         const double2 t = cmul(ld<T>(z + r + nX * q), HY[q * nY + c]);
@@ -217,7 +223,7 @@ int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const d
   double2* ynew = y + total;
   T2* zA = reinterpret_cast<T2*>(ynew + total);
   T2* zB = zA + total;
-  const int opsz = a.dense ? a.M * a.M : a.nX * a.nX + a.nY * a.nY;
+  const int opsz = heom_opsz(a);
   const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(T2));
   QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_heom_propagate: %zu bytes of LDS", shmem);
   if (shmem > 64 * 1024)
@@ -304,7 +310,7 @@ extern "C" int32_t qsx_heom_apply(int32_t B, int32_t n_ado, int32_t M, int32_t n
                      rel_site, rel_occ);
   if (st) return st;
   QSX_CHECK_ARG(y && out, "qsx_heom_apply: null pointer");
-  const int opsz = a.dense ? a.M * a.M : a.nX * a.nX + a.nY * a.nY;
+  const int opsz = heom_opsz(a);
   const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(double2));
   QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_heom_apply: %zu bytes of LDS", shmem);
   const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;

@@ -363,7 +363,7 @@ class HEOMModel(DynamicalModel):
         if XY is not None and (tb['M'] > DENSE_BLOCK_MAX or XY[0] == XY[1]):
             # commutator form: H_X and H_Y blocks of every member
             blocks = []
-            for Z in XY:
+            for Z in (XY[:1] if XY[0] == XY[1] else XY):     # square: H_Y = H_X
                 sl = self.hamiltonian.hilbert_subspace_index(Z, self.hilbert_subspace)
                 Hz = np.stack([h.H(self.hilbert_subspace)[sl, sl] for h in hams])
                 if heisenberg_picture:
