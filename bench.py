@@ -35,7 +35,7 @@ PEAK_TFLOPS = {'fp64': 78.6, 'mixed': 157.3}   # MI355X dense FP64 / FP32 (vecto
 def parse_args(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-baseline', choices=('auto', 'off'), default='auto')
     ap.add_argument('--cpu-budget', type=float, default=30.0,
@@ -95,7 +95,7 @@ def run_plain(qa, model, work):
 class FusedSpectrum(object):
     """The batched path: fused plan (one shared t1 trajectory; the t2 stage
     on the rows of the V1 map with cached RK4 propagator powers; t3
-    trajectories on a second stream; time-chunked stepping with complex64
+    trajectories on streams of their own; time-chunked stepping with complex64
     products and FP64 state; both Fourier transforms as GEMMs with the
     transform matrices), the whole 2-D spectrum replayed from one HIP graph."""
 

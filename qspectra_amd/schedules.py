@@ -267,12 +267,14 @@ def _save_cache():
         pass
 
 
-def autotune(key, candidates, make_trial, reps=5, graph=False):
+def autotune(key, candidates, make_trial, reps=5, graph=False, block=1):
     """The fastest of ``candidates`` for problem ``key`` on this device:
     each candidate's launch sequence (make_trial(candidate) -> callable) is
     run once to warm up, then timed ``reps`` times with device events -- as
     eager launches, or (graph=True) as replays of a captured HIP graph, the
-    way a plan runs it; the median decides.  Decisions are cached in memory
+    way a plan runs it; each sample is ``block`` calls back to back (the
+    steady state of repeated calls rather than one call's start-up
+    latency); the median decides.  Decisions are cached in memory
     and in a JSON file (QSX_TUNE_CACHE, default
     ~/.cache/qspectra_amd/schedules.json), keyed by device name and problem
     shape."""
@@ -298,10 +300,11 @@ def autotune(key, candidates, make_trial, reps=5, graph=False):
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record()
-            fn()
+            for _ in range(block):
+                fn()
             b.record()
             b.synchronize()
-            ms.append(a.elapsed_time(b))
+            ms.append(a.elapsed_time(b) / block)
         times[cand] = float(np.median(ms))
     best = min(times, key=times.get)
     _TUNED[full] = {'choice': list(best),

@@ -20,7 +20,9 @@ bra of pathway p), so a plan evaluates it in the cheapest association:
     Liouvillian);
   * W_s = Q_s V2_p^T for every pathway, written side by side so that all
     pathways contract with their t3 trajectories in one GEMM;
-  * each distinct t3 Heisenberg trajectory once, each on a stream of its own;
+  * each distinct t3 Heisenberg trajectory once, each on a stream of its own
+    (the largest block first), while the t2 and t1 factors run in order on
+    the main stream;
   * the Fourier transforms of two_dimensional_spectra (simulate/utils.py:
     154-219) as products with the transform matrices F1, F3 (the FFT
     pipeline applied to unit vectors), folded into the t1 and t3 factors:
@@ -199,8 +201,10 @@ class ThirdOrderPlan(object):
         # status slots of the stepper launches (t1, then each t3), cleared per run
         self.status = torch.zeros(1 + len(self.t3_stages), dtype=torch.int32, device=dev)
         self.steps = self._count_steps(paths)
+        self._t3_list = list(self.t3_stages.values())
+        self._t3_order = sorted(range(len(self._t3_list)),
+                                key=lambda i: -self._t3_list[i]['stage'].eom.L.shape[-1])
         self._sides = [torch.cuda.Stream() for _ in self.t3_stages]
-        self._t2_stream = torch.cuda.Stream()
         self._graph = None
         self._statuses = []
 
@@ -221,27 +225,25 @@ class ThirdOrderPlan(object):
         self.status.zero_()
         statuses = []
         # each t3 Heisenberg trajectory on its own stream, transformed along t3
-        # and written to its column range of G
-        for i, (st, side) in enumerate(zip(self.t3_stages.values(), self._sides)):
+        # and written to its column range of G; the largest block (the
+        # longest chain of the plan) is launched first
+        for i, side in zip(self._t3_order, self._sides):
+            st = self._t3_list[i]
             side.wait_stream(main)
             with torch.cuda.stream(side):
                 M3 = st['bra'].shape[-1]
                 G3 = st['stage'].run(st['bra'], statuses,
                                      self.status[1 + i:2 + i])[:, 0, :n3]      # [B, n3, M3]
                 _native.zgemm(self.F3, G3, self.G[:, :, st['col']:st['col'] + M3])
-        # t2 stage on the rows of V1^T and the V2 maps of every pathway, on a
-        # stream of its own (independent of the trajectories)
-        t2s = self._t2_stream
-        t2s.wait_stream(main)
-        with torch.cuda.stream(t2s):
-            for i, (k12, g) in enumerate(self.groups.items()):
-                _native.zgemm(g['V1T'], g['Pn'], self.Q[k12], transb=True)
-                _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=0.0 if i == 0 else 1.0)
-        # t1 trajectory, transformed along t1, then U_s = (F1 S1) W_s
+        # t2 stage on the rows of V1^T and the V2 maps of every pathway, then
+        # the t1 trajectory transformed along t1, in order on the main stream
+        # (a fourth stream costs more in cross-stream waits than it overlaps)
+        for i, (k12, g) in enumerate(self.groups.items()):
+            _native.zgemm(g['V1T'], g['Pn'], self.Q[k12], transb=True)
+            _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=0.0 if i == 0 else 1.0)
         stage1, y1 = next(iter(self.t1_stages.values()))
         S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
         _native.zgemm(self.F1, S1, self.S1F)
-        main.wait_stream(t2s)
         S1F = self.S1F if B == 1 else self.S1F.repeat_interleave(n2, dim=0)
         _native.zgemm(S1F, self.W, self.U)                                       # [B n2, N1, w]
         for side in self._sides:
@@ -287,8 +289,8 @@ def plan_chunk(model, coherence_time_max, population_time_max=None, population_t
     """The chunk length of the plan's time-chunked trajectory stages, chosen
     for the plan as a whole: the stages run concurrently on their streams,
     so the fastest schedule of each stage on its own is not the fastest
-    plan.  Candidates (8, 12, 16 and sqrt(n1)) are timed as replays of the
-    captured plan; the decision is cached like the stage tuner's (per
+    plan.  Candidates (8, 12, 16 and sqrt(n1)) are timed as blocks of 20
+    back-to-back replays of the captured plan; the decision is cached like the stage tuner's (per
     device and plan shape).  None when tuning is off or the schedule is not
     'auto' (each stage then takes its own default)."""
     from ..schedules import autotune, autotune_enabled
@@ -323,6 +325,6 @@ def plan_chunk(model, coherence_time_max, population_time_max=None, population_t
             built[cand] = p
         return p.run
     try:
-        return autotune(key, cands, trial, reps=5)[1]
+        return autotune(key, cands, trial, reps=5, block=20)[1]
     except PlanNotApplicable:
         return None
