@@ -105,7 +105,17 @@ class FusedSpectrum(object):
         kw = dict(population_times=work['population_times'], geometry=work['geometry'],
                   include_signal=work['include_signal'], dt=work['dt'], precision=precision,
                   fourier=True)
+        from qspectra_amd import schedules
+        schedules._load_cache()
+        before = set(schedules._TUNED)
         chunk = plan_chunk(model, work['coherence_time_max'], **kw)
+        self.chunk = chunk
+        # candidate times of the plan tuner (empty when the decision came
+        # from the on-disk cache)
+        self.chunk_ms = {}
+        for k in set(schedules._TUNED) - before:
+            if k.split(' | ', 1)[-1].startswith('plan '):
+                self.chunk_ms.update(schedules._TUNED[k].get('ms', {}))
         self.plan = ThirdOrderPlan(model, work['coherence_time_max'], chunk=chunk,
                                    **kw).capture()
 
@@ -534,6 +544,7 @@ def main(argv=None):
                      'algorithmic_flop': flops},
         'kernels': {k: {'count': v['count'], 'mean_ms': round(v['mean_ms'], 5)}
                     for k, v in summary.items()},
+        'plan_chunk': {'choice': fused.chunk, 'ms_per_spectrum': fused.chunk_ms},
     }
     wanted = ([n for n, _ in LEGS] if args.workloads == 'all' else
               [] if args.workloads == 'none' else args.workloads.split(','))

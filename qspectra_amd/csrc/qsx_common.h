@@ -82,6 +82,38 @@ __device__ inline double wave_uniform(double v) {
                           __builtin_amdgcn_readfirstlane(__double2loint(v)));
 }
 
+// Wait for all outstanding global loads (s_waitcnt vmcnt(0); the expcnt and
+// lgkmcnt fields at their maxima do not wait).  Placed in uniform control
+// flow before a loop, it leaves the compiler no load that it must assume
+// pending inside the loop.
+__device__ inline void wait_global_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Save-interval table (step count, step length) of a stepping loop, one
+// block of 64 intervals at a time held in registers: lane l of every
+// wavefront holds interval base + l.  load() waits for the block once, so
+// the intervals of the block are then read without waiting on global memory
+// (on gfx9 a wait for a load also waits for every store issued before it,
+// such as the states saved at the end of the previous interval).
+struct IntervalBlock {
+  int n_l;
+  double h_l;
+
+  // intervals base .. base + 63 of n_[0, count), h_[0, count)
+  __device__ void load(const int* n_, const double* h_, int count, int base) {
+    const int e = base + __lane_id();
+    n_l = e < count ? n_[e] : 0;
+    h_l = e < count ? h_[e] : 0.0;
+    // consume the loads here, before the loop that reads the block
+    asm volatile("" : : "v"(n_l), "v"(h_l));
+  }
+  // interval base + sl (sl wave-uniform, < 64)
+  __device__ int steps(int sl) const { return __builtin_amdgcn_readlane(n_l, sl); }
+  __device__ double length(int sl) const {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(h_l), sl),
+                            __builtin_amdgcn_readlane(__double2loint(h_l), sl));
+  }
+};
+
 // Lane move within rows of 16 lanes by a DPP control (gfx9 encoding:
 // 0x00-0xFF quad permutation, 0x140 row mirror, 0x141 half-row mirror).
 template <int CTRL>

@@ -10,8 +10,9 @@
 // power of two, so a row's lanes are adjacent in one wavefront), and the
 // stage vector is exchanged through a double-buffered LDS vector padded
 // with zeros to S*W entries (so the row segments read it without bounds
-// tests): one barrier per product.  The step counts and lengths of the next
-// save interval are fetched with vector loads one interval ahead.
+// tests): one barrier per product.  The step counts and lengths of the
+// save intervals are held in registers, 64 intervals per wavefront register
+// (IntervalBlock), so the loop does not wait on global memory between saves.
 //
 // Modes
 //   RK4  (PROP = false): y <- y + h/6 (k1 + 2 k2 + 2 k3 + k4), k = L z,
@@ -126,16 +127,16 @@ __global__ __launch_bounds__(NT) void step_kernel(
   bool bad = false;
   const int nslots = nov > 0 ? 2 : 1;
   double2* ob = out + ((int64_t)b * R + r) * nsave * M;
-  const int vz = opaque_zero();
-  int n_next = nsub[vz];
-  double h_next = hs[vz];
+  // every load of the set-up (some waited for only under lane-divergent
+  // branches) completes here, in uniform control flow: no wait on global
+  // memory is then left inside the loops, where it would also wait for the
+  // stores of the saved states
+  wait_global_loads();
+  IntervalBlock blk;
   for (int s = 0; s < nsave; ++s) {
-    const int n = max(wave_uniform(n_next), 0);
-    const double h = wave_uniform(h_next);
-    if (s + 1 < nsave) {
-      n_next = nsub[s + 1 + vz];
-      h_next = hs[s + 1 + vz];
-    }
+    if ((s & 63) == 0) blk.load(nsub, hs, nsave, s);
+    const int n = max(blk.steps(s & 63), 0);
+    const double h = blk.length(s & 63);
     for (int step = 0; step < n; ++step) {
       if (PROP) {
         const T2* zc = zb + cur * MP;

@@ -274,7 +274,8 @@ def autotune(key, candidates, make_trial, reps=5, graph=False, block=1):
     eager launches, or (graph=True) as replays of a captured HIP graph, the
     way a plan runs it; each sample is ``block`` calls back to back (the
     steady state of repeated calls rather than one call's start-up
-    latency); the median decides.  Decisions are cached in memory
+    latency; with block > 1 the candidates are sampled in turn); the median
+    decides.  Decisions are cached in memory
     and in a JSON file (QSX_TUNE_CACHE, default
     ~/.cache/qspectra_amd/schedules.json), keyed by device name and problem
     shape."""
@@ -283,8 +284,18 @@ def autotune(key, candidates, make_trial, reps=5, graph=False, block=1):
     hit = _TUNED.get(full)
     if hit is not None and tuple(hit['choice']) in [tuple(c) for c in candidates]:
         return tuple(hit['choice'])
-    times = {}
-    for cand in candidates:
+
+    def sample(fn):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(block):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / block
+
+    def prepare(cand):
         fn = make_trial(cand)
         fn()
         torch.cuda.synchronize()
@@ -295,17 +306,24 @@ def autotune(key, candidates, make_trial, reps=5, graph=False, block=1):
             fn = g.replay
             fn()
             torch.cuda.synchronize()
-        ms = []
+        return fn
+
+    samples = {}
+    if block > 1:
+        # whole plans (kept alive by the caller anyway): the candidates are
+        # sampled in turn, so that a drift of the clocks during tuning does
+        # not favour the ones timed last
+        fns = {cand: prepare(cand) for cand in candidates}
+        for cand in candidates:
+            samples[cand] = []
         for _ in range(reps):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(block):
-                fn()
-            b.record()
-            b.synchronize()
-            ms.append(a.elapsed_time(b) / block)
-        times[cand] = float(np.median(ms))
+            for cand in candidates:
+                samples[cand].append(sample(fns[cand]))
+    else:
+        for cand in candidates:
+            fn = prepare(cand)
+            samples[cand] = [sample(fn) for _ in range(reps)]
+    times = {cand: float(np.median(v)) for cand, v in samples.items()}
     best = min(times, key=times.get)
     _TUNED[full] = {'choice': list(best),
                     'ms': {'%s:%s' % c: round(v, 5) for c, v in times.items()}}

@@ -290,7 +290,8 @@ def plan_chunk(model, coherence_time_max, population_time_max=None, population_t
     for the plan as a whole: the stages run concurrently on their streams,
     so the fastest schedule of each stage on its own is not the fastest
     plan.  Candidates (8, 12, 16 and sqrt(n1)) are timed as blocks of 20
-    back-to-back replays of the captured plan; the decision is cached like the stage tuner's (per
+    back-to-back replays of the captured plan, sampled in turn (9 samples
+    each, the median decides); the decision is cached like the stage tuner's (per
     device and plan shape).  None when tuning is off or the schedule is not
     'auto' (each stage then takes its own default)."""
     from ..schedules import autotune, autotune_enabled
@@ -325,6 +326,6 @@ def plan_chunk(model, coherence_time_max, population_time_max=None, population_t
             built[cand] = p
         return p.run
     try:
-        return autotune(key, cands, trial, reps=5, block=20)[1]
+        return autotune(key, cands, trial, reps=9, block=20)[1]
     except PlanNotApplicable:
         return None
