@@ -115,8 +115,10 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
 // a work-group is WM x WN waves.  Operand fragments (gfx950 layout): A lane
 // l = A[l % 16][k0 + l / 16], B lane l = B[k0 + l / 16][l % 16]; result lane
 // l, element i = D[l / 16 + 4 i][l % 16].  Complex product = four real MFMAs
-// (re re, -im im, re im, im re) on real/imaginary planes staged in LDS, K
-// in chunks of TK (64) whose operand loads are all in flight together.
+// (re re, im im, re im, im re) into four accumulators on real/imaginary
+// planes staged in LDS, K in chunks of TK (64) whose operand loads are all
+// in flight together; the loads of chunk c + 1 are issued before the MFMAs
+// of chunk c.
 typedef double qsx_double4 __attribute__((ext_vector_type(4)));
 
 template <bool TA, bool TB, int WM, int WN, int TK>
@@ -134,11 +136,13 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_mfma_kernel(
   const double2* Ab = A + (int64_t)bz * sA;
   const double2* Bb = B + (int64_t)bz * sB;
   double2* Cb = C + (int64_t)bz * sC;
-  qsx_double4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+  // four independent accumulation chains (re re, im im, re im, im re), so
+  // consecutive MFMAs never wait for each other's result
+  qsx_double4 crr = {0.0, 0.0, 0.0, 0.0}, cii = crr, cri = crr, cir = crr;
   const int r16 = lane % 16, q4 = lane / 16;
   static_assert((TM * TK) % NT == 0 && (TN * TK) % NT == 0, "tile and chunk sizes");
-  for (int k0 = 0; k0 < k; k0 += TK) {
-    double2 av[TM * TK / NT], bv[TN * TK / NT];   // a chunk's loads, all in flight
+  double2 av[TM * TK / NT], bv[TN * TK / NT];     // a chunk's loads, all in flight
+  auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < TM * TK / NT; ++u) {
       const int e = tid + NT * u;
@@ -159,6 +163,9 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_mfma_kernel(
       const int64_t off = TB ? (int64_t)gj * ldb + gk : (int64_t)gk * ldb + gj;
       bv[u] = (gk < k && gj < n) ? Bb[off] : make_double2(0.0, 0.0);
     }
+  };
+  load_chunk(0);
+  for (int k0 = 0; k0 < k; k0 += TK) {
 #pragma unroll
     for (int u = 0; u < TM * TK / NT; ++u) {
       const int e = tid + NT * u;
@@ -174,17 +181,20 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_mfma_kernel(
       Bi[kk][nn] = bv[u].y;
     }
     __syncthreads();
+    // the next chunk's loads are in flight while this chunk's MFMAs run
+    if (k0 + TK < k) load_chunk(k0 + TK);
     const int kend = min(TK, k - k0);
     for (int kq = 0; kq < kend; kq += 4) {
       const double ar = Ar[kq + q4][wm * 16 + r16], ai = Ai[kq + q4][wm * 16 + r16];
       const double br = Br[kq + q4][wn * 16 + r16], bi = Bi[kq + q4][wn * 16 + r16];
-      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
-      cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cr, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
-      ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci, 0, 0, 0);
+      crr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, crr, 0, 0, 0);
+      cii = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, cii, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, cri, 0, 0, 0);
+      cir = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, cir, 0, 0, 0);
     }
     __syncthreads();
   }
+  const qsx_double4 cr = crr - cii, ci = cri + cir;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int gi = bm + wm * 16 + q4 + 4 * i, gj = bn + wn * 16 + r16;

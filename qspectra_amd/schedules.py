@@ -11,6 +11,7 @@ import torch
 from . import _native
 from .engine import rk4_schedule
 from .lti import propagator_power, transposed
+from .timing import capture
 
 
 def _mode(precision):
@@ -301,7 +302,7 @@ def autotune(key, candidates, make_trial, reps=5, graph=False, block=1):
         torch.cuda.synchronize()
         if graph:
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
+            with capture(g):
                 fn()
             fn = g.replay
             fn()

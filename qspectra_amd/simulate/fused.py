@@ -42,6 +42,7 @@ from ..lti import propagator_power
 from .response import (_members, _op_matrices, _parse_pathways, THIRD_ORDER_PATHWAYS,
                        third_order_grids)
 from ..schedules import Stage as _Stage
+from ..timing import capture as _capture
 
 
 def transform_matrix(t, rw_freq, unit_convert, sign, device):
@@ -259,7 +260,7 @@ class ThirdOrderPlan(object):
         self._launch()                      # warm up allocations and caches
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
+        with _capture(g):
             self._out = self._launch()
         self._graph = g
         return self

@@ -6,6 +6,7 @@ launched on; ``kt.summary()`` (after the work completes) gives count, total
 and mean milliseconds per launch label.
 """
 import contextlib
+import gc
 
 import torch
 
@@ -56,6 +57,23 @@ def launch(label, stream=None, relaunch=None):
             kt.relaunch[label] = relaunch
 
 
+@contextlib.contextmanager
+def capture(graph):
+    """torch.cuda.graph(graph) with Python's automatic garbage collection
+    suspended while the stream captures: a collection inside the capture can
+    destroy another, unreachable graph, which HIP refuses while a stream is
+    capturing (the destructor then aborts the process).  torch.cuda.graph
+    itself collects once before the capture begins."""
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.graph(graph):
+            yield
+    finally:
+        if enabled:
+            gc.enable()
+
+
 def back_to_back_ms(fn, n=20, reps=5):
     """Mean device time of one call of ``fn`` (a single kernel launch) when
     n calls are replayed back to back from a captured HIP graph, so that
@@ -63,7 +81,7 @@ def back_to_back_ms(fn, n=20, reps=5):
     fn()
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with capture(g):
         for _ in range(n):
             fn()
     g.replay()

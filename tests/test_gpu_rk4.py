@@ -205,9 +205,11 @@ def test_zgemm_and_propagator(gpu):
 def test_zgemm_transposed_operands_and_tile_shapes(gpu):
     from qspectra_amd import _native
     rs = np.random.RandomState(9)
-    # the last two shapes have >= 192 32x32 tiles: the FP64 matrix-core path
+    # the last three shapes have >= 192 32x32 tiles: the FP64 matrix-core
+    # path (one, three and five K chunks of 64, the last one partial)
     for (b, m, n, k) in [(1, 7, 255, 128), (3, 255, 7, 128), (2, 147, 147, 147),
-                         (1, 300, 200, 40), (5, 1, 3, 2), (4, 255, 255, 154), (1, 520, 400, 33)]:
+                         (1, 300, 200, 40), (5, 1, 3, 2), (4, 255, 255, 154), (1, 520, 400, 33),
+                         (3, 300, 260, 257)]:
         A = rs.randn(b, m, k) + 1j * rs.randn(b, m, k)
         B = rs.randn(b, k, n) + 1j * rs.randn(b, k, n)
         ref = A @ B
