@@ -138,10 +138,13 @@ def dominant_kernel(summary, relaunch, timing):
     spectrum; each launch's duration is measured back to back from a
     captured graph (its own time, without host gaps).  Algorithmic work:
     8 M^2 real flops per complex matrix-vector product, 4 per RK4 step or 1
-    per step with the RK4 propagator, times R B steps of the launch."""
+    per step with the RK4 propagator, times R B steps of the launch (the
+    label's `steps` counts one right-hand side; a chunked launch performs nc
+    chunks of `steps` products and the nc - 1 products of its chain of
+    chunk starts)."""
     best = None
     for label, info in summary.items():
-        if not label.startswith('propagate'):
+        if not label.startswith(('propagate', 'chunks')):
             continue
         ms = timing.back_to_back_ms(relaunch[label])
         total = ms * info['count']
@@ -149,9 +152,16 @@ def dominant_kernel(summary, relaunch, timing):
             best = (label, info, ms, total)
     label, info, ms, _ = best
     f = dict(re.findall(r'(\w+)=(\w+)', label))
-    M, R, B, steps = int(f['M']), int(f['R']), int(f['B']), int(f['steps'])
-    products = 4 if ' rk4 ' in label else 1
-    flops = 8.0 * products * M * M * R * B * steps
+    M, B, steps = int(f['M']), int(f['B']), int(f['steps'])
+    if label.startswith('chunks'):
+        # nc chunks of `steps` products each, plus the nc - 1 products of the
+        # chain of chunk starts (each work-group recomputes its own start:
+        # that repetition is not counted as work)
+        nc = int(f['nc'])
+        products = nc * steps + (nc - 1)
+    else:
+        products = int(f['R']) * steps * (4 if ' rk4 ' in label else 1)
+    flops = 8.0 * M * M * B * products
     return label, info, ms, flops
 
 
@@ -199,8 +209,8 @@ HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
 # scaled by 2.0 and WRITE by 1.0 (the factors that make a 256 MiB device copy
 # in the same runs read and write 256 MiB).  Keyed by launch label.
 PMC_TRAFFIC = {
-    'heom stage fp64': 3.839e9,
-    'heom stage mixed': 2.988e9,
+    'heom stage fp64': 3.525e9,
+    'heom stage mixed': 2.678e9,
 }
 # HBM bytes per launch of the 2-D plan's M = 147 trajectory launches, by
 # (precision, M, right-hand sides R, saves): rocprofv3 FETCH_SIZE x 2 +
@@ -212,14 +222,22 @@ PMC_PROPAGATE = {
     ('fp64', 147, 1, 8): 3.80e5,        # chunk starts, P^16 chain
     ('fp64', 147, 1, 11): 3.87e5,       # chunk starts, P^12 chain
 }
+# The same for the one-launch chunked trajectories, by (precision, M,
+# chunks nc, saves per chunk)
+PMC_CHUNKS = {
+    ('mixed', 147, 8, 16): 4.300e6,     # chunk 16: 8 work-groups
+    ('mixed', 147, 11, 12): 4.662e6,    # chunk 12: 11 work-groups
+}
 
 
 def launch_traffic(label):
     """Measured HBM bytes of a propagate launch (None if not profiled)."""
-    m = re.match(r'propagate \w+ (\w+) M=(\d+) R=(\d+) B=\d+ steps=\S+ nsave=(\d+)', label)
+    m = re.match(r'(propagate|chunks) \w+ (\w+) M=(\d+) (?:R|nc)=(\d+) B=\d+ steps=\S+ '
+                 r'nsave=(\d+)', label)
     if not m:
         return PMC_TRAFFIC.get(label)
-    return PMC_PROPAGATE.get((m.group(1), int(m.group(2)), int(m.group(3)), int(m.group(4))))
+    table = PMC_PROPAGATE if m.group(1) == 'propagate' else PMC_CHUNKS
+    return table.get((m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5))))
 
 
 def _host(x):

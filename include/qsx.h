@@ -158,6 +158,23 @@ int32_t qsx_propagate(int32_t mode, int32_t precision, const void* A, int64_t st
                       int32_t R, int32_t B, const int32_t* nsub, const double* h,
                       int32_t nsave, void* out, void* y_final, int32_t* status, void* stream);
 
+/* Time-chunked trajectory of one right-hand side per problem in one launch
+ * (QSX_MODE_PROPAGATOR): work-group j of problem b computes the chunk start
+ * (Pc)^j y0[b] with the FP64 power Pc = P^(q c) [B|1][M][M], then steps the
+ * chunk with A (P in FP64, P - I complex64 with the FP64 diagonal diag in
+ * MIXED) over the save intervals nsub/h (device, nsave entries): out
+ * [B][nc][nsave][M] complex128, the same values as qsx_propagate with Pc on
+ * the chain of chunk starts followed by qsx_propagate with A on the nc
+ * starts as right-hand sides.  Like qsx_propagate it replaces the zvode
+ * trajectories of qspectra/simulate/utils.py:16-109 (the t1 and t3
+ * propagations of qspectra/simulate/response.py:269-337), for trajectories
+ * saved at equally spaced points. */
+int32_t qsx_propagate_chunks(int32_t precision, const void* Pc, int64_t strideP, const void* A,
+                             int64_t strideA, const void* diag, int64_t strideD, int32_t M,
+                             const void* y0, int32_t nc, int32_t B, const int32_t* nsub,
+                             const double* h, int32_t nsave, void* out, int32_t* status,
+                             void* stream);
+
 /* Reverse mode of qsx_propagate in QSX_MODE_PROPAGATOR (FP64): given the
  * one-step propagator A = P [B|1][M][M], y0 [B][R][M], the step counts nsub
  * (device) and the cotangents gout [B][R][nsave][M] of the saved states

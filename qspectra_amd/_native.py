@@ -54,6 +54,8 @@ SIGNATURES = {
     'qsx_unitary_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_i32, c_i32, c_i32, c_vp, c_vp],
     'qsx_propagate': [c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32, c_i32,
                       c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp],
+    'qsx_propagate_chunks': [c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32,
+                             c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp],
     'qsx_adjoint_workspace_bytes': [c_i32, c_i32, c_i32, c_i64, c_i32],
     'qsx_propagate_adjoint': [c_vp, c_i64, c_i32, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp, c_i32,
                               c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
@@ -271,7 +273,8 @@ def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_f
                      stream=None, steps=None, status=None):
     """qsx_propagate: full states at every save point, out [B, R, nsave, M].
     status: optional int32 device slot (set to 1 on a non-finite state; the
-    caller clears it), allocated and cleared here when None."""
+    caller clears it), allocated and cleared here when None.  steps: RK4
+    steps of one right-hand side (recorded in the launch label)."""
     lib = require_native()
     B, R, M = y0.shape
     nsave = int(nsub.shape[0])
@@ -299,6 +302,46 @@ def propagate_states(mode, A, y0, nsub, h, precision=QSX_FP64, diag=None, want_f
         st = go()
     check(st, 'qsx_propagate')
     return out, yfin, status
+
+
+def propagate_chunks(Pc, A, y0, nc, nsub, h, precision=QSX_FP64, diag=None, stream=None,
+                     steps=None, status=None):
+    """qsx_propagate_chunks: trajectory of y0 [B, 1, M] in nc chunks of
+    nsave saves each, out [B, nc, nsave, M]; chunk j starts from
+    Pc^j y0 (Pc complex128 [B|1, M, M]) and is stepped with A over the
+    intervals nsub/h.  status as in propagate_states; steps: products of
+    one chunk (recorded in the launch label)."""
+    lib = require_native()
+    B, R, M = y0.shape
+    assert R == 1 and nc >= 1
+    nsave = int(nsub.shape[0])
+    adt = torch.complex128 if precision == QSX_FP64 else torch.complex64
+    assert A.dtype == adt and A.is_contiguous() and A.shape[-2:] == (M, M)
+    assert Pc.dtype == torch.complex128 and Pc.is_contiguous() and Pc.shape[-2:] == (M, M)
+    assert A.shape[0] in (1, B) and Pc.shape[0] in (1, B)
+    assert y0.dtype == torch.complex128 and y0.is_contiguous()
+    assert nsub.dtype == torch.int32 and h.dtype == torch.float64 and h.shape[0] == nsave
+    strideA = 0 if A.shape[0] == 1 else M * M
+    strideP = 0 if Pc.shape[0] == 1 else M * M
+    strideD = 0
+    if precision == QSX_MIXED:
+        assert diag is not None and diag.dtype == torch.complex128 and diag.is_contiguous()
+        assert diag.shape[-1] == M and diag.shape[0] in (1, B)
+        strideD = 0 if diag.shape[0] == 1 else M
+    out = torch.empty((B, nc, nsave, M), dtype=torch.complex128, device=y0.device)
+    if status is None:
+        status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    label = 'chunks prop %s M=%d nc=%d B=%d steps=%s nsave=%d' % (
+        'fp64' if precision == QSX_FP64 else 'mixed', M, nc, B, steps, nsave)
+
+    def go():
+        return lib.qsx_propagate_chunks(precision, ptr(Pc), strideP, ptr(A), strideA, ptr(diag),
+                                        strideD, M, ptr(y0), nc, B, ptr(nsub), ptr(h), nsave,
+                                        ptr(out), ptr(status), stream_ptr(stream))
+    with timing.launch(label, stream, go):
+        st = go()
+    check(st, 'qsx_propagate_chunks')
+    return out, status
 
 
 MAX_DRIVEN_BLOCK = 64

@@ -28,6 +28,8 @@
 //        dephasing) in complex128 applied to the thread's FP64 stage value;
 //        state and update in FP64.  In PROP mode the operator is P - I.
 
+#include <type_traits>
+
 #include "qsx_common.h"
 
 namespace qsx {
@@ -36,12 +38,15 @@ namespace {
 __constant__ double kRkW[4] = {1.0, 2.0, 2.0, 1.0};
 __constant__ double kRkC[3] = {0.5, 0.5, 1.0};
 
+// One propagation phase of a work-group: operator rows Ab (this problem's
+// A), FP64 diagonal diagb (MIXED), state y of the thread's rows (lead lanes;
+// in/out), save intervals nsub/hs (or, with nsub == nullptr, one interval of
+// n_single steps), saved states to ob [nsave][M] (none when ob == nullptr).
 template <typename T, int NT, int S, int W, bool MIXED, bool PROP>
-__global__ __launch_bounds__(NT) void step_kernel(
-    const cplx_t<T>* __restrict__ A, int64_t strideA, const double2* __restrict__ diag,
-    int64_t strideD, int M, const double2* __restrict__ y0, int R,
-    const int* __restrict__ nsub, const double* __restrict__ hs, int nsave,
-    double2* __restrict__ out, double2* __restrict__ yfin, int* __restrict__ status) {
+__device__ __forceinline__ void step_phase(
+    const cplx_t<T>* __restrict__ Ab, const double2* __restrict__ diagb, int M,
+    double2 (&y)[2], const int* __restrict__ nsub, const double* __restrict__ hs, int nsave,
+    int n_single, double2* __restrict__ ob, bool& bad) {
   using T2 = cplx_t<T>;
   constexpr int kRegRows = NT / S;
   constexpr int MP = S * W;                            // padded row length (>= M)
@@ -50,9 +55,7 @@ __global__ __launch_bounds__(NT) void step_kernel(
   T2* Aov = zb + 2 * MP;                               // [M - kRegRows][MP] overflow rows
 
   const int tid = threadIdx.x;
-  const int r = blockIdx.x, b = blockIdx.y;
   const int k = tid % S;
-  const T2* Ab = A + (int64_t)b * strideA;
   const int nov = M > kRegRows ? M - kRegRows : 0;
 
   // this thread's rows: slot 0 = primary row (registers), slot 1 = overflow row (LDS)
@@ -68,20 +71,19 @@ __global__ __launch_bounds__(NT) void step_kernel(
     const int j = k + S * c;
     a[c] = (has0 && j < M) ? Ab[(int64_t)row0 * M + j] : T2{0, 0};
   }
+  __syncthreads();                       // LDS of a previous phase is free
   for (int e = tid; e < nov * MP; e += NT) {
     const int j = e % MP;
     Aov[e] = j < M ? Ab[(int64_t)(kRegRows + e / MP) * M + j] : T2{0, 0};
   }
   for (int e = tid; e < 2 * MP; e += NT) zb[e] = T2{0, 0};
 
-  const double2* yb0 = y0 + ((int64_t)b * R + r) * M;
-  double2 y[2], acc[2], zi[2], dd[2];
+  double2 acc[2], zi[2], dd[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int row = q == 0 ? row0 : row1;
     const bool on = q == 0 ? has0 : has1;
-    y[q] = on ? yb0[row] : make_double2(0.0, 0.0);
-    dd[q] = (MIXED && on) ? diag[(int64_t)b * strideD + row] : make_double2(0.0, 0.0);
+    dd[q] = (MIXED && on) ? diagb[row] : make_double2(0.0, 0.0);
     acc[q] = zi[q] = make_double2(0.0, 0.0);
   }
   __syncthreads();
@@ -124,9 +126,7 @@ __global__ __launch_bounds__(NT) void step_kernel(
   };
 
   int cur = 0;
-  bool bad = false;
   const int nslots = nov > 0 ? 2 : 1;
-  double2* ob = out + ((int64_t)b * R + r) * nsave * M;
   // every load of the set-up (some waited for only under lane-divergent
   // branches) completes here, in uniform control flow: no wait on global
   // memory is then left inside the loops, where it would also wait for the
@@ -134,9 +134,13 @@ __global__ __launch_bounds__(NT) void step_kernel(
   wait_global_loads();
   IntervalBlock blk;
   for (int s = 0; s < nsave; ++s) {
-    if ((s & 63) == 0) blk.load(nsub, hs, nsave, s);
-    const int n = max(blk.steps(s & 63), 0);
-    const double h = blk.length(s & 63);
+    int n = n_single;
+    double h = 0.0;
+    if (nsub != nullptr) {
+      if ((s & 63) == 0) blk.load(nsub, hs, nsave, s);
+      n = max(blk.steps(s & 63), 0);
+      h = blk.length(s & 63);
+    }
     for (int step = 0; step < n; ++step) {
       if (PROP) {
         const T2* zc = zb + cur * MP;
@@ -198,10 +202,32 @@ __global__ __launch_bounds__(NT) void step_kernel(
       const bool on = q == 0 ? has0 : has1;
       if (!(on && lead)) continue;
       const int row = q == 0 ? row0 : row1;
-      ob[(int64_t)s * M + row] = y[q];
+      if (ob != nullptr) ob[(int64_t)s * M + row] = y[q];
       if (!cfinite(y[q])) bad = true;
     }
   }
+}
+
+template <typename T, int NT, int S, int W, bool MIXED, bool PROP>
+__global__ __launch_bounds__(NT) void step_kernel(
+    const cplx_t<T>* __restrict__ A, int64_t strideA, const double2* __restrict__ diag,
+    int64_t strideD, int M, const double2* __restrict__ y0, int R,
+    const int* __restrict__ nsub, const double* __restrict__ hs, int nsave,
+    double2* __restrict__ out, double2* __restrict__ yfin, int* __restrict__ status) {
+  const int tid = threadIdx.x;
+  const int r = blockIdx.x, b = blockIdx.y;
+  constexpr int kRegRows = NT / S;
+  const int nov = M > kRegRows ? M - kRegRows : 0;
+  const int row0 = tid / S, row1 = kRegRows + tid / S;
+  const bool has0 = row0 < M, has1 = (tid / S) < nov, lead = (tid % S) == 0;
+  const double2* yb0 = y0 + ((int64_t)b * R + r) * M;
+  double2 y[2];
+  y[0] = has0 ? yb0[row0] : make_double2(0.0, 0.0);
+  y[1] = has1 ? yb0[row1] : make_double2(0.0, 0.0);
+  bool bad = false;
+  step_phase<T, NT, S, W, MIXED, PROP>(A + (int64_t)b * strideA,
+                                       MIXED ? diag + (int64_t)b * strideD : nullptr, M, y, nsub,
+                                       hs, nsave, 0, out + ((int64_t)b * R + r) * nsave * M, bad);
   if (bad) status[0] = 1;
   if (yfin != nullptr) {
 #pragma unroll
@@ -212,6 +238,39 @@ __global__ __launch_bounds__(NT) void step_kernel(
       yfin[((int64_t)b * R + r) * M + row] = y[q];
     }
   }
+}
+
+// Time-chunked trajectory in one launch (PROPAGATOR mode): work-group j of
+// problem b first computes its chunk start (P^c)^j y0 with the FP64 power
+// Pc = P^(q c) -- the same products, in the same order, as a chain of chunk
+// starts -- then steps its chunk with A (P in FP64, P - I in MIXED) over the
+// save intervals nsub: out [B][nc][nsave][M].
+template <int NT, int S, int W, bool MIXED>
+__global__ __launch_bounds__(NT) void chunk_kernel(
+    const double2* __restrict__ Pc, int64_t strideP, const void* __restrict__ A,
+    int64_t strideA, const double2* __restrict__ diag, int64_t strideD, int M,
+    const double2* __restrict__ y0, const int* __restrict__ nsub, const double* __restrict__ hs,
+    int nsave, double2* __restrict__ out, int* __restrict__ status) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  const int tid = threadIdx.x;
+  const int j = blockIdx.x, nc = gridDim.x, b = blockIdx.y;
+  constexpr int kRegRows = NT / S;
+  const int nov = M > kRegRows ? M - kRegRows : 0;
+  const int row0 = tid / S, row1 = kRegRows + tid / S;
+  const bool has0 = row0 < M, has1 = (tid / S) < nov;
+  const double2* yb0 = y0 + (int64_t)b * M;
+  double2 y[2];
+  y[0] = has0 ? yb0[row0] : make_double2(0.0, 0.0);
+  y[1] = has1 ? yb0[row1] : make_double2(0.0, 0.0);
+  bool bad = false;
+  if (j > 0)
+    step_phase<double, NT, S, W, false, true>(Pc + (int64_t)b * strideP, nullptr, M, y, nullptr,
+                                              nullptr, 1, j, nullptr, bad);
+  step_phase<T, NT, S, W, MIXED, true>(
+      reinterpret_cast<const cplx_t<T>*>(A) + (int64_t)b * strideA,
+      MIXED ? diag + (int64_t)b * strideD : nullptr, M, y, nsub, hs, nsave, 0,
+      out + ((int64_t)b * nc + j) * nsave * M, bad);
+  if (bad) status[0] = 1;
 }
 
 struct Shape {
@@ -266,6 +325,42 @@ int launch_step(const Shape& sh, const void* A, int64_t strideA, const void* dia
   return QSX_EINVAL;
 }
 
+template <bool MIXED>
+int launch_chunks(const Shape& sh, const void* Pc, int64_t strideP, const void* A,
+                  int64_t strideA, const void* diag, int64_t strideD, int M, const void* y0,
+                  int nc, int B, const int* nsub, const double* h, int nsave, void* out,
+                  int* status, hipStream_t stream) {
+  const int rows_reg = sh.NT / sh.S;
+  const int nov = M > rows_reg ? M - rows_reg : 0;
+  const int mp = sh.S * sh.W;
+  // the FP64 phase needs the larger LDS (complex128 stage vector and rows)
+  const size_t shmem = (size_t)(2 * mp + nov * mp) * sizeof(double2);
+  QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_propagate_chunks: block M=%d needs %zu bytes of LDS",
+                M, shmem);
+  dim3 grid(nc, B);
+#define QSX_CHUNK(NT_, S_, W_)                                                                 \
+  if (sh.NT == NT_ && sh.S == S_ && sh.W == W_) {                                              \
+    auto kern = chunk_kernel<NT_, S_, W_, MIXED>;                                              \
+    if (shmem > 64 * 1024)                                                                     \
+      QSX_CHECK_HIP(hipFuncSetAttribute((const void*)kern,                                     \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem)); \
+    hipLaunchKernelGGL(kern, grid, dim3(NT_), shmem, stream, (const double2*)Pc, strideP, A,    \
+                       strideA, (const double2*)diag, strideD, M, (const double2*)y0, nsub, h,  \
+                       nsave, (double2*)out, status);                                          \
+    QSX_CHECK_HIP(hipGetLastError());                                                          \
+    return QSX_OK;                                                                             \
+  }
+  QSX_CHUNK(64, 8, 1)
+  QSX_CHUNK(64, 4, 4)
+  QSX_CHUNK(128, 4, 8)
+  QSX_CHUNK(256, 4, 16)
+  QSX_CHUNK(512, 4, 32)
+  QSX_CHUNK(1024, 8, 20)
+#undef QSX_CHUNK
+  set_error("qsx_propagate_chunks: no kernel for shape (%d, %d, %d)", sh.NT, sh.S, sh.W);
+  return QSX_EINVAL;
+}
+
 }  // namespace
 }  // namespace qsx
 
@@ -299,4 +394,33 @@ extern "C" int32_t qsx_propagate(int32_t mode, int32_t precision, const void* A,
                                                  nsave, out, y_final, status, s)
               : launch_step<double, false, false>(sh, A, strideA, nullptr, 0, M, y0, R, B, nsub,
                                                   h, nsave, out, y_final, status, s);
+}
+
+extern "C" int32_t qsx_propagate_chunks(int32_t precision, const void* Pc, int64_t strideP,
+                                        const void* A, int64_t strideA, const void* diag,
+                                        int64_t strideD, int32_t M, const void* y0, int32_t nc,
+                                        int32_t B, const int32_t* nsub, const double* h,
+                                        int32_t nsave, void* out, int32_t* status,
+                                        void* stream) {
+  using namespace qsx;
+  QSX_CHECK_ARG(precision == QSX_FP64 || precision == QSX_MIXED,
+                "qsx_propagate_chunks: unknown precision %d", precision);
+  QSX_CHECK_ARG(Pc && A && y0 && nsub && h && out && status, "qsx_propagate_chunks: null pointer");
+  QSX_CHECK_ARG(M >= 1 && nc >= 1 && B >= 1 && nsave >= 1 && B <= 65535 && nc <= (1 << 30),
+                "qsx_propagate_chunks: bad sizes M=%d nc=%d B=%d nsave=%d", M, nc, B, nsave);
+  QSX_CHECK_ARG((strideP == 0 || strideP >= (int64_t)M * M) &&
+                    (strideA == 0 || strideA >= (int64_t)M * M),
+                "qsx_propagate_chunks: bad operator stride");
+  Shape sh;
+  QSX_CHECK_ARG(pick_shape(M, sh), "qsx_propagate_chunks: M=%d exceeds the register stepper "
+                "(<= 160)", M);
+  hipStream_t s = as_stream(stream);
+  if (precision == QSX_MIXED) {
+    QSX_CHECK_ARG(diag != nullptr, "qsx_propagate_chunks: MIXED needs the FP64 diagonal");
+    QSX_CHECK_ARG(strideD == 0 || strideD >= M, "qsx_propagate_chunks: bad strideD");
+    return launch_chunks<true>(sh, Pc, strideP, A, strideA, diag, strideD, M, y0, nc, B, nsub, h,
+                               nsave, out, status, s);
+  }
+  return launch_chunks<false>(sh, Pc, strideP, A, strideA, nullptr, 0, M, y0, nc, B, nsub, h,
+                              nsave, out, status, s);
 }

@@ -39,6 +39,12 @@ def stepper_operator(eom, schedule, h0, precision):
     return cache[key]
 
 
+# 'chunks' in one launch (qsx_propagate_chunks: every work-group computes its
+# own chunk start, then steps its chunk) instead of a launch for the chain of
+# chunk starts followed by one for the chunks; both give the same values.
+FUSED_CHUNKS = True
+
+
 class Stage(object):
     """One propagation with its launch schedule resident on the device.
 
@@ -46,9 +52,10 @@ class Stage(object):
       'powers'   few save points: Y_s = P^(n_s - n_{s-1}) Y_{s-1}, one batched
                  GEMM per save interval with a cached power (FP64);
       'chunks'   one right-hand side saved at every one of many equally spaced
-                 points: chunk starts y_{kc} = (P^c)^k y_0 by the register
-                 stepper with P^c, then all chunks stepped in parallel with P,
-                 so the sequential chain is n/c + c products instead of n;
+                 points: chunk starts y_{kc} = (P^c)^k y_0 with P^c, then all
+                 chunks stepped in parallel with P, so the sequential chain
+                 is n/c + c products instead of n (one launch: work-group k
+                 computes its own start, then steps its chunk);
       'doubling' the same trajectory by doubling: rows [y_0 .. y_{m-1}] give
                  rows [y_m .. y_{2m-1}] in one GEMM with the cached power
                  (P^q)^m, so ceil(log2 n) GEMMs (FP64);
@@ -145,6 +152,7 @@ class Stage(object):
                                        device=dev)
             self.chunk_n = torch.tensor([0] + [q] * (chunk - 1), dtype=torch.int32, device=dev)
             self.chunk_h = torch.full((chunk,), self.h0, dtype=torch.float64, device=dev)
+            self.chunk_steps = q * (chunk - 1)          # products of one chunk
             self.A, self.diag = stepper_operator(eom, 'propagator', self.h0, self.prec_name)
         elif kind == 'dchunks':
             q = int(n[1])
@@ -157,6 +165,7 @@ class Stage(object):
                 m *= 2
             self.chunk_n = torch.tensor([0] + [q] * (chunk - 1), dtype=torch.int32, device=dev)
             self.chunk_h = torch.full((chunk,), self.h0, dtype=torch.float64, device=dev)
+            self.chunk_steps = q * (chunk - 1)
             self.A, self.diag = stepper_operator(eom, 'propagator', self.h0, self.prec_name)
         elif kind == 'doubling':
             q = int(n[1])
@@ -198,7 +207,14 @@ class Stage(object):
             _doubling(starts, y0[:, 0], self.pw)
             st1, _, s1 = _native.propagate_states(self.mode, self.A, starts, self.chunk_n,
                                                    self.chunk_h, self.precision, self.diag,
-                                                   steps=self.steps, status=status)
+                                                   steps=self.chunk_steps, status=status)
+            statuses.append(s1)
+            return st1.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
+        if self.kind == 'chunks' and FUSED_CHUNKS:
+            B, R, M = y0.shape
+            st1, s1 = _native.propagate_chunks(self.Pc, self.A, y0, self.nc, self.chunk_n,
+                                               self.chunk_h, self.precision, self.diag,
+                                               steps=self.chunk_steps, status=status)
             statuses.append(s1)
             return st1.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
         if self.kind == 'chunks':
@@ -209,7 +225,7 @@ class Stage(object):
             starts = st0.reshape(B, self.nc, M)
             st1, _, s1 = _native.propagate_states(self.mode, self.A, starts, self.chunk_n,
                                                    self.chunk_h, self.precision, self.diag,
-                                                   steps=self.steps, status=status)
+                                                   steps=self.chunk_steps, status=status)
             statuses += [s0] if status is not None else [s0, s1]
             return st1.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
         out, _, st = _native.propagate_states(self.mode, self.A, y0, self.nsub, self.hs,

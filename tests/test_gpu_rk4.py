@@ -175,6 +175,52 @@ def test_register_stepper_all_shapes(gpu, M):
     assert rel_err(out.cpu().numpy(), ref) < 2e-6, 'propagator mixed'
 
 
+@pytest.mark.parametrize('M', [1, 7, 13, 24, 49, 100, 147])
+def test_chunked_trajectory_one_launch_matches_two(gpu, M):
+    """qsx_propagate_chunks (each work-group computes its own chunk start,
+    then steps its chunk) against the chain of chunk starts followed by the
+    chunks as right-hand sides (two qsx_propagate launches), FP64 and MIXED,
+    shared and per-problem operators: the same products in the same order,
+    so the same values; and against dense RK4 in FP64."""
+    from qspectra_amd import _native, engine, lti
+    rs = np.random.RandomState(100 + M)
+    B, q, c, nc = 3, 2, 5, 4
+    h = 0.5
+    L = (rs.randn(B, M, M) + 1j * rs.randn(B, M, M)) * 0.02 - 0.05j * np.eye(M)
+    y0 = rs.randn(B, 1, M) + 1j * rs.randn(B, 1, M)
+    Ld = _native.to_device(L)
+    y = _native.to_device(y0)
+    Pm = lti.rk4_propagator(Ld, h)
+    Pc = torch.linalg.matrix_power(Pm, q * c).contiguous()
+    dev = y.device
+    starts_n = torch.tensor([0] + [1] * (nc - 1), dtype=torch.int32, device=dev)
+    starts_h = torch.full((nc,), h * q * c, dtype=torch.float64, device=dev)
+    chunk_n = torch.tensor([0] + [q] * (c - 1), dtype=torch.int32, device=dev)
+    chunk_h = torch.full((c,), h, dtype=torch.float64, device=dev)
+    t = np.arange(nc * c) * (q * h)
+    ref = np.stack([P.integrate_rk4(L[b], y0[b], t, t0=0.0, dt=h) for b in range(B)])
+    for prec in (_native.QSX_FP64, _native.QSX_MIXED):
+        if prec == _native.QSX_FP64:
+            A, dg = Pm, None
+        else:
+            A, dg = lti.split_diagonal(lti.rk4_increment(Ld, h))
+        for shared in (False, True):
+            Pc_, A_ = (Pc[:1].contiguous(), A[:1].contiguous()) if shared else (Pc, A)
+            dg_ = dg[:1].contiguous() if (shared and dg is not None) else dg
+            st0, _, s0 = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, Pc_, y,
+                                                  starts_n, starts_h)
+            starts = st0.reshape(B, nc, M).contiguous()
+            two, _, s1 = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, A_, starts,
+                                                  chunk_n, chunk_h, prec, dg_)
+            one, s2 = _native.propagate_chunks(Pc_, A_, y, nc, chunk_n, chunk_h, prec, dg_)
+            assert int(s0.item()) == 0 and int(s1.item()) == 0 and int(s2.item()) == 0
+            assert one.shape == (B, nc, c, M)
+            assert rel_err(one.cpu().numpy(), two.cpu().numpy()) < 1e-14, (prec, shared)
+            if prec == _native.QSX_FP64 and not shared:
+                got = one.reshape(B, nc * c, M).cpu().numpy()
+                assert rel_err(got, ref[:, 0]) < 1e-11
+
+
 def test_zgemm_and_propagator(gpu):
     from qspectra_amd import _native, lti
     rs = np.random.RandomState(5)
