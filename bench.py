@@ -151,18 +151,24 @@ def dominant_kernel(summary, relaunch, timing):
         if best is None or total > best[3]:
             best = (label, info, ms, total)
     label, info, ms, _ = best
+    return label, info, ms, launch_flops(label)
+
+
+def launch_flops(label):
+    """Algorithmic flops of a stepper launch from its label: 8 M^2 per complex
+    matrix-vector product.  `steps` counts one right-hand side: a propagate
+    launch performs R B steps of 4 products (rk4) or 1 (prop); a chunked
+    launch performs, per problem, nc chunks of `steps` products plus the
+    nc - 1 products of its chain of chunk starts (each work-group recomputes
+    its own start: that repetition is not counted as work)."""
     f = dict(re.findall(r'(\w+)=(\w+)', label))
     M, B, steps = int(f['M']), int(f['B']), int(f['steps'])
     if label.startswith('chunks'):
-        # nc chunks of `steps` products each, plus the nc - 1 products of the
-        # chain of chunk starts (each work-group recomputes its own start:
-        # that repetition is not counted as work)
         nc = int(f['nc'])
         products = nc * steps + (nc - 1)
     else:
         products = int(f['R']) * steps * (4 if ' rk4 ' in label else 1)
-    flops = 8.0 * M * M * B * products
-    return label, info, ms, flops
+    return 8.0 * M * M * B * products
 
 
 def cpu_baseline(budget, steps_per_spectrum, min_seconds=10.0):
