@@ -512,8 +512,8 @@ LEGS = (('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', g
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
-        print('bench.py measures one GPU: this build runs on a single MI355X (the path '
-              'shards by ensemble member; see DESIGN.md). Use --gpus 1.', file=sys.stderr)
+        print('bench.py measures one GPU: the propagation runs on a single MI355X (across '
+              'GPUs it would shard by ensemble member). Use --gpus 1.', file=sys.stderr)
         return 2
     import torch
     import qspectra_amd as qa

@@ -12,7 +12,7 @@
  *   RedfieldModel.evolution_super_operator (redfield.py:9-152),
  *   Hamiltonian.eig (hamiltonian.py:310-328).
  * The entry points below replace the arithmetic under that API; the Python
- * package qspectra_amd binds them with ctypes.
+ * package qspectra_amd binds them with ctypes (qspectra_amd/_native.py).
  *
  * Conventions
  *   - every function returns int32 status: QSX_OK or a negative QSX_E* code;
