@@ -9,15 +9,21 @@ cutoff 106 cm^-1, 77 K; t1 x t3 = 128 x 128 at 2 fs, t2 in {0, 100, 200,
 500} fs; every t1 branch is a right-hand side of the t2 stage.
 
 One benchmark step is one whole 2-D spectrum (third_order_response plus the
-two FFTs), with the Liouvillian blocks already built and resident in HBM.
-One propagated density-matrix step is one RK4 step (4 Liouvillian
-applications) of one state vector; a 2-D spectrum is 3 * (127 + 128*250 +
-127) = 96,762 of them.
+two FFTs) computed from the Liouvillian blocks resident in HBM: every replay
+of the captured plan rebuilds the RK4 propagators, their powers and the
+mixed-precision split from the blocks, then propagates and contracts.  One
+propagated density-matrix step is one 2 fs step of one state vector on the
+spectrum's time grid; a 2-D spectrum is 3 * (127 + 128*250 + 127) = 96,762
+of them.  Each 2 fs step is integrated as two classical RK4 steps of 1 fs,
+so that the spectrum is within 1e-6 of the reference's tight-tolerance
+result (tests/golden/spectra_2d_full.npz; RK4 at 2 fs is 4.4e-6 from it).
 
 Legs: 'mixed' (complex64 Liouvillian products, FP64 diagonal / state / RK
-update; the reported value), 'plain' (all complex128).  The CPU baseline is
-the oracle's restatement of the reference path (scipy zvode at default
-tolerance, serial loop over right-hand sides) on one host thread.
+update; the reported value), 'plain' (pathway by pathway, classical RK4 at
+the same 1 fs step, all complex128).  The CPU baseline is the oracle's
+restatement of the reference path (scipy zvode at default tolerance, serial
+loop over right-hand sides), timed on all host cores (default BLAS
+threading) and on one thread.
 """
 import argparse
 import json
@@ -30,6 +36,8 @@ import numpy as np
 
 METRIC = 'propagated density-matrix steps/sec, batched-FP32/64 vs plain FP64, @1e-6 rel err'
 PEAK_TFLOPS = {'fp64': 78.6, 'mixed': 157.3}   # MI355X dense FP64 / FP32 (vector and MFMA)
+REFERENCE_2D = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'tests', 'golden',
+                            'spectra_2d_full.npz')
 
 
 def parse_args(argv=None):
@@ -38,8 +46,9 @@ def parse_args(argv=None):
     ap.add_argument('--steps', type=int, default=100)
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--cpu-baseline', choices=('auto', 'off'), default='auto')
-    ap.add_argument('--cpu-budget', type=float, default=30.0,
-                    help='seconds of CPU work allowed for the baseline sample')
+    ap.add_argument('--cpu-budget', type=float, default=40.0,
+                    help='seconds of CPU work allowed for the baseline samples (all cores, '
+                         'then one thread)')
     ap.add_argument('--workloads', default='all',
                     help="'all', 'none' or a comma list of ensemble_absorption, "
                          "adjoint_gradient, heom, pump, example_dimer_2d, zofe_fmo, "
@@ -133,36 +142,63 @@ def time_leg(torch, fn, steps):
     return time.perf_counter() - t0, out
 
 
-def dominant_kernel(summary, relaunch, timing):
-    """The propagation launch with the largest total device time per
-    spectrum; each launch's duration is measured back to back from a
-    captured graph (its own time, without host gaps).  Algorithmic work:
-    8 M^2 real flops per complex matrix-vector product, 4 per RK4 step or 1
-    per step with the RK4 propagator, times R B steps of the launch (the
-    label's `steps` counts one right-hand side; a chunked launch performs nc
-    chunks of `steps` products and the nc - 1 products of its chain of
-    chunk starts)."""
+# launch labels that are one kernel each (the others -- 'rk4 propagator',
+# 'matrix powers' of blocks beyond one work-group -- enqueue several GEMMs)
+SINGLE_KERNEL = ('propagate ', 'chunks ', 'small propagator', 'zgemm ')
+# kernel behind each single-kernel label, as rocprofv3 names it
+KERNEL_NAMES = {'chunks': 'chunk_kernel', 'propagate powers': 'small_powers_apply_kernel',
+                'small propagator': 'small_program_kernel', 'zgemm': 'zgemm_kernel',
+                'propagate': 'step_kernel'}
+
+
+def dominant_kernel(summary, relaunch, timing, per):
+    """The single-kernel launch with the largest total device time per
+    spectrum (``per`` spectra in the summary); its duration is measured back
+    to back from a captured graph (its own time, without host gaps)."""
     best = None
     for label, info in summary.items():
-        if not label.startswith(('propagate', 'chunks')):
+        if not label.startswith(SINGLE_KERNEL) or label not in relaunch:
             continue
         ms = timing.back_to_back_ms(relaunch[label])
-        total = ms * info['count']
+        total = ms * info['count'] / per
         if best is None or total > best[3]:
             best = (label, info, ms, total)
-    label, info, ms, _ = best
-    return label, info, ms, launch_flops(label)
+    return best
+
+
+def _gemms_power(e):
+    """Products of binary powering to exponent e >= 1."""
+    e = int(e)
+    return (e.bit_length() - 1) + (bin(e).count('1') - 1)
 
 
 def launch_flops(label):
-    """Algorithmic flops of a stepper launch from its label: 8 M^2 per complex
-    matrix-vector product.  `steps` counts one right-hand side: a propagate
-    launch performs R B steps of 4 products (rk4) or 1 (prop); a chunked
-    launch performs, per problem, nc chunks of `steps` products plus the
-    nc - 1 products of its chain of chunk starts (each work-group recomputes
-    its own start: that repetition is not counted as work)."""
-    f = dict(re.findall(r'(\w+)=(\w+)', label))
-    M, B, steps = int(f['M']), int(f['B']), int(f['steps'])
+    """Algorithmic flops of a launch from its label (8 real flops per complex
+    multiply-add):
+      propagate (stepper): 8 M^2 per matrix-vector product, 4 per RK4 step
+        or 1 per propagator step, R B right-hand sides x `steps`;
+      chunks: per problem nc chunks of `steps` products plus the nc - 1
+        products of the chain of chunk starts (each work-group recomputing
+        its own start is not counted as work);
+      small propagator: 8 M^3 per product: Horner's three, the powering to
+        q, to c;
+      propagate powers: Horner's three and the squarings (8 M^3 each), and
+        8 M^2 R per state product of every set bit of every step count;
+      zgemm: 8 m n k per batch entry."""
+    f = dict(re.findall(r'(\w+)=([\w,]+)', label))
+    if label.startswith('zgemm'):
+        return 8.0 * int(f['m']) * int(f['n']) * int(f['k']) * int(f['batch'])
+    M, B = int(f['M']), int(f['B'])
+    if label.startswith('small propagator'):
+        q, c = int(f['q']), int(f['c'])
+        n = 3 + _gemms_power(q) + (_gemms_power(c) if c > 0 else 0)
+        return 8.0 * M ** 3 * B * n
+    if label.startswith('propagate powers'):
+        ex = [int(x) for x in f['e'].split(',')]
+        sq = max(ex).bit_length() - 1 if max(ex) > 0 else 0
+        apply = sum(bin(x).count('1') for x in ex)
+        return 8.0 * B * (M ** 3 * (3 + sq) + M * M * int(f['R']) * apply)
+    steps = int(f['steps'])
     if label.startswith('chunks'):
         nc = int(f['nc'])
         products = nc * steps + (nc - 1)
@@ -171,41 +207,108 @@ def launch_flops(label):
     return 8.0 * M * M * B * products
 
 
-def cpu_baseline(budget, steps_per_spectrum, min_seconds=10.0):
-    """The oracle's zvode restatement of the reference path, one thread.
+def kernel_bound(label):
+    """(bound, peak TFLOP/s) of a kernel: the FP64 GEMM of enough tiles runs
+    on the FP64 matrix cores; every other kernel here computes on the vector
+    ALUs (FP32 for MIXED stepping, FP64 otherwise) -- the MI355X vector peaks
+    equal the dense MFMA ones (157.3 FP32, 78.6 FP64)."""
+    f = dict(re.findall(r'(\w+)=([\w,]+)', label))
+    if label.startswith('zgemm'):
+        tiles32 = -(-int(f['m']) // 32) * -(-int(f['n']) // 32) * int(f['batch'])
+        fp64 = ' fp64 ' in label
+        if fp64 and tiles32 >= 192:
+            return 'mfma', PEAK_TFLOPS['fp64']
+        return 'valu', PEAK_TFLOPS['fp64' if fp64 else 'mixed']
+    if label.startswith(('small propagator', 'propagate powers')):
+        return 'valu', PEAK_TFLOPS['fp64']
+    return 'valu', PEAK_TFLOPS['fp64' if ' fp64 ' in label else 'mixed']
 
-    The sample is the whole 2-D spectrum, repeated until ``min_seconds`` of
-    CPU work; if one spectrum would exceed ``budget``, whole pathways are run
+
+def host_info():
+    """The host the CPU baseline runs on."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    from threadpoolctl import threadpool_info
+    blas = [dict(api=p.get('internal_api'), threads=p.get('num_threads'))
+            for p in threadpool_info()]
+    return dict(nproc=os.cpu_count(), sched_getaffinity=affinity, cpu_model=model, blas=blas)
+
+
+def _cpu_sample(model, work, per_path, budget, min_seconds):
+    """Whole pathways of the 2-D spectrum through the oracle's zvode path,
+    repeated until min_seconds of work (or the budget is spent)."""
+    from oracle import propagate as P
+    done_steps, elapsed, runs = 0, 0.0, 0
+    while elapsed < min_seconds and elapsed < budget:
+        for name in ('GSB', 'ESE', 'ESA'):
+            t0 = time.perf_counter()
+            P.third_order_response(model, work['coherence_time_max'],
+                                   population_times=work['population_times'],
+                                   geometry=work['geometry'], include_signal=name,
+                                   integrator='zvode')
+            elapsed += time.perf_counter() - t0
+            done_steps += per_path
+            if elapsed > budget:
+                break
+        runs += 1
+    return done_steps, elapsed, runs
+
+
+def cpu_baseline(budget, steps_per_spectrum, min_seconds=10.0):
+    """The oracle's zvode restatement of the reference path (scipy zvode at
+    its default rtol 1e-6, serial loop over t1 branches, Liouvillian built
+    outside the timed part), timed (i) on every host core with numpy's
+    default BLAS threading -- the reported value -- and (ii) on one thread.
+    Each sample is the whole 2-D spectrum, repeated until ``min_seconds``
+    of work; if a spectrum would exceed ``budget``, whole pathways are run
     until the budget is spent and the rate is taken over their steps."""
     from threadpoolctl import threadpool_limits
-    from oracle import system as S, liouville as Lv, propagate as P
+    from oracle import system as S, liouville as Lv
     from qspectra_amd import workloads as W
     H1, d, extra = W.aggregate7_hamiltonian()
     ham = S.ElectronicHamiltonian(H1, bath=S.DebyeBath(S.CM_K * 77, 35, 106), dipoles=d,
                                   energy_spread_extra=extra)
     model = Lv.LiouvilleModel(ham, unit_convert=S.CM_FS, secular=False)
+    model.evolution_super_operator        # Liouvillian built outside the timed part
     work = W.AGG7_2D
     per_path = steps_per_spectrum // 3
-    done_steps, elapsed, runs = 0, 0.0, 0
+    info = host_info()
+    steps_all, sec_all, runs_all = _cpu_sample(model, work, per_path, budget / 2, min_seconds)
     with threadpool_limits(limits=1):
-        model.evolution_super_operator        # Liouvillian built outside the timed part
-        while elapsed < min_seconds and elapsed < budget:
-            for name in ('GSB', 'ESE', 'ESA'):
-                t0 = time.perf_counter()
-                P.third_order_response(model, work['coherence_time_max'],
-                                       population_times=work['population_times'],
-                                       geometry=work['geometry'], include_signal=name,
-                                       integrator='zvode')
-                elapsed += time.perf_counter() - t0
-                done_steps += per_path
-                if elapsed > budget:
-                    break
-            runs += 1
-    return dict(value=done_steps / elapsed, unit='steps/s', cores=1, kind='port',
-                sample='%d repetitions of the benchmark 2-D spectrum (%d RK4-equivalent steps) '
-                       'through the oracle restatement of the reference path: scipy zvode at '
-                       'rtol 1e-6, serial loop over t1 branches, Liouvillian prebuilt; %.1f s on '
-                       'one host thread' % (runs, done_steps, elapsed))
+        steps_one, sec_one, runs_one = _cpu_sample(model, work, per_path, budget / 2,
+                                                   min_seconds)
+    cores = info['sched_getaffinity'] or info['nproc']
+    return dict(value=steps_all / sec_all, unit='steps/s', cores=cores, kind='port',
+                sample='%d repetitions of the benchmark 2-D spectrum (%d steps of the 2 fs '
+                       'grid) through the oracle restatement of the reference path: scipy '
+                       'zvode at rtol 1e-6, serial loop over t1 branches, Liouvillian prebuilt; '
+                       '%.1f s with numpy default BLAS threading on %s cores'
+                       % (runs_all, steps_all, sec_all, cores),
+                ms_per_spectrum=1e3 * sec_all * per_path * 3 / steps_all,
+                one_thread=dict(value=steps_one / sec_one, cores=1,
+                                ms_per_spectrum=1e3 * sec_one * per_path * 3 / steps_one,
+                                sample='%d repetitions, %.1f s on one thread'
+                                       % (runs_one, sec_one)),
+                host=info,
+                reference_default_tolerance_rel_err=_reference_default_err())
+
+
+def _reference_default_err():
+    try:
+        return float(np.load(REFERENCE_2D, allow_pickle=False)['default_tol_rel_err'])
+    except (OSError, KeyError):
+        return None
 
 
 HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
@@ -524,29 +627,44 @@ def main(argv=None):
         return 0
     model = W.aggregate7_model(secular=False)
     work = W.AGG7_2D
-    steps_per_spectrum = W.rk4_steps_2d()
+    steps_per_spectrum = W.rk4_steps_2d()                  # steps of the 2 fs grid
+    rk4_per_spectrum = W.rk4_steps_2d(dt=work['dt'])       # RK4 steps of 1 fs
+    ref = np.load(REFERENCE_2D, allow_pickle=False)
 
     fused = FusedSpectrum(model, work, 'mixed')
     for _ in range(max(args.warmup, 1)):
         run_plain(qa, model, work)
         fused()
     fused.plan.check_status()
+    f1, _, f3 = fused.plan.frequencies()
+    assert np.allclose(f1, ref['f1']) and np.allclose(f3, ref['f3'])
 
     t_plain, X64 = time_leg(torch, lambda: run_plain(qa, model, work), args.steps)
     t_mixed, X32 = time_leg(torch, fused, args.steps)
+    fused.plan.check_status()
     X32 = X32.cpu().numpy()
-    err = float(np.max(np.abs(X32 - X64)) / np.max(np.abs(X64)))
-    # per-kernel durations: the same launch sequence without the graph, with
-    # HIP events around every launch on the stream it runs on
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    # device time of the plain leg: its launches, summed
+    with timing.kernel_timer() as kt:
+        run_plain(qa, model, work)
+    plain_device_ms = sum(v['total_ms'] for v in kt.summary().values())
+    # per-kernel durations of the fused plan: the same launch sequence without
+    # the graph, with HIP events around every launch on the stream it runs on
     with timing.kernel_timer() as kt:
         for _ in range(args.steps):
             fused.plan._launch()
     summary = kt.summary()
-    label, info, launch_ms, flops = dominant_kernel(summary, kt.relaunch, timing)
+    label, info, launch_ms, kernel_ms_per_spectrum = dominant_kernel(summary, kt.relaunch,
+                                                                     timing, args.steps)
+    flops = launch_flops(label)
+    bound, peak = kernel_bound(label)
     achieved = flops / (launch_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS['fp64' if ' fp64 ' in label else 'mixed']
     value = steps_per_spectrum * args.steps / t_mixed
     plain = steps_per_spectrum * args.steps / t_plain
+    per_spec = {k: {'launches': v['count'] / args.steps, 'mean_ms': round(v['mean_ms'], 5)}
+                for k, v in summary.items()}
 
     line = {
         'metric': METRIC, 'value': value, 'unit': 'steps/s', 'n_gpus': 1,
@@ -556,18 +674,36 @@ def main(argv=None):
         'data': 'synthetic',
         'config': {'workload': '2-D spectrum (GSB+ESE+ESA, -++) of a 7-site aggregate, '
                                'non-secular Redfield g/e/f, t1 x t3 = 128 x 128 at 2 fs, '
-                               't2 in {0,100,200,500} fs, all t1 branches as RHS',
-                   'rk4_steps_per_spectrum': steps_per_spectrum, 'parallelism': 'single GPU'},
-        'plain_fp64': {'value': plain, 'ms_per_step': 1e3 * t_plain / args.steps},
+                               't2 in {0,100,200,500} fs, all t1 branches as RHS; RK4 at 1 fs',
+                   'steps_per_spectrum': steps_per_spectrum, 'grid_step_fs': 2.0,
+                   'rk4_step_fs': work['dt'], 'parallelism': 'single GPU'},
+        'timed_region': 'one captured graph per spectrum: RK4 propagators, their powers and '
+                        'the mixed split rebuilt from the Liouvillian blocks, trajectories, '
+                        'contractions, both Fourier transforms',
+        'ms_per_spectrum': 1e3 * t_mixed / args.steps,
+        'rk4_steps_per_spectrum_reference_algorithm': rk4_per_spectrum,
+        'executed_per_spectrum': dict(fused.plan.executed_products(),
+                                      launches=sum(v['launches'] for v in per_spec.values()),
+                                      device_ms_sum_over_streams=sum(
+                                          v['total_ms'] for v in summary.values()) / args.steps),
+        'plain_fp64': {'value': plain, 'ms_per_step': 1e3 * t_plain / args.steps,
+                       'device_ms_per_spectrum': plain_device_ms,
+                       'rel_err_vs_reference': rel(X64, ref['S2d'])},
         'speedup_vs_plain': value / plain,
-        'rel_err_vs_fp64': err,
-        'roofline': {'bound': 'mfma', 'kernel': label, 'achieved': achieved,
-                     'peak': peak, 'unit': 'TFLOP/s', 'frac': achieved / peak,
-                     'traffic': launch_traffic(label),
+        'rel_err_vs_fp64': rel(X32, X64),
+        'rel_err_vs_reference': rel(X32, ref['S2d']),
+        'reference': 'tests/golden/spectra_2d_full.npz: the reference two_dimensional_spectra at '
+                     'zvode rtol 1e-12, atol 1e-14',
+        'roofline': {'bound': bound, 'kernel': label,
+                     'rocprof_kernel': next((v for k, v in sorted(KERNEL_NAMES.items(),
+                                                                  key=lambda kv: -len(kv[0]))
+                                             if label.startswith(k)), None),
+                     'achieved': achieved, 'peak': peak, 'unit': 'TFLOP/s',
+                     'frac': achieved / peak, 'traffic': launch_traffic(label),
                      'launch_ms': launch_ms, 'launch_ms_in_sequence': info['mean_ms'],
+                     'kernel_ms_per_spectrum': kernel_ms_per_spectrum,
                      'algorithmic_flop': flops},
-        'kernels': {k: {'count': v['count'], 'mean_ms': round(v['mean_ms'], 5)}
-                    for k, v in summary.items()},
+        'kernels': per_spec,
         'plan_chunk': {'choice': fused.chunk, 'ms_per_spectrum': fused.chunk_ms},
     }
     wanted = ([n for n, _ in LEGS] if args.workloads == 'all' else
@@ -584,6 +720,7 @@ def main(argv=None):
     if args.cpu_baseline == 'auto':
         line['cpu_baseline'] = cpu_baseline(args.cpu_budget, steps_per_spectrum)
         line['vs_cpu'] = value / line['cpu_baseline']['value']
+        line['vs_cpu_one_thread'] = value / line['cpu_baseline']['one_thread']['value']
     else:
         line['cpu_baseline'] = None
     print(json.dumps(line))

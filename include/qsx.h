@@ -214,6 +214,60 @@ int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_t n, int32_
                   int32_t batch, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Propagation operators of a time-invariant block, built on the device and
+ * enqueued on `stream` (a captured graph rebuilds them on every replay).
+ * The reference has no such operators: its zvode integration (qspectra/
+ * simulate/utils.py:16-50) applies L = evolution_super_operator[idx][:, idx]
+ * (qspectra/dynamics/liouville_space.py:316-341) at every internal step.  A
+ * classical RK4 step of a constant L is exactly y <- P4(h L) y, so these
+ * replace the step loop's operator; the trajectories they produce are the
+ * RK4 ones up to rounding.
+ *
+ * Workspace of each: qsx_propagator_workspace_bytes(M, B) bytes
+ * (4 B M^2 complex128).  All matrices complex128 [B][M][M], row-major. */
+int64_t qsx_propagator_workspace_bytes(int32_t M, int32_t B);
+
+/* P[b] = P4(h L[b])^q, P4(X) = I + X + X^2/2 + X^3/6 + X^4/24 (Horner: three
+ * GEMMs), q >= 1 RK4 steps of size h per application (repeated squaring).
+ * strideL = 0: one L shared by the batch. */
+int32_t qsx_rk4_propagator(const void* L, int64_t strideL, int32_t M, int32_t B, double h,
+                           int32_t q, void* P, void* workspace, int64_t workspace_bytes,
+                           void* stream);
+
+/* out[b][s] = A[b]^exps[s] for s < ns (exps: HOST array, >= 0; exponent 0
+ * gives the identity), one shared chain of squarings. */
+int32_t qsx_matrix_powers(const void* A, int64_t strideA, int32_t M, int32_t B,
+                          const int64_t* exps, int32_t ns, void* out, void* workspace,
+                          int64_t workspace_bytes, void* stream);
+
+/* The MIXED operands of a propagator P [B][M][M]: off [B][M][M] complex64 =
+ * P with its diagonal zeroed, diag [B][M] complex128 = diag(P) - 1 (the
+ * stepper applies y + off y + diag * y). */
+int32_t qsx_split_increment(const void* P, int32_t M, int32_t B, void* off, void* diag,
+                            void* stream);
+
+/* Small blocks (M <= 64), one work-group per batch entry with the matrices in
+ * LDS, so the whole operator program is one launch:
+ *   qsx_small_propagator: P = P4(h L)^q into P [B][M][M] (may be NULL), its
+ *     power P^c into Pc [B][M][M] (c = 0: none), and the MIXED split of P
+ *     into off / diag as qsx_split_increment (both NULL: none);
+ *   qsx_propagate_powers: out[b][s][r] = P4(h L[b])^exps[s] y0[b][r] for R
+ *     states y0 [B][R][M] and ns <= 8 cumulative step counts exps (HOST
+ *     array): the few-save-point propagation of integrate
+ *     (qspectra/simulate/utils.py:16-109, e.g. the t2 stage of
+ *     qspectra/simulate/response.py:321-322), the squarings shared by all
+ *     save points.
+ * qsx_small_propagator_fits(M, R, ns) is 1 when a block fits these kernels
+ * (R = 0: qsx_small_propagator). */
+int32_t qsx_small_propagator_fits(int32_t M, int32_t R, int32_t ns);
+int32_t qsx_small_propagator(const void* L, int64_t strideL, int32_t M, int32_t B, double h,
+                             int32_t q, int64_t c, void* P, void* Pc, void* off, void* diag,
+                             void* stream);
+int32_t qsx_propagate_powers(const void* L, int64_t strideL, int32_t M, int32_t B, double h,
+                             const int64_t* exps, int32_t ns, const void* y0, int32_t R, void* out,
+                             void* stream);
+
+/* ---------------------------------------------------------------------------
  * HEOM for a Drude bath (qspectra/dynamics/heom.py:177-443), batched over B
  * samples that share the hierarchy structure.  State per sample: ADO-major
  * [n_ado][M] complex128.  Operator tables (device pointers):

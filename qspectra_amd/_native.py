@@ -40,7 +40,8 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
 # name -> argtypes (restype int32 unless listed in RESTYPES)
 RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64,
             'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64,
-            'qsx_redfield_dense_workspace_bytes': c_i64, 'qsx_fourier_workspace_bytes': c_i64}
+            'qsx_redfield_dense_workspace_bytes': c_i64, 'qsx_fourier_workspace_bytes': c_i64,
+            'qsx_propagator_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -78,6 +79,15 @@ SIGNATURES = {
     'qsx_fourier_workspace_bytes': [c_i32],
     'qsx_fourier_transform': [c_vp, c_i64, c_i32, c_i32, c_i32, c_i32, ctypes.c_double, c_i32,
                               c_vp, c_vp, c_vp],
+    'qsx_propagator_workspace_bytes': [c_i32, c_i32],
+    'qsx_rk4_propagator': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp, c_i64, c_vp],
+    'qsx_matrix_powers': [c_vp, c_i64, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_i64, c_vp],
+    'qsx_split_increment': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp],
+    'qsx_small_propagator_fits': [c_i32, c_i32, c_i32],
+    'qsx_small_propagator': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_i64, c_vp, c_vp, c_vp,
+                             c_vp, c_vp],
+    'qsx_propagate_powers': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_vp, c_i32, c_vp, c_i32, c_vp,
+                             c_vp],
     'qsx_zofe_workspace_bytes': [c_i32, c_i32, c_i32, c_i32, c_i32],
     'qsx_zofe_propagate': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp,
                            c_vp, c_i64, c_i32, c_dbl, c_i32, c_vp, c_i64, c_vp, c_i64, c_i64,
@@ -509,6 +519,115 @@ def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stre
         st = go()
     check(st, 'qsx_zgemm')
     return C
+
+
+# ------------------------------------------------- propagation operators
+def propagator_workspace(M, B, dev=None):
+    """Device workspace of qsx_rk4_propagator / qsx_matrix_powers."""
+    n = load_library().qsx_propagator_workspace_bytes(M, B)
+    if n < 0:
+        raise NativeError('qsx_propagator_workspace_bytes(%d, %d) failed' % (M, B))
+    dev = device() if dev is None else dev
+    return torch.empty(n // 16, dtype=torch.complex128, device=dev)
+
+
+def rk4_propagator_into(L, h, q, P, workspace, stream=None):
+    """P [B, M, M] <- P4(h L)^q on the device (L [B|1, M, M] complex128)."""
+    lib = require_native()
+    B, M, _ = P.shape
+    assert L.dtype == torch.complex128 and L.is_contiguous() and L.shape[-2:] == (M, M)
+    assert L.shape[0] in (1, B) and P.dtype == torch.complex128 and P.is_contiguous()
+    assert workspace.dtype == torch.complex128
+    strideL = 0 if L.shape[0] == 1 else M * M
+
+    def go():
+        return lib.qsx_rk4_propagator(ptr(L), strideL, M, B, float(h), int(q), ptr(P),
+                                      ptr(workspace), workspace.numel() * 16, stream_ptr(stream))
+    with timing.launch('rk4 propagator M=%d B=%d q=%d' % (M, B, q), stream, go):
+        st = go()
+    check(st, 'qsx_rk4_propagator')
+    return P
+
+
+def matrix_powers_into(A, exps, out, workspace, stream=None):
+    """out [B, ns, M, M] <- A^exps[s] (A [B|1, M, M] complex128)."""
+    lib = require_native()
+    B, ns, M, _ = out.shape
+    assert A.dtype == torch.complex128 and A.is_contiguous() and A.shape[-2:] == (M, M)
+    assert A.shape[0] in (1, B) and out.is_contiguous() and len(exps) == ns
+    e = (ctypes.c_int64 * ns)(*[int(x) for x in exps])
+    strideA = 0 if A.shape[0] == 1 else M * M
+
+    def go():
+        return lib.qsx_matrix_powers(ptr(A), strideA, M, B, e, ns, ptr(out), ptr(workspace),
+                                     workspace.numel() * 16, stream_ptr(stream))
+    with timing.launch('matrix powers M=%d B=%d e=%s' % (M, B, ','.join(map(str, exps))),
+                       stream, go):
+        st = go()
+    check(st, 'qsx_matrix_powers')
+    return out
+
+
+def split_increment_into(P, off, diag, stream=None):
+    """off [B, M, M] complex64 <- P with zero diagonal; diag [B, M] <- diag(P) - 1."""
+    lib = require_native()
+    B, M, _ = P.shape
+    assert P.dtype == torch.complex128 and P.is_contiguous()
+    assert off.dtype == torch.complex64 and off.shape == P.shape and off.is_contiguous()
+    assert diag.dtype == torch.complex128 and diag.shape == (B, M) and diag.is_contiguous()
+
+    def go():
+        return lib.qsx_split_increment(ptr(P), M, B, ptr(off), ptr(diag), stream_ptr(stream))
+    with timing.launch('split increment M=%d B=%d' % (M, B), stream, go):
+        st = go()
+    check(st, 'qsx_split_increment')
+    return off, diag
+
+
+def small_fits(M, R=0, ns=0):
+    """True when a block of M states fits the one-work-group kernels
+    (qsx_small_propagator with R = 0, qsx_propagate_powers otherwise)."""
+    return bool(load_library().qsx_small_propagator_fits(M, R, ns))
+
+
+def small_propagator_into(L, h, q, c=0, P=None, Pc=None, off=None, diag=None, stream=None):
+    """One launch for blocks of M <= 64: P <- P4(h L)^q, Pc <- P^c, and the
+    MIXED split (off complex64, diag complex128) of P; outputs may be None."""
+    lib = require_native()
+    Bl, M, _ = L.shape
+    assert L.dtype == torch.complex128 and L.is_contiguous()
+    for X, dt, shp in ((P, torch.complex128, (Bl, M, M)), (Pc, torch.complex128, (Bl, M, M)),
+                       (off, torch.complex64, (Bl, M, M)), (diag, torch.complex128, (Bl, M))):
+        if X is not None:
+            assert X.dtype == dt and X.is_contiguous() and X.numel() == int(np.prod(shp))
+
+    def go():
+        return lib.qsx_small_propagator(ptr(L), M * M, M, Bl, float(h), int(q), int(c), ptr(P),
+                                        ptr(Pc), ptr(off), ptr(diag), stream_ptr(stream))
+    with timing.launch('small propagator M=%d B=%d q=%d c=%d' % (M, Bl, q, c), stream, go):
+        st = go()
+    check(st, 'qsx_small_propagator')
+
+
+def propagate_powers_into(L, h, exps, y0, out, stream=None):
+    """out [B, ns, R, M] <- P4(h L)^exps[s] y0 [B, R, M] (blocks of M <= 64)."""
+    lib = require_native()
+    B, R, M = y0.shape
+    ns = len(exps)
+    assert L.dtype == torch.complex128 and L.is_contiguous() and L.shape[-2:] == (M, M)
+    assert L.shape[0] in (1, B) and y0.dtype == torch.complex128 and y0.is_contiguous()
+    assert out.dtype == torch.complex128 and out.is_contiguous() and out.numel() == B * ns * R * M
+    e = (ctypes.c_int64 * ns)(*[int(x) for x in exps])
+    strideL = 0 if L.shape[0] == 1 else M * M
+
+    def go():
+        return lib.qsx_propagate_powers(ptr(L), strideL, M, B, float(h), e, ns, ptr(y0), R,
+                                        ptr(out), stream_ptr(stream))
+    with timing.launch('propagate powers M=%d B=%d R=%d e=%s' % (
+            M, B, R, ','.join(map(str, exps))), stream, go):
+        st = go()
+    check(st, 'qsx_propagate_powers')
+    return out
 
 
 # ------------------------------------------------------------------- HEOM

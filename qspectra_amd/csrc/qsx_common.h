@@ -34,6 +34,13 @@ void set_error(const char* fmt, ...);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// qsx_zgemm's validation and launch, for use by other entry points of the
+// library (zgemm.hip): enqueues C = alpha op(A) op(B) + beta C + gamma I.
+int zgemm_enqueue(int precision, int trans, int m, int n, int k, double alpha_re,
+                  double alpha_im, const void* A, int64_t lda, int64_t sA, const void* B,
+                  int64_t ldb, int64_t sB, double beta_re, double beta_im, void* C, int64_t ldc,
+                  int64_t sC, double gamma_re, double gamma_im, int batch, void* stream);
+
 // --------------------------------------------------------- complex algebra
 template <typename T> struct cplx;
 template <> struct cplx<double> { using type = double2; };

@@ -21,7 +21,8 @@ namespace {
 // TM x TN output tile per 256-thread workgroup, each thread an (TM/16) x
 // (TN/16) block, K in chunks of TK.  op(A) = A or A^T (TA), op(B) = B or B^T
 // (TB); the LDS staging loads walk the contiguous index of the stored
-// operand and are unrolled, so a chunk's loads are all in flight together.
+// operand and are unrolled, so a chunk's loads are all in flight together,
+// and the loads of chunk c + 1 are issued before the FMAs of chunk c.
 template <typename T, int TM, int TN, int TK, bool TA, bool TB>
 __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2 alpha,
                                                     const cplx_t<T>* __restrict__ A, int64_t lda,
@@ -45,8 +46,8 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
   for (int i = 0; i < RM; ++i)
 #pragma unroll
     for (int j = 0; j < RN; ++j) acc[i][j] = T2{0, 0};
-  for (int k0 = 0; k0 < k; k0 += TK) {
-    T2 av[TM * TK / 256], bv[TN * TK / 256];
+  T2 av[TM * TK / 256], bv[TN * TK / 256];      // a chunk's loads, all in flight
+  auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < TM * TK / 256; ++u) {
       const int e = tid + 256 * u;
@@ -67,6 +68,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
       const int64_t off = TB ? (int64_t)gj * ldb + gk : (int64_t)gk * ldb + gj;
       bv[u] = (gk < k && gj < n) ? Bb[off] : T2{0, 0};
     }
+  };
+  load_chunk(0);
+  for (int k0 = 0; k0 < k; k0 += TK) {
 #pragma unroll
     for (int u = 0; u < TM * TK / 256; ++u) {
       const int e = tid + 256 * u;
@@ -80,6 +84,8 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double2
       else Bs[e / TN][e % TN] = bv[u];
     }
     __syncthreads();
+    // the next chunk's loads are in flight while this chunk's FMAs run
+    if (k0 + TK < k) load_chunk(k0 + TK);
     const int kend = min(TK, k - k0);
 #pragma unroll 4
     for (int kk = 0; kk < kend; ++kk) {
@@ -246,11 +252,11 @@ void launch_zgemm(int trans, dim3 grid, hipStream_t s, int m, int n, int k, doub
 }  // namespace
 }  // namespace qsx
 
-extern "C" int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_t n, int32_t k,
-                             double alpha_re, double alpha_im, const void* A, int64_t lda,
-                             int64_t sA, const void* B, int64_t ldb, int64_t sB, double beta_re,
-                             double beta_im, void* C, int64_t ldc, int64_t sC, double gamma_re,
-                             double gamma_im, int32_t batch, void* stream) {
+int qsx::zgemm_enqueue(int precision, int trans, int m, int n, int k, double alpha_re,
+                       double alpha_im, const void* A, int64_t lda, int64_t sA, const void* B,
+                       int64_t ldb, int64_t sB, double beta_re, double beta_im, void* C,
+                       int64_t ldc, int64_t sC, double gamma_re, double gamma_im, int batch,
+                       void* stream) {
   using namespace qsx;
   QSX_CHECK_ARG(precision == QSX_FP64 || precision == QSX_MIXED, "qsx_zgemm: bad precision");
   QSX_CHECK_ARG(trans >= 0 && trans <= 3, "qsx_zgemm: bad transpose flags %d", trans);
@@ -295,4 +301,13 @@ extern "C" int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_
 #undef QSX_ZL
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
+}
+
+extern "C" int32_t qsx_zgemm(int32_t precision, int32_t trans, int32_t m, int32_t n, int32_t k,
+                             double alpha_re, double alpha_im, const void* A, int64_t lda,
+                             int64_t sA, const void* B, int64_t ldb, int64_t sB, double beta_re,
+                             double beta_im, void* C, int64_t ldc, int64_t sC, double gamma_re,
+                             double gamma_im, int32_t batch, void* stream) {
+  return qsx::zgemm_enqueue(precision, trans, m, n, k, alpha_re, alpha_im, A, lda, sA, B, ldb,
+                            sB, beta_re, beta_im, C, ldc, sC, gamma_re, gamma_im, batch, stream);
 }

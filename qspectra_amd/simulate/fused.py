@@ -16,8 +16,7 @@ bra of pathway p), so a plan evaluates it in the cheapest association:
     '-++' geometry share it), computed by the register stepper;
   * the t2 stage acts on the M1 rows of V1^T instead of on the n1 t1
     branches: Q_s = V1^T (P2^N_s)^T, one batched GEMM with the RK4
-    propagator powers (cached on the equation of motion, like the
-    Liouvillian);
+    propagator powers;
   * W_s = Q_s V2_p^T for every pathway, written side by side so that all
     pathways contract with their t3 trajectories in one GEMM;
   * each distinct t3 Heisenberg trajectory once, each on a stream of its own
@@ -32,16 +31,26 @@ bra of pathway p), so a plan evaluates it in the cheapest association:
 Exact algebra: the result equals the pathway-by-pathway evaluation up to
 rounding (the same RK4 schedule, only the order of the linear maps
 differs).  The launch sequence can be captured into a HIP graph.
+
+Every operator derived from a Liouvillian block is rebuilt from that block
+by every launch sequence (and so by every graph replay): the one-step RK4
+propagators P4(h L) (Horner, qsx_rk4_propagator), the save-interval
+propagators P4(h L)^q of q RK4 steps (repeated squaring), the t2 powers and
+the chunk-start powers (qsx_matrix_powers), and the complex64 / FP64 split
+of MIXED precision (qsx_split_increment).  Only the dipole maps, the
+initial state and the Fourier matrices -- none of which depends on the
+Liouvillian -- are prepared once.  Overwriting the Liouvillian blocks in
+place (``liouvillians()``) and replaying therefore computes the response of
+the new generator.
 """
 import numpy as np
 import torch
 
 from .. import _native
 from ..engine import rk4_schedule
-from ..lti import propagator_power
 from .response import (_members, _op_matrices, _parse_pathways, THIRD_ORDER_PATHWAYS,
                        third_order_grids)
-from ..schedules import Stage as _Stage
+from ..schedules import _mode
 from ..timing import capture as _capture
 
 
@@ -58,6 +67,78 @@ def transform_matrix(t, rw_freq, unit_convert, sign, device):
 class PlanNotApplicable(ValueError):
     """The response cannot be evaluated by a fused plan (non-uniform t2 step,
     schedule 'rk4', pathways with different t1 propagations)."""
+
+
+class _Trajectory(object):
+    """A trajectory of one right-hand side per member saved at equally spaced
+    points (the t1 or a t3 stage of a plan), in nc chunks of c saves.  build()
+    enqueues the operators from the Liouvillian block L: the save-interval
+    propagator P = P4(h L)^q (q RK4 steps of size h per save interval), the
+    chunk-start power P^c and, for MIXED, the complex64 off-diagonal part of
+    P with its FP64 diagonal minus one; run() the chunked stepping
+    (qsx_propagate_chunks: work-group j reaches its start (P^c)^j y0, then
+    takes c - 1 steps with P)."""
+    kind = 'chunks'
+
+    def __init__(self, eom, t, dt, precision, chunk):
+        n, h = rk4_schedule(t, None, dt)
+        if len(n) < 2 or n[0] != 0 or not np.all(n[1:] == n[1]) or n[1] < 1:
+            raise PlanNotApplicable('the fused plan needs trajectories saved at equally '
+                                    'spaced points from t = 0')
+        hs = h[1:]
+        if not np.all(np.abs(hs - hs[0]) <= 1e-12 * abs(hs[0])):
+            raise PlanNotApplicable('the fused plan needs a uniform step')
+        self.eom = eom
+        self.L = eom.L
+        self.q, self.h = int(n[1]), float(hs[0])
+        self.nsave = len(t)
+        self.rk4_steps = int(n.sum())
+        if chunk is None:
+            chunk = max(4, int(np.ceil(np.sqrt(self.nsave))))
+        self.c = int(min(chunk, self.nsave))
+        self.nc = -(-self.nsave // self.c)
+        self.precision = _mode(precision)
+        Bl, M, _ = self.L.shape
+        dev = self.L.device
+        self.M = M
+        self.P = torch.empty((Bl, M, M), dtype=torch.complex128, device=dev)
+        self.Pc = torch.empty((Bl, 1, M, M), dtype=torch.complex128, device=dev)
+        self.ws = _native.propagator_workspace(M, Bl, dev)
+        if self.precision == _native.QSX_MIXED:
+            self.off = torch.empty((Bl, M, M), dtype=torch.complex64, device=dev)
+            self.diag = torch.empty((Bl, M), dtype=torch.complex128, device=dev)
+        else:
+            self.off = self.diag = None
+        self.chunk_n = torch.tensor([0] + [1] * (self.c - 1), dtype=torch.int32, device=dev)
+        self.chunk_h = torch.full((self.c,), self.h * self.q, dtype=torch.float64, device=dev)
+
+    def products(self):
+        """Propagator products the chunked stepping executes per member:
+        nc chunks of c - 1 steps plus the nc - 1 products of the chain of
+        chunk starts."""
+        return self.nc * (self.c - 1) + (self.nc - 1)
+
+    def build(self):
+        if _native.small_fits(self.M):
+            # one launch: Horner, the q-th and (q c)-th powers, the split
+            _native.small_propagator_into(self.L, self.h, self.q,
+                                          self.c if self.nc > 1 else 0, self.P,
+                                          self.Pc if self.nc > 1 else None, self.off, self.diag)
+            return
+        _native.rk4_propagator_into(self.L, self.h, self.q, self.P, self.ws)
+        if self.nc > 1:
+            _native.matrix_powers_into(self.P, [self.c], self.Pc, self.ws)
+        if self.off is not None:
+            _native.split_increment_into(self.P, self.off, self.diag)
+
+    def run(self, y0, statuses, status):
+        B, _, M = y0.shape
+        A = self.P if self.off is None else self.off
+        out, st = _native.propagate_chunks(self.Pc[:, 0] if self.nc > 1 else self.P, A, y0,
+                                           self.nc, self.chunk_n, self.chunk_h, self.precision,
+                                           self.diag, steps=self.c - 1, status=status)
+        statuses.append(st)
+        return out.reshape(B, 1, self.nc * self.c, M)[:, :, :self.nsave]
 
 
 class ThirdOrderPlan(object):
@@ -120,35 +201,40 @@ class ThirdOrderPlan(object):
                                   'commutator')
                 y1 = np.einsum('bij,j->bi', V0, rho0)[:, None, :]
                 y1 = np.broadcast_to(y1, (B,) + y1.shape[1:])
-                stage = _Stage(model.equation_of_motion(s[1]), self.t1, None, dt, precision,
-                               schedule, 1, chunk=chunk, graph=True)
+                stage = _Trajectory(model.equation_of_motion(s[1]), self.t1, dt, precision,
+                                    chunk)
                 self.t1_stages[k1] = (stage, dev_t(y1))
             k12 = tuple(s[:3])
             if k12 not in self.groups:
                 V1 = _op_matrices(model, '%s->%s' % (s[1], s[2]), polarization[1], trans[1],
                                   'commutator')                        # [B|1, M2, M1]
                 eom2 = model.equation_of_motion(s[2])
-                M2 = eom2.L.shape[-1]
-                pw = []
-                for N in self.N2:
-                    if N == 0:
-                        pw.append(torch.eye(M2, dtype=torch.complex128, device=dev)
-                                  .expand(eom2.L.shape[0], M2, M2))
-                    else:
-                        pw.append(propagator_power(eom2, h2, int(N)))
-                Pn = torch.stack(pw, dim=1).reshape(-1, M2, M2).contiguous()   # [B*n2|n2, ..]
-                if Pn.shape[0] != B * n2:
-                    Pn = Pn.repeat(B, 1, 1).contiguous() if Pn.shape[0] == n2 else Pn
+                Bl, M2 = eom2.L.shape[0], eom2.L.shape[-1]
+                if Bl not in (1, B):
+                    raise PlanNotApplicable('Liouvillian batch does not match the members')
+                # t2 operators, rebuilt by every launch: P2 = P4(h2 L2), its
+                # powers P2^N_s at the population times, [B*n2, M2, M2]
                 V1T = per_batch(dev_t(V1.transpose(0, 2, 1)), n2)           # [B*n2|1, M1, M2]
-                self.groups[k12] = dict(t1=k1, V1T=V1T, Pn=Pn)
+                g = dict(t1=k1, V1T=V1T, L=eom2.L, h=h2, rows=None)
+                M1 = V1.shape[2]                                           # V1 [B|1, M2, M1]
+                if _native.small_fits(M2, M1, n2):
+                    # one launch: the rows of V1^T propagated to every population
+                    # time (Q_s = V1^T (P2^N_s)^T, shared squarings)
+                    rows = dev_t(V1.transpose(0, 2, 1))
+                    g['rows'] = rows.expand(B, -1, -1).contiguous() if rows.shape[0] != B \
+                        else rows
+                else:
+                    g['Pn'] = torch.empty((B, n2, M2, M2), dtype=torch.complex128, device=dev)
+                    g['P'] = torch.empty((Bl, M2, M2), dtype=torch.complex128, device=dev)
+                    g['ws'] = _native.propagator_workspace(M2, Bl, dev)
+                self.groups[k12] = g
             k3 = tuple(s[3:])
             if k3 not in self.t3_stages:
                 b3 = _op_matrices(model, '%s->%s' % (s[3], s[4]), polarization[3], trans[3],
                                   'bra')[:, None, :]
                 b3 = np.broadcast_to(b3, (B,) + b3.shape[1:])
-                stage = _Stage(model.equation_of_motion(s[3], heisenberg_picture=True),
-                               self.t3, None, dt, precision, schedule, 1, chunk=chunk,
-                               graph=True)
+                stage = _Trajectory(model.equation_of_motion(s[3], heisenberg_picture=True),
+                                    self.t3, dt, precision, chunk)
                 self.t3_stages[k3] = dict(stage=stage, bra=dev_t(b3), col=None)
             V2 = _op_matrices(model, '%s->%s' % (s[2], s[3]), polarization[2], trans[2],
                               'commutator')                            # [B|1, M3, M2]
@@ -204,19 +290,42 @@ class ThirdOrderPlan(object):
         self.steps = self._count_steps(paths)
         self._t3_list = list(self.t3_stages.values())
         self._t3_order = sorted(range(len(self._t3_list)),
-                                key=lambda i: -self._t3_list[i]['stage'].eom.L.shape[-1])
+                                key=lambda i: -self._t3_list[i]['stage'].M)
         self._sides = [torch.cuda.Stream() for _ in self.t3_stages]
+        self._t2side = torch.cuda.Stream()
         self._graph = None
         self._statuses = []
 
     def _count_steps(self, paths):
-        """RK4 steps of the reference's pathway-by-pathway algorithm (one
-        right-hand side per t1 branch in the t2 stage)."""
+        """RK4 steps of the reference's pathway-by-pathway algorithm at the
+        plan's step (one right-hand side per t1 branch in the t2 stage)."""
         n1 = len(self.t1)
-        s1 = next(iter(self.t1_stages.values()))[0].steps
+        s1 = next(iter(self.t1_stages.values()))[0].rk4_steps
         s2 = int(self.N2[-1])
-        s3 = {k: v['stage'].steps for k, v in self.t3_stages.items()}
+        s3 = {k: v['stage'].rk4_steps for k, v in self.t3_stages.items()}
         return sum(s1 + n1 * s2 + s3[tuple(p.split('->')[3:])] for p in paths)
+
+    def liouvillians(self):
+        """The Liouvillian blocks the plan reads, {(subspace, heisenberg): L}
+        (device tensors [B|1, M, M]); writing into them in place changes the
+        generator of the next run or replay."""
+        out = {}
+        for (s0, s1), (stage, _) in self.t1_stages.items():
+            out[(s1, False)] = stage.L
+        for (s0, s1, s2), g in self.groups.items():
+            out[(s2, False)] = g['L']
+        for (s3, s4), st in self.t3_stages.items():
+            out[(s3, True)] = st['stage'].L
+        return out
+
+    def executed_products(self):
+        """Per member: propagator products of the trajectories (chunked
+        stepping and chunk-start chains) and GEMMs of the operator build and
+        the contractions, by kind."""
+        traj = [st for st, _ in self.t1_stages.values()] + \
+            [st['stage'] for st in self.t3_stages.values()]
+        return {'trajectory_products': {'M=%d' % t.M: t.products() for t in traj},
+                'trajectories': len(traj)}
 
     # ------------------------------------------------------------------ run
     def _launch(self):
@@ -233,19 +342,24 @@ class ThirdOrderPlan(object):
             side.wait_stream(main)
             with torch.cuda.stream(side):
                 M3 = st['bra'].shape[-1]
+                st['stage'].build()
                 G3 = st['stage'].run(st['bra'], statuses,
                                      self.status[1 + i:2 + i])[:, 0, :n3]      # [B, n3, M3]
                 _native.zgemm(self.F3, G3, self.G[:, :, st['col']:st['col'] + M3])
-        # t2 stage on the rows of V1^T and the V2 maps of every pathway, then
-        # the t1 trajectory transformed along t1, in order on the main stream
-        # (a fourth stream costs more in cross-stream waits than it overlaps)
-        for i, (k12, g) in enumerate(self.groups.items()):
-            _native.zgemm(g['V1T'], g['Pn'], self.Q[k12], transb=True)
-            _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=0.0 if i == 0 else 1.0)
+        # t2 stage on the rows of V1^T and the V2 maps of every pathway (its
+        # operator programs are the main stream's longest chain) on a stream of
+        # its own, while the t1 trajectory, transformed along t1, runs on the
+        # main stream
+        t2side = self._t2side
+        t2side.wait_stream(main)
+        with torch.cuda.stream(t2side):
+            self._t2_stage(B)
         stage1, y1 = next(iter(self.t1_stages.values()))
+        stage1.build()
         S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
         _native.zgemm(self.F1, S1, self.S1F)
         S1F = self.S1F if B == 1 else self.S1F.repeat_interleave(n2, dim=0)
+        main.wait_stream(t2side)
         _native.zgemm(S1F, self.W, self.U)                                       # [B n2, N1, w]
         for side in self._sides:
             main.wait_stream(side)
@@ -254,6 +368,23 @@ class ThirdOrderPlan(object):
         self._statuses = statuses
         N1, N3 = self.X.shape[1], self.X.shape[2]
         return self.X.view(B, n2, N1, N3).permute(0, 2, 1, 3)
+
+    def _t2_stage(self, B):
+        """W = sum over t2 groups of Q_g V2_g^T, Q_g the rows of V1^T
+        propagated to every population time."""
+        for i, (k12, g) in enumerate(self.groups.items()):
+            if g['rows'] is not None:
+                _native.propagate_powers_into(g['L'], g['h'], self.N2, g['rows'], self.Q[k12])
+            else:
+                _native.rk4_propagator_into(g['L'], g['h'], 1, g['P'], g['ws'])
+                Pn = g['Pn']
+                if g['P'].shape[0] == 1 and B > 1:
+                    _native.matrix_powers_into(g['P'], self.N2, Pn[:1], g['ws'])
+                    Pn[1:].copy_(Pn[:1].expand(B - 1, -1, -1, -1))
+                else:
+                    _native.matrix_powers_into(g['P'], self.N2, Pn, g['ws'])
+                _native.zgemm(g['V1T'], Pn.view(-1, *Pn.shape[2:]), self.Q[k12], transb=True)
+            _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=0.0 if i == 0 else 1.0)
 
     def capture(self):
         """Records the launch sequence into a HIP graph, replayed by run()."""

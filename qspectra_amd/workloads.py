@@ -68,16 +68,23 @@ def aggregate7_model(secular=False, disorder_sigma=None, kelvin=77.0):
     return RedfieldModel(ham, hilbert_subspace='gef', unit_convert=CM_FS, secular=secular)
 
 
-# the 2-D workload: t1 = t3 = 128 points at 2 fs, four population times
+# the 2-D workload: t1 = t3 = 128 points at 2 fs (the model's time step),
+# four population times.  The RK4 step is 1 fs: two RK4 steps per 2 fs grid
+# interval put the spectrum within 3e-7 of the reference's tight-tolerance
+# zvode result (RK4 at 2 fs: 4.4e-6; the reference at its default zvode
+# tolerance: 1.7e-6), inside the 1e-6 the metric is quoted at.
 AGG7_2D = dict(coherence_time_max=2 * 127.5, population_times=[0., 100., 200., 500.],
-               geometry='-++', include_signal='GSB,ESE,ESA', dt=2.0)
+               geometry='-++', include_signal='GSB,ESE,ESA', dt=1.0)
+AGG7_2D_GRID_STEP = 2.0
 
 
 def rk4_steps_2d(n1=128, t2=(0., 100., 200., 500.), n3=128, dt=2.0, pathways=3):
-    """Propagated state-steps of one 2-D spectrum: per pathway 1 RHS over
-    the t1 grid, n1 RHS over the t2 grid (from t0 = 0), 1 RHS over t3."""
+    """Propagated state-steps of one 2-D spectrum at step dt: per pathway 1
+    RHS over the t1 grid, n1 RHS over the t2 grid (from t0 = 0), 1 RHS over
+    t3 (with the 2 fs grid spacing and dt = 2: 96,762)."""
+    q = int(round(AGG7_2D_GRID_STEP / dt))
     s2 = int(round(max(t2) / dt))
-    return pathways * ((n1 - 1) + n1 * s2 + (n3 - 1))
+    return pathways * (q * (n1 - 1) + n1 * s2 + q * (n3 - 1))
 
 
 def ensemble_absorption_hamiltonians(n_disorder=4096, n_aggregates=1024, sigma=100.0):
