@@ -234,6 +234,20 @@ int32_t qsx_rk4_propagator(const void* L, int64_t strideL, int32_t M, int32_t B,
                            int32_t q, void* P, void* workspace, int64_t workspace_bytes,
                            void* stream);
 
+/* No counterpart in the reference: qsx_rk4_propagator for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * Same result as qsx_rk4_propagator (the same kernels in the same order); its
+ * workspace is a device array inside the library, QSX_PROPAGATOR_SCRATCH_BYTES
+ * (32 MiB) per GPU, held until the process ends.  Rejects problems whose
+ * qsx_propagator_workspace_bytes(M, B) exceeds it (e.g. M = 147 with B > 24).
+ * Calls no allocating routine.  All calls on one GPU share that array:
+ * calls that overlap in time on one GPU (concurrent streams, threads) race
+ * and must be serialised by the caller.  The stream must belong to the
+ * current device. */
+#define QSX_PROPAGATOR_SCRATCH_BYTES (32ll << 20)
+int32_t qsx_rk4_propagator_noalloc(const void* L, int64_t strideL, int32_t M, int32_t B,
+                                   double h, int32_t q, void* P, void* stream);
+
 /* out[b][s] = A[b]^exps[s] for s < ns (exps: HOST array, >= 0; exponent 0
  * gives the identity), one shared chain of squarings. */
 int32_t qsx_matrix_powers(const void* A, int64_t strideA, int32_t M, int32_t B,

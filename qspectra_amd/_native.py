@@ -81,6 +81,7 @@ SIGNATURES = {
                               c_vp, c_vp, c_vp],
     'qsx_propagator_workspace_bytes': [c_i32, c_i32],
     'qsx_rk4_propagator': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp, c_i64, c_vp],
+    'qsx_rk4_propagator_noalloc': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp],
     'qsx_matrix_powers': [c_vp, c_i64, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_i64, c_vp],
     'qsx_split_increment': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp],
     'qsx_small_propagator_fits': [c_i32, c_i32, c_i32],

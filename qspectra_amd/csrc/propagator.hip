@@ -16,9 +16,16 @@
 // for the large batched products); the element-wise steps are the kernels
 // below.  Nothing here allocates: the caller passes the workspace.
 
+#include <mutex>
 #include <vector>
 
 #include "qsx_common.h"
+
+// Workspace of qsx_rk4_propagator_noalloc: one copy per GPU, at its own
+// address, held until the process ends.  Host code never passes this name
+// to a kernel (on the host it is a stand-in, not a device address): the
+// per-device address comes from hipGetSymbolAddress (scratch_address).
+static __device__ __align__(256) unsigned char scratch[QSX_PROPAGATOR_SCRATCH_BYTES];
 
 namespace qsx {
 namespace {
@@ -452,19 +459,12 @@ extern "C" int64_t qsx_propagator_workspace_bytes(int32_t M, int32_t B) {
   return (int64_t)4 * B * M * M * (int64_t)sizeof(double2);
 }
 
-extern "C" int32_t qsx_rk4_propagator(const void* L, int64_t strideL, int32_t M, int32_t B,
-                                      double h, int32_t q, void* P, void* workspace,
-                                      int64_t workspace_bytes, void* stream) {
-  using namespace qsx;
-  QSX_CHECK_ARG(L && P && workspace, "qsx_rk4_propagator: null pointer");
-  QSX_CHECK_ARG(M >= 1 && B >= 1 && B <= 65535 && q >= 1,
-                "qsx_rk4_propagator: bad sizes M=%d B=%d q=%d", M, B, q);
-  QSX_CHECK_ARG(strideL == 0 || strideL >= (int64_t)M * M, "qsx_rk4_propagator: bad strideL");
-  QSX_CHECK_ARG(h == h && h > -1e300 && h < 1e300, "qsx_rk4_propagator: bad step %g", h);
-  QSX_CHECK_ARG(workspace_bytes >= qsx_propagator_workspace_bytes(M, B),
-                "qsx_rk4_propagator: workspace of %lld bytes, %lld needed",
-                (long long)workspace_bytes, (long long)qsx_propagator_workspace_bytes(M, B));
-  hipStream_t s = as_stream(stream);
+namespace qsx {
+namespace {
+
+// P = P4(h L)^q with the 4 B M^2 complex128 workspace ws (arguments checked)
+int rk4_propagator_impl(const void* L, int64_t strideL, int M, int B, double h, int q, void* P,
+                        void* workspace, hipStream_t s) {
   const int64_t mm = (int64_t)M * M;
   double2* ws = static_cast<double2*>(workspace);
   double2 *Ta = ws, *Tb = ws + mm * B, *Tc = ws + 2 * mm * B, *Td = ws + 3 * mm * B;
@@ -486,6 +486,31 @@ extern "C" int32_t qsx_rk4_propagator(const void* L, int64_t strideL, int32_t M,
   if (q == 1) return QSX_OK;
   const int64_t e = q;
   return powers_impl(Tb, mm, M, B, &e, 1, Pp, 0, mm, Ta, Tc, Td, s);
+}
+
+int check_rk4_args(const char* who, const void* L, int64_t strideL, int M, int B, double h,
+                   int q, const void* P) {
+  QSX_CHECK_ARG(L && P, "%s: null pointer", who);
+  QSX_CHECK_ARG(M >= 1 && B >= 1 && B <= 65535 && q >= 1, "%s: bad sizes M=%d B=%d q=%d", who,
+                M, B, q);
+  QSX_CHECK_ARG(strideL == 0 || strideL >= (int64_t)M * M, "%s: bad strideL", who);
+  QSX_CHECK_ARG(h == h && h > -1e300 && h < 1e300, "%s: bad step %g", who, h);
+  return QSX_OK;
+}
+
+}  // namespace
+}  // namespace qsx
+
+extern "C" int32_t qsx_rk4_propagator(const void* L, int64_t strideL, int32_t M, int32_t B,
+                                      double h, int32_t q, void* P, void* workspace,
+                                      int64_t workspace_bytes, void* stream) {
+  using namespace qsx;
+  QSX_TRY(check_rk4_args("qsx_rk4_propagator", L, strideL, M, B, h, q, P));
+  QSX_CHECK_ARG(workspace != nullptr, "qsx_rk4_propagator: null workspace");
+  QSX_CHECK_ARG(workspace_bytes >= qsx_propagator_workspace_bytes(M, B),
+                "qsx_rk4_propagator: workspace of %lld bytes, %lld needed",
+                (long long)workspace_bytes, (long long)qsx_propagator_workspace_bytes(M, B));
+  return rk4_propagator_impl(L, strideL, M, B, h, q, P, workspace, as_stream(stream));
 }
 
 extern "C" int32_t qsx_matrix_powers(const void* A, int64_t strideA, int32_t M, int32_t B,
@@ -519,4 +544,52 @@ extern "C" int32_t qsx_split_increment(const void* P, int32_t M, int32_t B, void
                      static_cast<float2*>(off), static_cast<double2*>(diag));
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
+}
+
+// ----------------------------------------------------------------------
+// Allocation-free qsx_rk4_propagator for callers that cannot allocate
+// (code under graph capture, serving loops): the workspace is `scratch`.
+namespace qsx {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device
+// (with that device current) and kept.
+int scratch_address(void** out) {
+  int dev = -1;
+  QSX_CHECK_HIP(hipGetDevice(&dev));
+  QSX_CHECK_ARG(dev >= 0 && dev < kMaxDevices,
+                "qsx_rk4_propagator_noalloc: device ordinal %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_address[dev] == nullptr) {
+    void* p = nullptr;
+    QSX_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QSX_CHECK_ARG(p != nullptr, "qsx_rk4_propagator_noalloc: no device address for the scratch "
+                  "array on device %d", dev);
+    g_scratch_address[dev] = p;
+  }
+  *out = g_scratch_address[dev];
+  return QSX_OK;
+}
+
+}  // namespace
+}  // namespace qsx
+
+extern "C" int32_t qsx_rk4_propagator_noalloc(const void* L, int64_t strideL, int32_t M,
+                                              int32_t B, double h, int32_t q, void* P,
+                                              void* stream) {
+  using namespace qsx;
+  QSX_TRY(check_rk4_args("qsx_rk4_propagator_noalloc", L, strideL, M, B, h, q, P));
+  const int64_t need = qsx_propagator_workspace_bytes(M, B);
+  QSX_CHECK_ARG(need <= (int64_t)QSX_PROPAGATOR_SCRATCH_BYTES,
+                "qsx_rk4_propagator_noalloc: M=%d, B=%d needs %lld bytes of workspace; the "
+                "scratch array holds %lld (use qsx_rk4_propagator with a workspace)",
+                M, B, (long long)need, (long long)QSX_PROPAGATOR_SCRATCH_BYTES);
+  void* ws = nullptr;
+  QSX_TRY(scratch_address(&ws));
+  // every workspace element is written before it is read: nothing to clear
+  return rk4_propagator_impl(L, strideL, M, B, h, q, P, ws, as_stream(stream));
 }
