@@ -50,7 +50,7 @@ def parse_args(argv=None):
                     help='seconds of CPU work allowed for the baseline samples (all cores, '
                          'then one thread)')
     ap.add_argument('--workloads', default='all',
-                    help="'all', 'none' or a comma list of ensemble_absorption, "
+                    help="'all', 'none' or a comma list of batched_2d, ensemble_absorption, "
                          "adjoint_gradient, heom, pump, example_dimer_2d, zofe_fmo, "
                          "jonas_vibronic: the further workloads to measure")
     ap.add_argument('--counters', action='store_true',
@@ -145,6 +145,10 @@ def time_leg(torch, fn, steps):
 # launch labels that are one kernel each (the others -- 'rk4 propagator',
 # 'matrix powers' of blocks beyond one work-group -- enqueue several GEMMs)
 SINGLE_KERNEL = ('propagate ', 'chunks ', 'small propagator', 'zgemm ')
+# launches whose work is one kind of kernel repeated (the split propagator:
+# three Horner GEMMs and the squarings on the FP32 matrix cores, beside
+# element-wise kernels a hundred times shorter)
+MFMA_LAUNCHES = ('split propagator',)
 # kernel behind each single-kernel label, as rocprofv3 names it
 KERNEL_NAMES = {'chunks': 'chunk_kernel', 'propagate powers': 'small_powers_apply_kernel',
                 'small propagator': 'small_program_kernel', 'zgemm': 'zgemm_kernel',
@@ -157,7 +161,7 @@ def dominant_kernel(summary, relaunch, timing, per):
     to back from a captured graph (its own time, without host gaps)."""
     best = None
     for label, info in summary.items():
-        if not label.startswith(SINGLE_KERNEL) or label not in relaunch:
+        if not label.startswith(SINGLE_KERNEL + MFMA_LAUNCHES) or label not in relaunch:
             continue
         ms = timing.back_to_back_ms(relaunch[label])
         total = ms * info['count'] / per
@@ -189,6 +193,9 @@ def launch_flops(label):
     if label.startswith('zgemm'):
         return 8.0 * int(f['m']) * int(f['n']) * int(f['k']) * int(f['batch'])
     M, B = int(f['M']), int(f['B'])
+    if label.startswith('split propagator'):
+        # three Horner GEMMs and one squaring per doubling of q
+        return 8.0 * M ** 3 * B * (3 + int(f['q']).bit_length() - 1)
     if label.startswith('small propagator'):
         q, c = int(f['q']), int(f['c'])
         n = 3 + _gemms_power(q) + (_gemms_power(c) if c > 0 else 0)
@@ -221,6 +228,8 @@ def kernel_bound(label):
         return 'valu', PEAK_TFLOPS['fp64' if fp64 else 'mixed']
     if label.startswith(('small propagator', 'propagate powers')):
         return 'valu', PEAK_TFLOPS['fp64']
+    if label.startswith('split propagator'):
+        return 'mfma', PEAK_TFLOPS['mixed']
     return 'valu', PEAK_TFLOPS['fp64' if ' fp64 ' in label else 'mixed']
 
 
@@ -359,6 +368,61 @@ def _sync_time(torch, fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def batched_2d_leg(torch, members=256, replays=10):
+    """The batched 2-D path: the ensemble-mean 2-D spectrum of `members`
+    static-disorder members (sigma = 100 cm^-1) of the 7-site aggregate on
+    the headline's grid, one captured graph per ensemble spectrum, end to end
+    from the members' Hamiltonians resident in HBM: the Liouvillian blocks
+    (eigensystems, Redfield tensors, basis changes), the RK4 propagators
+    (MIXED: the 147-element block's split on the FP32 matrix cores), the
+    trajectories, the contraction over members and both Fourier transforms.
+    FP64 and MIXED; error of the MIXED spectrum against the FP64 one."""
+    from qspectra_amd import workloads as W, timing
+    from qspectra_amd.simulate.fused import ThirdOrderPlan
+    work = W.AGG7_2D
+    m = W.aggregate7_model(secular=False, disorder_sigma=100.0)
+    mb = m.with_hamiltonians(list(m.hamiltonian.sample_ensemble(members)))
+    steps = members * W.rk4_steps_2d()
+    out = {'workload': '2-D spectrum (GSB+ESE+ESA) ensemble mean over %d static-disorder '
+                       'members (sigma 100 cm^-1) of the 7-site aggregate, 128 x 4 x 128 grid, '
+                       'RK4 at 1 fs; end to end from the Hamiltonians in one graph' % members,
+           'members': members, 'steps_per_ensemble_spectrum': steps}
+    res, plans = {}, {}
+    for prec in ('mixed', 'fp64'):
+        plan = ThirdOrderPlan(mb, work['coherence_time_max'],
+                              population_times=work['population_times'],
+                              geometry=work['geometry'], include_signal=work['include_signal'],
+                              dt=work['dt'], precision=prec, fourier=True, ensemble_mean=True,
+                              rebuild=True).capture()
+        for _ in range(2):
+            plan.run()
+        t, X = time_leg(torch, plan.run, replays)
+        plan.check_status()
+        res[prec] = (t / replays, X.cpu().numpy())
+        plans[prec] = plan
+    X32, X64 = res['mixed'][1], res['fp64'][1]
+    out['rel_err_mixed_vs_fp64'] = float(np.abs(X32 - X64).max() / np.abs(X64).max())
+    for prec in ('mixed', 'fp64'):
+        out[prec] = {'ms_per_ensemble_spectrum': 1e3 * res[prec][0],
+                     'value': steps / res[prec][0], 'unit': 'steps/s'}
+    with timing.kernel_timer() as kt:
+        for _ in range(3):
+            plans['mixed']._launch()
+    summary = kt.summary()
+    label, info, launch_ms, per = dominant_kernel(summary, kt.relaunch, timing, 3)
+    flops = launch_flops(label)
+    bound, peak = kernel_bound(label)
+    achieved = flops / (launch_ms * 1e-3) / 1e12
+    out['roofline'] = {'bound': bound, 'kernel': label, 'achieved': achieved, 'peak': peak,
+                       'unit': 'TFLOP/s', 'frac': achieved / peak, 'launch_ms': launch_ms,
+                       'kernel_ms_per_spectrum': per, 'algorithmic_flop': flops,
+                       'traffic': launch_traffic(label)}
+    out['kernels'] = {k: {'launches': v['count'] / 3, 'mean_ms': round(v['mean_ms'], 5)}
+                      for k, v in summary.items()}
+    out['device_ms_sum_per_spectrum'] = sum(v['total_ms'] for v in summary.values()) / 3
+    return out
 
 
 def ensemble_absorption_leg(torch):
@@ -607,7 +671,8 @@ def jonas_leg(torch):
     return out
 
 
-LEGS = (('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', gradient_leg),
+LEGS = (('batched_2d', batched_2d_leg),
+        ('ensemble_absorption', ensemble_absorption_leg), ('adjoint_gradient', gradient_leg),
         ('heom', heom_leg), ('pump', pump_leg), ('example_dimer_2d', example_dimer_2d_leg),
         ('zofe_fmo', zofe_fmo_leg), ('jonas_vibronic', jonas_leg))
 

@@ -260,6 +260,18 @@ int32_t qsx_matrix_powers(const void* A, int64_t strideA, int32_t M, int32_t B,
 int32_t qsx_split_increment(const void* P, int32_t M, int32_t B, void* off, void* diag,
                             void* stream);
 
+/* MIXED operands (off complex64 [B][M][M], diag complex128 [B][M], as
+ * qsx_split_increment) of P4(h L)^q built on the FP32 matrix cores without an
+ * FP64 P: with L = D + O, Horner's rule runs on the differences from the
+ * exact FP64 diagonal recursion (three complex FP32-MFMA GEMMs whose
+ * epilogues add the column-scaled O), then q (a power of two) RK4 steps per
+ * application by squaring the split.  Workspace:
+ * qsx_split_propagator_workspace_bytes(M, B). */
+int64_t qsx_split_propagator_workspace_bytes(int32_t M, int32_t B);
+int32_t qsx_rk4_propagator_split(const void* L, int64_t strideL, int32_t M, int32_t B,
+                                 double h, int32_t q, void* off, void* diag, void* workspace,
+                                 int64_t workspace_bytes, void* stream);
+
 /* Small blocks (M <= 64), one work-group per batch entry with the matrices in
  * LDS, so the whole operator program is one launch:
  *   qsx_small_propagator: P = P4(h L)^q into P [B][M][M] (may be NULL), its

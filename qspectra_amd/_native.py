@@ -41,7 +41,8 @@ c_i32, c_i64, c_dbl, c_vp, c_sz = (ctypes.c_int32, ctypes.c_int64, ctypes.c_doub
 RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c_i64,
             'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64,
             'qsx_redfield_dense_workspace_bytes': c_i64, 'qsx_fourier_workspace_bytes': c_i64,
-            'qsx_propagator_workspace_bytes': c_i64}
+            'qsx_propagator_workspace_bytes': c_i64,
+            'qsx_split_propagator_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -84,6 +85,9 @@ SIGNATURES = {
     'qsx_rk4_propagator_noalloc': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp],
     'qsx_matrix_powers': [c_vp, c_i64, c_i32, c_i32, c_vp, c_i32, c_vp, c_vp, c_i64, c_vp],
     'qsx_split_increment': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp],
+    'qsx_split_propagator_workspace_bytes': [c_i32, c_i32],
+    'qsx_rk4_propagator_split': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp, c_vp,
+                                 c_i64, c_vp],
     'qsx_small_propagator_fits': [c_i32, c_i32, c_i32],
     'qsx_small_propagator': [c_vp, c_i64, c_i32, c_i32, c_dbl, c_i32, c_i64, c_vp, c_vp, c_vp,
                              c_vp, c_vp],
@@ -247,18 +251,33 @@ def block_eigh(H1, X, rw=None, stream=None):
     return E, U
 
 
-def redfield_block(H1, rw, T, lam, gam, X, Y, flags, unit_convert, stream=None):
-    """Redfield Liouvillian block XY, complex128 [B, M, M]."""
+def redfield_block(H1, rw, T, lam, gam, X, Y, flags, unit_convert, stream=None, L=None,
+                   ws=None):
+    """Redfield Liouvillian block XY, complex128 [B, M, M] (written into L
+    and built with the workspace ws when given: redfield_workspace)."""
     lib = require_native()
     B, n, _ = H1.shape
     assert H1.dtype == torch.float64 and H1.is_contiguous()
     M = block_size(X, n) * block_size(Y, n)
-    L = torch.empty((B, M, M), dtype=torch.complex128, device=H1.device)
-    ws = _workspace(n, X, Y, B, H1.device)
-    check(lib.qsx_redfield_block(ptr(H1), n, B, ptr(rw), float(T), float(lam), float(gam),
-                                 int(flags), float(unit_convert), X, Y, ptr(L), ptr(ws),
-                                 stream_ptr(stream)), 'qsx_redfield_block')
+    if L is None:
+        L = torch.empty((B, M, M), dtype=torch.complex128, device=H1.device)
+    assert L.shape == (B, M, M) and L.dtype == torch.complex128 and L.is_contiguous()
+    if ws is None:
+        ws = _workspace(n, X, Y, B, H1.device)
+
+    def go():
+        return lib.qsx_redfield_block(ptr(H1), n, B, ptr(rw), float(T), float(lam), float(gam),
+                                      int(flags), float(unit_convert), X, Y, ptr(L), ptr(ws),
+                                      stream_ptr(stream))
+    with timing.launch('redfield block %s%s n=%d B=%d' % ('gef'[X], 'gef'[Y], n, B), stream, go):
+        st = go()
+    check(st, 'qsx_redfield_block')
     return L
+
+
+def redfield_workspace(n, X, Y, B, dev=None):
+    """Device workspace of qsx_redfield_block for block XY of n sites."""
+    return _workspace(n, X, Y, B, device() if dev is None else dev)
 
 
 def unitary_block(H1, rw, unit_convert, X, Y, heisenberg=False, stream=None):
@@ -582,6 +601,35 @@ def split_increment_into(P, off, diag, stream=None):
     with timing.launch('split increment M=%d B=%d' % (M, B), stream, go):
         st = go()
     check(st, 'qsx_split_increment')
+    return off, diag
+
+
+def split_propagator_workspace(M, B, dev=None):
+    """Device workspace of qsx_rk4_propagator_split."""
+    n = load_library().qsx_split_propagator_workspace_bytes(M, B)
+    if n < 0:
+        raise NativeError('qsx_split_propagator_workspace_bytes(%d, %d) failed' % (M, B))
+    dev = device() if dev is None else dev
+    return torch.empty(-(-n // 16), dtype=torch.complex128, device=dev)
+
+
+def rk4_propagator_split_into(L, h, q, off, diag, workspace, stream=None):
+    """MIXED operands of P4(h L)^q built on the FP32 matrix cores: off
+    [B, M, M] complex64, diag [B, M] complex128 (q a power of two)."""
+    lib = require_native()
+    B, M, _ = off.shape
+    assert L.dtype == torch.complex128 and L.is_contiguous() and L.shape[-2:] == (M, M)
+    assert L.shape[0] in (1, B) and off.dtype == torch.complex64 and off.is_contiguous()
+    assert diag.dtype == torch.complex128 and diag.shape == (B, M) and diag.is_contiguous()
+    strideL = 0 if L.shape[0] == 1 else M * M
+
+    def go():
+        return lib.qsx_rk4_propagator_split(ptr(L), strideL, M, B, float(h), int(q), ptr(off),
+                                            ptr(diag), ptr(workspace), workspace.numel() * 16,
+                                            stream_ptr(stream))
+    with timing.launch('split propagator M=%d B=%d q=%d' % (M, B, q), stream, go):
+        st = go()
+    check(st, 'qsx_rk4_propagator_split')
     return off, diag
 
 

@@ -382,6 +382,29 @@ class LiouvilleSpaceModel(DynamicalModel):
             L[:, s[:, None], s[None, :]] = blk[:, l[:, None], l[None, :]]
         return L
 
+    def block_builder(self, liouville_subspace, heisenberg_picture=False):
+        """(inputs, rebuild) for a single-block Liouville subspace of an
+        electronic Hamiltonian: inputs = (H1 [B, n, n], rw [B]) device
+        tensors, rebuild(L, stream=None) re-runs the builder from them into L
+        [B, M, M] (the block liouvillian() returns).  A caller that writes
+        new Hamiltonians into H1 in place and rebuilds gets their blocks;
+        the pair of tensors is shared by every block of the model."""
+        n = self.hamiltonian.n_sites
+        layout, idx = block_layout(liouville_subspace, self.hilbert_subspace, n,
+                                   self.hamiltonian.n_vibrational_states)
+        if hasattr(self.hamiltonian, 'dense_block') or len(layout) != 1 or \
+                not np.array_equal(layout[0][3], np.arange(idx.size)):
+            raise NotImplementedError('rebuilds need a single Liouville block of an '
+                                      'electronic Hamiltonian')
+        inputs = self.__dict__.get('_builder_inputs')
+        if inputs is None:
+            inputs = self.__dict__['_builder_inputs'] = self._device_inputs()
+        X, Y = ('gef'.index(layout[0][0]), 'gef'.index(layout[0][1]))
+        return inputs, self._block_rebuilder(X, Y, heisenberg_picture, inputs)
+
+    def _block_rebuilder(self, X, Y, heisenberg, inputs):
+        raise NotImplementedError('%s has no block rebuilder' % type(self).__name__)
+
     @imemoize
     def equation_of_motion(self, liouville_subspace, heisenberg_picture=False):
         return DeviceEOM(self.liouvillian(liouville_subspace, heisenberg_picture),

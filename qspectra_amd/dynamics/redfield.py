@@ -35,6 +35,17 @@ class RedfieldModel(LiouvilleSpaceModel):
         return _native.redfield_block(H1, rw, T, lam, gam, X, Y, self._flags(heisenberg),
                                       self.unit_convert)
 
+    def _block_rebuilder(self, X, Y, heisenberg, inputs):
+        T, lam, gam = debye_parameters(self.hamiltonian.bath)
+        H1, rw = inputs
+        ws = _native.redfield_workspace(H1.shape[-1], X, Y, H1.shape[0], H1.device)
+        flags = self._flags(heisenberg)
+
+        def rebuild(L, stream=None):
+            _native.redfield_block(H1, rw, T, lam, gam, X, Y, flags, self.unit_convert,
+                                   stream=stream, L=L, ws=ws)
+        return rebuild
+
     def _build_dense_block(self, X, Y, heisenberg, dense):
         T, lam, gam = debye_parameters(self.hamiltonian.bath)
         (HX, oX), (HY, oY) = dense[X], dense[Y]

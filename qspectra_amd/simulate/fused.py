@@ -111,6 +111,12 @@ class _Trajectory(object):
             self.off = self.diag = None
         self.chunk_n = torch.tensor([0] + [1] * (self.c - 1), dtype=torch.int32, device=dev)
         self.chunk_h = torch.full((self.c,), self.h * self.q, dtype=torch.float64, device=dev)
+        # one chunk (no start powers): a large block in MIXED precision gets
+        # its split straight from the FP32 matrix cores, without an FP64 P
+        self.split_mfma = (self.nc == 1 and self.off is not None and not _native.small_fits(M)
+                           and self.q & (self.q - 1) == 0)
+        if self.split_mfma:
+            self.ws = _native.split_propagator_workspace(M, Bl, dev)
 
     def products(self):
         """Propagator products the chunked stepping executes per member:
@@ -119,6 +125,10 @@ class _Trajectory(object):
         return self.nc * (self.c - 1) + (self.nc - 1)
 
     def build(self):
+        if self.split_mfma:
+            _native.rk4_propagator_split_into(self.L, self.h, self.q, self.off, self.diag,
+                                              self.ws)
+            return
         if _native.small_fits(self.M):
             # one launch: Horner, the q-th and (q c)-th powers, the split
             _native.small_propagator_into(self.L, self.h, self.q,
@@ -134,6 +144,13 @@ class _Trajectory(object):
     def run(self, y0, statuses, status):
         B, _, M = y0.shape
         A = self.P if self.off is None else self.off
+        if self.nc == 1:
+            # one chunk: the register stepper, one product per save interval
+            out, _, st = _native.propagate_states(_native.QSX_MODE_PROPAGATOR, A, y0,
+                                                  self.chunk_n, self.chunk_h, self.precision,
+                                                  self.diag, steps=self.c - 1, status=status)
+            statuses.append(st)
+            return out
         out, st = _native.propagate_chunks(self.Pc[:, 0] if self.nc > 1 else self.P, A, y0,
                                            self.nc, self.chunk_n, self.chunk_h, self.precision,
                                            self.diag, steps=self.c - 1, status=status)
@@ -146,12 +163,26 @@ class ThirdOrderPlan(object):
     spectrum (fourier=True, run() -> [B, N1, n2, N3], frequencies()) of a
     model; run() launches on the current stream, capture() records a graph.
     schedule 'rk4' (no propagator) is served by the pathway-by-pathway
-    evaluation and not by a plan."""
+    evaluation and not by a plan.
+
+    ensemble_mean: the mean over the B members (the ensemble average of
+    simulate/decorators.py:40-64) -> [1, ...]: contracted in the time domain,
+        R_s = (1/B) sum_b S1_b (W_bs G3_b^T) = (1/B) [S1_1 .. S1_B] [Y_1s; ..; Y_Bs],
+    one GEMM over all members per population time, then the two Fourier
+    products once.
+    rebuild: every launch first re-runs the Liouvillian builder (eigensystem,
+    Redfield tensor, basis change: qsx_redfield_block) from the members'
+    Hamiltonians (model.block_builder), so the whole path from H1 runs in
+    the launch sequence.
+    With chunk=None and at least 32 members the trajectories are stepped in
+    one chunk per member (each member's chain fills a compute unit; no
+    chunk-start powers), and MIXED blocks too large for one work-group take
+    their split from the FP32 matrix cores (qsx_rk4_propagator_split)."""
 
     def __init__(self, model, coherence_time_max, population_time_max=None,
                  population_times=None, geometry='-++', polarization='xxxx',
                  include_signal=None, dt=None, precision='fp64', schedule='auto',
-                 fourier=False, chunk=None):
+                 fourier=False, chunk=None, ensemble_mean=False, rebuild=False):
         if schedule == 'rk4':
             raise PlanNotApplicable("the fused plan needs the RK4 propagator; use "
                                     "plan='pathways' for schedule='rk4'")
@@ -164,6 +195,9 @@ class ThirdOrderPlan(object):
         self.B = B = len(_members(model))
         self.precision = precision
         self.fourier = fourier
+        self.ensemble_mean = ensemble_mean
+        if chunk is None and B >= 32:
+            chunk = max(len(self.t1), len(self.t3))          # one chunk per trajectory
         dt = dt if dt is not None else model.time_step
         rho0 = model.thermal_state('gg')
         trans = geometry + '-'
@@ -267,6 +301,19 @@ class ThirdOrderPlan(object):
         if len(self.t1_stages) != 1:
             raise PlanNotApplicable('pathways of one plan must share the t1 propagation')
 
+        self._rebuilds = []
+        if rebuild:
+            seen = set()
+            blocks = [(k1[1], False, st.L) for k1, (st, _) in self.t1_stages.items()]
+            blocks += [(k12[2], False, g['L']) for k12, g in self.groups.items()]
+            blocks += [(k3[0], True, st['stage'].L) for k3, st in self.t3_stages.items()]
+            for sub, heis, L in blocks:
+                if (sub, heis) in seen:
+                    continue
+                seen.add((sub, heis))
+                inputs, fn = model.block_builder(sub, heis)
+                self._rebuilds.append((fn, L))
+            self.hamiltonian_inputs = inputs
         rw, uc = model.rw_freq, model.unit_convert
         if fourier:
             self.f1, self.F1 = transform_matrix(self.t1, rw, uc, -1, dev)
@@ -278,13 +325,24 @@ class ThirdOrderPlan(object):
         self.F1, self.F3 = self.F1[None].contiguous(), self.F3[None].contiguous()
         N1, N3 = self.F1.shape[1], self.F3.shape[1]
         # device buffers reused by every run (fixed addresses for graph replay)
-        self.G = torch.empty((B, N3, width), dtype=torch.complex128, device=dev)
         self.W = torch.empty((B * n2, self.M1, width), dtype=torch.complex128, device=dev)
         self.Q = {k: torch.empty((B * n2, self.M1, g['V1T'].shape[2]), dtype=torch.complex128,
                                  device=dev) for k, g in self.groups.items()}
-        self.U = torch.empty((B * n2, N1, width), dtype=torch.complex128, device=dev)
-        self.S1F = torch.empty((B, N1, self.M1), dtype=torch.complex128, device=dev)
-        self.X = torch.empty((B * n2, N1, N3), dtype=torch.complex128, device=dev)
+        if not ensemble_mean:
+            # per-member spectra: t3 factors, t1 factors, products, result
+            self.G = torch.empty((B, N3, width), dtype=torch.complex128, device=dev)
+            self.U = torch.empty((B * n2, N1, width), dtype=torch.complex128, device=dev)
+            self.S1F = torch.empty((B, N1, self.M1), dtype=torch.complex128, device=dev)
+            self.X = torch.empty((B * n2, N1, N3), dtype=torch.complex128, device=dev)
+        else:
+            # time-domain contraction buffers: t3 trajectories, per-member
+            # products Y_s, the stacked t1 trajectories, the mean response
+            self.G3 = torch.empty((B, n3, width), dtype=torch.complex128, device=dev)
+            self.Y = torch.empty((n2, B * self.M1, n3), dtype=torch.complex128, device=dev)
+            self.S1cat = torch.empty((n1, B, self.M1), dtype=torch.complex128, device=dev)
+            self.R = torch.empty((n2, n1, n3), dtype=torch.complex128, device=dev)
+            self.RF = torch.empty((n2, N1, n3), dtype=torch.complex128, device=dev)
+            self.XE = torch.empty((n2, N1, N3), dtype=torch.complex128, device=dev)
         # status slots of the stepper launches (t1, then each t3), cleared per run
         self.status = torch.zeros(1 + len(self.t3_stages), dtype=torch.int32, device=dev)
         self.steps = self._count_steps(paths)
@@ -334,6 +392,10 @@ class ThirdOrderPlan(object):
         main = torch.cuda.current_stream()
         self.status.zero_()
         statuses = []
+        for fn, L in self._rebuilds:
+            fn(L)
+        if self.ensemble_mean:
+            return self._launch_ensemble(main, statuses)
         # each t3 Heisenberg trajectory on its own stream, transformed along t3
         # and written to its column range of G; the largest block (the
         # longest chain of the plan) is launched first
@@ -368,6 +430,45 @@ class ThirdOrderPlan(object):
         self._statuses = statuses
         N1, N3 = self.X.shape[1], self.X.shape[2]
         return self.X.view(B, n2, N1, N3).permute(0, 2, 1, 3)
+
+    def _launch_ensemble(self, main, statuses):
+        """The ensemble-mean response: trajectories per member, then one
+        contraction over all members per population time, then (fourier)
+        the transforms of the mean."""
+        B, n2 = self.B, len(self.t2)
+        n1, n3, M1 = len(self.t1), len(self.t3), self.M1
+        for i, side in zip(self._t3_order, self._sides):
+            st = self._t3_list[i]
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                M3 = st['bra'].shape[-1]
+                st['stage'].build()
+                G3 = st['stage'].run(st['bra'], statuses,
+                                     self.status[1 + i:2 + i])[:, 0, :n3]      # [B, n3, M3]
+                self.G3[:, :, st['col']:st['col'] + M3].copy_(G3)
+        t2side = self._t2side
+        t2side.wait_stream(main)
+        with torch.cuda.stream(t2side):
+            self._t2_stage(B)
+        stage1, y1 = next(iter(self.t1_stages.values()))
+        stage1.build()
+        S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
+        self.S1cat.copy_(S1.permute(1, 0, 2))
+        main.wait_stream(t2side)
+        for side in self._sides:
+            main.wait_stream(side)
+        W = self.W.view(B, n2, M1, -1)
+        for s in range(n2):
+            # Y_s[b] = W_bs G3_b^T  [M1, n3]
+            _native.zgemm(W[:, s], self.G3, self.Y[s].view(B, M1, n3), transb=True)
+        # R_s = (1/B) S1cat Y_s, one GEMM over all members per population time
+        _native.zgemm(self.S1cat.view(1, n1, B * M1), self.Y, self.R, alpha=1.0 / B)
+        self._statuses = statuses
+        if not self.fourier:
+            return self.R.permute(1, 0, 2)[None]
+        _native.zgemm(self.F1, self.R, self.RF)                                 # [n2, N1, n3]
+        _native.zgemm(self.RF, self.F3, self.XE, transb=True)                   # [n2, N1, N3]
+        return self.XE.permute(1, 0, 2)[None]
 
     def _t2_stage(self, B):
         """W = sum over t2 groups of Q_g V2_g^T, Q_g the rows of V1^T
