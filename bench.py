@@ -320,6 +320,12 @@ def _reference_default_err():
         return None
 
 
+# HEOM bytes per state element and RK4 step.  Algorithmic: four stages each
+# reading and writing the stage operand (16 B FP64, 8 B MIXED) plus one read
+# and one write of the FP64 state: 160 / 96 B.  The stage kernel as written
+# moves more: every stage reads the state and the RK accumulator and writes
+# the accumulator and the next stage operand (72 / 56 B per element-stage).
+HEOM_ALGORITHMIC_BYTES = {'fp64': 160, 'mixed': 96}
 HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
 
 # HBM bytes per launch (read + written) measured with rocprofv3 --pmc
@@ -531,13 +537,18 @@ def heom_leg(torch):
            'hierarchy_steps': 256 * steps, 'rel_err_mixed_vs_fp64': err}
     for prec in ('fp64', 'mixed'):
         sec = res[prec][0]
-        gbs = 256 * S * HEOM_BYTES_PER_ELEMENT_STEP[prec] * steps / sec / 1e9
+        algo = HEOM_ALGORITHMIC_BYTES[prec]
+        gbs = 256 * S * algo * steps / sec / 1e9
+        per_launch = 256 * S * algo / 4
+        traffic = PMC_TRAFFIC['heom stage ' + prec]
         out[prec] = {'seconds': sec, 'hierarchy_steps_per_s': 256 * steps / sec,
                      'achieved_GBps': gbs, 'hbm_frac': gbs / 8000.0,
-                     'bytes_per_element_step': HEOM_BYTES_PER_ELEMENT_STEP[prec],
-                     'traffic_bytes_per_stage_launch': PMC_TRAFFIC['heom stage ' + prec],
-                     'algorithmic_bytes_per_stage_launch':
-                         256 * S * HEOM_BYTES_PER_ELEMENT_STEP[prec] / 4}
+                     'algorithmic_bytes_per_element_step': algo,
+                     'kernel_bytes_per_element_step': HEOM_BYTES_PER_ELEMENT_STEP[prec],
+                     'kernel_GBps': gbs * HEOM_BYTES_PER_ELEMENT_STEP[prec] / algo,
+                     'traffic_bytes_per_stage_launch': traffic,
+                     'algorithmic_bytes_per_stage_launch': per_launch,
+                     'traffic_over_algorithmic': traffic / per_launch}
     return out
 
 

@@ -246,7 +246,7 @@ struct SmallGemm {
 
 __device__ inline void lds_copy(const double2* src, double2* dst, int M) {
   const int ld = M + 1;
-  for (int e = threadIdx.x; e < M * M; e += kSmallThreads)
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x)
     dst[(e / M) * ld + e % M] = src[(e / M) * ld + e % M];
 }
 
@@ -254,7 +254,7 @@ __device__ inline void lds_copy(const double2* src, double2* dst, int M) {
 __device__ void small_horner(const double2* __restrict__ Lb, double h, int M, double2* X,
                              double2* Y) {
   const int ld = M + 1;
-  for (int e = threadIdx.x; e < M * M; e += kSmallThreads) {
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
     const int i = e / M, j = e % M;
     const double2 v = Lb[e];
     X[i * ld + j] = v;
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_program_kernel(
   small_horner(L + b * strideL, h, M, X, Y);
   double2* Pq = q == 1 ? Y : small_power(Y, X, q, M);
   double2* other = Pq == Y ? X : Y;
-  for (int e = threadIdx.x; e < M * M; e += kSmallThreads) {
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
     const int i = e / M, j = e % M;
     const double2 v = Pq[i * ld + j];
     if (P != nullptr) P[b * mm + e] = v;
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_program_kernel(
   if (Pc == nullptr || c < 1) return;
   __syncthreads();
   double2* Rc = small_power(Pq, other, c, M);
-  for (int e = threadIdx.x; e < M * M; e += kSmallThreads)
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x)
     Pc[b * mm + e] = Rc[(e / M) * ld + e % M];
 }
 
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_powers_apply_kernel(
   double2* W = Z + ns * RM;                      // [R][M] products before copy-back
   long long emax = 0;
   for (int s = 0; s < ns; ++s) emax = ex.e[s] > emax ? ex.e[s] : emax;
-  for (int e = threadIdx.x; e < ns * RM; e += kSmallThreads)
+  for (int e = threadIdx.x; e < ns * RM; e += blockDim.x)
     Z[e] = y0[(int64_t)b * RM + e % RM];
   small_horner(L + b * strideL, h, M, X, Y);    // Y = P; X = L is free from here
   SmallGemm g(M);
@@ -356,14 +356,14 @@ __global__ __launch_bounds__(kSmallThreads) void small_powers_apply_kernel(
     for (int s = 0; s < ns; ++s) {
       if (!(ex.e[s] & (1LL << k))) continue;
       double2* Zs = Z + s * RM;
-      for (int e = threadIdx.x; e < RM; e += kSmallThreads) {
+      for (int e = threadIdx.x; e < RM; e += blockDim.x) {
         const int r = e / M, i = e % M;
         double2 acc = make_double2(0.0, 0.0);
         for (int j = 0; j < M; ++j) cfma<double>(acc, Y[i * ld + j], Zs[r * M + j]);
         W[e] = acc;
       }
       __syncthreads();
-      for (int e = threadIdx.x; e < RM; e += kSmallThreads) Zs[e] = W[e];
+      for (int e = threadIdx.x; e < RM; e += blockDim.x) Zs[e] = W[e];
       __syncthreads();
     }
     if ((2LL << k) <= emax) {
@@ -373,11 +373,19 @@ __global__ __launch_bounds__(kSmallThreads) void small_powers_apply_kernel(
       __syncthreads();
     }
   }
-  for (int e = threadIdx.x; e < ns * RM; e += kSmallThreads)
+  for (int e = threadIdx.x; e < ns * RM; e += blockDim.x)
     out[(int64_t)b * ns * RM + e] = Z[e];
 }
 
 size_t small_program_lds(int M) { return (size_t)2 * M * (M + 1) * sizeof(double2); }
+
+// threads of a small-block work-group: the T x T product grid (T =
+// ceil(M / 2)) rounded up to whole wavefronts, at most kSmallThreads
+int small_threads(int M) {
+  const int T = (M + RB - 1) / RB;
+  const int t = ((T * T + 63) / 64) * 64;
+  return t < kSmallThreads ? t : kSmallThreads;
+}
 size_t small_apply_lds(int M, int R, int ns) {
   return small_program_lds(M) + (size_t)(ns + 1) * R * M * sizeof(double2);
 }
@@ -416,8 +424,8 @@ extern "C" int32_t qsx_small_propagator(const void* L, int64_t strideL, int32_t 
   const int64_t sL = B > 1 ? strideL : 0;
 auto kern = small_program_kernel;
   QSX_TRY(set_lds(kern, lds));
-  hipLaunchKernelGGL(kern, dim3(B), dim3(kSmallThreads), lds, s, static_cast<const double2*>(L),
-                     sL, M, h, (long long)q, (long long)c, static_cast<double2*>(P),
+  hipLaunchKernelGGL(kern, dim3(B), dim3(small_threads(M)), lds, s,
+                     static_cast<const double2*>(L), sL, M, h, (long long)q, (long long)c, static_cast<double2*>(P),
                      static_cast<double2*>(Pc), static_cast<float2*>(off),
                      static_cast<double2*>(diag));
   QSX_CHECK_HIP(hipGetLastError());
@@ -447,8 +455,8 @@ extern "C" int32_t qsx_propagate_powers(const void* L, int64_t strideL, int32_t 
   const int64_t sL = B > 1 ? strideL : 0;
 auto kern = small_powers_apply_kernel;
   QSX_TRY(set_lds(kern, lds));
-  hipLaunchKernelGGL(kern, dim3(B), dim3(kSmallThreads), lds, s, static_cast<const double2*>(L),
-                     sL, M, h, ex, static_cast<const double2*>(y0), R,
+  hipLaunchKernelGGL(kern, dim3(B), dim3(small_threads(M)), lds, s,
+                     static_cast<const double2*>(L), sL, M, h, ex, static_cast<const double2*>(y0), R,
                      static_cast<double2*>(out));
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;

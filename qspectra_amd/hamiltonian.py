@@ -2,10 +2,17 @@
 (Hamiltonian, ElectronicHamiltonian).
 
 Eigensystems come from the batched FP64 Jacobi kernel (qsx_block_eigh), one
-excitation manifold at a time; the thermal state is built from them.  The
-rotating-frame / sampling bookkeeping follows hamiltonian.py:96-274: a
-sampled Hamiltonian keeps the rotating-wave frequency and time step of its
-parent, and ``sample`` and ``in_rotating_frame`` commute.
+excitation manifold at a time; the thermal state is built from them.
+
+Variants.  Every Hamiltonian is a variant of the one a user constructed
+(its *base*: lab frame, no disorder drawn), identified by a key (draw,
+rw_freq): ``draw`` is None or (n, random_orientations) for the n-th disorder
+sample, ``rw_freq`` the rotating-wave frequency (0 = lab frame).  One
+factory on the base builds and caches every variant, so rotating and
+sampling act on the key and commute (they even return the same object),
+drawing a sample of a sample replaces the draw (disorder is always added to
+the base), and a sample keeps its parent's rw_freq and hence its time and
+frequency grids -- the semantics of hamiltonian.py:96-274.
 """
 from abc import ABCMeta, abstractmethod
 from numbers import Number
@@ -29,10 +36,14 @@ class HamiltonianError(Exception):
 
 
 def check_hermitian(matrix):
-    matrix = np.asarray(matrix)
-    if not np.allclose(matrix.conj().T, matrix):
-        raise ValueError('matrix input must to be Hermitian')
-    return matrix
+    """The matrix as an array; ValueError unless it is square and equals its
+    conjugate transpose to numpy's default tolerances (hamiltonian.py:18-22)."""
+    m = np.asarray(matrix)
+    square = m.ndim == 2 and m.shape[0] == m.shape[1]
+    if not (square and np.allclose(m, np.conj(m).T)):
+        raise ValueError('expected a Hermitian matrix, got one of shape %s that is not'
+                         % (m.shape,))
+    return m
 
 
 def _state_from_eig(E, U, temperature):
@@ -72,7 +83,7 @@ def electronic_thermal_states(hamiltonians, subspace):
     for u in Us:
         U[:, k:k + u.shape[1], k:k + u.shape[1]] = u
         k += u.shape[1]
-    T = np.array([getattr(getattr(h._not_rotating, 'bath', None), 'temperature', 0) or 0
+    T = np.array([getattr(getattr(h.lab_frame, 'bath', None), 'temperature', 0) or 0
                   for h in hams], dtype=float)
     dE = E - E.min(axis=1, keepdims=True)
     hot = T > 0
@@ -92,21 +103,74 @@ def add_braket(labels):
 
 
 class Hamiltonian(metaclass=ABCMeta):
-    """Base class; subclasses provide ``H(subspace)``."""
+    """Base class; subclasses provide ``H(subspace)`` and, to support the
+    variants described in the module docstring, ``_in_rotating_frame`` and
+    ``_sample`` (each building a new instance from the base's data)."""
 
     _implements_eq = False
 
     def __init__(self, energy_spread_extra=None, site_labels=None):
         self.energy_spread_extra = energy_spread_extra
-        self._not_sampled = self
-        self._not_rotating = self
-        self.rw_freq = 0
         self.site_labels = site_labels
+        self.rw_freq = 0
+        self._variant_of = None      # the base this is a variant of (None: itself)
+        self._draw = None            # (n, random_orientations) of a disorder sample
+
+    # ------------------------------------------------------------ variants
+    @property
+    def base(self):
+        """The Hamiltonian as constructed: lab frame, no disorder drawn."""
+        return self if self._variant_of is None else self._variant_of
+
+    def variant(self, draw, rw_freq):
+        """The base's variant with disorder ``draw`` (None, or (n,
+        random_orientations)) in the frame rotating at ``rw_freq``; built
+        once per key and cached on the base."""
+        base = self.base
+        key = (draw, float(rw_freq))
+        if key == (None, 0.0):
+            return base
+        table = base.__dict__.setdefault('_variants', {})
+        ham = table.get(key)
+        if ham is None:
+            ham = base._sample(*draw) if draw is not None else base
+            if key[1] != 0.0:
+                ham = ham._in_rotating_frame(key[1])
+            ham._variant_of, ham._draw, ham.rw_freq = base, draw, rw_freq
+            table[key] = ham
+        return ham
 
     @property
-    def _original(self):
-        return self._not_rotating._not_sampled
+    def lab_frame(self):
+        """This variant without the rotating frame (same disorder draw)."""
+        return self.variant(self._draw, 0.0)
 
+    @property
+    def undisordered(self):
+        """This variant without the disorder draw (same frame)."""
+        return self.variant(None, self.rw_freq)
+
+    def in_rotating_frame(self, rw_freq=None):
+        """This Hamiltonian in the frame rotating at rw_freq (default: the
+        base's mean one-exciton energy); applying it again re-targets the
+        frame instead of shifting twice."""
+        if rw_freq is None:
+            rw_freq = self.base.transition_energy
+        return self.variant(self._draw, rw_freq)
+
+    def sample_ensemble(self, ensemble_size=1, random_orientations=False):
+        for n in range(ensemble_size):
+            yield self.sample(n, random_orientations)
+
+    def sample(self, n=None, random_orientations=False):
+        """The n-th static-disorder sample (deterministic in n; random n by
+        default), in this Hamiltonian's frame: it keeps rw_freq, so its time
+        and frequency grids are this Hamiltonian's."""
+        if n is None:
+            n = np.random.randint(2 ** 30)
+        return self.variant((int(n), bool(random_orientations)), self.rw_freq)
+
+    # ------------------------------------------------------------ identity
     def __repr__(self):
         return inspect_repr(self)
 
@@ -119,12 +183,18 @@ class Hamiltonian(metaclass=ABCMeta):
     __hash__ = object.__hash__
 
     def _eq(self, other, max_depth):
+        """Data equality (subclasses compare their fields and call this);
+        variants must also agree in frame, draw and base."""
         if not self._implements_eq:
             return self is other
-        return (not max_depth
-                or (self.rw_freq == other.rw_freq and self.site_labels == other.site_labels
-                    and self._not_sampled._eq(other._not_sampled, max_depth - 1)
-                    and self._not_rotating._eq(other._not_rotating, max_depth - 1)))
+        if not max_depth:
+            return True
+        if (self.rw_freq, self.site_labels, self._draw) != \
+                (other.rw_freq, other.site_labels, other._draw):
+            return False
+        if (self._variant_of is None) != (other._variant_of is None):
+            return False
+        return self._variant_of is None or self.base._eq(other.base, max_depth - 1)
 
     @abstractmethod
     def H(self, subspace):
@@ -133,38 +203,9 @@ class Hamiltonian(metaclass=ABCMeta):
     def n_states(self, subspace):
         return len(self.H(subspace))
 
-    @imemoize
-    def in_rotating_frame(self, rw_freq=None):
-        """Copy shifted to the rotating frame at rw_freq (default: the mean
-        one-exciton energy of the original Hamiltonian); idempotent."""
-        if rw_freq is None:
-            rw_freq = self._original.transition_energy
-        ham = self._not_rotating._in_rotating_frame(rw_freq)
-        ham._not_rotating = self._not_rotating
-        if self._not_sampled is not self:
-            ham._not_sampled = self._not_sampled.in_rotating_frame(rw_freq)
-        ham.rw_freq = rw_freq
-        return ham
-
     def _in_rotating_frame(self, rw_freq):
         raise NotImplementedError('%s does not implement rotating frame transformations'
                                   % type(self).__name__)
-
-    def sample_ensemble(self, ensemble_size=1, random_orientations=False):
-        for n in range(ensemble_size):
-            yield self.sample(n, random_orientations)
-
-    def sample(self, n=None, random_orientations=False):
-        """The n-th static-disorder sample (deterministic in n); it keeps this
-        Hamiltonian's rotating-wave frequency and time step."""
-        if n is None:
-            n = np.random.randint(2 ** 30)
-        ham = self._not_sampled._sample(n, random_orientations)
-        if self._not_rotating is not self:
-            ham._not_rotating = self._not_rotating.sample(n, random_orientations)
-        ham._not_sampled = self._not_sampled
-        ham.rw_freq = self.rw_freq
-        return ham
 
     def _sample(self, n, random_orientations):
         raise NotImplementedError('%s does not implement ensemble sampling' % type(self).__name__)
@@ -217,8 +258,8 @@ class Hamiltonian(metaclass=ABCMeta):
         """Nyquist sampling frequency covering every energy of the
         non-sampled Hamiltonian plus ``energy_spread_extra`` (default 1% of
         the original transition energy); fixed under sampling."""
-        E = self._not_sampled.E('gef')
-        extra = (0.01 * self._original.transition_energy if self.energy_spread_extra is None
+        E = self.undisordered.E('gef')
+        extra = (0.01 * self.base.transition_energy if self.energy_spread_extra is None
                  else self.energy_spread_extra)
         return 2 * (max(E.max(), -E.min()) + extra)
 
@@ -228,18 +269,16 @@ class Hamiltonian(metaclass=ABCMeta):
 
     @imemoize
     def ground_state(self, subspace):
-        E, U = self._not_rotating.eig(subspace)
+        E, U = self.lab_frame.eig(subspace)
         return _state_from_eig(E, U, 0)
 
     @imemoize
     def thermal_state(self, subspace):
-        """Boltzmann state of the non-rotating Hamiltonian at the bath
+        """Boltzmann state of the lab-frame Hamiltonian at the bath
         temperature (ground state without a bath temperature)."""
-        try:
-            T = self._not_rotating.bath.temperature
-        except AttributeError:
-            T = 0
-        E, U = self._not_rotating.eig(subspace)
+        lab = self.lab_frame
+        T = getattr(getattr(lab, 'bath', None), 'temperature', 0) or 0
+        E, U = lab.eig(subspace)
         return _state_from_eig(E, U, T)
 
     def basis_labels(self, subspace, braket=False):
@@ -250,10 +289,21 @@ class Hamiltonian(metaclass=ABCMeta):
                                       self.n_vibrational_states)
 
 
-def diagonal_gaussian_disorder(fwhm, n_sites):
-    def disorder(random_state):
-        return np.diag((fwhm * GAUSSIAN_SD_FWHM) * random_state.randn(n_sites))
-    return disorder
+class GaussianSiteDisorder(object):
+    """Independent Gaussian shifts of the n site energies, of full width at
+    half maximum ``fwhm`` (hamiltonian.py:458-461): called with a
+    RandomState, draws n standard normals in one call and returns the
+    diagonal disorder matrix."""
+
+    def __init__(self, fwhm, n_sites):
+        self.sigma = float(fwhm) * GAUSSIAN_SD_FWHM
+        self.n_sites = int(n_sites)
+
+    def __call__(self, random_state):
+        return np.diag(random_state.randn(self.n_sites) * self.sigma)
+
+
+diagonal_gaussian_disorder = GaussianSiteDisorder
 
 
 class ElectronicHamiltonian(Hamiltonian):
@@ -264,15 +314,14 @@ class ElectronicHamiltonian(Hamiltonian):
 
     _implements_eq = True
 
+    n_vibrational_states = 1
+
     def __init__(self, H_1exc, bath=None, dipoles=None, disorder=None, random_seed=0,
                  energy_spread_extra=None, site_labels=None):
-        self.H_1exc = check_hermitian(H_1exc)
-        self.bath = bath
-        self.dipoles = np.asarray(dipoles) if dipoles is not None else None
-        self.disorder = disorder
-        self.random_seed = random_seed
-        self.n_vibrational_states = 1
         super(ElectronicHamiltonian, self).__init__(energy_spread_extra, site_labels)
+        self.H_1exc = check_hermitian(H_1exc)
+        self.dipoles = None if dipoles is None else np.asarray(dipoles)
+        self.bath, self.disorder, self.random_seed = bath, disorder, random_seed
 
     def _eq(self, other, max_depth):
         return (isinstance(other, ElectronicHamiltonian)
@@ -303,32 +352,37 @@ class ElectronicHamiltonian(Hamiltonian):
                           self.dipoles, self.disorder, self.random_seed,
                           self.energy_spread_extra, self.site_labels)
 
-    def _sample(self, n, random_orientations):
+    def _disorder_draw(self):
+        """The disorder as a function of a RandomState (a FWHM number becomes
+        Gaussian site disorder; None draws nothing)."""
         if self.disorder is None:
-            if not random_orientations:
-                warnings.warn('called sample with `disorder=None` and `random_orientations='
-                              'False`: sampled Hamiltonian is identical to original',
-                              RuntimeWarning, stacklevel=3)
-            disorder = lambda rs: 0
-        elif isinstance(self.disorder, Number):
-            disorder = diagonal_gaussian_disorder(self.disorder, self.n_sites)
-        else:
-            disorder = self.disorder
-        seed = list(np.atleast_1d(self.random_seed)) + [n]
-        rs = check_random_state(seed)
-        H_1exc = self.H_1exc + disorder(rs)
-        dipoles = self.dipoles
-        if random_orientations:
-            dipoles = self.dipoles @ random_rotation_matrix(rs).T
-        return type(self)(H_1exc, self.bath, dipoles, self.disorder, seed,
-                          self.energy_spread_extra, self.site_labels)
+            return None
+        if isinstance(self.disorder, Number):
+            return GaussianSiteDisorder(self.disorder, self.n_sites)
+        return self.disorder
+
+    def _sample(self, n, random_orientations):
+        """Member n: a RandomState seeded with the seed list extended by n
+        (hamiltonian.py:566-567) draws the disorder, then the orientation."""
+        draw = self._disorder_draw()
+        if draw is None and not random_orientations:
+            warnings.warn('sampling a Hamiltonian without disorder or random orientations '
+                          'returns a copy of it', RuntimeWarning, stacklevel=4)
+        seed = np.atleast_1d(self.random_seed).tolist() + [n]
+        rng = check_random_state(seed)
+        H = self.H_1exc if draw is None else self.H_1exc + draw(rng)
+        dipoles = (self.dipoles @ random_rotation_matrix(rng).T if random_orientations
+                   else self.dipoles)
+        return type(self)(H, bath=self.bath, dipoles=dipoles, disorder=self.disorder,
+                          random_seed=seed, energy_spread_extra=self.energy_spread_extra,
+                          site_labels=self.site_labels)
 
     # ---------------------------------------------------------- device data
     @property
     def H1_nonrotating(self):
         """One-exciton Hamiltonian in the lab frame (with this sample's
         disorder), the input of the GPU eigen- and Liouvillian builders."""
-        return np.asarray(self._not_rotating.H_1exc, dtype=float)
+        return np.asarray(self.lab_frame.H_1exc, dtype=float)
 
     def _eig_blocks(self, subspace):
         if np.iscomplexobj(self.H_1exc) and np.any(np.imag(self.H_1exc) != 0):

@@ -16,24 +16,29 @@ MAGIC_ANGLE = np.arccos(1 / np.sqrt(3))
 
 
 def polarization_vector(p):
-    """'x'/'y'/'z', an angle in the x-y plane, or a 3-vector -> 3-vector."""
-    try:
-        if isinstance(p, str):
+    """'x'/'y'/'z', an angle in the x-y plane, or a 3-vector -> 3-vector;
+    ValueError for anything else."""
+    if isinstance(p, str):
+        if p in _AXES:
             return _AXES[p]
-        if isinstance(p, Number):
-            return np.array([np.cos(p), np.sin(p), 0])
-        v = np.asanyarray(p, float).reshape(-1)
-        if len(v) != 3:
-            raise ValueError('polarization vectors must have length 3')
-        return v
-    except Exception:
-        raise ValueError('invalid polarization {}'.format(p))
+    elif isinstance(p, Number):
+        return np.array([np.cos(p), np.sin(p), 0])
+    else:
+        try:
+            v = np.asanyarray(p, float).reshape(-1)
+        except (TypeError, ValueError):
+            v = None
+        if v is not None and v.size == 3:
+            return v
+    raise ValueError("cannot read %r as a polarization ('x', 'y', 'z', an angle in the x-y "
+                     "plane or a 3-vector)" % (p,))
 
 
 def check_polarizations(p, length):
+    """Polarization vectors [length, 3] of a sequence of `length` items."""
     vecs = np.array([polarization_vector(x) for x in p])
     if len(vecs) != length:
-        raise ValueError('%s polarizations required' % length)
+        raise ValueError('expected %d polarizations, got %d' % (length, len(vecs)))
     return vecs
 
 
@@ -49,8 +54,8 @@ def invariant_weights_4th_order(polarizations):
 def invariant_polarizations(invariant):
     """The 9 lab polarizations summed to evaluate one invariant."""
     if invariant not in FOURTH_ORDER_INVARIANTS:
-        raise ValueError('`invariant` is not one of the three 4th order tensor invariants %r'
-                         % FOURTH_ORDER_INVARIANTS)
+        raise ValueError('unknown fourth-order invariant %r (expected one of %r)'
+                         % (invariant, FOURTH_ORDER_INVARIANTS))
     return [''.join(p) for p in product('xyz', repeat=4)
             if all(p[a] == p[b] for a, b in invariant)]
 

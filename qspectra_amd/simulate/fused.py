@@ -125,6 +125,8 @@ class _Trajectory(object):
         return self.nc * (self.c - 1) + (self.nc - 1)
 
     def build(self):
+        if getattr(self, 'rebuild', None) is not None:
+            self.rebuild()
         if self.split_mfma:
             _native.rk4_propagator_split_into(self.L, self.h, self.q, self.off, self.diag,
                                               self.ws)
@@ -301,19 +303,23 @@ class ThirdOrderPlan(object):
         if len(self.t1_stages) != 1:
             raise PlanNotApplicable('pathways of one plan must share the t1 propagation')
 
-        self._rebuilds = []
+        # rebuild: each Liouvillian block is rebuilt on the stream that uses it
+        for st, _ in self.t1_stages.values():
+            st.rebuild = None
+        for g in self.groups.values():
+            g['rebuild'] = None
+        for st in self.t3_stages.values():
+            st['stage'].rebuild = None
         if rebuild:
-            seen = set()
-            blocks = [(k1[1], False, st.L) for k1, (st, _) in self.t1_stages.items()]
-            blocks += [(k12[2], False, g['L']) for k12, g in self.groups.items()]
-            blocks += [(k3[0], True, st['stage'].L) for k3, st in self.t3_stages.items()]
-            for sub, heis, L in blocks:
-                if (sub, heis) in seen:
-                    continue
-                seen.add((sub, heis))
-                inputs, fn = model.block_builder(sub, heis)
-                self._rebuilds.append((fn, L))
-            self.hamiltonian_inputs = inputs
+            for k1, (st, _) in self.t1_stages.items():
+                self.hamiltonian_inputs, fn = model.block_builder(k1[1], False)
+                st.rebuild = (lambda f, L: lambda: f(L))(fn, st.L)
+            for k12, g in self.groups.items():
+                _, fn = model.block_builder(k12[2], False)
+                g['rebuild'] = (lambda f, L: lambda: f(L))(fn, g['L'])
+            for k3, st in self.t3_stages.items():
+                _, fn = model.block_builder(k3[0], True)
+                st['stage'].rebuild = (lambda f, L: lambda: f(L))(fn, st['stage'].L)
         rw, uc = model.rw_freq, model.unit_convert
         if fourier:
             self.f1, self.F1 = transform_matrix(self.t1, rw, uc, -1, dev)
@@ -392,8 +398,6 @@ class ThirdOrderPlan(object):
         main = torch.cuda.current_stream()
         self.status.zero_()
         statuses = []
-        for fn, L in self._rebuilds:
-            fn(L)
         if self.ensemble_mean:
             return self._launch_ensemble(main, statuses)
         # each t3 Heisenberg trajectory on its own stream, transformed along t3
@@ -474,6 +478,8 @@ class ThirdOrderPlan(object):
         """W = sum over t2 groups of Q_g V2_g^T, Q_g the rows of V1^T
         propagated to every population time."""
         for i, (k12, g) in enumerate(self.groups.items()):
+            if g['rebuild'] is not None:
+                g['rebuild']()
             if g['rows'] is not None:
                 _native.propagate_powers_into(g['L'], g['h'], self.N2, g['rows'], self.Q[k12])
             else:
@@ -485,7 +491,15 @@ class ThirdOrderPlan(object):
                 else:
                     _native.matrix_powers_into(g['P'], self.N2, Pn, g['ws'])
                 _native.zgemm(g['V1T'], Pn.view(-1, *Pn.shape[2:]), self.Q[k12], transb=True)
-            _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=0.0 if i == 0 else 1.0)
+            beta = 0.0 if i == 0 else 1.0
+            if g['V2T'].shape[0] == 1:
+                # a shared V2 map: one tall GEMM over the stacked rows of every
+                # member and population time
+                Q, W = self.Q[k12], self.W
+                _native.zgemm(Q.view(1, -1, Q.shape[-1]), g['V2T'], W.view(1, -1, W.shape[-1]),
+                              beta=beta)
+            else:
+                _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=beta)
 
     def capture(self):
         """Records the launch sequence into a HIP graph, replayed by run()."""

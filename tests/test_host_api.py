@@ -15,6 +15,7 @@ from qspectra_amd import utils
 from qspectra_amd.dynamics import liouville_space as ls
 from qspectra_amd.simulate import decorators, utils as su
 from qspectra_amd import engine
+import qspectra_amd as qa
 from oracle import system as S
 from oracle import liouville as Lv
 from oracle import propagate as P
@@ -250,3 +251,35 @@ def test_ell_pack_of_field_operators():
     assert col.shape[2] == union
     col0, coef0 = _native.ell_pack(np.zeros((1, 1, 3, 3)))
     assert col0.shape == (1, 3, 1) and not coef0.any()
+
+
+def test_hamiltonian_variants_commute_and_keep_frames():
+    """Rotating-frame and disorder-sample variants (hamiltonian.py:96-274
+    semantics): they commute, a sample of a sample replaces the draw, member
+    n draws its disorder from RandomState(seed + [n]) with sigma = FWHM *
+    GAUSSIAN_SD_FWHM, samples keep rw_freq, rotating again re-targets."""
+    import warnings
+    from qspectra_amd.constants import GAUSSIAN_SD_FWHM
+    from qspectra_amd.hamiltonian import check_hermitian
+    H1 = np.array([[12400., 80., 10.], [80., 12500., -30.], [10., -30., 12600.]])
+    H = qa.ElectronicHamiltonian(H1, disorder=100.0, random_seed=5)
+    a = H.in_rotating_frame(12500.0).sample(3)
+    b = H.sample(3).in_rotating_frame(12500.0)
+    assert a is b and a.rw_freq == 12500.0 and a.base is H
+    d = np.diag(100.0 * GAUSSIAN_SD_FWHM * np.random.RandomState([5, 3]).randn(3))
+    assert np.array_equal(H.sample(3).H_1exc, H1 + d)
+    assert np.array_equal(a.H_1exc, H1 + d - 12500.0 * np.eye(3))
+    assert a.lab_frame is H.sample(3) and a.undisordered is H.in_rotating_frame(12500.0)
+    assert H.sample(3).sample(4) is H.sample(4)
+    assert H.in_rotating_frame(100.0).in_rotating_frame(200.0) is H.in_rotating_frame(200.0)
+    assert np.array_equal(a.H1_nonrotating, H1 + d)
+    twin = qa.ElectronicHamiltonian(H1, disorder=100.0, random_seed=5)
+    assert twin.sample(3) == H.sample(3) and twin.sample(3) != H.sample(2)
+    assert twin.in_rotating_frame(12500.0) != twin
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter('always')
+        qa.ElectronicHamiltonian(H1).sample(0)
+    assert any(issubclass(x.category, RuntimeWarning) for x in w)
+    for bad in ([[1., 2.], [0., 1.]], np.ones((2, 3))):
+        with pytest.raises(ValueError):
+            check_hermitian(bad)
