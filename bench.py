@@ -56,31 +56,41 @@ def parse_args(argv=None):
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
                          'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
-                         'launch sequence with chunk lengths 12 and 16, two HEOM RK4 steps '
+                         'launch sequence, the 256-member ensemble plan, two HEOM RK4 steps '
                          'of 256 members)')
     return ap.parse_args(argv)
 
 
 def counter_sequence(torch, qa):
     """Kernel sequence profiled with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE):
-    the calibration copy reads and writes exactly 256 MiB per dispatch."""
+    the calibration copy reads and writes exactly 256 MiB per dispatch; then
+    three launch sequences of the headline 2-D plan (MIXED, the plan's
+    chunk length 16), two of the 256-member ensemble plan (MIXED, rebuilt
+    from the Hamiltonians), and two HEOM RK4 steps of 256 members (FP64 and
+    MIXED)."""
     from qspectra_amd import workloads as W
     from qspectra_amd.simulate.utils import propagate_heom
+    from qspectra_amd.simulate.fused import ThirdOrderPlan
     src = torch.ones(32 * 1024 * 1024, dtype=torch.complex64, device='cuda')   # 256 MiB
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    from qspectra_amd.simulate.fused import ThirdOrderPlan
     work = W.AGG7_2D
-    for chunk in (12, 16):                 # the two chunk lengths the plan tuner picks
-        plan = ThirdOrderPlan(W.aggregate7_model(secular=False), work['coherence_time_max'],
-                              population_times=work['population_times'],
-                              geometry=work['geometry'], include_signal=work['include_signal'],
-                              dt=work['dt'], precision='mixed', fourier=True, chunk=chunk)
-        for _ in range(3):
-            plan._launch()
-        torch.cuda.synchronize()
+    kw = dict(population_times=work['population_times'], geometry=work['geometry'],
+              include_signal=work['include_signal'], dt=work['dt'], precision='mixed',
+              fourier=True)
+    plan = ThirdOrderPlan(W.aggregate7_model(secular=False), work['coherence_time_max'],
+                          chunk=16, **kw)
+    for _ in range(3):
+        plan._launch()
+    torch.cuda.synchronize()
+    m = W.aggregate7_model(secular=False, disorder_sigma=100.0)
+    mb = m.with_hamiltonians(list(m.hamiltonian.sample_ensemble(256)))
+    ens = ThirdOrderPlan(mb, work['coherence_time_max'], ensemble_mean=True, rebuild=True, **kw)
+    for _ in range(2):
+        ens._launch()
+    torch.cuda.synchronize()
     m = W.heom_aggregate7_model()
     eom = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(256)]) \
         .equation_of_motion('ee')
@@ -223,7 +233,7 @@ def kernel_bound(label):
     if label.startswith('zgemm'):
         tiles32 = -(-int(f['m']) // 32) * -(-int(f['n']) // 32) * int(f['batch'])
         fp64 = ' fp64 ' in label
-        if fp64 and tiles32 >= 192:
+        if fp64 and tiles32 >= 192 and int(f['m']) >= 16 and int(f['n']) >= 16:
             return 'mfma', PEAK_TFLOPS['fp64']
         return 'valu', PEAK_TFLOPS['fp64' if fp64 else 'mixed']
     if label.startswith(('small propagator', 'propagate powers')):
@@ -313,6 +323,77 @@ def cpu_baseline(budget, steps_per_spectrum, min_seconds=10.0):
                 reference_default_tolerance_rel_err=_reference_default_err())
 
 
+def _fixed_step_model(ham, rw):
+    """An oracle Liouville-space model on the 'ge' space whose time step is
+    the absorption workload's 1 fs grid."""
+    from oracle import system as S, liouville as Lv
+
+    class FixedStep(Lv.LiouvilleModel):
+        time_step = 1.0
+    return FixedStep(ham, rw_freq=rw, hilbert_subspace='ge', unit_convert=S.CM_FS)
+
+
+def cpu_extrapolated(budget=10.0):
+    """Extrapolated CPU figures of the two long workloads (too long to run in
+    full on the host), each from a bounded sample timed through the oracle
+    restatement of the reference path on all host cores and scaled linearly:
+      absorption ensemble: members of the 4096 + 1024 set, absorption over
+        4096 points at 1 fs through scipy zvode (rtol 1e-6), one member at a
+        time as the reference's ensemble loop runs them, Liouvillian built in
+        the timed part (the reference rebuilds it per member);
+      HEOM: one member's 3060-ADO hierarchy propagated through scipy zvode on
+        the sparse generator (built outside the timed part) over as many
+        whole 0.5 fs output steps of the 1 ps trajectory as the budget
+        allows, scaled to 2000 steps x 256 members."""
+    from oracle import system as S, propagate as P, liouville as Lv
+    from qspectra_amd import workloads as W
+    out = {}
+    hams = W.ensemble_absorption_hamiltonians()
+    parts = {}
+    for name, idx in (('disorder', range(0, 4096, 16)), ('aggregates', range(4096, 5120, 4))):
+        done, sec = 0, 0.0
+        for i in idx:
+            h = hams[i]
+            oh = S.ElectronicHamiltonian(h.H1_nonrotating,
+                                         bath=S.DebyeBath(S.CM_K * 77, 35, 106), dipoles=h.dipoles)
+            a = time.perf_counter()
+            P.absorption_spectra(_fixed_step_model(oh, 12450.0), 4096.0, integrator='zvode')
+            sec += time.perf_counter() - a
+            done += 1
+            if sec > budget / 2:
+                break
+        parts[name] = (done, sec)
+    (nd, sd), (na, sa) = parts['disorder'], parts['aggregates']
+    total = sd / nd * 4096 + sa / na * 1024
+    out['ensemble_absorption'] = {
+        'seconds_extrapolated': total, 'steps_per_s': len(hams) * 4095 / total,
+        'sample': '%d of the 4096 disorder members (%.1f s) and %d of the 1024 aggregates of '
+                  '2..24 sites (%.1f s), absorption over 4096 points through zvode (rtol 1e-6), '
+                  'one member at a time; each part scaled by its count' % (nd, sd, na, sa)}
+    H1, d, _ = W.aggregate7_hamiltonian()
+    oh = S.ElectronicHamiltonian(H1, bath=S.DebyeBath(S.CM_K * 300, 35, 106),
+                                 dipoles=d).in_rotating_frame()
+    L = Lv.heom_evolve_matrix(oh, 'ge', 'ee', S.CM_FS, K=1, level_cutoff=5)
+    y0 = np.zeros(L.shape[0], complex)
+    y0[0] = 1.0
+    f = lambda t, v: L @ v
+    # a short run sizes the timed one (saved every 0.5 fs, as the trajectory)
+    a = time.perf_counter()
+    P.integrate_zvode(f, y0, 0.5 * np.arange(11))
+    rate = 10 / (time.perf_counter() - a)
+    steps = int(min(2000, max(10, rate * budget)))
+    a = time.perf_counter()
+    P.integrate_zvode(f, y0, 0.5 * np.arange(steps + 1))
+    sec = time.perf_counter() - a
+    out['heom'] = {'seconds_extrapolated': sec / steps * 2000 * 256,
+                   'hierarchy_steps_timed': steps, 'seconds_timed': sec,
+                   'hierarchy_steps_per_s': steps / sec,
+                   'sample': '%d output steps of 0.5 fs of one 3060-ADO hierarchy through zvode '
+                             '(rtol 1e-6) on the sparse generator, %.1f s; x 2000 steps x 256 '
+                             'members' % (steps, sec)}
+    return out
+
+
 def _reference_default_err():
     try:
         return float(np.load(REFERENCE_2D, allow_pickle=False)['default_tol_rel_err'])
@@ -331,37 +412,26 @@ HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
 # HBM bytes per launch (read + written) measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE over the `bench.py --counters` sequence, FETCH
 # scaled by 2.0 and WRITE by 1.0 (the factors that make a 256 MiB device copy
-# in the same runs read and write 256 MiB).  Keyed by launch label.
+# in the same runs read and write 256 MiB), keyed by launch label; a launch
+# of several kernels counts all of them.
 PMC_TRAFFIC = {
-    'heom stage fp64': 3.525e9,
+    'heom stage fp64': 3.528e9,
     'heom stage mixed': 2.678e9,
-}
-# HBM bytes per launch of the 2-D plan's M = 147 trajectory launches, by
-# (precision, M, right-hand sides R, saves): rocprofv3 FETCH_SIZE x 2 +
-# WRITE_SIZE, the copy-calibrated factors (profiles/hbm_traffic_counters.md);
-# the plan's chunk length (12 or 16, tuned per box) selects the shapes
-PMC_PROPAGATE = {
-    ('mixed', 147, 8, 16): 1.807e6,     # 8 chunks x 16 saved steps
-    ('mixed', 147, 11, 12): 1.825e6,    # 11 chunks x 12 saved steps
-    ('fp64', 147, 1, 8): 3.80e5,        # chunk starts, P^16 chain
-    ('fp64', 147, 1, 11): 3.87e5,       # chunk starts, P^12 chain
-}
-# The same for the one-launch chunked trajectories, by (precision, M,
-# chunks nc, saves per chunk)
-PMC_CHUNKS = {
-    ('mixed', 147, 8, 16): 4.300e6,     # chunk 16: 8 work-groups
-    ('mixed', 147, 11, 12): 4.662e6,    # chunk 12: 11 work-groups
+    # the headline plan's t2 operator program (one work-group, M = 49)
+    'propagate powers M=49 B=1 R=7 e=0,100,200,500': 8.21e4,
+    'chunks prop mixed M=147 nc=8 B=1 steps=15 nsave=16': 4.30e6,
+    # the 256-member ensemble plan
+    'propagate powers M=49 B=256 R=7 e=0,100,200,500': 1.704e7,
+    'propagate prop mixed M=147 R=1 B=256 steps=127 nsave=128': 1.235e8,
+    # prep 1.366e8 + three Horner GEMMs 9.716e8 each + squaring 5.307e8
+    # + diagonal 1.2e6 + final split 9.25e7
+    'split propagator M=147 B=256 q=2': 3.676e9,
 }
 
 
 def launch_traffic(label):
-    """Measured HBM bytes of a propagate launch (None if not profiled)."""
-    m = re.match(r'(propagate|chunks) \w+ (\w+) M=(\d+) (?:R|nc)=(\d+) B=\d+ steps=\S+ '
-                 r'nsave=(\d+)', label)
-    if not m:
-        return PMC_TRAFFIC.get(label)
-    table = PMC_PROPAGATE if m.group(1) == 'propagate' else PMC_CHUNKS
-    return table.get((m.group(2), int(m.group(3)), int(m.group(4)), int(m.group(5))))
+    """Measured HBM bytes of a launch (None if not profiled)."""
+    return PMC_TRAFFIC.get(label)
 
 
 def _host(x):
@@ -797,6 +867,9 @@ def main(argv=None):
         line['cpu_baseline'] = cpu_baseline(args.cpu_budget, steps_per_spectrum)
         line['vs_cpu'] = value / line['cpu_baseline']['value']
         line['vs_cpu_one_thread'] = value / line['cpu_baseline']['one_thread']['value']
+        if wanted:
+            print('bench: cpu extrapolations', file=sys.stderr, flush=True)
+            line['cpu_baseline']['extrapolated'] = cpu_extrapolated()
     else:
         line['cpu_baseline'] = None
     print(json.dumps(line))

@@ -213,34 +213,42 @@ __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __re
 // ------------------------------------------------------------ 2. C(w) and K
 // Debye bath correlation function, qspectra/bath.py:84-102 (complex, 1000-term
 // Matsubara sum, k = 0 term zero) and bath.py:17-31 (real, detailed balance).
-__device__ double2 debye_corr(double x, double T, double lam, double gam, int real_corr) {
-  if (real_corr) {
-    if (x == 0.0) return make_double2(T * 2.0 * lam / gam, 0.0);
-    const double J = 2.0 * lam * gam * x / (gam * gam + x * x);  // odd: J_anti = J
-    return make_double2((1.0 / expm1(x / T) + 1.0) * J, 0.0);
-  }
-  if (x == 0.0) return make_double2(lam * 2.0 * T / gam, -lam);
-  // (cot(g/2T) - i) / (g - i x)
-  const double ct = 1.0 / tan(gam / (2.0 * T));
-  const double den = gam * gam + x * x;
-  double re = (ct * gam + x) / den;
-  double im = (ct * x - gam) / den;
-  // 4T sum_k nu_k / ((nu_k^2 - g^2)(nu_k - i x)), Kahan-compensated
+// sum_{k=1}^{999} nu_k / ((nu_k^2 - g^2)(nu_k - i x)), nu_k = 2 pi k T,
+// Kahan-compensated; its value at -x is the complex conjugate (each term's
+// imaginary part is odd in x, its real part even), exactly so in floating
+// point since only signs change
+__device__ double2 matsubara_sum(double x, double T, double gam) {
   double sr = 0.0, si = 0.0, cr = 0.0, ci = 0.0;
   const double twopiT = 2.0 * M_PI * T;
   for (int k = 1; k < 1000; ++k) {
     const double nu = twopiT * k;
-    const double a = (nu * nu - gam * gam) * (nu * nu + x * x);
-    const double tr = nu * nu / a - cr;
-    const double ti = nu * x / a - ci;
+    const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
+    const double tr = nu * nu * inv - cr;
+    const double ti = nu * x * inv - ci;
     const double ur = sr + tr, ui = si + ti;
     cr = (ur - sr) - tr;
     ci = (ui - si) - ti;
     sr = ur;
     si = ui;
   }
-  re += 4.0 * T * sr;
-  im += 4.0 * T * si;
+  return make_double2(sr, si);
+}
+
+// C(x) given the Matsubara sum S(x) (unused for the real correlation
+// function and at x = 0)
+__device__ double2 debye_corr_with(double x, double2 S, double T, double lam, double gam,
+                                   int real_corr) {
+  if (real_corr) {
+    if (x == 0.0) return make_double2(T * 2.0 * lam / gam, 0.0);
+    const double J = 2.0 * lam * gam * x / (gam * gam + x * x);  // odd: J_anti = J
+    return make_double2((1.0 / expm1(x / T) + 1.0) * J, 0.0);
+  }
+  if (x == 0.0) return make_double2(lam * 2.0 * T / gam, -lam);
+  // (cot(g/2T) - i) / (g - i x) + 4T S(x)
+  const double ct = 1.0 / tan(gam / (2.0 * T));
+  const double den = gam * gam + x * x;
+  const double re = (ct * gam + x) / den + 4.0 * T * S.x;
+  const double im = (ct * x - gam) / den + 4.0 * T * S.y;
   return make_double2(lam * gam * re, lam * gam * im);
 }
 
@@ -252,9 +260,21 @@ __global__ __launch_bounds__(kThreads) void block_bath_kernel(int n, int X, int 
                                                               size_t rstride) {
   const int b = blockIdx.x, tid = threadIdx.x;
   Rec rec = rec_at(ws + (size_t)b * rstride, nb, n);
-  for (int e = tid; e < nb * nb; e += kThreads) {
-    const int d = e / nb, c = e % nb;
-    rec.C[e] = debye_corr(rec.E[d] - rec.E[c], T, lam, gam, real_corr);
+  // pairs (d, c) with d <= c: one Matsubara sum serves C[d][c] and C[c][d]
+  const int npairs = nb * (nb + 1) / 2;
+  for (int e = tid; e < npairs; e += kThreads) {
+    int d = 0, r = e;
+    while (r >= nb - d) {
+      r -= nb - d;
+      ++d;
+    }
+    const int c = d + r;
+    const double x = rec.E[d] - rec.E[c];
+    const bool sum = !real_corr && x != 0.0;
+    const double2 S = sum ? matsubara_sum(x, T, gam) : make_double2(0.0, 0.0);
+    rec.C[d * nb + c] = debye_corr_with(x, S, T, lam, gam, real_corr);
+    if (c != d) rec.C[c * nb + d] = debye_corr_with(-x, make_double2(S.x, -S.y), T, lam, gam,
+                                                    real_corr);
   }
   // K_i[a][m] = sum over states s of X that contain site i of U[s][a] U[s][m]
   const int nk = n * nb * nb;
@@ -335,12 +355,42 @@ __device__ void sandwich(const double2* Z, double2* tmp, double2* outp, const do
 // R = d_bd Gs_X[a][c] + d_ac conj(Gs_Y[b][d])
 //     - sum_i K^X_i[a][c] K^Y_i[b][d] (conj(C_Y[d][b]) + C_X[c][a])
 // (redfield.py:61-69 with K real symmetric), secular mask redfield.py:77-83.
+// The Heisenberg block is the transpose; since T = kron(U_Y, U_X) is real,
+// T L^T T^T = (T L T^T)^T, so both passes work on A = L_eig or L_eig^T and
+// every global write is a contiguous row: pass 1 writes column q of W = T A
+// as row q of W^T, pass 2 reads row p of W (a strided read of W^T, L2
+// resident) and writes row p of T A T^T.
 enum { kSecular = 1, kRealCorr = 2, kEigenBasis = 4, kHeisenberg = 8 };
 
+__device__ inline double2 leig_entry(const Rec& rx, const Rec& ry, int n, int nX, int nY,
+                                     int same, int flags, double unit, int p, int q) {
+  const int a = p % nX, bb = p / nX, c = q % nX, d = q / nX;
+  double2 r = make_double2(0.0, 0.0);
+  if (bb == d) r = cadd(r, rx.Gs[a * nX + c]);
+  if (a == c) {
+    const double2 g = ry.Gs[bb * nY + d];
+    r = cadd(r, make_double2(g.x, -g.y));
+  }
+  const double2 cy = ry.C[d * nY + bb], cx = rx.C[c * nX + a];
+  const double2 cs = make_double2(cy.x + cx.x, -cy.y + cx.y);
+  double kk = 0.0;
+  for (int i = 0; i < n; ++i)
+    kk = fma(rx.K[((size_t)i * nX + a) * nX + c], ry.K[((size_t)i * nY + bb) * nY + d], kk);
+  r.x -= kk * cs.x;
+  r.y -= kk * cs.y;
+  const bool keep = (a == c && bb == d) || (same && a == bb && c == d);
+  if ((flags & kSecular) && !keep) r = make_double2(0.0, 0.0);
+  double2 v = make_double2(-r.x, -r.y);
+  if (a == c && bb == d) v.y -= rx.E[a] - ry.E[bb];
+  return make_double2(unit * v.x, unit * v.y);
+}
+
+// block q of problem b: eigen basis -> row q of A into L; site basis ->
+// column q of A, times T, into row q of W^T
 __global__ __launch_bounds__(kThreads) void liouv_cols_kernel(
     int n, int nX, int nY, int same, const double* __restrict__ wsX,
     const double* __restrict__ wsY, size_t rstride, int flags, double unit,
-    double2* __restrict__ W, double2* __restrict__ L) {
+    double2* __restrict__ WT, double2* __restrict__ L) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int q = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
   const int M = nX * nY;
@@ -351,49 +401,29 @@ __global__ __launch_bounds__(kThreads) void liouv_cols_kernel(
   double2* w = tmp + M;
   double* UX = reinterpret_cast<double*>(w + M);
   double* UY = UX + nX * nX;
-  const int c = q % nX, d = q / nX;
-  const bool secular = flags & kSecular;
-  for (int p = tid; p < M; p += kThreads) {
-    const int a = p % nX, bb = p / nX;
-    double2 r = make_double2(0.0, 0.0);
-    if (bb == d) r = cadd(r, rx.Gs[a * nX + c]);
-    if (a == c) {
-      const double2 g = ry.Gs[bb * nY + d];
-      r = cadd(r, make_double2(g.x, -g.y));
-    }
-    const double2 cy = ry.C[d * nY + bb], cx = rx.C[c * nX + a];
-    const double2 cs = make_double2(cy.x + cx.x, -cy.y + cx.y);
-    double kk = 0.0;
-    for (int i = 0; i < n; ++i)
-      kk = fma(rx.K[((size_t)i * nX + a) * nX + c], ry.K[((size_t)i * nY + bb) * nY + d], kk);
-    r.x -= kk * cs.x;
-    r.y -= kk * cs.y;
-    const bool keep = (a == c && bb == d) || (same && a == bb && c == d);
-    if (secular && !keep) r = make_double2(0.0, 0.0);
-    double2 v = make_double2(-r.x, -r.y);
-    if (a == c && bb == d) v.y -= rx.E[a] - ry.E[bb];
-    v = make_double2(unit * v.x, unit * v.y);
-    if (flags & kEigenBasis) {
-      double2* Lb = L + (size_t)b * M * M;
-      if (flags & kHeisenberg) Lb[(size_t)q * M + p] = v;
-      else Lb[(size_t)p * M + q] = v;
-    } else {
-      z[p] = v;
-    }
+  const bool heis = flags & kHeisenberg;
+  if (flags & kEigenBasis) {
+    double2* Lrow = L + (size_t)b * M * M + (size_t)q * M;
+    for (int p = tid; p < M; p += kThreads)       // A[q][p]
+      Lrow[p] = heis ? leig_entry(rx, ry, n, nX, nY, same, flags, unit, p, q)
+                     : leig_entry(rx, ry, n, nX, nY, same, flags, unit, q, p);
+    return;
   }
-  if (flags & kEigenBasis) return;
+  for (int p = tid; p < M; p += kThreads)         // A[p][q]
+    z[p] = heis ? leig_entry(rx, ry, n, nX, nY, same, flags, unit, q, p)
+                : leig_entry(rx, ry, n, nX, nY, same, flags, unit, p, q);
   for (int e = tid; e < nX * nX; e += kThreads) UX[e] = rx.U[e];
   for (int e = tid; e < nY * nY; e += kThreads) UY[e] = ry.U[e];
   __syncthreads();
   sandwich(z, tmp, w, UX, UY, nX, nY, tid);
-  double2* Wb = W + (size_t)b * M * M;
-  for (int p = tid; p < M; p += kThreads) Wb[(size_t)p * M + q] = w[p];
+  double2* WTrow = WT + (size_t)b * M * M + (size_t)q * M;
+  for (int p = tid; p < M; p += kThreads) WTrow[p] = w[p];
 }
 
 // ----------------------------------------------------- 5. site rows, pass 2
 __global__ __launch_bounds__(kThreads) void liouv_rows_kernel(
     int n, int nX, int nY, const double* __restrict__ wsX, const double* __restrict__ wsY,
-    size_t rstride, int flags, const double2* __restrict__ W, double2* __restrict__ L) {
+    size_t rstride, int flags, const double2* __restrict__ WT, double2* __restrict__ L) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
   const int M = nX * nY;
@@ -404,17 +434,14 @@ __global__ __launch_bounds__(kThreads) void liouv_rows_kernel(
   double2* w = tmp + M;
   double* UX = reinterpret_cast<double*>(w + M);
   double* UY = UX + nX * nX;
-  const double2* Wb = W + (size_t)b * M * M;
-  for (int e = tid; e < M; e += kThreads) z[e] = Wb[(size_t)p * M + e];
+  const double2* WTb = WT + (size_t)b * M * M;
+  for (int e = tid; e < M; e += kThreads) z[e] = WTb[(size_t)e * M + p];   // row p of W
   for (int e = tid; e < nX * nX; e += kThreads) UX[e] = rx.U[e];
   for (int e = tid; e < nY * nY; e += kThreads) UY[e] = ry.U[e];
   __syncthreads();
   sandwich(z, tmp, w, UX, UY, nX, nY, tid);
-  double2* Lb = L + (size_t)b * M * M;
-  for (int q = tid; q < M; q += kThreads) {
-    if (flags & kHeisenberg) Lb[(size_t)q * M + p] = w[q];
-    else Lb[(size_t)p * M + q] = w[q];
-  }
+  double2* Lrow = L + (size_t)b * M * M + (size_t)p * M;
+  for (int q = tid; q < M; q += kThreads) Lrow[q] = w[q];
 }
 
 // ------------------------------------------------------------------ unitary

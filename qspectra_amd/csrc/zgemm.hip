@@ -277,7 +277,9 @@ int qsx::zgemm_enqueue(int precision, int trans, int m, int n, int k, double alp
   // skinny products are latency-bound and run faster on the vector path
   // (measured on MI355X: 255 x 255 x 154 x 4 16 us vs 21 us; 255 x 147 x 128
   // 16 us vs 9 us)
-  if (precision == QSX_FP64 && tiles32 >= 192) {
+  // (and output tiles at least half full: a 7-row product padded to 32 rows
+  // wastes the matrix cores four times over)
+  if (precision == QSX_FP64 && tiles32 >= 192 && m >= 16 && n >= 16) {
     launch_zgemm_mfma<2, 2, 64>(trans, s, m, n, k, al, A, lda, sA, B, ldb, sB, be, C, ldc, sC,
                                 ga, batch);
     QSX_CHECK_HIP(hipGetLastError());

@@ -36,11 +36,9 @@ def test_propagate_launch_counts(bench):
         8.0 * 7 * 7 * 4 * 10 * 3 * 2
 
 
-def test_traffic_lookup_by_launch_shape(bench):
-    assert bench.launch_traffic('chunks prop mixed M=147 nc=8 B=1 steps=15 nsave=16') == \
-        bench.PMC_CHUNKS[('mixed', 147, 8, 16)]
-    assert bench.launch_traffic('propagate prop fp64 M=147 R=1 B=1 steps=7 nsave=8') == \
-        bench.PMC_PROPAGATE[('fp64', 147, 1, 8)]
+def test_traffic_lookup_by_launch_label(bench):
+    label = 'propagate powers M=49 B=1 R=7 e=0,100,200,500'
+    assert bench.launch_traffic(label) == bench.PMC_TRAFFIC[label]
     assert bench.launch_traffic('chunks prop mixed M=31 nc=8 B=1 steps=15 nsave=16') is None
 
 
