@@ -131,6 +131,33 @@ int32_t qsx_redfield_block(const double* H1, int32_t n, int32_t B, const double*
                            double unit_convert, int32_t X, int32_t Y, void* L,
                            void* workspace, void* stream);
 
+/* The same block for any bath, its correlation function tabulated by the
+ * caller: ctabX [B][nX][nX], ctabY [B][nY][nY] complex128 with
+ * ctab[b][d][c] = C(E_d - E_c) over the eigen-energies of Hilbert block X
+ * (Y) of problem b (as qsx_block_eigh returns them).  Replaces the
+ * bath.corr_func_real / corr_func_complex calls of redfield_tensor
+ * (qspectra/dynamics/redfield.py:48-53) for ArbitraryBath and other
+ * Python-defined baths (qspectra/bath.py:8-57). */
+int32_t qsx_redfield_block_tabulated(const double* H1, int32_t n, int32_t B, const double* rw,
+                                     const void* ctabX, const void* ctabY, int32_t flags,
+                                     double unit_convert, int32_t X, int32_t Y, void* L,
+                                     void* workspace, void* stream);
+
+/* Reverse mode of qsx_redfield_block (Debye bath, same flags and blocks):
+ * given a cotangent gL [B][M][M] of the block it returned, writes
+ * gH1 [B][n][n], the cotangent of each one-exciton Hamiltonian (symmetric),
+ * and glam [B] (may be null), the derivative with respect to the
+ * reorganization energy, both in torch's convention dl = Re sum conj(g) dx.
+ * The reference computes the tensor only (qspectra/dynamics/redfield.py:9-104,
+ * qspectra/bath.py:84-102); this is the derivative of that same map, for
+ * gradients of spectra with respect to site energies, couplings and lam.
+ * Workspace: qsx_redfield_vjp_workspace_bytes. */
+int64_t qsx_redfield_vjp_workspace_bytes(int32_t n, int32_t X, int32_t Y, int32_t B);
+int32_t qsx_redfield_block_vjp(const double* H1, int32_t n, int32_t B, const double* rw,
+                               double T, double lam, double gam, int32_t flags,
+                               double unit_convert, int32_t X, int32_t Y, const void* gL,
+                               double* gH1, double* glam, void* workspace, void* stream);
+
 /* Unitary Liouvillian block -i*unit_convert*[H, .] in the rotating frame
  * (UnitaryModel.evolution_super_operator, qspectra/dynamics/unitary.py:5-9);
  * heisenberg != 0 returns the transpose. */
@@ -194,6 +221,23 @@ int32_t qsx_propagate_adjoint(const void* A, int64_t strideA, int32_t M, const v
                               const void* gout, int32_t ckpt, void* workspace,
                               int64_t workspace_bytes, void* gA, void* gy0, int32_t* status,
                               void* stream);
+
+/* Linear-response signals of many problems of unequal size in one launch:
+ * for problem b, y_{k+1} = P_b y_k from y0_b and s_k = v_b . y_k,
+ * k = 0 .. nt-1, written to out [B][nt] complex128 (FP64 products).
+ * waves int32 [nwaves][4] (device): first problem, problem count, M, G per
+ * wavefront, with G a power of two above M, G * count <= 64 and M at most
+ * qsx_response_buckets_max_states(); offsets int64 [B][2] (device): offset
+ * of P_b (M^2 complex elements, row-major) in P and of y0_b, v_b (M each) in
+ * y0 and v.  status int32 [B]: 1 where a signal value was not finite.
+ * Replaces the per-member zvode runs of linear_response / absorption_spectra
+ * (qspectra/simulate/response.py:15-43, :103-154; simulate/utils.py:16-109)
+ * over an ensemble: P_b = P4(h L_b)^q of the member's Heisenberg 'eg' block,
+ * y0_b = -bra, v_b = V rho0. */
+int32_t qsx_response_buckets(const int32_t* waves, int32_t nwaves, const int64_t* offsets,
+                             const void* P, const void* y0, const void* v, int32_t nt,
+                             void* out, int32_t* status, void* stream);
+int32_t qsx_response_buckets_max_states(void);
 
 /* Batched complex GEMM, row-major storage:
  *   C[b] = alpha * op(A[b]) (m x k) * op(B[b]) (k x n) + beta * C[b] + gamma * I
@@ -339,6 +383,26 @@ int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t 
                            const double* rel_occ, const void* y0, const int32_t* nsub_host,
                            const double* h_host, int32_t nsave, void* out, void* y_final,
                            void* workspace, int32_t* status, void* stream);
+
+/* qsx_heom_propagate under classical fields for a dense hierarchy block
+ * (M <= 64, the multi-block subspaces such as 'gg,ge,eg,ee'): every ADO
+ * evolves with the stage operator Hop[b] + sum_i f_i(t) C_i, C [B|1][P][M][M]
+ * the dense dipole commutators (strideC 0 or P*M*M), f_i = -i E_i(t)
+ * (conjugated for a '+' interaction) sampled by the host at the stage times
+ * t, t + h/2, t + h of every step: F [B|1][nF][3][P] (strideF 0 or nF*3*P,
+ * nF >= the total step count).  Hwork [B][M][M] complex128 holds the stage
+ * operators.  simulate_with_fields / simulate_pump of a HEOM model
+ * (qspectra/simulate/eom.py:79-103 with the HEOMSpaceOperator commutator of
+ * qspectra/dynamics/heom.py:22-58, which maps over ADOs). */
+int32_t qsx_heom_propagate_driven(int32_t precision, int32_t B, int32_t n_ado, int32_t M,
+                                  const void* Hop, const double* shift, const void* dvec,
+                                  int32_t nbins, int32_t K1, const int32_t* bin_src,
+                                  const void* bin_coef, const void* ck, int32_t nrel,
+                                  const int32_t* rel_site, const double* rel_occ, int32_t P,
+                                  const void* C, int64_t strideC, const void* F, int64_t strideF,
+                                  int64_t nF, const void* y0, const int32_t* nsub_host,
+                                  const double* h_host, int32_t nsave, void* out, void* y_final,
+                                  void* Hwork, void* workspace, int32_t* status, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Fixed-step RK4 under classical fields (rotating wave approximation):

@@ -191,6 +191,26 @@ class DebyeBath(object):
                           + 4 * T * np.sum(nu / ((nu ** 2 - g ** 2) * (nu - 1j * x))))
 
 
+class ArbitraryBath(object):
+    """qspectra/bath.py:50-56 with Bath.corr_func_real :17-31: a bath given
+    by a spectral density function J(x) and lim J(x)/x at 0.  Only the real
+    correlation function exists (no corr_func_complex, as in the reference)."""
+
+    def __init__(self, temperature, spectral_density_func, spectral_density_limit_at_zero):
+        self.temperature = temperature
+        self.spectral_density_func = spectral_density_func
+        self.spectral_density_limit_at_zero = spectral_density_limit_at_zero
+
+    def corr_func_real(self, x):
+        """bath.py:17-31: (n(x) + 1) J_anti(x), T J0 at x = 0."""
+        T = self.temperature
+        if x == 0:
+            return T * self.spectral_density_limit_at_zero
+        J = self.spectral_density_func
+        Janti = J(x) if x >= 0 else -J(-x)
+        return (1 / np.expm1(x / T) + 1) * Janti
+
+
 # -------------------------------------------------------------- Hamiltonian
 def thermal_state(H, temperature):
     """qspectra/hamiltonian.py:49-78 (+ ground_state :25-46)."""

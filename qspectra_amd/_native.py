@@ -42,7 +42,8 @@ RESTYPES = {'qsx_redfield_workspace_bytes': c_i64, 'qsx_heom_workspace_bytes': c
             'qsx_adjoint_workspace_bytes': c_i64, 'qsx_zofe_workspace_bytes': c_i64,
             'qsx_redfield_dense_workspace_bytes': c_i64, 'qsx_fourier_workspace_bytes': c_i64,
             'qsx_propagator_workspace_bytes': c_i64,
-            'qsx_split_propagator_workspace_bytes': c_i64}
+            'qsx_split_propagator_workspace_bytes': c_i64,
+            'qsx_redfield_vjp_workspace_bytes': c_i64}
 SIGNATURES = {
     'qsx_version': [],
     'qsx_last_error': [ctypes.c_char_p, c_sz],
@@ -54,6 +55,17 @@ SIGNATURES = {
     'qsx_redfield_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_dbl, c_dbl, c_i32, c_dbl,
                            c_i32, c_i32, c_vp, c_vp, c_vp],
     'qsx_unitary_block': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_i32, c_i32, c_i32, c_vp, c_vp],
+    'qsx_redfield_block_tabulated': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_dbl, c_i32,
+                                     c_i32, c_vp, c_vp, c_vp],
+    'qsx_redfield_vjp_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
+    'qsx_response_buckets': [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp],
+    'qsx_heom_propagate_driven': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp,
+                                  c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64,
+                                  c_i64, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                  c_vp],
+    'qsx_response_buckets_max_states': [],
+    'qsx_redfield_block_vjp': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_dbl, c_dbl, c_i32, c_dbl, c_i32,
+                               c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     'qsx_propagate': [c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32, c_i32,
                       c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp],
     'qsx_propagate_chunks': [c_i32, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i32, c_vp, c_i32,
@@ -273,6 +285,76 @@ def redfield_block(H1, rw, T, lam, gam, X, Y, flags, unit_convert, stream=None, 
         st = go()
     check(st, 'qsx_redfield_block')
     return L
+
+
+def redfield_block_tabulated(H1, rw, ctabX, ctabY, X, Y, flags, unit_convert, stream=None):
+    """Redfield block XY with the bath correlation function tabulated over
+    each member's eigen-energy differences (ctab complex128 [B, nb, nb])."""
+    lib = require_native()
+    B, n, _ = H1.shape
+    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    nX, nY = block_size(X, n), block_size(Y, n)
+    for C, nb in ((ctabX, nX), (ctabY, nY)):
+        assert C.dtype == torch.complex128 and C.is_contiguous() and C.shape == (B, nb, nb)
+    L = torch.empty((B, nX * nY, nX * nY), dtype=torch.complex128, device=H1.device)
+    ws = _workspace(n, X, Y, B, H1.device)
+
+    def go():
+        return lib.qsx_redfield_block_tabulated(ptr(H1), n, B, ptr(rw), ptr(ctabX), ptr(ctabY),
+                                                int(flags), float(unit_convert), X, Y, ptr(L),
+                                                ptr(ws), stream_ptr(stream))
+    with timing.launch('redfield block tabulated %s%s n=%d B=%d' % ('gef'[X], 'gef'[Y], n, B),
+                       stream, go):
+        st = go()
+    check(st, 'qsx_redfield_block_tabulated')
+    return L
+
+
+def redfield_block_vjp(H1, rw, T, lam, gam, X, Y, flags, unit_convert, gL, stream=None):
+    """Cotangents (gH1 [B, n, n] float64, glam [B] float64) of the Redfield
+    block XY that redfield_block returns for the same arguments, given its
+    cotangent gL (complex128 [B, M, M])."""
+    lib = require_native()
+    B, n, _ = H1.shape
+    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    M = block_size(X, n) * block_size(Y, n)
+    assert gL.dtype == torch.complex128 and gL.is_contiguous() and gL.shape == (B, M, M)
+    gH1 = torch.empty_like(H1)
+    glam = torch.empty((B,), dtype=torch.float64, device=H1.device)
+    nbytes = lib.qsx_redfield_vjp_workspace_bytes(n, X, Y, B)
+    ws = torch.empty((max(int(nbytes), 8) + 7) // 8, dtype=torch.float64, device=H1.device)
+
+    def go():
+        return lib.qsx_redfield_block_vjp(ptr(H1), n, B, ptr(rw), float(T), float(lam),
+                                          float(gam), int(flags), float(unit_convert), X, Y,
+                                          ptr(gL), ptr(gH1), ptr(glam), ptr(ws),
+                                          stream_ptr(stream))
+    with timing.launch('redfield block vjp %s%s n=%d B=%d' % ('gef'[X], 'gef'[Y], n, B),
+                       stream, go):
+        st = go()
+    check(st, 'qsx_redfield_block_vjp')
+    return gH1, glam
+
+
+def response_buckets_into(waves, offsets, P, y0, v, out, status, stream=None):
+    """out [B, nt] <- s_k = v_b . P_b^k y0_b for problems of unequal size in
+    one launch (qsx_response_buckets; the tables are device int32 [nw, 4] and
+    int64 [B, 2], P, y0, v flat complex128)."""
+    lib = require_native()
+    assert waves.dtype == torch.int32 and waves.is_contiguous() and waves.shape[1] == 4
+    assert offsets.dtype == torch.int64 and offsets.is_contiguous()
+    for X in (P, y0, v, out):
+        assert X.dtype == torch.complex128 and X.is_contiguous()
+    assert status.dtype == torch.int32 and status.numel() == out.shape[0]
+    nt = out.shape[1]
+
+    def go():
+        return lib.qsx_response_buckets(ptr(waves), waves.shape[0], ptr(offsets), ptr(P), ptr(y0),
+                                        ptr(v), nt, ptr(out), ptr(status), stream_ptr(stream))
+    with timing.launch('response buckets B=%d waves=%d nt=%d' % (out.shape[0], waves.shape[0], nt),
+                       stream, go):
+        st = go()
+    check(st, 'qsx_response_buckets')
 
 
 def redfield_workspace(n, X, Y, B, dev=None):
@@ -522,6 +604,16 @@ def zgemm(A, B, C=None, alpha=1.0, beta=0.0, gamma=0.0, precision=QSX_FP64, stre
     assert bA in (1, batch) and bB in (1, batch)
     dt = torch.complex128 if precision == QSX_FP64 else torch.complex64
     assert A.dtype == dt and B.dtype == dt
+    if m == 0 or n == 0:
+        return C
+    if k == 0:              # empty contraction: only the beta and identity terms
+        if complex(beta) == 0:
+            C.zero_()
+        else:
+            C.mul_(complex(beta))
+        if complex(gamma) != 0:
+            C.diagonal(dim1=1, dim2=2).add_(complex(gamma))
+        return C
     lda, sA = _mat_strides(A)
     ldb, sB = _mat_strides(B)
     ldc, sC = _mat_strides(C)
@@ -724,6 +816,48 @@ def heom_propagate(eom, y0, nsub, h, precision=QSX_FP64, want_final=False, strea
             h.ctypes.data_as(c_vp), nsave, ptr(out), ptr(yfin), ptr(ws), ptr(status),
             stream_ptr(stream))
     check(st, 'qsx_heom_propagate')
+    return out, yfin, status
+
+
+def heom_propagate_driven(eom, C, F, y0, nsub, h, precision=QSX_FP64, want_final=False,
+                          stream=None):
+    """qsx_heom_propagate under classical fields (dense hierarchy blocks):
+    C [B|1, P, M, M] the dipole commutators, F [B|1, nF, 3, P] the field
+    samples (complex128 device tensors); nsub, h host arrays.  Returns
+    (ADO 0 at every save point [B, nsave, M], final states or None, status)."""
+    lib = require_native()
+    if eom.n_comm != 0:
+        raise ValueError('fields need the dense hierarchy form (several Liouville blocks, '
+                         'M <= 64)')
+    B = eom.Hop.shape[0]
+    M, S = eom.M, eom.n_ado * eom.M
+    assert y0.shape == (B, S) and y0.dtype == torch.complex128
+    assert C.dtype == torch.complex128 and C.is_contiguous() and C.shape[0] in (1, B)
+    assert F.dtype == torch.complex128 and F.is_contiguous() and F.shape[0] in (1, B)
+    P, nF = C.shape[1], F.shape[1]
+    assert C.shape[2:] == (M, M) and F.shape[2:] == (3, P)
+    y0 = y0.contiguous()
+    nsub = np.ascontiguousarray(nsub, dtype=np.int32)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    nsave = int(nsub.size)
+    nbytes = lib.qsx_heom_workspace_bytes(precision, B, eom.n_ado, M)
+    if nbytes < 0:
+        raise NativeError('invalid HEOM workspace request')
+    ws = torch.empty((nbytes + 15) // 16, dtype=torch.complex128, device=y0.device)
+    Hwork = torch.empty((B, M, M), dtype=torch.complex128, device=y0.device)
+    out = torch.empty((B, nsave, M), dtype=torch.complex128, device=y0.device)
+    yfin = torch.empty_like(y0) if want_final else None
+    status = torch.zeros(1, dtype=torch.int32, device=y0.device)
+    n_ado, M_, _, *rest = _heom_ptrs(eom)
+    label = 'heom driven %s B=%d n_ado=%d M=%d P=%d steps=%d' % (
+        'fp64' if precision == QSX_FP64 else 'mixed', B, eom.n_ado, M, P, int(nsub.sum()))
+    with timing.launch(label, stream):
+        st = lib.qsx_heom_propagate_driven(
+            precision, B, n_ado, M_, *rest, P, ptr(C), 0 if C.shape[0] == 1 else P * M * M,
+            ptr(F), 0 if F.shape[0] == 1 else nF * 3 * P, nF, ptr(y0),
+            nsub.ctypes.data_as(c_vp), h.ctypes.data_as(c_vp), nsave, ptr(out), ptr(yfin),
+            ptr(Hwork), ptr(ws), ptr(status), stream_ptr(stream))
+    check(st, 'qsx_heom_propagate_driven')
     return out, yfin, status
 
 

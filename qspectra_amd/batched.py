@@ -49,3 +49,169 @@ def absorption_batch(hamiltonians, time_max, rw_freq, dt, model_cls=RedfieldMode
             out = np.empty((len(hamiltonians), X.shape[-1]))
         out[idx] = X
     return f, out
+
+
+def _group_size(M):
+    """Lanes per problem in qsx_response_buckets: the power of two above M."""
+    G = 2
+    while G < M + 1:
+        G *= 2
+    return G
+
+
+class AbsorptionEnsemblePlan(object):
+    """Absorption spectra of many Hamiltonians of unequal site counts as one
+    captured HIP graph, spectra left on the device.
+
+    Per site count (bucket): the Redfield 'eg' blocks (Heisenberg picture)
+    rebuilt from the members' Hamiltonians (qsx_redfield_block) and their
+    one-step propagators P4(dt L) (qsx_small_propagator).  Then one launch
+    propagates every member of every bucket and records its linear-response
+    signal (qsx_response_buckets), and the Fourier transform of all signals
+    gives the spectra.  The values are those of absorption_batch
+    (absorption_spectra member by member, response.py:103-154) with FP64
+    propagation at the step dt.
+
+        plan = AbsorptionEnsemblePlan(hams, 4096.0, rw_freq=12450.0, dt=1.0)
+        X = plan.run()          # [len(hams), len(plan.f)] device tensor
+        plan.check_status()
+
+    Writing new Hamiltonians into plan.hamiltonian_inputs(n)[0] in place
+    and replaying gives their spectra (the graph rebuilds every operator)."""
+
+    def __init__(self, hamiltonians, time_max, rw_freq, dt, model_cls=RedfieldModel,
+                 model_kwargs=None, polarization='xx', correlation_decay_time=None,
+                 unit_convert=CM_FS, graph=True):
+        from .simulate.response import _op_matrices
+        lib = _native.require_native()
+        max_m = int(lib.qsx_response_buckets_max_states())
+        dev = _native.device()
+        kw = dict(hilbert_subspace='ge', unit_convert=unit_convert)
+        kw.update(model_kwargs or {})
+        groups = {}
+        for i, h in enumerate(hamiltonians):
+            groups.setdefault(h.n_sites, []).append(i)
+        self.t = np.arange(0, time_max, dt)
+        nt = len(self.t)
+        self.dt, self.rw_freq, self.unit_convert = float(dt), float(rw_freq), unit_convert
+        self.decay = correlation_decay_time
+        self.buckets = []
+        order, waves, offs, y0s, vs = [], [], [], [], []
+        offP = offY = 0
+        for n, idx in sorted(groups.items()):
+            hams = [hamiltonians[i].in_rotating_frame(rw_freq) for i in idx]
+            model = model_cls(hams[0], rw_freq=rw_freq, **kw).with_hamiltonians(hams)
+            model.time_step_override = dt
+            inputs, rebuild = model.block_builder('eg', heisenberg_picture=True)
+            B = len(idx)
+            M = n
+            if M > max_m:
+                raise ValueError('AbsorptionEnsemblePlan handles up to %d sites per system '
+                                 '(use absorption_batch for larger ones)' % max_m)
+            rho0 = model.thermal_state('gg')
+            V0 = _op_matrices(model, 'gg->eg', polarization[0], '+', 'commutator')
+            bra = _op_matrices(model, 'eg->gg', polarization[1], '-', 'bra')
+            v = np.einsum('bij,j->bi', V0, rho0)
+            y0s.append(np.broadcast_to(-bra, (B, M)).reshape(-1))
+            vs.append(np.broadcast_to(v, (B, M)).reshape(-1))
+            G = _group_size(M)
+            per = 64 // G
+            first = len(order)
+            for w0 in range(0, B, per):
+                waves.append((first + w0, min(per, B - w0), M, G))
+            for k in range(B):
+                offs.append((offP + k * M * M, offY + k * M))
+            self.buckets.append(dict(n=n, B=B, M=M, inputs=inputs, rebuild=rebuild,
+                                     L=torch.empty((B, M, M), dtype=torch.complex128,
+                                                   device=dev),
+                                     p0=offP, model=model))
+            order.extend(idx)
+            offP += B * M * M
+            offY += B * M
+        self.order = np.asarray(order)
+        self.B = len(order)
+        self.P = torch.empty(offP, dtype=torch.complex128, device=dev)
+        for bk in self.buckets:
+            bk['P'] = self.P[bk['p0']:bk['p0'] + bk['B'] * bk['M'] ** 2].view(
+                bk['B'], bk['M'], bk['M'])
+        self.y0 = _native.to_device(np.concatenate(y0s), dev=dev)
+        self.v = _native.to_device(np.concatenate(vs), dev=dev)
+        self.waves = torch.as_tensor(np.asarray(waves, dtype=np.int32)).to(dev)
+        self.offsets = torch.as_tensor(np.asarray(offs, dtype=np.int64)).to(dev)
+        self.signal = torch.empty((self.B, nt), dtype=torch.complex128, device=dev)
+        self.status = torch.zeros(self.B, dtype=torch.int32, device=dev)
+        inv = np.empty(self.B, dtype=np.int64)
+        inv[self.order] = np.arange(self.B)
+        self.inverse = torch.as_tensor(inv).to(dev)        # input row -> plan row
+        if correlation_decay_time is not None:
+            self._decay = torch.as_tensor(np.exp(-self.t / correlation_decay_time)).to(dev)
+        self._check_tables(waves, offs, offP, offY)
+        # the buckets' builds are independent: largest first, round-robin over
+        # side streams, joined before the propagation launch
+        self._build_order = sorted(range(len(self.buckets)),
+                                   key=lambda k: -self.buckets[k]['B'] * self.buckets[k]['M'] ** 3)
+        self._sides = [torch.cuda.Stream() for _ in range(min(4, len(self.buckets)))]
+        self.f = None
+        self._graph = None
+        self._out = None
+        if graph:
+            self.capture()
+
+    @staticmethod
+    def _check_tables(waves, offs, nP, nY):
+        """Host check that every wavefront's problems and every offset stay
+        inside the packed buffers the kernel reads."""
+        B = len(offs)
+        for first, count, M, G in waves:
+            assert G * count <= 64 and G >= M + 1 and 0 <= first and first + count <= B
+            for b in range(first, first + count):
+                p, y = offs[b]
+                assert 0 <= p and p + M * M <= nP and 0 <= y and y + M <= nY
+
+    def hamiltonian_inputs(self, n):
+        """(H1 [B, n, n], rw [B]) device inputs of the n-site bucket."""
+        for bk in self.buckets:
+            if bk['n'] == n:
+                return bk['inputs']
+        raise KeyError(n)
+
+    def _launch(self):
+        main = torch.cuda.current_stream()
+        for side in self._sides:
+            side.wait_stream(main)
+        for j, k in enumerate(self._build_order):
+            bk = self.buckets[k]
+            with torch.cuda.stream(self._sides[j % len(self._sides)]):
+                bk['rebuild'](bk['L'])
+                _native.small_propagator_into(bk['L'], self.dt, 1, 0, P=bk['P'])
+        for side in self._sides:
+            main.wait_stream(side)
+        _native.response_buckets_into(self.waves, self.offsets, self.P, self.y0, self.v,
+                                      self.signal, self.status)
+        x = self.signal if self.decay is None else self.signal * self._decay
+        f, X = fourier_transform_device(self.t, -x, -1, self.rw_freq, self.unit_convert)
+        self.f = f
+        return torch.index_select(X.real, 0, self.inverse)
+
+    def capture(self):
+        """Records the whole plan into one HIP graph, replayed by run()."""
+        from .timing import capture as _capture
+        self._launch()                      # warm up allocations and the FFT plan
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with _capture(g):
+            self._out = self._launch()
+        self._graph = g
+        return self
+
+    def run(self):
+        """Spectra [len(hamiltonians), len(f)] (device, input order)."""
+        if self._graph is not None:
+            self._graph.replay()
+            return self._out
+        return self._launch()
+
+    def check_status(self):
+        from .simulate.utils import IntegratorError
+        if int(self.status.max().item()) != 0:
+            raise IntegratorError('integration failed: non-finite linear-response signal')

@@ -31,8 +31,17 @@ class Bath(object):
 
 
 class ArbitraryBath(Bath):
-    """A bath given by a spectral density function (host-side only; the GPU
-    Redfield builder supports the Debye form)."""
+    """A bath given by a spectral density function J(x) (bath.py:50-57); only
+    the real correlation function follows from it (detailed balance), so
+    Redfield models use it with discard_imag_corr=True.  The host evaluates
+    it at each member's eigen-energy differences for the GPU builder.
+
+    The reference's constructor assigns spectral_density_limit_at_zero over
+    the base class's read-only property (bath.py:42-47, :55) and so raises
+    AttributeError; here the class attribute below shadows that property, so
+    the value given is stored."""
+
+    spectral_density_limit_at_zero = None
 
     def __init__(self, temperature, spectral_density_func, spectral_density_limit_at_zero):
         self.temperature = temperature
@@ -88,6 +97,12 @@ class DebyeBath(Bath):
         nu = 2 * np.pi * T * np.arange(1, matsubara_cutoff)
         tail = np.sum(nu / ((nu ** 2 - g ** 2) * (nu - 1j * x)))
         return lam * g * ((1 / np.tan(g / (2 * T)) - 1j) / (g - 1j * x) + 4 * T * tail)
+
+
+def is_debye_form(bath):
+    """True for the baths the GPU builder evaluates itself (Debye C(w) with
+    its Matsubara sum, or no coupling)."""
+    return isinstance(bath, (DebyeBath, UncoupledBath))
 
 
 def debye_parameters(bath):

@@ -34,6 +34,8 @@
 
 #include "qsx_common.h"
 
+#include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 namespace qsx {
@@ -201,6 +203,34 @@ __global__ void heom_init_kernel(const double2* __restrict__ y0, double2* __rest
   }
 }
 
+// Classical fields on a dense hierarchy block (simulate_with_fields of a HEOM
+// model, qspectra/simulate/eom.py:79-103 with heom.py:22-58: the dipole
+// commutators act on every ADO): the stage operator of sample b at stage
+// sample sidx of global step g is Hop[b] + sum_i f_i C_i[b], f_i = F[b][g][sidx][i].
+struct HeomFields {
+  int P = 0;                    // field terms (0: no fields)
+  const double2* C = nullptr;   // [B|1][P][M][M] dense commutators (times unit)
+  int64_t strideC = 0;          // P*M*M or 0
+  const double2* F = nullptr;   // [B|1][nF][3][P] -i E(t) (conjugated for '+')
+  int64_t strideF = 0;          // nF*3*P or 0
+  double2* Hwork = nullptr;     // [B][M][M] stage operators
+};
+
+__global__ void heom_field_op_kernel(const double2* __restrict__ Hop,
+                                     const double2* __restrict__ C, int64_t strideC,
+                                     const double2* __restrict__ F, int64_t strideF, int64_t g,
+                                     int sidx, int P, int B, int opsz,
+                                     double2* __restrict__ Hout) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)B * opsz) return;
+  const int64_t b = i / opsz, e = i % opsz;
+  const double2* f = F + b * strideF + (g * 3 + sidx) * P;
+  const double2* Cb = C + b * strideC;
+  double2 v = Hop[i];
+  for (int k = 0; k < P; ++k) v = cadd(v, cmul(f[k], Cb[(int64_t)k * opsz + e]));
+  Hout[i] = v;
+}
+
 // out[b][s][0:M] = ADO 0 of sample b; status on non-finite values
 __global__ void heom_save_kernel(const double2* __restrict__ y, int n_ado, int M, int B, int s,
                                  int nsave, double2* __restrict__ out, int* __restrict__ status) {
@@ -212,46 +242,85 @@ __global__ void heom_save_kernel(const double2* __restrict__ y, int n_ado, int M
   if (!cfinite(v)) status[0] = 1;
 }
 
+// Samples per pass: the whole hierarchy of a pass (y, ynew and the two stage
+// vectors) is kept small enough to stay resident in the 256 MB
+// memory-attached last-level cache while all its steps run, so the stage
+// vectors and the neighbour gathers are served from there instead of HBM;
+// a multiple of the XCD count so that every XCD holds whole samples.
+// QSX_HEOM_SAMPLES_PER_PASS overrides the choice (0 = all samples at once).
+inline int heom_samples_per_pass(int B, int64_t bytes_per_sample) {
+  const char* env = getenv("QSX_HEOM_SAMPLES_PER_PASS");
+  if (env != nullptr) {
+    const int v = atoi(env);
+    return v <= 0 ? B : std::min(B, v);
+  }
+  const int64_t budget = 160ll << 20;
+  int c = (int)std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes_per_sample));
+  if (c >= kXcds) c -= c % kXcds;
+  return std::min(B, c);
+}
+
 template <typename T>
 int run_heom(const HeomArgs& a0, const double2* y0, const int32_t* nsub, const double* h,
              int nsave, double2* out, double2* yfin, void* workspace, int* status,
-             hipStream_t stream) {
+             hipStream_t stream, const HeomFields& fl = HeomFields()) {
   using T2 = cplx_t<T>;
-  HeomArgs a = a0;
-  const int64_t total = (int64_t)a.B * a.n_ado * a.M;
+  const int64_t per = (int64_t)a0.n_ado * a0.M;               // elements per sample
+  const int Bc = heom_samples_per_pass(a0.B, per * (2 * sizeof(double2) + 2 * sizeof(T2)));
+  const int64_t ctotal = (int64_t)Bc * per;
   double2* y = static_cast<double2*>(workspace);
-  double2* ynew = y + total;
-  T2* zA = reinterpret_cast<T2*>(ynew + total);
-  T2* zB = zA + total;
-  const int opsz = heom_opsz(a);
-  const size_t shmem = heom_lds_bytes(opsz, a.ado_per_tile, a.M, a.nbins, sizeof(T2));
+  double2* ynew = y + ctotal;
+  T2* zA = reinterpret_cast<T2*>(ynew + ctotal);
+  T2* zB = zA + ctotal;
+  const int opsz = heom_opsz(a0);
+  const size_t shmem = heom_lds_bytes(opsz, a0.ado_per_tile, a0.M, a0.nbins, sizeof(T2));
   QSX_CHECK_ARG(shmem <= 160 * 1024, "qsx_heom_propagate: %zu bytes of LDS", shmem);
   if (shmem > 64 * 1024)
     QSX_CHECK_HIP(hipFuncSetAttribute((const void*)heom_stage_kernel<T>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-  const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles : a.B * a.tiles;
-  hipLaunchKernelGGL(heom_init_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     stream, y0, y, zA, total);
-  const int sblocks = (a.B * a.M + 255) / 256;
-  for (int s = 0; s < nsave; ++s) {
-    for (int step = 0; step < nsub[s]; ++step) {
-      const T2* zi = zA;
-      T2* zo = zB;
-      for (int st = 0; st < 4; ++st) {
-        hipLaunchKernelGGL(heom_stage_kernel<T>, dim3(ngroups), dim3(kThreads), shmem, stream, a,
-                           zi, zo, y, ynew, h[s], st);
-        const T2* t = zi;
-        zi = zo;
-        zo = const_cast<T2*>(t);
+  static const int kStageSample[4] = {0, 1, 1, 2};   // t, t + h/2, t + h/2, t + h
+  for (int b0 = 0; b0 < a0.B; b0 += Bc) {
+    HeomArgs a = a0;
+    a.B = std::min(Bc, a0.B - b0);
+    a.xcd_map = a.B >= kXcds;
+    a.Hop = a0.Hop + (int64_t)b0 * opsz;
+    const double2* Hbase = a.Hop;
+    if (fl.P > 0) a.Hop = fl.Hwork;
+    const unsigned fblocks = (unsigned)(((int64_t)a.B * opsz + 255) / 256);
+    int64_t g = 0;                                      // global step (row of F)
+    const int64_t total = (int64_t)a.B * per;
+    const int ngroups = a.xcd_map ? kXcds * ((a.B + kXcds - 1) / kXcds) * a.tiles
+                                  : a.B * a.tiles;
+    hipLaunchKernelGGL(heom_init_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       stream, y0 + (int64_t)b0 * per, y, zA, total);
+    const int sblocks = (a.B * a.M + 255) / 256;
+    double2* outc = out + (int64_t)b0 * nsave * a.M;
+    for (int s = 0; s < nsave; ++s) {
+      for (int step = 0; step < nsub[s]; ++step) {
+        const T2* zi = zA;
+        T2* zo = zB;
+        for (int st = 0; st < 4; ++st) {
+          if (fl.P > 0)
+            hipLaunchKernelGGL(heom_field_op_kernel, dim3(fblocks), dim3(256), 0, stream, Hbase,
+                               fl.C + (int64_t)b0 * fl.strideC, fl.strideC,
+                               fl.F + (int64_t)b0 * fl.strideF, fl.strideF, g,
+                               kStageSample[st], fl.P, a.B, opsz, fl.Hwork);
+          hipLaunchKernelGGL(heom_stage_kernel<T>, dim3(ngroups), dim3(kThreads), shmem, stream,
+                             a, zi, zo, y, ynew, h[s], st);
+          const T2* t = zi;
+          zi = zo;
+          zo = const_cast<T2*>(t);
+        }
+        ++g;
       }
+      hipLaunchKernelGGL(heom_save_kernel, dim3(sblocks), dim3(256), 0, stream, y, a.n_ado, a.M,
+                         a.B, s, nsave, outc, status);
     }
-    hipLaunchKernelGGL(heom_save_kernel, dim3(sblocks), dim3(256), 0, stream, y, a.n_ado, a.M,
-                       a.B, s, nsave, out, status);
+    QSX_CHECK_HIP(hipGetLastError());
+    if (yfin != nullptr)
+      QSX_CHECK_HIP(hipMemcpyAsync(yfin + (int64_t)b0 * per, y, total * sizeof(double2),
+                                   hipMemcpyDeviceToDevice, stream));
   }
-  QSX_CHECK_HIP(hipGetLastError());
-  if (yfin != nullptr)
-    QSX_CHECK_HIP(hipMemcpyAsync(yfin, y, total * sizeof(double2), hipMemcpyDeviceToDevice,
-                                 stream));
   return QSX_OK;
 }
 
@@ -354,4 +423,48 @@ extern "C" int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ad
                             (double2*)y_final, workspace, status, st);
   return run_heom<float>(a, (const double2*)y0, nsub_host, h_host, nsave, (double2*)out,
                          (double2*)y_final, workspace, status, st);
+}
+
+extern "C" int32_t qsx_heom_propagate_driven(
+    int32_t precision, int32_t B, int32_t n_ado, int32_t M, const void* Hop, const double* shift,
+    const void* dvec, int32_t nbins, int32_t K1, const int32_t* bin_src, const void* bin_coef,
+    const void* ck, int32_t nrel, const int32_t* rel_site, const double* rel_occ, int32_t P,
+    const void* C, int64_t strideC, const void* F, int64_t strideF, int64_t nF, const void* y0,
+    const int32_t* nsub_host, const double* h_host, int32_t nsave, void* out, void* y_final,
+    void* Hwork, void* workspace, int32_t* status, void* stream) {
+  using namespace qsx;
+  QSX_CHECK_ARG(precision == QSX_FP64 || precision == QSX_MIXED,
+                "qsx_heom_propagate_driven: precision");
+  QSX_CHECK_ARG(y0 && nsub_host && h_host && out && workspace && status && C && F && Hwork,
+                "qsx_heom_propagate_driven: null pointer");
+  QSX_CHECK_ARG(nsave >= 1 && P >= 1 && P <= 4 && nF >= 1,
+                "qsx_heom_propagate_driven: nsave=%d P=%d nF=%lld", nsave, P, (long long)nF);
+  HeomArgs a;
+  int rc = fill_args(a, B, n_ado, M, 0, Hop, shift, dvec, nbins, K1, bin_src, bin_coef, ck, nrel,
+                     rel_site, rel_occ);
+  if (rc) return rc;
+  int64_t steps = 0;
+  for (int s = 0; s < nsave; ++s) {
+    QSX_CHECK_ARG(nsub_host[s] >= 0, "qsx_heom_propagate_driven: negative step count");
+    steps += nsub_host[s];
+  }
+  QSX_CHECK_ARG(steps <= nF, "qsx_heom_propagate_driven: %lld steps but field samples for %lld",
+                (long long)steps, (long long)nF);
+  QSX_CHECK_ARG(strideC == 0 || strideC == (int64_t)P * M * M,
+                "qsx_heom_propagate_driven: strideC must be 0 or P*M*M");
+  QSX_CHECK_ARG(strideF == 0 || strideF == nF * 3 * P,
+                "qsx_heom_propagate_driven: strideF must be 0 or nF*3*P");
+  HeomFields fl;
+  fl.P = P;
+  fl.C = static_cast<const double2*>(C);
+  fl.strideC = strideC;
+  fl.F = static_cast<const double2*>(F);
+  fl.strideF = strideF;
+  fl.Hwork = static_cast<double2*>(Hwork);
+  hipStream_t st = as_stream(stream);
+  if (precision == QSX_FP64)
+    return run_heom<double>(a, (const double2*)y0, nsub_host, h_host, nsave, (double2*)out,
+                            (double2*)y_final, workspace, status, st, fl);
+  return run_heom<float>(a, (const double2*)y0, nsub_host, h_host, nsave, (double2*)out,
+                         (double2*)y_final, workspace, status, st, fl);
 }

@@ -257,11 +257,15 @@ __global__ __launch_bounds__(kThreads) void block_bath_kernel(int n, int X, int 
                                                               double T, double lam,
                                                               double gam, int real_corr,
                                                               double* __restrict__ ws,
-                                                              size_t rstride) {
+                                                              size_t rstride,
+                                                              const double2* __restrict__ ctab) {
   const int b = blockIdx.x, tid = threadIdx.x;
   Rec rec = rec_at(ws + (size_t)b * rstride, nb, n);
+  // a tabulated bath: C[d][c] = C(E_d - E_c) given per problem
+  if (ctab != nullptr)
+    for (int e = tid; e < nb * nb; e += kThreads) rec.C[e] = ctab[(size_t)b * nb * nb + e];
   // pairs (d, c) with d <= c: one Matsubara sum serves C[d][c] and C[c][d]
-  const int npairs = nb * (nb + 1) / 2;
+  const int npairs = ctab != nullptr ? 0 : nb * (nb + 1) / 2;
   for (int e = tid; e < npairs; e += kThreads) {
     int d = 0, r = e;
     while (r >= nb - d) {
@@ -444,6 +448,140 @@ __global__ __launch_bounds__(kThreads) void liouv_rows_kernel(
   for (int q = tid; q < M; q += kThreads) Lrow[q] = w[q];
 }
 
+// ---------------------------------------- 4'. non-secular block, one pass
+// Without the secular mask every term of L_eig factorises over (a,c) x (b,d):
+//   -i(E_a - E_b) d_ac d_bd - R = sum_t A_t[a][c] B_t[b][d],
+//   t = 0:      A = -i E_X - Gs_X,                  B = I
+//   t = 1:      A = I,                              B = i E_Y - conj(Gs_Y)
+//   t = 2+i:    A = K^X_i,                          B[b][d] = K^Y_i[b][d] conj(C_Y[d][b])
+//   t = 2+n+i:  A[a][c] = K^X_i[a][c] C_X[c][a],    B = K^Y_i
+// and the basis change of a Kronecker product factorises too:
+// T (A (x) B) T^T = (U_X A U_X^T) (x) (U_Y B U_Y^T).  So the site-basis block
+// is L[(i,j),(k,l)] = unit sum_t Ah_t[i][k] Bh_t[j][l] with the 2 + 2n small
+// transformed factors held in LDS: one work-group per problem, one pass, no
+// intermediate in global memory.  Thread r owns index pairs (i, k) (or
+// (k, i) for the Heisenberg transpose, so that consecutive threads write
+// consecutive addresses), keeps its 2 + 2n coefficients Ah_t[i][k] in
+// registers and reads Bh_t[j][l] as LDS broadcasts.
+constexpr int kKronMaxSites = 9;
+
+__device__ inline void small_mul(const double* U, const double2* A, double2* out, double2* tmp,
+                                 int nb, int tid) {
+  // out = U A U^T (U real nb x nb, A complex nb x nb; out may alias A)
+  for (int e = tid; e < nb * nb; e += kThreads) {   // tmp[a][k] = sum_c A[a][c] U[k][c]
+    const int a = e / nb, k = e % nb;
+    double2 t = make_double2(0.0, 0.0);
+    for (int c = 0; c < nb; ++c) {
+      const double u = U[k * nb + c];
+      const double2 v = A[a * nb + c];
+      t.x = fma(u, v.x, t.x);
+      t.y = fma(u, v.y, t.y);
+    }
+    tmp[e] = t;
+  }
+  __syncthreads();
+  for (int e = tid; e < nb * nb; e += kThreads) {   // out[i][k] = sum_a U[i][a] tmp[a][k]
+    const int i = e / nb, k = e % nb;
+    double2 t = make_double2(0.0, 0.0);
+    for (int a = 0; a < nb; ++a) {
+      const double u = U[i * nb + a];
+      const double2 v = tmp[a * nb + k];
+      t.x = fma(u, v.x, t.x);
+      t.y = fma(u, v.y, t.y);
+    }
+    out[e] = t;
+  }
+  __syncthreads();
+}
+
+__host__ __device__ inline size_t kron_lds_bytes(int n, int nX, int nY) {
+  const int nt = 2 + 2 * n, big = nX > nY ? nX : nY;
+  return (size_t)nt * (nX * nX + nY * nY) * sizeof(double2) + (size_t)big * big * sizeof(double2) +
+         (size_t)(nX * nX + nY * nY) * sizeof(double);
+}
+
+template <int NS>   // sites: 2 + 2 NS terms
+__global__ __launch_bounds__(kThreads) void liouv_kron_kernel(
+    int nX, int nY, const double* __restrict__ wsX, const double* __restrict__ wsY,
+    size_t rstride, int flags, double unit, double2* __restrict__ L) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int n = NS, nt = 2 + 2 * NS;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int M = nX * nY, nX2 = nX * nX, nY2 = nY * nY;
+  Rec rx = rec_at(const_cast<double*>(wsX) + (size_t)b * rstride, nX, n);
+  Rec ry = rec_at(const_cast<double*>(wsY) + (size_t)b * rstride, nY, n);
+  double2* Ah = reinterpret_cast<double2*>(smem);       // [nt][nX2]
+  double2* Bh = Ah + (size_t)nt * nX2;                   // [nt][nY2]
+  double2* tmp = Bh + (size_t)nt * nY2;                  // [max(nX,nY)^2]
+  double* UX = reinterpret_cast<double*>(tmp + (nX > nY ? nX2 : nY2));
+  double* UY = UX + nX2;
+  for (int e = tid; e < nX2; e += kThreads) UX[e] = rx.U[e];
+  for (int e = tid; e < nY2; e += kThreads) UY[e] = ry.U[e];
+  for (int e = tid; e < nt * nX2; e += kThreads) {
+    const int t = e / nX2, r = e % nX2, a = r / nX, c = r % nX;
+    double2 v;
+    if (t == 0) {
+      const double2 g = rx.Gs[a * nX + c];
+      v = make_double2(-g.x, (a == c ? -rx.E[a] : 0.0) - g.y);
+    } else if (t == 1) {
+      v = make_double2(a == c ? 1.0 : 0.0, 0.0);
+    } else if (t < 2 + n) {
+      v = make_double2(rx.K[((size_t)(t - 2) * nX + a) * nX + c], 0.0);
+    } else {
+      const double k = rx.K[((size_t)(t - 2 - n) * nX + a) * nX + c];
+      const double2 cx = rx.C[c * nX + a];
+      v = make_double2(k * cx.x, k * cx.y);
+    }
+    Ah[e] = v;
+  }
+  for (int e = tid; e < nt * nY2; e += kThreads) {
+    const int t = e / nY2, r = e % nY2, bb = r / nY, d = r % nY;
+    double2 v;
+    if (t == 0) {
+      v = make_double2(bb == d ? 1.0 : 0.0, 0.0);
+    } else if (t == 1) {
+      const double2 g = ry.Gs[bb * nY + d];
+      v = make_double2(-g.x, (bb == d ? ry.E[bb] : 0.0) + g.y);
+    } else if (t < 2 + n) {
+      const double k = ry.K[((size_t)(t - 2) * nY + bb) * nY + d];
+      const double2 cy = ry.C[d * nY + bb];
+      v = make_double2(k * cy.x, -k * cy.y);
+    } else {
+      v = make_double2(ry.K[((size_t)(t - 2 - n) * nY + bb) * nY + d], 0.0);
+    }
+    Bh[e] = v;
+  }
+  __syncthreads();
+  if (!(flags & kEigenBasis)) {
+    // the identity factors (Bh_0, Ah_1) are invariant under U . U^T
+    for (int t = 0; t < nt; ++t) {
+      if (t != 1) small_mul(UX, Ah + (size_t)t * nX2, Ah + (size_t)t * nX2, tmp, nX, tid);
+      if (t != 0) small_mul(UY, Bh + (size_t)t * nY2, Bh + (size_t)t * nY2, tmp, nY, tid);
+    }
+  }
+  const bool heis = flags & kHeisenberg;
+  double2* Lb = L + (size_t)b * M * M;
+  for (int pr = tid; pr < nX2; pr += kThreads) {
+    // Heisenberg: consecutive threads take consecutive i (consecutive
+    // columns of the transposed output row); otherwise consecutive k
+    const int i = heis ? pr % nX : pr / nX, k = heis ? pr / nX : pr % nX;
+    double2 a[nt];
+#pragma unroll
+    for (int t = 0; t < nt; ++t) a[t] = Ah[(size_t)t * nX2 + i * nX + k];
+    for (int j = 0; j < nY; ++j) {
+      for (int l = 0; l < nY; ++l) {
+        double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int t = 0; t < nt; ++t) cfma<double>(v, a[t], Bh[(size_t)t * nY2 + j * nY + l]);
+        const int p = i + nX * j, q = k + nX * l;
+        const double2 out = make_double2(unit * v.x, unit * v.y);
+        if (heis) Lb[(size_t)q * M + p] = out;
+        else Lb[(size_t)p * M + q] = out;
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ unitary
 // L[(i,j),(k,l)] = -i unit (H_X[i][k] d_jl - d_ik H_Y[l][j]) in the rotating
 // frame (qspectra/dynamics/unitary.py:5-9, liouville_space.py:93-131).
@@ -508,6 +646,464 @@ __global__ __launch_bounds__(kThreads) void unitary_dense_kernel(const double* _
   else Lb[e] = val;
 }
 
+// ------------------------------------------------ vector-Jacobian product
+// Reverse mode of a Redfield block with respect to the one-exciton
+// Hamiltonian H1 and the reorganization energy (Debye bath).  With
+// Lu = leig_entry (unit * mask o L_eig, eigen basis) and the site block
+// L = T Lu T^T, T = kron(U_Y, U_X), a cotangent G of L (dl = Re sum conj(G) o dL,
+// torch's convention) pulls back as
+//   W   = T^T G T                        cotangent of Lu (G itself in the eigen basis)
+//   Lam = unit * mask o W                cotangent of -i w - R in leig_entry
+//   U_X^T Ubar_X[a'][a] = Re sum_b (W Lu^H + W^H Lu)[(a',b),(a,b)]  (and Y alike)
+// then through -i(E_a - E_b), Gs_X[a][c] = sum_{i,m} K_i[a][m] K_i[m][c] C[c][m],
+// the Kronecker terms K^X_i[a][c] K^Y_i[b][d] (conj(C_Y[d][b]) + C_X[c][a]),
+// K_i = U^T P_i U (U^T Ubar += K_i (Kbar_i + Kbar_i^T)) and C[d][c] = C(E_d - E_c)
+// (Ebar_d += Re(Cbar[d][c] conj(C'(x))), Ebar_c -= the same), and reaches the
+// block Hamiltonian through the symmetric eigenproblem's VJP
+//   Hbar = U (diag(Ebar) + F o (Om - Om^T) / 2) U^T,  F[a][c] = 1 / (E_c - E_a),
+// Om = U^T Ubar (F = 0 where E_c = E_a); the f block's Hbar is gathered back
+// onto H1 through operator_1_to_2's four terms.
+
+// C(x)/lam and dC/dx of the Debye bath (the x = 0 branch is a constant)
+__device__ void debye_corr_grad(double x, double T, double lam, double gam, int real_corr,
+                                double2& c1, double2& dc) {
+  if (real_corr) {
+    if (x == 0.0) {
+      c1 = make_double2(T * 2.0 / gam, 0.0);
+      dc = make_double2(0.0, 0.0);
+      return;
+    }
+    const double den = gam * gam + x * x;
+    const double J1 = 2.0 * gam * x / den;                       // J / lam
+    const double dJ1 = 2.0 * gam * (gam * gam - x * x) / (den * den);
+    const double nb = 1.0 / expm1(x / T);
+    c1 = make_double2((nb + 1.0) * J1, 0.0);
+    dc = make_double2(lam * ((nb + 1.0) * dJ1 - nb * (nb + 1.0) * J1 / T), 0.0);
+    return;
+  }
+  if (x == 0.0) {
+    c1 = make_double2(2.0 * T / gam, -1.0);
+    dc = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2 S = matsubara_sum(x, T, gam);
+  const double ct = 1.0 / tan(gam / (2.0 * T));
+  const double den = gam * gam + x * x;
+  c1 = make_double2(gam * ((ct * gam + x) / den + 4.0 * T * S.x),
+                    gam * ((ct * x - gam) / den + 4.0 * T * S.y));
+  // d/dx (ct - i)/(g - i x) = (1 + i ct)(g^2 - x^2 + 2 i g x) / (g^2 + x^2)^2
+  const double pr = gam * gam - x * x, pi = 2.0 * gam * x, d2 = den * den;
+  double dr = (pr - ct * pi) / d2, di = (ct * pr + pi) / d2;
+  // dS/dx = sum_k nu/(nu^2 - g^2) (-2 nu x + i (nu^2 - x^2)) / (nu^2 + x^2)^2
+  double sr = 0.0, si = 0.0;
+  const double twopiT = 2.0 * M_PI * T;
+  for (int k = 1; k < 1000; ++k) {
+    const double nu = twopiT * k;
+    const double q = nu * nu + x * x;
+    const double f = nu / ((nu * nu - gam * gam) * q * q);
+    sr = fma(f, -2.0 * nu * x, sr);
+    si = fma(f, nu * nu - x * x, si);
+  }
+  dr += 4.0 * T * sr;
+  di += 4.0 * T * si;
+  dc = make_double2(lam * gam * dr, lam * gam * di);
+}
+
+// cotangent of L_eig's bracket: Lam(p, q) = unit * mask * W(p, q)
+__device__ inline double2 vjp_w(const double2* Wb, const double2* Gb, int M, int flags, int p,
+                                int q) {
+  if (flags & kEigenBasis)
+    return (flags & kHeisenberg) ? Gb[(size_t)q * M + p] : Gb[(size_t)p * M + q];
+  return Wb[(size_t)p * M + q];
+}
+__device__ inline double2 vjp_lam(const double2* Wb, const double2* Gb, int M, int nX,
+                                  int same, int flags, double unit, int a, int bb, int c,
+                                  int d) {
+  if ((flags & kSecular) && !((a == c && bb == d) || (same && a == bb && c == d)))
+    return make_double2(0.0, 0.0);
+  const double2 w = vjp_w(Wb, Gb, M, flags, a + nX * bb, c + nX * d);
+  return make_double2(unit * w.x, unit * w.y);
+}
+
+// T^T applied to column q of G (the caller's block transposed back for the
+// Heisenberg picture): row q of W1^T, W1 = T^T G
+__global__ __launch_bounds__(kThreads) void vjp_cols_kernel(
+    int n, int nX, int nY, const double* __restrict__ wsX, const double* __restrict__ wsY,
+    size_t rstride, int flags, const double2* __restrict__ G, double2* __restrict__ W1T) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int q = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int M = nX * nY;
+  Rec rx = rec_at(const_cast<double*>(wsX) + (size_t)b * rstride, nX, n);
+  Rec ry = rec_at(const_cast<double*>(wsY) + (size_t)b * rstride, nY, n);
+  double2* z = reinterpret_cast<double2*>(smem);
+  double2* tmp = z + M;
+  double2* w = tmp + M;
+  double* UXt = reinterpret_cast<double*>(w + M);
+  double* UYt = UXt + nX * nX;
+  const double2* Gb = G + (size_t)b * M * M;
+  const bool heis = flags & kHeisenberg;
+  for (int p = tid; p < M; p += kThreads)
+    z[p] = heis ? Gb[(size_t)q * M + p] : Gb[(size_t)p * M + q];
+  for (int e = tid; e < nX * nX; e += kThreads) UXt[e] = rx.U[(e % nX) * nX + e / nX];
+  for (int e = tid; e < nY * nY; e += kThreads) UYt[e] = ry.U[(e % nY) * nY + e / nY];
+  __syncthreads();
+  sandwich(z, tmp, w, UXt, UYt, nX, nY, tid);
+  double2* row = W1T + (size_t)b * M * M + (size_t)q * M;
+  for (int p = tid; p < M; p += kThreads) row[p] = w[p];
+}
+
+// row p of W = W1 T: T^T applied to row p of W1 (column p of W1^T)
+__global__ __launch_bounds__(kThreads) void vjp_rows_kernel(
+    int n, int nX, int nY, const double* __restrict__ wsX, const double* __restrict__ wsY,
+    size_t rstride, const double2* __restrict__ W1T, double2* __restrict__ W) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int M = nX * nY;
+  Rec rx = rec_at(const_cast<double*>(wsX) + (size_t)b * rstride, nX, n);
+  Rec ry = rec_at(const_cast<double*>(wsY) + (size_t)b * rstride, nY, n);
+  double2* z = reinterpret_cast<double2*>(smem);
+  double2* tmp = z + M;
+  double2* w = tmp + M;
+  double* UXt = reinterpret_cast<double*>(w + M);
+  double* UYt = UXt + nX * nX;
+  const double2* src = W1T + (size_t)b * M * M;
+  for (int q = tid; q < M; q += kThreads) z[q] = src[(size_t)q * M + p];
+  for (int e = tid; e < nX * nX; e += kThreads) UXt[e] = rx.U[(e % nX) * nX + e / nX];
+  for (int e = tid; e < nY * nY; e += kThreads) UYt[e] = ry.U[(e % nY) * nY + e / nY];
+  __syncthreads();
+  sandwich(z, tmp, w, UXt, UYt, nX, nY, tid);
+  double2* row = W + (size_t)b * M * M + (size_t)p * M;
+  for (int q = tid; q < M; q += kThreads) row[q] = w[q];
+}
+
+// per-problem reductions of Lam (complex):
+//   dX[nX]  dY[nY]                 sum over the other index of Lam's diagonal
+//   GX[nX2] GY[nY2]                cotangents of Gs_X, Gs_Y
+//   OX[nX2] OY[nY2]                U^T Ubar from the basis change (real part)
+//   V[n][nX2] V2[n][nX2]           sum_{b,d} Lam K^Y_i[b][d] (x C_Y[d][b])
+//   Z[n][nY2] Z2[n][nY2]           sum_{a,c} Lam K^X_i[a][c] (x conj(C_X[c][a]))
+__host__ __device__ inline size_t vjp_red_count(int n, int nX, int nY) {
+  const size_t s2 = (size_t)nX * nX + (size_t)nY * nY;
+  return (size_t)nX + nY + 2 * s2 + 2 * (size_t)n * s2;
+}
+
+__global__ __launch_bounds__(kThreads) void vjp_reduce_kernel(
+    int n, int nX, int nY, int same, int flags, double unit, const double* __restrict__ wsX,
+    const double* __restrict__ wsY, size_t rstride, const double2* __restrict__ G,
+    const double2* __restrict__ W, const double2* __restrict__ Lu, double2* __restrict__ red,
+    size_t rcount) {
+  const int b = blockIdx.y;
+  const size_t e0 = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (e0 >= rcount) return;
+  const int M = nX * nY, nX2 = nX * nX, nY2 = nY * nY;
+  Rec rx = rec_at(const_cast<double*>(wsX) + (size_t)b * rstride, nX, n);
+  Rec ry = rec_at(const_cast<double*>(wsY) + (size_t)b * rstride, nY, n);
+  const double2* Wb = W + (size_t)b * M * M;
+  const double2* Gb = G + (size_t)b * M * M;
+  const double2* Lb = Lu + (size_t)b * M * M;
+  auto lam_at = [&](int a, int bb, int c, int d) {
+    return vjp_lam(Wb, Gb, M, nX, same, flags, unit, a, bb, c, d);
+  };
+  size_t e = e0;
+  double2 acc = make_double2(0.0, 0.0);
+  if (e < (size_t)nX) {
+    const int a = (int)e;
+    for (int bb = 0; bb < nY; ++bb) acc = cadd(acc, lam_at(a, bb, a, bb));
+  } else if ((e -= nX) < (size_t)nY) {
+    const int bb = (int)e;
+    for (int a = 0; a < nX; ++a) acc = cadd(acc, lam_at(a, bb, a, bb));
+  } else if ((e -= nY) < (size_t)nX2) {
+    const int a = (int)e / nX, c = (int)e % nX;
+    for (int bb = 0; bb < nY; ++bb) acc = cadd(acc, lam_at(a, bb, c, bb));
+    acc = make_double2(-acc.x, -acc.y);
+  } else if ((e -= nX2) < (size_t)nY2) {
+    const int bb = (int)e / nY, d = (int)e % nY;
+    for (int a = 0; a < nX; ++a) acc = cadd(acc, lam_at(a, bb, a, d));
+    acc = make_double2(-acc.x, acc.y);
+  } else if ((e -= nY2) < (size_t)(nX2 + nY2)) {
+    // Re sum (W(p', q) conj(Lu(p, q)) + conj(W(q, p')) Lu(q, p)) over the
+    // other block's index and q (site basis only)
+    if (!(flags & kEigenBasis)) {
+      const bool xs = e < (size_t)nX2;
+      const int r = xs ? (int)e / nX : (int)(e - nX2) / nY;
+      const int s = xs ? (int)e % nX : (int)(e - nX2) % nY;
+      const int nothers = xs ? nY : nX;
+      double v = 0.0;
+      for (int o = 0; o < nothers; ++o) {
+        const int pp = xs ? r + nX * o : o + nX * r;
+        const int p = xs ? s + nX * o : o + nX * s;
+        for (int q = 0; q < M; ++q) {
+          const double2 w1 = Wb[(size_t)pp * M + q], l1 = Lb[(size_t)p * M + q];
+          const double2 w2 = Wb[(size_t)q * M + pp], l2 = Lb[(size_t)q * M + p];
+          v += w1.x * l1.x + w1.y * l1.y + w2.x * l2.x + w2.y * l2.y;
+        }
+      }
+      acc = make_double2(v, 0.0);
+    }
+  } else if ((e -= nX2 + nY2) < (size_t)2 * n * nX2) {
+    const bool withc = e >= (size_t)n * nX2;
+    const size_t f = withc ? e - (size_t)n * nX2 : e;
+    const int i = (int)(f / nX2), a = (int)(f % nX2) / nX, c = (int)(f % nX2) % nX;
+    const double* Ky = ry.K + (size_t)i * nY2;
+    for (int bb = 0; bb < nY; ++bb)
+      for (int d = 0; d < nY; ++d) {
+        const double k = Ky[bb * nY + d];
+        if (k == 0.0) continue;
+        double2 l = lam_at(a, bb, c, d);
+        l = make_double2(k * l.x, k * l.y);
+        acc = cadd(acc, withc ? cmul(l, ry.C[d * nY + bb]) : l);
+      }
+  } else {
+    e -= (size_t)2 * n * nX2;
+    const bool withc = e >= (size_t)n * nY2;
+    const size_t f = withc ? e - (size_t)n * nY2 : e;
+    const int i = (int)(f / nY2), bb = (int)(f % nY2) / nY, d = (int)(f % nY2) % nY;
+    const double* Kx = rx.K + (size_t)i * nX2;
+    for (int a = 0; a < nX; ++a)
+      for (int c = 0; c < nX; ++c) {
+        const double k = Kx[a * nX + c];
+        if (k == 0.0) continue;
+        double2 l = lam_at(a, bb, c, d);
+        l = make_double2(k * l.x, k * l.y);
+        if (withc) {
+          const double2 cx = rx.C[c * nX + a];
+          l = cmul(l, make_double2(cx.x, -cx.y));
+        }
+        acc = cadd(acc, l);
+      }
+  }
+  red[(size_t)b * rcount + e0] = acc;
+}
+
+// per-(problem, Hilbert block) slab of the chain (doubles):
+//   Cbar dC C1 Gsbar (complex nb^2 each) | Om Kbar Mid Tmp Hb (nb^2) | Ebar (nb) | glam
+__host__ __device__ inline size_t vjp_slab_doubles(int nb) {
+  return (13 * (size_t)nb * nb + nb + 1 + 1) & ~(size_t)1;
+}
+
+// one work-group per (problem, block k): the block's Hbar and its share of dl/dlam
+__global__ __launch_bounds__(kThreads) void vjp_chain_kernel(
+    int n, int nX, int nY, int same, double T, double lam, double gam, int flags,
+    const double* __restrict__ wsX, const double* __restrict__ wsY, size_t rstride,
+    const double2* __restrict__ red, size_t rcount, double* __restrict__ chain,
+    size_t cstride) {
+  __shared__ double part[kThreads];
+  const int b = blockIdx.x, k = blockIdx.y, tid = threadIdx.x;
+  const int nb = k == 0 ? nX : nY, nb2 = nb * nb;
+  const int nX2 = nX * nX, nY2 = nY * nY;
+  const bool isx = k == 0 || same, isy = k == 1 || same;
+  Rec r = rec_at(const_cast<double*>(k == 0 ? wsX : wsY) + (size_t)b * rstride, nb, n);
+  double* slab = chain + ((size_t)b * 2 + k) * cstride;
+  double2* Cbar = reinterpret_cast<double2*>(slab);
+  double2* dC = Cbar + nb2;
+  double2* C1 = dC + nb2;
+  double2* Gsb = C1 + nb2;
+  double* Om = reinterpret_cast<double*>(Gsb + nb2);
+  double* Kbar = Om + nb2;
+  double* Mid = Kbar + nb2;
+  double* Tmp = Mid + nb2;
+  double* Hb = Tmp + nb2;
+  double* Ebar = Hb + nb2;
+  double* glam = Ebar + nb;
+  const double2* rb = red + (size_t)b * rcount;
+  const double2* dX = rb;
+  const double2* dY = dX + nX;
+  const double2* GX = dY + nY;
+  const double2* GY = GX + nX2;
+  const double2* OX = GY + nY2;
+  const double2* OY = OX + nX2;
+  const double2* V = OY + nY2;
+  const double2* V2 = V + (size_t)n * nX2;
+  const double2* Z = V2 + (size_t)n * nX2;
+  const double2* Z2 = Z + (size_t)n * nY2;
+  const int real_corr = (flags & kRealCorr) != 0;
+  // phase 0: eigenvalue cotangents of -i w, Gs cotangent, basis-change Om,
+  // C(x)/lam and C'(x) (pairs d <= c: S(-x) = conj S(x), S'(-x) = -conj S'(x)
+  // hold for the Debye sums, but the closed-form part is not symmetric, so
+  // each ordered pair is evaluated)
+  for (int a = tid; a < nb; a += kThreads)
+    Ebar[a] = (isx ? -dX[a].y : 0.0) + (isy ? dY[a].y : 0.0);
+  for (int e = tid; e < nb2; e += kThreads) {
+    double2 g = make_double2(0.0, 0.0);
+    if (isx) g = cadd(g, GX[e]);
+    if (isy) g = cadd(g, GY[e]);
+    Gsb[e] = g;
+    Om[e] = (isx ? OX[e].x : 0.0) + (isy ? OY[e].x : 0.0);
+    const int d = e / nb, c = e % nb;
+    double2 c1, dc;
+    debye_corr_grad(r.E[d] - r.E[c], T, lam, gam, real_corr, c1, dc);
+    C1[e] = c1;
+    dC[e] = dc;
+  }
+  __syncthreads();
+  // phase 1: Cbar[d][c] (index [r][s]) from the Kronecker terms and Gs
+  for (int e = tid; e < nb2; e += kThreads) {
+    const int rr = e / nb, ss = e % nb;
+    double2 cb = make_double2(0.0, 0.0);
+    for (int i = 0; i < n; ++i) {
+      const double* Ki = r.K + (size_t)i * nb2;
+      const double kk = Ki[ss * nb + rr];
+      if (isx) {
+        const double2 v = V[(size_t)i * nX2 + ss * nb + rr];
+        cb.x += kk * v.x;
+        cb.y += kk * v.y;
+      }
+      if (isy) {
+        const double2 z = Z[(size_t)i * nY2 + ss * nb + rr];
+        cb.x += kk * z.x;
+        cb.y -= kk * z.y;
+      }
+      // Gs: Cbar[c][m] += K_i[m][c] sum_a K_i[m][a] Gsbar[a][c]
+      double2 t = make_double2(0.0, 0.0);
+      for (int a = 0; a < nb; ++a) {
+        const double km = Ki[ss * nb + a];
+        const double2 g = Gsb[a * nb + rr];
+        t.x = fma(km, g.x, t.x);
+        t.y = fma(km, g.y, t.y);
+      }
+      cb.x += kk * t.x;
+      cb.y += kk * t.y;
+    }
+    Cbar[e] = cb;
+  }
+  __syncthreads();
+  // phase 2: per site, Kbar_i, then Om += K_i (Kbar_i + Kbar_i^T)
+  for (int i = 0; i < n; ++i) {
+    const double* Ki = r.K + (size_t)i * nb2;
+    for (int e = tid; e < nb2; e += kThreads) {
+      const int rr = e / nb, ss = e % nb;
+      const double2 csr = r.C[ss * nb + rr];
+      double kb = 0.0;
+      if (isx) {
+        const double2 v = V[(size_t)i * nX2 + e], v2 = V2[(size_t)i * nX2 + e];
+        kb += csr.x * v.x + csr.y * v.y + v2.x;            // Re(conj(C[s][r]) V + V2)
+      }
+      if (isy) {
+        const double2 z = Z[(size_t)i * nY2 + e], z2 = Z2[(size_t)i * nY2 + e];
+        kb += csr.x * z.x - csr.y * z.y + z2.x;            // Re(C[s][r] Z + Z2)
+      }
+      // Gs: Re sum_c Gsbar[r][c] K_i[s][c] conj(C[c][s])
+      //   + Re conj(C[s][r]) sum_a Gsbar[a][s] K_i[a][r]
+      double t1 = 0.0;
+      double2 t2 = make_double2(0.0, 0.0);
+      for (int c = 0; c < nb; ++c) {
+        const double2 g = Gsb[rr * nb + c];
+        const double2 cc = r.C[c * nb + ss];
+        const double kc = Ki[ss * nb + c];
+        t1 += kc * (g.x * cc.x + g.y * cc.y);
+        const double2 g2 = Gsb[c * nb + ss];
+        const double ka = Ki[c * nb + rr];
+        t2.x = fma(ka, g2.x, t2.x);
+        t2.y = fma(ka, g2.y, t2.y);
+      }
+      kb += t1 + csr.x * t2.x + csr.y * t2.y;
+      Kbar[e] = kb;
+    }
+    __syncthreads();
+    for (int e = tid; e < nb2; e += kThreads) {
+      const int rr = e / nb, ss = e % nb;
+      double o = 0.0;
+      for (int t = 0; t < nb; ++t) o += Ki[rr * nb + t] * (Kbar[t * nb + ss] + Kbar[ss * nb + t]);
+      Om[e] += o;
+    }
+    __syncthreads();
+  }
+  // phase 3: eigenvalue cotangents through C(E_d - E_c), and dl/dlam
+  double gl = 0.0;
+  for (int e = tid; e < nb2; e += kThreads) {
+    const double2 cb = Cbar[e], c1 = C1[e];
+    gl += cb.x * c1.x + cb.y * c1.y;
+  }
+  part[tid] = gl;
+  for (int d = tid; d < nb; d += kThreads) {
+    double eb = 0.0;
+    for (int c = 0; c < nb; ++c) {
+      if (c == d) continue;
+      const double2 a1 = Cbar[d * nb + c], g1 = dC[d * nb + c];
+      const double2 a2 = Cbar[c * nb + d], g2 = dC[c * nb + d];
+      eb += (a1.x * g1.x + a1.y * g1.y) - (a2.x * g2.x + a2.y * g2.y);
+    }
+    Ebar[d] += eb;
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (tid < w) part[tid] += part[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) glam[0] = part[0];
+  // phase 4: Hbar = U (diag(Ebar) + F o (Om - Om^T)/2) U^T
+  for (int e = tid; e < nb2; e += kThreads) {
+    const int a = e / nb, c = e % nb;
+    double v;
+    if (a == c) {
+      v = Ebar[a];
+    } else {
+      const double gap = r.E[c] - r.E[a];
+      v = gap != 0.0 ? 0.5 * (Om[e] - Om[c * nb + a]) / gap : 0.0;
+    }
+    Mid[e] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < nb2; e += kThreads) {
+    const int p = e / nb, c = e % nb;
+    double v = 0.0;
+    for (int a = 0; a < nb; ++a) v += r.U[p * nb + a] * Mid[a * nb + c];
+    Tmp[e] = v;
+  }
+  __syncthreads();
+  for (int e = tid; e < nb2; e += kThreads) {
+    const int p = e / nb, q = e % nb;
+    double v = 0.0;
+    for (int c = 0; c < nb; ++c) v += Tmp[p * nb + c] * r.U[q * nb + c];
+    Hb[e] = v;
+  }
+}
+
+// Hbar of the blocks onto H1: e block directly, f block through the four
+// terms of operator_1_to_2 (H_f[(i,j),(k,l)] = d_ik H[j][l] + d_il H[j][k] +
+// d_jk H[i][l] + d_jl H[i][k]); g has no H1 dependence
+__device__ double block_hbar_to_h1(const double* Hb, int n, int X, int p, int q) {
+  if (X == 1) return Hb[p * n + q];
+  if (X != 2) return 0.0;
+  const int nf = n * (n - 1) / 2;
+  auto pair_index = [n](int lo, int hi) { return lo * n - lo * (lo + 1) / 2 + (hi - lo - 1); };
+  double v = 0.0;
+  // H[p][q] appears in H_f[(p,m),(q,m)] for every site m other than p and q
+  // (and, for p = q, on the diagonal H_f[(p,m),(p,m)])
+  for (int m = 0; m < n; ++m) {
+    if (m == p || m == q) continue;
+    const int s = pair_index(min(p, m), max(p, m)), t = pair_index(min(q, m), max(q, m));
+    v += Hb[(size_t)s * nf + t];
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void vjp_map_kernel(
+    int n, int X, int Y, int nX, int nY, const double* __restrict__ chain, size_t cstride,
+    double* __restrict__ gH1, double* __restrict__ glam) {
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  const int nk = X == Y ? 1 : 2;
+  if (e < n * n) {
+    const int p = e / n, q = e % n;
+    double v = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      const int nb = k == 0 ? nX : nY;
+      const double* Hb = chain + ((size_t)b * 2 + k) * cstride + 12 * (size_t)nb * nb;
+      v += block_hbar_to_h1(Hb, n, k == 0 ? X : Y, p, q);
+    }
+    gH1[(size_t)b * n * n + e] = v;
+  }
+  if (e == 0 && glam != nullptr) {
+    double g = 0.0;
+    for (int k = 0; k < nk; ++k) {
+      const int nb = k == 0 ? nX : nY;
+      g += chain[((size_t)b * 2 + k) * cstride + 13 * (size_t)nb * nb + nb];
+    }
+    glam[b] = g;
+  }
+}
+
 size_t ws_doubles_dense(int n, int nX, int nY, int B) {
   const size_t r = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
   const size_t M = (size_t)nX * nY;
@@ -561,6 +1157,11 @@ extern "C" int32_t qsx_block_eigh(const double* H1, int32_t n, int32_t B, int32_
   return QSX_OK;
 }
 
+static int redfield_block_impl(const double* H1, int n, int B, const double* rw, double T,
+                               double lam, double gam, int flags, double unit_convert, int X,
+                               int Y, const void* ctabX, const void* ctabY, void* L,
+                               void* workspace, hipStream_t s);
+
 extern "C" int32_t qsx_redfield_block(const double* H1, int32_t n, int32_t B, const double* rw,
                                       double T, double lam, double gam, int32_t flags,
                                       double unit_convert, int32_t X, int32_t Y, void* L,
@@ -571,22 +1172,66 @@ extern "C" int32_t qsx_redfield_block(const double* H1, int32_t n, int32_t B, co
   QSX_CHECK_ARG((flags & kRealCorr) ? T > 0 : (T > 0 && gam > 0),
                 "qsx_redfield_block: bath needs T > 0 and cutoff > 0");
   QSX_CHECK_ARG(gam > 0, "qsx_redfield_block: cutoff frequency must be > 0");
+  return redfield_block_impl(H1, n, B, rw, T, lam, gam, flags, unit_convert, X, Y, nullptr,
+                             nullptr, L, workspace, as_stream(stream));
+}
+
+extern "C" int32_t qsx_redfield_block_tabulated(const double* H1, int32_t n, int32_t B,
+                                                const double* rw, const void* ctabX,
+                                                const void* ctabY, int32_t flags,
+                                                double unit_convert, int32_t X, int32_t Y,
+                                                void* L, void* workspace, void* stream) {
+  QSX_CHECK_ARG(H1 && L && workspace && ctabX && ctabY,
+                "qsx_redfield_block_tabulated: null pointer");
+  int st = check_block_args(n, X, Y, B);
+  if (st) return st;
+  return redfield_block_impl(H1, n, B, rw, 1.0, 0.0, 1.0, flags, unit_convert, X, Y, ctabX,
+                             ctabY, L, workspace, as_stream(stream));
+}
+
+static int redfield_block_impl(const double* H1, int n, int B, const double* rw, double T,
+                               double lam, double gam, int flags, double unit_convert, int X,
+                               int Y, const void* ctabX, const void* ctabY, void* L,
+                               void* workspace, hipStream_t s) {
   const int nX = block_size(X, n), nY = block_size(Y, n), M = nX * nY;
   const size_t rs = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
   double* wsX = static_cast<double*>(workspace);
   double* wsY = (X == Y) ? wsX : wsX + rs * B;
   double2* W = reinterpret_cast<double2*>(wsX + 2 * rs * B);
-  hipStream_t s = as_stream(stream);
   const int blocks[2] = {X, Y};
   double* recs[2] = {wsX, wsY};
+  const double2* ctabs[2] = {static_cast<const double2*>(ctabX),
+                             static_cast<const double2*>(ctabY)};
   for (int k = 0; k < (X == Y ? 1 : 2); ++k) {
     const int nb = block_size(blocks[k], n);
     hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, H1, n, blocks[k], rw,
                        (const double*)nullptr, 0, recs[k], rs);
     hipLaunchKernelGGL(block_bath_kernel, dim3(B), dim3(kThreads), 0, s, n, blocks[k], nb,
                        (const double*)nullptr, T, lam, gam, (int)((flags & kRealCorr) != 0),
-                       recs[k], rs);
+                       recs[k], rs, ctabs[k]);
     hipLaunchKernelGGL(block_gs_kernel, dim3(B), dim3(kThreads), 0, s, n, nb, recs[k], rs);
+  }
+  // non-secular blocks whose transformed factors fit the LDS: one pass
+  const size_t kron = kron_lds_bytes(n, nX, nY);
+  if (!(flags & kSecular) && n <= kKronMaxSites && kron <= 160 * 1024) {
+#define QSX_KRON(NS_)                                                                          \
+  case NS_: {                                                                                  \
+    auto kern = liouv_kron_kernel<NS_>;                                                        \
+    if (kron > 64 * 1024)                                                                      \
+      QSX_CHECK_HIP(hipFuncSetAttribute((const void*)kern,                                     \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kron)); \
+    hipLaunchKernelGGL(kern, dim3(B), dim3(kThreads), kron, s, nX, nY, wsX, wsY, rs, flags,    \
+                       unit_convert, (double2*)L);                                             \
+    break;                                                                                     \
+  }
+    switch (n) {
+      QSX_KRON(1) QSX_KRON(2) QSX_KRON(3) QSX_KRON(4) QSX_KRON(5) QSX_KRON(6) QSX_KRON(7)
+      QSX_KRON(8) QSX_KRON(9)
+      default: break;
+    }
+#undef QSX_KRON
+    QSX_CHECK_HIP(hipGetLastError());
+    return QSX_OK;
   }
   const size_t shmem = 3 * (size_t)M * sizeof(double2) + (size_t)(nX * nX + nY * nY) * sizeof(double);
   QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_redfield_block: block too large (M=%d)", M);
@@ -665,7 +1310,8 @@ extern "C" int32_t qsx_redfield_block_dense(const double* HX, const double* HY, 
     hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, (const double*)nullptr,
                        n, 0, (const double*)nullptr, Hs[k], nbs[k], recs[k], rs);
     hipLaunchKernelGGL(block_bath_kernel, dim3(B), dim3(kThreads), 0, s, n, 0, nbs[k], occs[k],
-                       T, lam, gam, (int)((flags & kRealCorr) != 0), recs[k], rs);
+                       T, lam, gam, (int)((flags & kRealCorr) != 0), recs[k], rs,
+                       (const double2*)nullptr);
     hipLaunchKernelGGL(block_gs_kernel, dim3(B), dim3(kThreads), 0, s, n, nbs[k], recs[k], rs);
   }
   const size_t shmem = 3 * (size_t)M * sizeof(double2) + (size_t)(nX * nX + nY * nY) * sizeof(double);
@@ -689,6 +1335,80 @@ extern "C" int32_t qsx_unitary_block_dense(const double* HX, const double* HY, i
   const int64_t blocks = ((int64_t)M * M + kThreads - 1) / kThreads;
   hipLaunchKernelGGL(unitary_dense_kernel, dim3((unsigned)blocks, B), dim3(kThreads), 0,
                      as_stream(stream), HX, HY, nX, nY, unit_convert, heisenberg, (double2*)L);
+  QSX_CHECK_HIP(hipGetLastError());
+  return QSX_OK;
+}
+
+// -------------------------------------------------------------------- VJP
+static size_t vjp_ws_doubles(int n, int X, int Y, int B) {
+  const int nX = block_size(X, n), nY = block_size(Y, n);
+  const size_t rs = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
+  const size_t M = (size_t)nX * nY;
+  return 2 * rs * B + 3 * 2 * M * M * B + 2 * vjp_red_count(n, nX, nY) * B +
+         2 * vjp_slab_doubles(std::max(nX, nY)) * B;
+}
+
+extern "C" int64_t qsx_redfield_vjp_workspace_bytes(int32_t n, int32_t X, int32_t Y,
+                                                    int32_t B) {
+  if (n < 1 || B < 1 || X < 0 || X > 2 || Y < 0 || Y > 2) return -1;
+  return (int64_t)(vjp_ws_doubles(n, X, Y, B) * sizeof(double));
+}
+
+extern "C" int32_t qsx_redfield_block_vjp(const double* H1, int32_t n, int32_t B,
+                                          const double* rw, double T, double lam, double gam,
+                                          int32_t flags, double unit_convert, int32_t X,
+                                          int32_t Y, const void* gL, double* gH1, double* glam,
+                                          void* workspace, void* stream) {
+  QSX_CHECK_ARG(H1 && gL && gH1 && workspace, "qsx_redfield_block_vjp: null pointer");
+  int st = check_block_args(n, X, Y, B);
+  if (st) return st;
+  QSX_CHECK_ARG(T > 0 && gam > 0, "qsx_redfield_block_vjp: bath needs T > 0 and cutoff > 0");
+  const int nX = block_size(X, n), nY = block_size(Y, n), M = nX * nY;
+  const int same = X == Y;
+  const size_t rs = std::max(rec_doubles(nX, n), rec_doubles(nY, n));
+  const size_t shmem = 3 * (size_t)M * sizeof(double2) + (size_t)(nX * nX + nY * nY) * sizeof(double);
+  QSX_CHECK_ARG(shmem <= 64 * 1024, "qsx_redfield_block_vjp: block too large (M=%d)", M);
+  hipStream_t s = as_stream(stream);
+  double* wsX = static_cast<double*>(workspace);
+  double* wsY = same ? wsX : wsX + rs * B;
+  double2* W1T = reinterpret_cast<double2*>(wsX + 2 * rs * B);
+  double2* W = W1T + (size_t)M * M * B;
+  double2* Lu = W + (size_t)M * M * B;
+  double2* red = Lu + (size_t)M * M * B;
+  const size_t rcount = vjp_red_count(n, nX, nY);
+  double* chain = reinterpret_cast<double*>(red + rcount * B);
+  const size_t cstride = vjp_slab_doubles(std::max(nX, nY));
+  const int blocks[2] = {X, Y};
+  double* recs[2] = {wsX, wsY};
+  for (int k = 0; k < (same ? 1 : 2); ++k) {
+    const int nb = block_size(blocks[k], n);
+    hipLaunchKernelGGL(block_eigh_kernel, dim3(B), dim3(kThreads), 0, s, H1, n, blocks[k], rw,
+                       (const double*)nullptr, 0, recs[k], rs);
+    hipLaunchKernelGGL(block_bath_kernel, dim3(B), dim3(kThreads), 0, s, n, blocks[k], nb,
+                       (const double*)nullptr, T, lam, gam, (int)((flags & kRealCorr) != 0),
+                       recs[k], rs, (const double2*)nullptr);
+    hipLaunchKernelGGL(block_gs_kernel, dim3(B), dim3(kThreads), 0, s, n, nb, recs[k], rs);
+  }
+  const double2* G = static_cast<const double2*>(gL);
+  if (!(flags & kEigenBasis)) {
+    hipLaunchKernelGGL(vjp_cols_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY, wsX,
+                       wsY, rs, flags, G, W1T);
+    hipLaunchKernelGGL(vjp_rows_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY, wsX,
+                       wsY, rs, (const double2*)W1T, W);
+    // the eigen-basis block itself (Schroedinger picture) for the basis-change term
+    hipLaunchKernelGGL(liouv_cols_kernel, dim3(M, B), dim3(kThreads), shmem, s, n, nX, nY, same,
+                       wsX, wsY, rs, (flags | kEigenBasis) & ~kHeisenberg, unit_convert,
+                       (double2*)nullptr, Lu);
+  }
+  const unsigned rblocks = (unsigned)((rcount + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(vjp_reduce_kernel, dim3(rblocks, B), dim3(kThreads), 0, s, n, nX, nY, same,
+                     flags, unit_convert, wsX, wsY, rs, G, (const double2*)W, (const double2*)Lu,
+                     red, rcount);
+  hipLaunchKernelGGL(vjp_chain_kernel, dim3(B, same ? 1 : 2), dim3(kThreads), 0, s, n, nX, nY,
+                     same, T, lam, gam, flags, wsX, wsY, rs, (const double2*)red, rcount, chain,
+                     cstride);
+  hipLaunchKernelGGL(vjp_map_kernel, dim3((n * n + kThreads - 1) / kThreads, B), dim3(kThreads),
+                     0, s, n, X, Y, nX, nY, (const double*)chain, cstride, gH1, glam);
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }

@@ -5,7 +5,7 @@ to the site basis, for a whole batch of Hamiltonians per launch."""
 import numpy as np
 
 from .. import _native
-from ..bath import debye_parameters
+from ..bath import debye_parameters, is_debye_form
 from .liouville_space import LiouvilleSpaceModel, all_liouville_subspaces
 
 
@@ -31,11 +31,43 @@ class RedfieldModel(LiouvilleSpaceModel):
                 | (_native.QSX_HEISENBERG if heisenberg else 0))
 
     def _build_block(self, X, Y, heisenberg, H1, rw):
-        T, lam, gam = debye_parameters(self.hamiltonian.bath)
+        bath = self.hamiltonian.bath
+        if not is_debye_form(bath):
+            return self._build_tabulated_block(X, Y, heisenberg, H1, rw)
+        T, lam, gam = debye_parameters(bath)
         return _native.redfield_block(H1, rw, T, lam, gam, X, Y, self._flags(heisenberg),
                                       self.unit_convert)
 
+    def _correlation_function(self):
+        """The bath function redfield_tensor evaluates (redfield.py:48-53):
+        the real one for discard_imag_corr, else the complex one (a bath
+        without it raises AttributeError, as in the reference)."""
+        bath = self.hamiltonian.bath
+        return bath.corr_func_real if self.discard_imag_corr else bath.corr_func_complex
+
+    def _build_tabulated_block(self, X, Y, heisenberg, H1, rw):
+        """A bath given by Python functions (ArbitraryBath, user baths): the
+        eigen-energies come from the GPU eigensolver, the host evaluates
+        C(E_d - E_c) for each member's pairs, the rest of the build runs on
+        the GPU (qsx_redfield_block_tabulated)."""
+        corr = self._correlation_function()
+        tabs = []
+        for blk in (X, Y):
+            E, _ = _native.block_eigh(H1, blk, rw)
+            E = E.cpu().numpy()
+            tab = np.empty(E.shape + E.shape[-1:], dtype=complex)
+            for b in range(E.shape[0]):
+                for d in range(E.shape[1]):
+                    for c in range(E.shape[1]):
+                        tab[b, d, c] = corr(E[b, d] - E[b, c])
+            tabs.append(_native.to_device(tab))
+        return _native.redfield_block_tabulated(H1, rw, tabs[0], tabs[1], X, Y,
+                                                self._flags(heisenberg), self.unit_convert)
+
     def _block_rebuilder(self, X, Y, heisenberg, inputs):
+        if not is_debye_form(self.hamiltonian.bath):
+            raise NotImplementedError('in-graph rebuilds need the GPU form of the bath '
+                                      '(DebyeBath or UncoupledBath)')
         T, lam, gam = debye_parameters(self.hamiltonian.bath)
         H1, rw = inputs
         ws = _native.redfield_workspace(H1.shape[-1], X, Y, H1.shape[0], H1.device)

@@ -19,8 +19,9 @@ qsx_propagate); the reverse pass uses
     for the time propagation,
   * the closed-form vector-Jacobian product of the Horner polynomial P4 (on
     qsx_zgemm),
-  * torch's autograd through a torch restatement of the Redfield block
-    (eigh, C(w), Redfield tensor, basis change) for dL/dtheta.
+  * the hand-written reverse mode of the Redfield block
+    (qsx_redfield_block_vjp: basis change, Redfield tensor, C(w) and the
+    symmetric eigenproblem) for dL/dtheta.
 """
 import numpy as np
 import torch
@@ -33,8 +34,6 @@ from .dynamics.redfield import RedfieldModel
 from .simulate.response import _op_matrices
 from .simulate.utils import fourier_transform_device
 from .utils import check_random_state
-
-_BLOCKS = {'g': 0, 'e': 1}
 
 
 # --------------------------------------------------------- time propagation
@@ -120,90 +119,39 @@ def propagate(A, y0, nsub, ckpt=64):
     return states
 
 
-# ---------------------------------------------------- Redfield block (torch)
-def debye_correlation(x, lam, gamma, T, real=False, matsubara_cutoff=1000):
-    """Debye bath correlation C(x) on a real tensor x (bath.py:70-102): the
-    complex form with the reference's truncated Matsubara sum (the k = 0
-    term is zero) or, for real=True, (1/expm1(x/T) + 1) J(x) with
-    J(x) = 2 lam gamma x / (gamma^2 + x^2); C(0) = lam (2T/gamma - i) or
-    2 lam T / gamma.  Differentiable in x and lam."""
-    zero = x == 0
-    xs = torch.where(zero, torch.ones_like(x), x)
-    if real:
-        J = 2 * lam * gamma * xs / (gamma ** 2 + xs ** 2)
-        val = (1 / torch.expm1(xs / T) + 1) * J
-        c0 = 2 * lam * T / gamma
-        return torch.where(zero, c0 * torch.ones_like(x), val).to(torch.complex128)
-    nu = 2 * np.pi * T * torch.arange(1, matsubara_cutoff, dtype=torch.float64, device=x.device)
-    a = nu / (nu ** 2 - gamma ** 2)
-    xc = xs.to(torch.complex128)[..., None]
-    msum = (a / (nu - 1j * xc)).sum(-1)
-    cot = 1.0 / np.tan(gamma / (2 * T))
-    val = lam * gamma * ((cot - 1j) / (gamma - 1j * xs.to(torch.complex128)) + 4 * T * msum)
-    c0 = lam * (2 * T / gamma - 1j)
-    return torch.where(zero, c0 * torch.ones_like(val), val)
+# ------------------------------------------------------------ Redfield block
+class _RedfieldBlock(torch.autograd.Function):
+    """Redfield block XY of one-exciton Hamiltonians H1 [B, n, n] (float64)
+    and a Debye bath: values from qsx_redfield_block, cotangents of H1 and of
+    the reorganization energy from its hand-written reverse mode
+    (qsx_redfield_block_vjp: basis change, Redfield tensor, C(w) and the
+    symmetric eigenproblem's VJP on the GPU)."""
+
+    @staticmethod
+    def forward(ctx, H1, lam, rw, T, gamma, X, Y, flags, unit_convert):
+        H1d = H1.detach().contiguous()
+        lam_v = float(lam)
+        ctx.args = (rw, T, lam_v, gamma, X, Y, flags, unit_convert)
+        ctx.lam_is_tensor = torch.is_tensor(lam)
+        ctx.save_for_backward(H1d)
+        return _native.redfield_block(H1d, rw, T, lam_v, gamma, X, Y, flags, unit_convert)
+
+    @staticmethod
+    def backward(ctx, gL):
+        (H1,) = ctx.saved_tensors
+        rw, T, lam, gamma, X, Y, flags, unit_convert = ctx.args
+        gH1, glam = _native.redfield_block_vjp(H1, rw, T, lam, gamma, X, Y, flags, unit_convert,
+                                               gL.contiguous())
+        g_lam = glam.sum() if ctx.lam_is_tensor else None
+        return gH1, g_lam, None, None, None, None, None, None, None
 
 
-def _block_eig(H1, X, rw):
-    """(E rotating frame [B, nX], U [B, nX, nX], number operators' K [B, N, nX, nX])
-    for Hilbert block X ('g' or 'e') of the one-exciton Hamiltonians H1."""
-    B, N, _ = H1.shape
-    if X == 'g':
-        z = torch.zeros((B, 1), dtype=torch.float64, device=H1.device)
-        return z, torch.ones((B, 1, 1), dtype=torch.float64, device=H1.device), \
-            torch.zeros((B, N, 1, 1), dtype=torch.float64, device=H1.device)
-    E, U = torch.linalg.eigh(H1)
-    K = U[:, :, None, :] * U[:, :, :, None]          # K_i[a][c] = U[i,a] U[i,c]
-    return E - rw, U, K
-
-
-def redfield_block(H1, lam, gamma, T, rw, X, Y, secular=False, discard_imag_corr=False,
-                   unit_convert=CM_FS):
-    """Redfield Liouvillian block XY (X, Y in 'g', 'e') in the site basis for
-    one-exciton Hamiltonians H1 [B, N, N] (lab frame, float64 tensor):
-    L[(a + nX b), (c + nX d)] of -i[H, .] - R (redfield.py:9-104,
-    liouville_space.py:68-131), times unit_convert.  Differentiable in H1
-    and lam (torch autograd)."""
-    EX, UX, KX = _block_eig(H1, X, rw)
-    EY, UY, KY = (EX, UX, KX) if Y == X else _block_eig(H1, Y, rw)
-
-    def corr(E):
-        return debye_correlation(E[:, :, None] - E[:, None, :], lam, gamma, T,
-                                 real=discard_imag_corr)           # C[c][a] = C(E_c - E_a)
-
-    def gs(K, C):
-        # Gs[a][c] = sum_i sum_b K_i[a][b] K_i[b][c] C[c][b]
-        Kc = K.to(torch.complex128)
-        return torch.einsum('niab,nibc,ncb->nac', Kc, Kc, C)
-
-    CX = corr(EX)
-    CY = CX if Y == X else corr(EY)
-    GX, GY = gs(KX, CX), (gs(KX, CX) if Y == X else gs(KY, CY))
-    nX, nY = EX.shape[1], EY.shape[1]
-    dev = H1.device
-    IX = torch.eye(nX, dtype=torch.complex128, device=dev)
-    IY = torch.eye(nY, dtype=torch.complex128, device=dev)
-    KXc, KYc = KX.to(torch.complex128), KY.to(torch.complex128)
-    R = (torch.einsum('bd,nac->nabcd', IY, GX)
-         + torch.einsum('ac,nbd->nabcd', IX, GY.conj())
-         - torch.einsum('niac,nibd,ndb->nabcd', KXc, KYc, CY.conj())
-         - torch.einsum('niac,nibd,nca->nabcd', KXc, KYc, CX))
-    w = (EX[:, :, None] - EY[:, None, :]).to(torch.complex128)       # [n, a, b]
-    L = -1j * torch.einsum('nab,ac,bd->nabcd', w, IX, IY) - R
-    if secular:
-        ia = torch.arange(nX, device=dev)
-        ib = torch.arange(nY, device=dev)
-        keep = (ia[:, None, None, None] == ia[None, None, :, None]) & \
-               (ib[None, :, None, None] == ib[None, None, None, :])
-        if X == Y:
-            keep = keep | ((ia[:, None, None, None] == ib[None, :, None, None])
-                           & (ia[None, None, :, None] == ib[None, None, None, :]))
-        L = L * keep
-    UXc, UYc = UX.to(torch.complex128), UY.to(torch.complex128)
-    L = torch.einsum('npa,nqb,nabcd,nrc,nsd->npqrs', UXc, UYc, L, UXc, UYc)
-    B = L.shape[0]
-    L = L.permute(0, 2, 1, 4, 3).reshape(B, nX * nY, nX * nY)
-    return L * unit_convert
+def redfield_block(H1, lam, gamma, T, rw, X, Y, flags, unit_convert=CM_FS):
+    """Differentiable Redfield block (see _RedfieldBlock); rw [B] float64
+    tensor, X, Y Hilbert block indices (0 g, 1 e, 2 f), flags the QSX_* bits
+    of qsx_redfield_block."""
+    return _RedfieldBlock.apply(H1, lam, rw, float(T), float(gamma), int(X), int(Y),
+                                int(flags), float(unit_convert))
 
 
 # ------------------------------------------------------------ spectra
@@ -282,21 +230,14 @@ class RedfieldAbsorption(object):
         return H[None] + torch.diag_embed(torch.as_tensor(self.shifts).to(H.device))
 
     def liouvillian(self, H1, lam):
-        """Heisenberg 'eg' blocks [B, M, M]: values from qsx_redfield_block,
-        derivatives from the torch restatement."""
+        """Heisenberg 'eg' blocks [B, M, M] (qsx_redfield_block; reverse mode
+        qsx_redfield_block_vjp)."""
         flags = ((_native.QSX_SECULAR if self.secular else 0)
                  | (_native.QSX_REAL_CORR if self.discard_imag_corr else 0)
                  | _native.QSX_HEISENBERG)
         rw = torch.full((H1.shape[0],), self.rw_freq, dtype=torch.float64, device=H1.device)
-        lam_v = float(lam.detach()) if torch.is_tensor(lam) else float(lam)
-        L_dev = _native.redfield_block(H1.detach().contiguous(), rw, self.T, lam_v,
-                                       self.gamma, 1, 0, flags, self.unit_convert)
-        if not (H1.requires_grad or (torch.is_tensor(lam) and lam.requires_grad)):
-            return L_dev
-        L_t = redfield_block(H1, lam, self.gamma, self.T, self.rw_freq, 'e', 'g',
-                             self.secular, self.discard_imag_corr,
-                             self.unit_convert).transpose(-1, -2)
-        return L_dev + (L_t - L_t.detach())
+        return redfield_block(H1.contiguous(), lam, self.gamma, self.T, rw, 1, 0, flags,
+                              self.unit_convert)
 
     def signal(self, E, J, lam):
         """Ensemble-mean linear response x(t) [len(t)] (complex128)."""

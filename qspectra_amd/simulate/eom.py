@@ -122,6 +122,112 @@ def _field_step(L, C, terms, rw_freq, span, rtol):
     return (120.0 * rtol / (rho * span)) ** 0.25 / rho
 
 
+def _heom_with_fields(model, eom, pulses, geometry, polarization, time_extra, times,
+                      liouville_subspace, save_func, dt, precision, rtol):
+    """simulate_with_fields of a HEOM model (eom.py:79-103): the dipole
+    commutators act on every ADO (heom.py:22-58), the hierarchy in its dense
+    form (several Liouville blocks, M <= 64); (t, full hierarchy states
+    [B, len(t), n_ado * M]) as the reference's integrate returns them."""
+    from .response import _members, _op_matrices
+    from .utils import calibrate_step, heom_rate
+    pairs = list(zip(polarization, geometry))
+    terms = list(zip(pulses, geometry))[:len(pairs)]
+    pairs = pairs[:len(terms)]
+    if not terms:
+        raise ValueError('at least one pulse is needed')
+    if len(terms) > _native.MAX_FIELD_TERMS:
+        raise ValueError('at most %d field terms are supported' % _native.MAX_FIELD_TERMS)
+    hams = _members(model)
+    B, M = len(hams), eom.M
+    dev = eom.Hop.device
+    mats = [_op_matrices(model, liouville_subspace, pol, trans, 'commutator')
+            for pol, trans in pairs]                                   # each [B|1, M, M]
+    Bc = max(m.shape[0] for m in mats)
+    C = np.stack([np.broadcast_to(m, (Bc, M, M)) for m in mats], axis=1)   # [B|1, P, M, M]
+    Cd = _native.to_device(np.ascontiguousarray(C), dev=dev)
+    rho0 = (model.thermal_states(liouville_subspace) if hasattr(model, 'thermal_states')
+            else np.asarray(model.thermal_state(liouville_subspace))[None])
+    y0 = _native.to_device(np.ascontiguousarray(np.broadcast_to(rho0, (B, eom.size))), dev=dev)
+    if eom.n_comm != 0:
+        # one Liouville block: a dipole commutator maps it out of itself, so
+        # its restriction (the field term) vanishes and the hierarchy evolves
+        # freely, as the reference's integrate of eom + 0 does
+        if np.any(C != 0):
+            raise NotImplementedError('field terms inside one Liouville block')
+        t0 = min(p.t_init for p in pulses)
+        tf = max(p.t_final for p in pulses)
+        t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
+             else tf + np.asarray(times, dtype=float))
+        t = np.atleast_1d(np.asarray(t, dtype=float))
+        out = propagate_heom(eom, y0, t, t0, dt, precision, full_states=True, rtol=rtol)
+        if save_func is not None:
+            arr = out.cpu().numpy()
+            return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in arr])
+        return t, out
+    t0 = min(p.t_init for p in pulses)
+    tf = max(p.t_final for p in pulses)
+    t = (np.arange(t0, tf + time_extra, model.time_step) if times is None
+         else tf + np.asarray(times, dtype=float))
+    t = np.atleast_1d(np.asarray(t, dtype=float))
+    if np.any(t < t0):
+        raise ValueError('output times must not precede the first pulse (t=%g)' % t0)
+    rw_freq = model.rw_freq
+    mode = _native.QSX_FP64 if precision in ('fp64', 'double', None) else None
+    if precision == 'mixed':
+        mode = _native.QSX_MIXED
+    if mode is None:
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+
+    def run(y, tp, t_start, h):
+        ni, hi = rk4_schedule(tp, t_start, h)
+        tg, hg = stage_times(tp, t_start, ni, hi)
+        if int(ni.sum()) == 0:
+            return y, 0
+        F = _native.to_device(field_samples(terms, rw_freq, tg, hg)[None], dev=dev)
+        _, yf, st = _native.heom_propagate_driven(eom, Cd, F, y, ni, hi, mode, want_final=True)
+        return yf, int(st.item())
+
+    if dt is None:
+        span = float(t[-1]) - float(t0)
+        rho = heom_rate(eom)
+        field = 0.0
+        for i, (pulse, _) in enumerate(terms):
+            amp, r = field_rate(pulse, rw_freq)
+            rho += amp * float(np.abs(C[:, i]).sum(axis=-1).max())
+            field = max(field, r)
+        rho += field
+        if rho > 0 and span > 0:
+            h_safe = (120.0 * rtol / (rho * span)) ** 0.25 / rho
+
+            def probe(H, n_p):
+                ends = []
+                for hp in (H, H / 2):
+                    yf, st = run(y0, np.array([t0 + n_p * H]), t0, hp)
+                    if st != 0:
+                        return None
+                    ends.append(yf)
+                return ends
+            dt = calibrate_step(probe, h_safe, 2.0 / rho, span, rtol,
+                                window=float(tf) - float(t0))
+        gaps = np.diff(np.concatenate([[t0], t]))
+        gaps = gaps[gaps > 0]
+        if gaps.size:
+            dt = gaps.min() if dt is None else min(dt, float(gaps.min()))
+    states, y, start = [], y0, float(t0)
+    for tk in t:
+        y, st = run(y, np.array([tk]), start, dt)
+        if st != 0:
+            raise IntegratorError('integration failed: non-finite state between t=%g and t=%g'
+                                  % (start, tk))
+        states.append(y)
+        start = float(tk)
+    out = torch.stack(states, dim=1)                                   # [B, nt, S]
+    if save_func is not None:
+        arr = out.cpu().numpy()
+        return t, np.stack([np.asarray([save_func(v) for v in ob]) for ob in arr])
+    return t, out
+
+
 def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_extra, times,
                           liouville_subspace, save_func, **integrate_kwargs):
     """(t, states [B, len(t), ...]) for a (possibly batched) model; the
@@ -136,8 +242,8 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
         raise TypeError('unexpected integrate options: %s' % sorted(unknown))
     eom = model.equation_of_motion(liouville_subspace)
     if isinstance(eom, DeviceHEOM):
-        raise NotImplementedError('field-driven propagation of hierarchy (HEOM) models is not '
-                                  'implemented; use a Redfield, unitary or ZOFE model')
+        return _heom_with_fields(model, eom, pulses, geometry, polarization, time_extra, times,
+                                 liouville_subspace, save_func, dt, precision, rtol)
     if isinstance(eom, DeviceZOFE):
         return _zofe_with_fields(model, eom, pulses, geometry, polarization, time_extra, times,
                                  save_func, dt, rtol)
@@ -226,7 +332,7 @@ def _simulate_with_fields(dynamical_model, pulses, geometry, polarization, time_
     if isinstance(save_func, LinearMap):
         mat = np.asarray(save_func.matrix)
         proj = _native.to_device(np.atleast_2d(mat), dev=dev)
-        saved = torch.matmul(states, proj.transpose(0, 1).contiguous())
+        saved = _native.zgemm(states.contiguous(), proj[None], transb=True)
         return t, saved[..., 0] if mat.ndim == 1 else saved
     if save_func is not None:
         states = states.cpu().numpy()

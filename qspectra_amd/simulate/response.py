@@ -173,7 +173,7 @@ def _batched_linear_response(dynamical_model, liouv_space_path, time_max, initia
             ado0 = propagate_heom(eom, _dev(y0), t, dt=kw.get('dt'),
                                   precision=kw.get('precision', 'fp64'),
                                   rtol=kw.get('rtol', 1e-6))               # [B, nt, M1]
-            out = torch.matmul(ado0, _expand(proj, B).transpose(1, 2).contiguous())
+            out = _native.zgemm(ado0.contiguous(), proj, transb=True)    # [B, nt, K]
             s = out.permute(0, 2, 1).reshape((B,) + lead + (len(t),))
         else:
             y0 = _expand(_dev(-bra[:, None, :]), B)                      # [B, 1, M1]
@@ -355,10 +355,9 @@ def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
             G3 = propagate(dense[2], _expand(_dev(bh[:, None, :]), B), t3, **kw)[:, 0]
         else:
             def per_ado(x, Vm):          # x [B, ..., n_ado*M] -> [B, ..., n_ado*Mto]
-                Vt = _expand(_dev(Vm), B).transpose(1, 2).contiguous()    # [B, M, Mto]
                 lead = x.shape[1:-1]
                 xs = x.reshape(B, -1, Vm.shape[-1])
-                return torch.matmul(xs, Vt).reshape((B,) + lead + (-1,))
+                return _native.zgemm(xs, _dev(Vm), transb=True).reshape((B,) + lead + (-1,))
             M0 = V[0].shape[-1]
             y0 = torch.zeros((B, eoms[0].size), dtype=torch.complex128, device=_native.device())
             y0[:, :V[0].shape[1]] = _expand(_dev(np.einsum('bij,j->bi', V[0], rho0[:M0])), B)
@@ -373,7 +372,7 @@ def _heom_pathways(model, t1, t2, t3, paths, geometry, polarization, kw):
             G3 = propagate_heom(eoms[2], g0, t3, dt=dt, precision=prec, full_states=True,
                                 rtol=rtol)
         S3 = G3.shape[-1]
-        prod = torch.matmul(V_rho2.reshape(B, -1, S3), G3.transpose(1, 2).contiguous())
+        prod = _native.zgemm(V_rho2.reshape(B, -1, S3), G3, transb=True)
         prod = prod.reshape(B, len(t1), len(t2), len(t3))
         total = prod if total is None else total + prod
     return (t1, t2, t3), total
@@ -441,8 +440,7 @@ def _pathway_by_pathway(model, coherence_time_max, population_time_max, populati
         G3 = propagate(eom3, _expand(_dev(b3[:, None, :]), B), t3, **kw)      # [B,1,n3,M3]
         # signal[b, a, c, d] = sum_i G3[b, d, i] * V_rho2[b, a, c, i]
         M3 = G3.shape[-1]
-        prod = torch.matmul(V_rho2.reshape(B, -1, M3),
-                            G3[:, 0].transpose(1, 2).contiguous())           # [B, n1*n2, n3]
+        prod = _native.zgemm(V_rho2.reshape(B, -1, M3), G3[:, 0], transb=True)   # [B, n1*n2, n3]
         prod = prod.reshape(B, len(t1), len(t2), len(t3))
         total = prod if total is None else total + prod
     return (t1, t2, t3), total

@@ -5,6 +5,7 @@ Run only in the build container, where the reference sources are present:
     python tests/golden/make_golden.py /root/reference            # every file
     python tests/golden/make_golden.py /root/reference fields     # fields.npz only
     python tests/golden/make_golden.py /root/reference heom_spectra
+    python tests/golden/make_golden.py /root/reference heom_fields
     python tests/golden/make_golden.py /root/reference zofe
     python tests/golden/make_golden.py /root/reference vibronic
 
@@ -18,6 +19,7 @@ known-answer test pins are written:
 * zvode-integrated trajectories and end-to-end spectra (absorption,
   third-order response, 2-D spectra, isotropic and ensemble averages),
 * linear and third-order responses of HEOM models (dimer, 'gef'),
+* field-driven HEOM trajectories (pump, two pulses; full hierarchy states),
 * ZOFE (pseudomode) dynamics, absorption, pump and third-order outputs,
 * vibronic Hamiltonians: Redfield and unitary Liouvillian blocks,
   dynamics, absorption and third-order GSB responses,
@@ -122,6 +124,28 @@ def write_heom_spectra(qs, HEOMModel, dim, extra):
             out['t1'], out['t2'], out['t3'] = t1, t2, t3
     out['time_step'] = hm.time_step
     np.savez_compressed(os.path.join(HERE, 'heom_spectra.npz'), **out)
+
+
+def write_heom_fields(qs, HEOMModel, dim, extra):
+    """heom_fields.npz: field-driven HEOM trajectories (simulate_pump and
+    simulate_with_fields of a HEOMModel: the dipole commutators act on every
+    ADO, eom.py:79-103 with heom.py:22-58); full hierarchy states."""
+    out = {}
+    ham = qs.ElectronicHamiltonian(dim['H1'], bath=qs.DebyeBath(dim['T'], dim['lam'],
+                                                              dim['gamma']),
+                                   dipoles=dim['dipoles'], energy_spread_extra=extra)
+    hm = HEOMModel(ham, hilbert_subspace='gef', unit_convert=qs.CM_FS, level_cutoff=3, K=1)
+    pump = qs.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=qs.CM_FS)
+    t, y = qs.simulate_pump(hm, pump, 'x', time_extra=30., **TIGHT)
+    out['pump_t'], out['pump_y'] = t, y
+    p1 = qs.GaussianPulse(12700., 15., t_peak=-10., scale=0.015, freq_convert=qs.CM_FS)
+    p2 = qs.GaussianPulse(12900., 25., t_peak=5., scale=0.01 * np.exp(0.6j),
+                          freq_convert=qs.CM_FS, t_limits_multiple=2.5)
+    t, y = qs.simulate_with_fields(hm, [p1, p2], '-+', 'xy', times=[0., 4., 10.], **TIGHT)
+    out['two_t'], out['two_y'] = t, y
+    out['ado_count'] = hm.ado_count
+    out['time_step'] = hm.time_step
+    np.savez_compressed(os.path.join(HERE, 'heom_fields.npz'), **out)
 
 
 PM_OMEGA = [-500., -200., -90., 1., 21., 60., 80., 130., 200., 300., 400., 500., 600., 800.,
@@ -327,6 +351,12 @@ def main(ref_path, only=None):
         write_heom_spectra(qs, HEOMModel, dim, extra_for_dt(qs, dim['H1'], 1.0))
         write_manifest(ref_path)
         return
+    if only == 'heom_fields':
+        from qspectra.dynamics.heom import HEOMModel
+        dim = dimer_inputs()
+        write_heom_fields(qs, HEOMModel, dim, extra_for_dt(qs, dim['H1'], 1.0))
+        write_manifest(ref_path)
+        return
     if only == 'fields':
         dim, agg = dimer_inputs(), aggregate7_inputs()
         ham = ham_of(dim, extra_for_dt(qs, dim['H1'], 1.0))
@@ -480,6 +510,7 @@ def main(ref_path, only=None):
 
     write_fields(qs, ham, ham7)
     write_heom_spectra(qs, HEOMModel, dim, extra)
+    write_heom_fields(qs, HEOMModel, dim, extra_for_dt(qs, dim['H1'], 1.0))
     write_zofe(qs, agg)
     write_vibronic(qs)
     write_manifest(ref_path)

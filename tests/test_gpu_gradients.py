@@ -212,3 +212,96 @@ def test_absorption_gradient_matches_oracle_differences(gpu):
     lm = ((oracle_spectrum(H, BATH[1] - h) - target) ** 2).sum()
     fd = (lp - lm) / (2 * h)
     assert abs(float(lam.grad) - fd) <= 1e-5 * abs(fd)
+
+
+def _sym_h(n, B, seed):
+    rng = np.random.RandomState(seed)
+    H = rng.normal(0.0, 60.0, (B, n, n))
+    H = (H + H.transpose(0, 2, 1)) / 2
+    for b in range(B):
+        H[b][np.diag_indices(n)] = rng.uniform(12200, 12700, n)
+    return H
+
+
+def _flags(secular=False, real=False, heis=False, eigen=False):
+    return ((_native.QSX_SECULAR if secular else 0) | (_native.QSX_REAL_CORR if real else 0)
+            | (_native.QSX_HEISENBERG if heis else 0) | (_native.QSX_EIGEN_BASIS if eigen else 0))
+
+
+@pytest.mark.parametrize('X,Y', [(1, 0), (0, 1), (1, 1), (0, 0)])
+@pytest.mark.parametrize('secular,real,heis', [(False, False, False), (True, False, True),
+                                               (False, True, True), (True, True, False)])
+def test_redfield_vjp_matches_torch_autograd(gpu, X, Y, secular, real, heis):
+    """qsx_redfield_block_vjp against torch's autograd through the torch
+    restatement of the block (eigh, C(w), tensor, basis change) on the CPU:
+    the symmetric part of dl/dH1 and dl/dlam, 1e-9 relative."""
+    from tests import torch_redfield as TR
+    n, B = 5, 3
+    H = _sym_h(n, B, 7 + X + 2 * Y)
+    rw = np.array([12450.0, 12380.0, 12510.0])
+    rng = np.random.RandomState(11)
+    flags = _flags(secular, real, heis)
+    Hd = torch.as_tensor(H).cuda()
+    rwd = torch.as_tensor(rw).cuda()
+    L = _native.redfield_block(Hd, rwd, BATH[0], BATH[1], BATH[2], X, Y, flags, S.CM_FS)
+    gL = torch.as_tensor(_cplx(rng, *L.shape)).cuda()
+    gH, glam = _native.redfield_block_vjp(Hd, rwd, BATH[0], BATH[1], BATH[2], X, Y, flags,
+                                          S.CM_FS, gL)
+    gH, glam = gH.cpu().numpy(), glam.cpu().numpy()
+    for b in range(B):
+        Ht = torch.tensor(H[b:b + 1], requires_grad=True)
+        lam = torch.tensor(BATH[1], dtype=torch.float64, requires_grad=True)
+        Lt = TR.redfield_block(Ht, lam, BATH[2], BATH[0], rw[b], 'ge'[X], 'ge'[Y], secular,
+                               real, S.CM_FS)
+        if heis:
+            Lt = Lt.transpose(-1, -2)
+        assert rel_err(L[b].cpu().numpy(), Lt[0].detach().numpy()) < 1e-12
+        loss = (torch.conj(gL[b:b + 1].cpu()) * Lt).real.sum()
+        loss.backward()
+        # the g block does not depend on H1 (and 'gg' on nothing)
+        ref = Ht.grad[0].numpy() if Ht.grad is not None else np.zeros((n, n))
+        ref = (ref + ref.T) / 2
+        ref_lam = float(lam.grad) if lam.grad is not None else 0.0
+        scale = max(np.abs(ref).max(), 1e-300)
+        if X or Y:
+            assert np.abs(gH[b] - ref).max() / scale < 1e-9, (b, np.abs(gH[b] - ref).max(), scale)
+        else:
+            assert np.abs(gH[b]).max() == 0.0
+        assert abs(glam[b] - ref_lam) <= 1e-9 * abs(ref_lam), b
+
+
+@pytest.mark.parametrize('X,Y,secular,heis,eigen', [
+    (2, 1, False, True, False), (2, 1, True, False, False), (1, 2, False, False, True),
+    (2, 2, False, False, False), (2, 2, True, True, False), (1, 1, False, False, True),
+    (2, 0, False, True, False), (1, 0, False, False, True)])
+def test_redfield_vjp_matches_finite_differences(gpu, X, Y, secular, heis, eigen):
+    """Blocks with the two-exciton space, the secular mask and the eigen
+    basis: dl/dH1 along symmetric directions and dl/dlam against central
+    differences (h = 1e-3 cm^-1, 1e-4 for lam) of the GPU forward pass."""
+    n, B = 4, 2
+    H = _sym_h(n, B, 3 + X + 3 * Y)
+    rwd = torch.as_tensor(np.array([12450.0, 12400.0])).cuda()
+    flags = _flags(secular, False, heis, eigen)
+    rng = np.random.RandomState(5)
+
+    def block(Hn, lam=BATH[1]):
+        return _native.redfield_block(torch.as_tensor(Hn).cuda(), rwd, BATH[0], lam, BATH[2],
+                                      X, Y, flags, S.CM_FS)
+    L0 = block(H)
+    gL = torch.as_tensor(_cplx(rng, *L0.shape)).cuda()
+    gH, glam = _native.redfield_block_vjp(torch.as_tensor(H).cuda(), rwd, BATH[0], BATH[1],
+                                          BATH[2], X, Y, flags, S.CM_FS, gL)
+    gH, glam = gH.cpu().numpy(), glam.cpu().numpy()
+
+    def loss(Hn, lam=BATH[1]):
+        return (torch.conj(gL) * block(Hn, lam)).real.sum(dim=(1, 2)).cpu().numpy()
+    h = 1e-3
+    for (p, q) in [(0, 0), (1, 2), (0, 3), (3, 3), (2, 1)]:
+        D = np.zeros((n, n))
+        D[p, q] = D[q, p] = 1.0
+        fd = (loss(H + h * D) - loss(H - h * D)) / (2 * h)
+        an = (gH * D).sum(axis=(1, 2))
+        scale = np.abs(gH).max(axis=(1, 2))
+        assert np.all(np.abs(fd - an) <= 1e-6 * scale), (p, q, fd, an)
+    fdl = (loss(H, BATH[1] + 1e-4) - loss(H, BATH[1] - 1e-4)) / 2e-4
+    assert np.all(np.abs(fdl - glam) <= 1e-6 * np.abs(fdl)), (fdl, glam)

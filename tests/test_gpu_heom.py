@@ -110,6 +110,27 @@ def test_heom_ensemble_batch_one_launch(gpu):
     assert rel_err(avg, got[:, :, :49].mean(axis=0).reshape(-1, 7, 7, order='F')) < 1e-12
 
 
+@pytest.mark.parametrize('per_pass', ['3', '8', '0'])
+def test_heom_sample_passes_match_one_pass(gpu, monkeypatch, per_pass):
+    """Samples propagated in passes of a few (each pass runs all its steps
+    before the next starts) equal the single pass over all samples, FP64 and
+    MIXED, with an uneven last pass."""
+    m = _model('redfield_7site', rw=float(golden('heom')['agg7_rw_freq']), level_cutoff=2)
+    rho0 = np.zeros((7, 7))
+    rho0[1, 1] = 1
+    t = np.arange(0, 10.0, 0.5)
+    mb = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(11)])
+    eom = mb.equation_of_motion('ee')
+    y0 = np.zeros((11, eom.size), complex)
+    y0[:, :49] = rho0.reshape(-1, order='F')
+    for prec in ('fp64', 'mixed'):
+        monkeypatch.setenv('QSX_HEOM_SAMPLES_PER_PASS', '0')
+        ref = qa.integrate(eom, y0, t, dt=0.5, precision=prec)
+        monkeypatch.setenv('QSX_HEOM_SAMPLES_PER_PASS', per_pass)
+        got = qa.integrate(eom, y0, t, dt=0.5, precision=prec)
+        assert np.array_equal(got, ref), (prec, per_pass)
+
+
 def _spectra_models(**kw):
     from oracle import system as S, liouville as Lv
     g = golden('redfield_dimer')
@@ -223,3 +244,44 @@ def test_heom_7site_esa_third_order_matches_oracle(gpu):
     _, Ro = P.third_order_response(mo, 5.5, dt=dt, **kw)
     assert R.shape == (len(t1), 2, len(t3)) and len(t1) >= 2
     assert rel_err(R, Ro) < 1e-10
+
+
+def test_heom_fields_match_oracle_and_reference(gpu):
+    """simulate_pump / simulate_with_fields of a HEOM model (the dipole
+    commutators on every ADO, eom.py:79-103 with heom.py:22-58): FP64 against
+    the oracle's time-dependent RK4 on the same schedule (1e-10), MIXED 1e-6,
+    and the default step against the reference's tight zvode hierarchy
+    states (tests/golden/heom_fields.npz, 1e-5)."""
+    from oracle import system as S, liouville as Lv
+    g = golden('heom_fields')
+    gd = golden('redfield_dimer')
+    hp = qa.ElectronicHamiltonian(gd['in_H1'], bath=qa.DebyeBath(qa.CM_K * 77, 35, 106),
+                                  dipoles=gd['in_dipoles'], energy_spread_extra=float(gd['extra']))
+    ho = S.ElectronicHamiltonian(gd['in_H1'], bath=S.DebyeBath(S.CM_K * 77, 35, 106),
+                                 dipoles=gd['in_dipoles'], energy_spread_extra=float(gd['extra']))
+    mp = qa.HEOMModel(hp, hilbert_subspace='gef', unit_convert=qa.CM_FS, level_cutoff=3, K=1)
+    mo = Lv.HEOMOracleModel(ho, hilbert_subspace='gef', unit_convert=S.CM_FS, level_cutoff=3, K=1)
+    assert mp.time_step == float(g['time_step'])
+    pp = qa.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=qa.CM_FS)
+    po = S.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=S.CM_FS)
+    dt = mp.time_step / 4
+    t, y = qa.simulate_pump(mp, pp, 'x', time_extra=30., dt=dt)
+    to, yo = P.simulate_pump(mo, po, 'x', time_extra=30., dt=dt)
+    np.testing.assert_allclose(t, to)
+    assert np.asarray(y).shape == yo.shape
+    assert rel_err(np.asarray(y), yo) < 1e-10
+    _, ym = qa.simulate_pump(mp, pp, 'x', time_extra=30., dt=dt, precision='mixed')
+    assert rel_err(np.asarray(ym), yo) < 1e-6
+    t, y = qa.simulate_pump(mp, pp, 'x', time_extra=30.)          # default step
+    np.testing.assert_allclose(t, g['pump_t'])
+    assert rel_err(np.asarray(y), g['pump_y']) < 1e-5
+    p1 = qa.GaussianPulse(12700., 15., t_peak=-10., scale=0.015, freq_convert=qa.CM_FS)
+    p2 = qa.GaussianPulse(12900., 25., t_peak=5., scale=0.01 * np.exp(0.6j),
+                          freq_convert=qa.CM_FS, t_limits_multiple=2.5)
+    t, y = qa.simulate_with_fields(mp, [p1, p2], '-+', 'xy', times=[0., 4., 10.])
+    np.testing.assert_allclose(t, g['two_t'])
+    assert rel_err(np.asarray(y), g['two_y']) < 1e-5
+    # one Liouville block: the field term vanishes, the hierarchy evolves freely
+    t, y = qa.simulate_pump(mp, pp, 'x', time_extra=10., liouville_subspace='gg', dt=dt)
+    to, yo = P.simulate_pump(mo, po, 'x', time_extra=10., liouville_subspace='gg', dt=dt)
+    assert rel_err(np.asarray(y), yo) < 1e-10

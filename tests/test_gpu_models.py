@@ -101,6 +101,52 @@ def test_redfield_7site_blocks_and_batch(gpu):
             assert rel_err(Lb[n], om.sample(n).evolve_matrix(s)) < 1e-12, (s, n)
 
 
+def _debye_j(lam, gam):
+    return lambda x: 2 * lam * gam * x / (gam ** 2 + x ** 2)
+
+
+def _superohmic_j(x):
+    return 35.0 * x ** 3 / 150.0 ** 2 * np.exp(-x / 150.0)
+
+
+def test_arbitrary_bath_blocks(gpu):
+    """ArbitraryBath (bath.py:50-56) through the tabulated builder: with the
+    Debye J it reproduces the reference's discard_imag_corr blocks; with a
+    super-ohmic J it matches the oracle member by member over a disorder
+    batch, secular and not; the complex correlation function it lacks
+    raises AttributeError as in the reference (redfield.py:48-53)."""
+    g, inp = _inputs('redfield_dimer')
+    T = S.CM_K * 77
+    ham = qa.ElectronicHamiltonian(inp['H1'], bath=qa.ArbitraryBath(T, _debye_j(35, 106),
+                                                                    2 * 35 / 106.),
+                                   dipoles=inp['dipoles'], energy_spread_extra=inp['extra'])
+    for sec in (True, False):
+        m = qa.RedfieldModel(ham, unit_convert=qa.CM_FS, secular=sec, discard_imag_corr=True)
+        for s in ['gg', 'eg', 'ge', 'ee', 'fg', 'fe']:
+            ref = g['L_%s_%d1' % (s, sec)]
+            assert rel_err(m.liouvillian(s)[0].cpu().numpy(), ref) < 1e-12, (s, sec)
+            got = m.liouvillian(s, heisenberg_picture=True)[0].cpu().numpy()
+            assert rel_err(got, ref.T) < 1e-12, (s, sec, 'heisenberg')
+    with pytest.raises(AttributeError):
+        qa.RedfieldModel(ham, unit_convert=qa.CM_FS).liouvillian('eg')
+
+    _, inp = _inputs('redfield_7site')
+    pham = qa.ElectronicHamiltonian(inp['H1'], bath=qa.ArbitraryBath(T, _superohmic_j, 0.0),
+                                    dipoles=inp['dipoles'], energy_spread_extra=inp['extra'],
+                                    disorder=100 / qa.GAUSSIAN_SD_FWHM)
+    oham = S.ElectronicHamiltonian(inp['H1'], bath=S.ArbitraryBath(T, _superohmic_j, 0.0),
+                                   dipoles=inp['dipoles'], energy_spread_extra=inp['extra'],
+                                   disorder=100 / S.GAUSSIAN_SD_FWHM)
+    for sec in (False, True):
+        m = qa.RedfieldModel(pham, unit_convert=qa.CM_FS, secular=sec, discard_imag_corr=True)
+        mb = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(4)])
+        om = Lv.LiouvilleModel(oham, unit_convert=S.CM_FS, secular=sec, discard_imag_corr=True)
+        for s in ('eg', 'ee', 'fe'):
+            Lb = mb.liouvillian(s).cpu().numpy()
+            for n in range(4):
+                assert rel_err(Lb[n], om.sample(n).evolve_matrix(s)) < 1e-12, (s, n, sec)
+
+
 def test_unitary_blocks_match_oracle(gpu):
     _, inp = _inputs('redfield_7site')
     m = qa.UnitaryModel(_product_ham(inp), unit_convert=qa.CM_FS)

@@ -1,13 +1,15 @@
 """Host-side checks of the differentiable Redfield restatement used by the
-gradient path: its blocks against the oracle (redfield.py:9-104), its
-derivatives against finite differences (torch.autograd.gradcheck), and the
-ensemble/Hamiltonian helpers against the model classes."""
+GPU reverse mode's tests: its blocks against the oracle (redfield.py:9-104),
+its derivatives against finite differences (torch.autograd.gradcheck), and
+the ensemble/Hamiltonian helpers of the gradient path against the model
+classes."""
 import numpy as np
 import pytest
 import torch
 
 import qspectra_amd as qa
 from qspectra_amd import gradients as G
+from tests import torch_redfield as TR
 from oracle import system as S
 from oracle import liouville as Lv
 from tests.conftest import rel_err
@@ -33,7 +35,7 @@ def test_torch_redfield_blocks_match_oracle(secular, real_corr):
                            secular=secular, discard_imag_corr=real_corr)
     Ht = torch.as_tensor(H)[None]
     for X, Y in [('e', 'g'), ('g', 'e'), ('e', 'e'), ('g', 'g')]:
-        L = G.redfield_block(Ht, BATH[1], BATH[2], BATH[0], rw, X, Y, secular, real_corr,
+        L = TR.redfield_block(Ht, BATH[1], BATH[2], BATH[0], rw, X, Y, secular, real_corr,
                              S.CM_FS)[0].numpy()
         ref = om.evolve_matrix(X + Y)
         assert L.shape == ref.shape
@@ -43,8 +45,8 @@ def test_torch_redfield_blocks_match_oracle(secular, real_corr):
 def test_debye_correlation_matches_oracle():
     bath = S.DebyeBath(*BATH)
     x = np.array([0.0, -350.0, -20.5, 1e-3, 17.0, 480.0])
-    Cc = G.debye_correlation(torch.as_tensor(x), BATH[1], BATH[2], BATH[0]).numpy()
-    Cr = G.debye_correlation(torch.as_tensor(x), BATH[1], BATH[2], BATH[0], real=True).numpy()
+    Cc = TR.debye_correlation(torch.as_tensor(x), BATH[1], BATH[2], BATH[0]).numpy()
+    Cr = TR.debye_correlation(torch.as_tensor(x), BATH[1], BATH[2], BATH[0], real=True).numpy()
     for i, xi in enumerate(x):
         assert abs(Cc[i] - bath.corr_func_complex(xi)) <= 1e-13 * abs(bath.corr_func_complex(xi))
         assert abs(Cr[i] - bath.corr_func_real(xi)) <= 1e-13 * abs(bath.corr_func_real(xi))
@@ -60,7 +62,7 @@ def test_torch_redfield_derivatives_gradcheck(X, Y):
 
     def f(E, J, lam):
         Hs = G.site_hamiltonian(E, J)[None]
-        L = G.redfield_block(Hs, lam, BATH[2], BATH[0], 12450.0, X, Y, False, False, S.CM_FS)
+        L = TR.redfield_block(Hs, lam, BATH[2], BATH[0], 12450.0, X, Y, False, False, S.CM_FS)
         return torch.view_as_real(L)
 
     assert torch.autograd.gradcheck(f, (E, J, lam), eps=1e-4, atol=1e-9, rtol=1e-5)

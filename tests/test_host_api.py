@@ -283,3 +283,18 @@ def test_hamiltonian_variants_commute_and_keep_frames():
     for bad in ([[1., 2.], [0., 1.]], np.ones((2, 3))):
         with pytest.raises(ValueError):
             check_hermitian(bad)
+
+
+def test_arbitrary_bath_host():
+    """ArbitraryBath (bath.py:50-56) is constructible (the reference's own
+    constructor trips over a read-only property) and its real correlation
+    function with the Debye J is the Debye one; it has no complex one."""
+    from oracle import system as S
+    T, lam, gam = S.CM_K * 77, 35.0, 106.0
+    b = qa.ArbitraryBath(T, lambda x: 2 * lam * gam * x / (gam ** 2 + x ** 2), 2 * lam / gam)
+    ref = S.DebyeBath(T, lam, gam)
+    for x in (0.0, -250.0, 3.5, 900.0):
+        assert abs(b.corr_func_real(x) - ref.corr_func_real(x)) <= 1e-14 * abs(ref.corr_func_real(x))
+    assert not hasattr(b, 'corr_func_complex')
+    assert b.spectral_density_limit_at_zero == 2 * lam / gam
+    assert not qa.bath.is_debye_form(b)

@@ -299,6 +299,29 @@ def test_heom_spectra_match_reference():
             assert rel_err(R, g[key]) < 1e-7, key
 
 
+def test_heom_fields_match_reference():
+    """Field-driven HEOM trajectories (the dipole commutators applied to
+    every ADO) against the reference's tight zvode outputs
+    (tests/golden/heom_fields.npz), full hierarchy states."""
+    g = golden('heom_fields')
+    hm = Lv.HEOMOracleModel(_dimer(), hilbert_subspace='gef', unit_convert=S.CM_FS,
+                            level_cutoff=3, K=1)
+    assert hm.ado_count == int(g['ado_count'])
+    pump = S.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=S.CM_FS)
+    t, y = P.simulate_pump(hm, pump, 'x', time_extra=30., integrator='zvode', **TIGHT)
+    assert_allclose(t, g['pump_t'])
+    assert rel_err(y, g['pump_y']) < 1e-9
+    t, y = P.simulate_pump(hm, pump, 'x', time_extra=30., dt=hm.time_step / 8)
+    assert rel_err(y, g['pump_y']) < 1e-7
+    p1 = S.GaussianPulse(12700., 15., t_peak=-10., scale=0.015, freq_convert=S.CM_FS)
+    p2 = S.GaussianPulse(12900., 25., t_peak=5., scale=0.01 * np.exp(0.6j),
+                         freq_convert=S.CM_FS, t_limits_multiple=2.5)
+    t, y = P.simulate_with_fields(hm, [p1, p2], '-+', 'xy', times=[0., 4., 10.],
+                                  integrator='zvode', **TIGHT)
+    assert_allclose(t, g['two_t'])
+    assert rel_err(y, g['two_y']) < 1e-9
+
+
 def _pm_arrays(g, n_sites):
     """The 16 pseudomodes of the reference's ZOFE examples, per site."""
     on = np.ones(n_sites, complex)

@@ -108,3 +108,25 @@ def test_bath_against_reference_and_closed_forms():
         a = complex(float(cot), -1.0) / complex(gam, -xv)
         ref = lam * gam * (a + 4 * T * complex(float(sr), float(si)))
         assert abs(b77.corr_func_complex(xv) - ref) < 1e-12 * abs(ref)
+
+
+def test_arbitrary_bath_pins():
+    """ArbitraryBath (bath.py:50-56) with the Debye spectral density is the
+    Debye real correlation function, itself pinned by the golden C(w); for a
+    super-ohmic J the detailed-balance identities C(x) - C(-x) = J(x) and
+    C(-x) = exp(-x/T) C(x) fix the rest, and x = 0 gives T J'(0)."""
+    g = golden('bath')
+    T, lam, gam = S.CM_K * 77, 35.0, 106.0
+    debye = S.ArbitraryBath(T, lambda x: 2 * lam * gam * x / (gam ** 2 + x ** 2),
+                            2 * lam / gam)
+    assert_allclose([debye.corr_func_real(v) for v in g['x']], g['r77'], rtol=1e-13)
+    assert debye.corr_func_real(0) == T * 2 * lam / gam
+    wc = 150.0
+    J = lambda x: lam * x ** 3 / wc ** 2 * np.exp(-x / wc)
+    sup = S.ArbitraryBath(T, J, 0.0)
+    assert sup.corr_func_real(0) == 0.0
+    for x in (0.5, 12.0, 80.0, 410.0, 1500.0):
+        cp, cm = sup.corr_func_real(x), sup.corr_func_real(-x)
+        assert abs((cp - cm) - J(x)) <= 1e-12 * abs(J(x))
+        assert abs(cm - np.exp(-x / T) * cp) <= 1e-12 * abs(cp)
+    assert not hasattr(sup, 'corr_func_complex')
