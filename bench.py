@@ -404,10 +404,12 @@ def _reference_default_err():
 # HEOM bytes per state element and RK4 step.  Algorithmic: four stages each
 # reading and writing the stage operand (16 B FP64, 8 B MIXED) plus one read
 # and one write of the FP64 state: 160 / 96 B.  The stage kernel as written
-# moves more: every stage reads the state and the RK accumulator and writes
-# the accumulator and the next stage operand (72 / 56 B per element-stage).
+# moves more (no accumulator buffer): FP64 reads y in stages 2..4 and the
+# stage vectors z2, z3 again in stage 4 (48 + 48 + 48 + 64 = 208 B); MIXED
+# writes the stage derivatives k1..k3 in complex64 and reads them, the FP64
+# state and the complex64 copy of y back (24 + 32 + 32 + 72 = 160 B).
 HEOM_ALGORITHMIC_BYTES = {'fp64': 160, 'mixed': 96}
-HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 288, 'mixed': 224}
+HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 208, 'mixed': 160}
 
 # HBM bytes per launch (read + written) measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE over the `bench.py --counters` sequence, FETCH

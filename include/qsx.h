@@ -384,6 +384,34 @@ int32_t qsx_heom_propagate(int32_t precision, int32_t B, int32_t n_ado, int32_t 
                            const double* h_host, int32_t nsave, void* out, void* y_final,
                            void* workspace, int32_t* status, void* stream);
 
+/* The same two operations with the system part in Hilbert-commutator form,
+ * for Liouville subspaces of several blocks beyond the dense form's 64
+ * elements (the reference's HEOM_tensor on any subspace, heom.py:298-443):
+ * Hop [B][N][N] = -i unit H over the Hilbert space (H^T in the Heisenberg
+ * picture, block diagonal by excitation number), elem_states [M][2] the row
+ * and column state of element p, state_pos [N][N] the element of (row,
+ * column) or -1, state_block [N][2] the first and end state of each state's
+ * Hilbert block; every (q, column) and (row, q) with q in the block of the
+ * row (column) state must be in the subspace.  N <= 128. */
+int32_t qsx_heom_apply_hilbert(int32_t B, int32_t n_ado, int32_t M, int32_t N,
+                               const int32_t* elem_states, const int32_t* state_pos,
+                               const int32_t* state_block, const void* Hop, const double* shift,
+                               const void* dvec, int32_t nbins, int32_t K1,
+                               const int32_t* bin_src, const void* bin_coef, const void* ck,
+                               int32_t nrel, const int32_t* rel_site, const double* rel_occ,
+                               const void* y, void* out, void* stream);
+int32_t qsx_heom_propagate_hilbert(int32_t precision, int32_t B, int32_t n_ado, int32_t M,
+                                   int32_t N, const int32_t* elem_states,
+                                   const int32_t* state_pos, const int32_t* state_block,
+                                   const void* Hop, const double* shift, const void* dvec,
+                                   int32_t nbins, int32_t K1, const int32_t* bin_src,
+                                   const void* bin_coef, const void* ck, int32_t nrel,
+                                   const int32_t* rel_site, const double* rel_occ,
+                                   const void* y0, const int32_t* nsub_host,
+                                   const double* h_host, int32_t nsave, void* out,
+                                   void* y_final, void* workspace, int32_t* status,
+                                   void* stream);
+
 /* qsx_heom_propagate under classical fields for a dense hierarchy block
  * (M <= 64, the multi-block subspaces such as 'gg,ge,eg,ee'): every ADO
  * evolves with the stage operator Hop[b] + sum_i f_i(t) C_i, C [B|1][P][M][M]

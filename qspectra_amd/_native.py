@@ -59,6 +59,12 @@ SIGNATURES = {
                                      c_i32, c_vp, c_vp, c_vp],
     'qsx_redfield_vjp_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
     'qsx_response_buckets': [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp],
+    'qsx_heom_apply_hilbert': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                               c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
+                               c_vp],
+    'qsx_heom_propagate_hilbert': [c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
+                                   c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp,
+                                   c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp],
     'qsx_heom_propagate_driven': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp,
                                   c_vp, c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, c_i64, c_vp, c_i64,
                                   c_i64, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp,
@@ -456,7 +462,7 @@ def propagate_chunks(Pc, A, y0, nc, nsub, h, precision=QSX_FP64, diag=None, stre
     return out, status
 
 
-MAX_DRIVEN_BLOCK = 64
+MAX_DRIVEN_BLOCK = 1536
 MAX_FIELD_TERMS = 4
 
 
@@ -778,6 +784,18 @@ def _heom_ptrs(eom):
             ptr(eom.rel_site), ptr(eom.rel_occ))
 
 
+def _heom_entry(lib, eom, name):
+    """(entry point, leading arguments) of qsx_heom_<name> for the form of
+    eom: the Hilbert-commutator entries take the state tables in place of
+    the commutator block size."""
+    hb = getattr(eom, 'hilbert', None)
+    if hb is None:
+        return getattr(lib, 'qsx_heom_' + name), _heom_ptrs(eom)
+    rest = _heom_ptrs(eom)[3:]
+    return (getattr(lib, 'qsx_heom_%s_hilbert' % name),
+            (eom.n_ado, eom.M, hb['N'], ptr(hb['est']), ptr(hb['pos']), ptr(hb['blk'])) + rest)
+
+
 def heom_apply(eom, y, stream=None):
     """F(y) for hierarchy states y [B, n_ado * M] (complex128)."""
     lib = require_native()
@@ -785,7 +803,8 @@ def heom_apply(eom, y, stream=None):
     assert y.shape == (B, eom.n_ado * eom.M) and y.dtype == torch.complex128
     y = y.contiguous()
     out = torch.empty_like(y)
-    check(lib.qsx_heom_apply(B, *_heom_ptrs(eom), ptr(y), ptr(out), stream_ptr(stream)),
+    fn, args = _heom_entry(lib, eom, 'apply')
+    check(fn(B, *args, ptr(y), ptr(out), stream_ptr(stream)),
           'qsx_heom_apply')
     return out
 
@@ -811,8 +830,9 @@ def heom_propagate(eom, y0, nsub, h, precision=QSX_FP64, want_final=False, strea
     label = 'heom %s B=%d n_ado=%d M=%d steps=%d' % (
         'fp64' if precision == QSX_FP64 else 'mixed', B, eom.n_ado, eom.M, int(nsub.sum()))
     with timing.launch(label, stream):
-        st = lib.qsx_heom_propagate(
-            precision, B, *_heom_ptrs(eom), ptr(y0), nsub.ctypes.data_as(c_vp),
+        fn, args = _heom_entry(lib, eom, 'propagate')
+        st = fn(
+            precision, B, *args, ptr(y0), nsub.ctypes.data_as(c_vp),
             h.ctypes.data_as(c_vp), nsave, ptr(out), ptr(yfin), ptr(ws), ptr(status),
             stream_ptr(stream))
     check(st, 'qsx_heom_propagate')
@@ -829,6 +849,8 @@ def heom_propagate_driven(eom, C, F, y0, nsub, h, precision=QSX_FP64, want_final
     if eom.n_comm != 0:
         raise ValueError('fields need the dense hierarchy form (several Liouville blocks, '
                          'M <= 64)')
+    if getattr(eom, 'hilbert', None) is not None:
+        raise NotImplementedError('fields on hierarchies of more than 64 elements per ADO')
     B = eom.Hop.shape[0]
     M, S = eom.M, eom.n_ado * eom.M
     assert y0.shape == (B, S) and y0.dtype == torch.complex128

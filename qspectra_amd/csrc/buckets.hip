@@ -13,7 +13,8 @@
 // exchanged through LDS (double-buffered, one barrier per step), and the
 // signal values are staged in LDS and stored kFlush steps at a time.
 //
-// A one-step product is 4 M FP64 FMAs per lane in four independent chains; with packed FP32 disabled, FP32 FMAs issue at the FP64 rate
+// A one-step product is 4 M FP64 FMAs per lane in four independent chains
+// after the step's M LDS reads (straight-line code per M); with packed FP32 disabled, FP32 FMAs issue at the FP64 rate
 // on the vector ALUs, so the products stay FP64.
 //
 // Work description (device, built by the host once per plan):
@@ -30,30 +31,19 @@ namespace qsx {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kMaxGroup = 32;   // rows M plus the projection row
+constexpr int kMaxStates = 31;  // rows M plus the projection row in 32 lanes
 constexpr int kFlush = 32;      // signal values staged in LDS between stores
 
-// acc_q += row[j] * y_j over j = q (mod 4) < M (four independent chains)
-#define QSX_BUCKET_TERM(q)                                       \
-  if (j + q < M) {                                               \
-    const double2 yv = ys[cur][base + j + q];                    \
-    acc[q].x = fma(row[j + q].x, yv.x, acc[q].x);                \
-    acc[q].x = fma(-row[j + q].y, yv.y, acc[q].x);               \
-    acc[q].y = fma(row[j + q].x, yv.y, acc[q].y);                \
-    acc[q].y = fma(row[j + q].y, yv.x, acc[q].y);                \
-  }
-
-__global__ __launch_bounds__(kWave) void response_buckets_kernel(
-    const int4* __restrict__ waves, const int64_t* __restrict__ offs,
-    const double2* __restrict__ P, const double2* __restrict__ y0,
-    const double2* __restrict__ v, int nt, double2* __restrict__ out,
-    int32_t* __restrict__ status) {
-  __shared__ double2 ys[2][kWave];
-  // s_k of every group, kFlush steps at a time: the stores leave the step
-  // loop, so its barriers do not wait on them, and go out coalesced
-  __shared__ double2 sbuf[kFlush][kWave / 2];
-  const int4 w = waves[blockIdx.x];
-  const int lane = threadIdx.x, M = w.z, G = w.w;
+// One wavefront's problems of M states through all nt steps.  M is a
+// template parameter so that the row product is straight-line code: the M
+// LDS reads of a step issue together before its multiply-adds.
+template <int M>
+__device__ void bucket_wave(const int4 w, const int64_t* __restrict__ offs,
+                            const double2* __restrict__ P, const double2* __restrict__ y0,
+                            const double2* __restrict__ v, int nt, double2* __restrict__ out,
+                            int32_t* __restrict__ status, double2 (*ys)[kWave],
+                            double2 (*sbuf)[kWave / 2]) {
+  const int lane = threadIdx.x, G = w.w;
   const int g = lane / G, i = lane - g * G;
   const bool live = g < w.y;
   const int64_t mem = (int64_t)w.x + g;
@@ -63,14 +53,12 @@ __global__ __launch_bounds__(kWave) void response_buckets_kernel(
     offP = offs[2 * mem];
     offY = offs[2 * mem + 1];
   }
-  double2 row[kMaxGroup];
+  double2 row[M];
 #pragma unroll
-  for (int j = 0; j < kMaxGroup; ++j) {
+  for (int j = 0; j < M; ++j) {
     double2 r = make_double2(0.0, 0.0);
-    if (j < M) {
-      if (srow) r = P[offP + (int64_t)i * M + j];
-      else if (prow) r = v[offY + j];
-    }
+    if (srow) r = P[offP + (int64_t)i * M + j];
+    else if (prow) r = v[offY + j];
     row[j] = r;
   }
   ys[0][lane] = srow ? y0[offY + i] : make_double2(0.0, 0.0);
@@ -82,14 +70,25 @@ __global__ __launch_bounds__(kWave) void response_buckets_kernel(
     double2 acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
-    // fully unrolled (row[] must stay in registers); M is uniform over the
-    // wavefront, so the bounds are scalar branches
+    // the state in chunks of up to 8 entries: a chunk's reads issue together
 #pragma unroll
-    for (int j = 0; j < kMaxGroup; j += 4) {
-      QSX_BUCKET_TERM(0)
-      QSX_BUCKET_TERM(1)
-      QSX_BUCKET_TERM(2)
-      QSX_BUCKET_TERM(3)
+    for (int j0 = 0; j0 < M; j0 += 8) {
+      constexpr int kChunk = 8;
+      double2 yv[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j)
+        if (j0 + j < M) yv[j] = ys[cur][base + j0 + j];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        if (j0 + j < M) {
+          double2& a = acc[j & 3];
+          const double2 r = row[j0 + j];
+          a.x = fma(r.x, yv[j].x, a.x);
+          a.x = fma(-r.y, yv[j].y, a.x);
+          a.y = fma(r.x, yv[j].y, a.y);
+          a.y = fma(r.y, yv[j].x, a.y);
+        }
+      }
     }
     const double2 sum = make_double2((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x),
                                      (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y));
@@ -112,7 +111,31 @@ __global__ __launch_bounds__(kWave) void response_buckets_kernel(
   }
   if (prow) status[mem] = bad ? 1 : 0;
 }
-#undef QSX_BUCKET_TERM
+
+__global__ __launch_bounds__(kWave) void response_buckets_kernel(
+    const int4* __restrict__ waves, const int64_t* __restrict__ offs,
+    const double2* __restrict__ P, const double2* __restrict__ y0,
+    const double2* __restrict__ v, int nt, double2* __restrict__ out,
+    int32_t* __restrict__ status) {
+  __shared__ double2 ys[2][kWave];
+  // s_k of every group, kFlush steps at a time: the stores leave the step
+  // loop, so its barriers do not wait on them, and go out coalesced
+  __shared__ double2 sbuf[kFlush][kWave / 2];
+  const int4 w = waves[blockIdx.x];
+  switch (w.z) {   // M, the same for every problem of the wavefront
+#define QSX_BUCKET_M(m_) \
+  case m_: bucket_wave<m_>(w, offs, P, y0, v, nt, out, status, ys, sbuf); break;
+    QSX_BUCKET_M(1) QSX_BUCKET_M(2) QSX_BUCKET_M(3) QSX_BUCKET_M(4) QSX_BUCKET_M(5)
+    QSX_BUCKET_M(6) QSX_BUCKET_M(7) QSX_BUCKET_M(8) QSX_BUCKET_M(9) QSX_BUCKET_M(10)
+    QSX_BUCKET_M(11) QSX_BUCKET_M(12) QSX_BUCKET_M(13) QSX_BUCKET_M(14) QSX_BUCKET_M(15)
+    QSX_BUCKET_M(16) QSX_BUCKET_M(17) QSX_BUCKET_M(18) QSX_BUCKET_M(19) QSX_BUCKET_M(20)
+    QSX_BUCKET_M(21) QSX_BUCKET_M(22) QSX_BUCKET_M(23) QSX_BUCKET_M(24) QSX_BUCKET_M(25)
+    QSX_BUCKET_M(26) QSX_BUCKET_M(27) QSX_BUCKET_M(28) QSX_BUCKET_M(29) QSX_BUCKET_M(30)
+    QSX_BUCKET_M(31)
+#undef QSX_BUCKET_M
+    default: break;
+  }
+}
 
 }  // namespace
 }  // namespace qsx
@@ -135,4 +158,4 @@ extern "C" int32_t qsx_response_buckets(const int32_t* waves, int32_t nwaves,
   return QSX_OK;
 }
 
-extern "C" int32_t qsx_response_buckets_max_states(void) { return kMaxGroup - 1; }
+extern "C" int32_t qsx_response_buckets_max_states(void) { return kMaxStates; }

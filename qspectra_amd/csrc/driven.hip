@@ -20,7 +20,8 @@
 // of L split into S-lane segments of W registers and the row's ELL slots into
 // S-lane segments of KW registers; wave shuffle reductions; double-buffered
 // LDS stage vector); blocks up to M = 64 and up to 4 field terms (kernels for
-// at most 2 and at most 4).  Each stage computes
+// at most 2 and at most 4); larger blocks (up to 1536) in driven_large_kernel.
+// Each stage computes
 //     sum_j a_ij z_j + sum_k (sum_i f_i(t) coef_ik) z_col(k)
 // with two accumulators to halve the dependent FMA chain.
 // MIXED: L off-diagonal and the ELL coefficients in complex64 (the C_i have
@@ -161,6 +162,129 @@ __global__ __launch_bounds__(NT) void driven_kernel(
   if (bad) status[0] = 1;
 }
 
+// Blocks above 64 elements (up to kDrivenLargeMax): one work-group of four
+// wavefronts per right-hand side; the stage vector, the state and the RK4
+// accumulator in LDS; each wavefront takes every fourth row, its lanes
+// stride over the row of L (read from memory, coalesced) and over the row's
+// ELL slots, and a wavefront reduction gives the row's derivative.
+constexpr int kDrivenLargeMax = 1536;   // 5 M complex128 of LDS
+constexpr int kDrLargeThreads = 256;
+
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(kDrLargeThreads) void driven_large_kernel(
+    const cplx_t<T>* __restrict__ A, int64_t strideA, const double2* __restrict__ diag,
+    int64_t strideD, int K, const int* __restrict__ Ecol, const cplx_t<T>* __restrict__ Ecoef,
+    int64_t strideE, int P, const double2* __restrict__ F, int64_t strideF, int64_t nF, int M,
+    const double2* __restrict__ y0, int R, const int* __restrict__ nsub,
+    const double* __restrict__ hs, int nsave, double2* __restrict__ out,
+    int* __restrict__ status) {
+  using T2 = cplx_t<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double2* ys = reinterpret_cast<double2*>(smem);          // [M] state
+  double2* acc = ys + M;                                     // [M] RK4 accumulator
+  double2* zi = acc + M;                                     // [M] FP64 stage value
+  T2* zb = reinterpret_cast<T2*>(zi + M);                    // [2][M] stage vector
+  __shared__ T2 fc[3 * kMaxFieldTerms];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int r = blockIdx.x, b = blockIdx.y;
+  const T2* Ab = A + (int64_t)b * strideA;
+  const T2* Eb = Ecoef + (int64_t)b * strideE * P;
+  const double2* Fb = F + (int64_t)b * strideF;
+  const double2* yb0 = y0 + ((int64_t)b * R + r) * M;
+  for (int i = tid; i < M; i += kDrLargeThreads) {
+    ys[i] = yb0[i];
+    zb[i] = from_d2<T>(yb0[i]);
+  }
+  __syncthreads();
+  int cur = 0;
+  int64_t g = 0;
+  bool bad = false;
+  double2* ob = out + ((int64_t)b * R + r) * nsave * M;
+  for (int s = 0; s < nsave; ++s) {
+    const int n = max(nsub[s], 0);
+    const double h = hs[s];
+    for (int step = 0; step < n; ++step, ++g) {
+      if (g >= nF) {                       // fewer sampled steps than scheduled
+        if (tid == 0) status[0] = 2;
+        return;
+      }
+      if (tid < 3 * P) fc[tid] = from_d2<T>(Fb[g * 3 * P + tid]);
+      __syncthreads();
+      for (int st = 0; st < 4; ++st) {
+        const int ti = st == 0 ? 0 : (st == 3 ? 2 : 1);
+        const T2* zc = zb + cur * M;
+        T2* zn = zb + (cur ^ 1) * M;
+        for (int row = wv; row < M; row += kDrLargeThreads / 64) {
+          const T2* Ar = Ab + (int64_t)row * M;
+          T2 sum = {0, 0}, sum2 = {0, 0};
+          for (int j = lane; j < M; j += 64) cfma<T>((j & 64) ? sum2 : sum, Ar[j], zc[j]);
+          for (int kk = lane; kk < K; kk += 64) {
+            const int64_t slot = (int64_t)row * K + kk;
+            T2 e = {0, 0};
+            for (int i = 0; i < P; ++i) cfma<T>(e, fc[ti * P + i], Eb[slot * P + i]);
+            cfma<T>(sum2, e, zc[Ecol[slot]]);
+          }
+          double sr = (double)sum.x + (double)sum2.x, si = (double)sum.y + (double)sum2.y;
+          for (int off = 32; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off);
+            si += __shfl_xor(si, off);
+          }
+          if (lane == 0) {
+            const double2 yv = ys[row];
+            const double2 zv = st == 0 ? yv : zi[row];
+            double2 kv = make_double2(sr, si);
+            if (MIXED) kv = cadd(kv, cmul(diag[(int64_t)b * strideD + row], zv));
+            const double2 a0 = st == 0 ? make_double2(0.0, 0.0) : acc[row];
+            const double2 an = caxpy(kDrW[st], kv, a0);
+            if (st < 3) {
+              acc[row] = an;
+              const double2 zz = caxpy(kDrC[st] * h, kv, yv);
+              zi[row] = zz;
+              zn[row] = from_d2<T>(zz);
+            } else {
+              const double2 yn = caxpy(h / 6.0, an, yv);
+              ys[row] = yn;
+              zn[row] = from_d2<T>(yn);
+            }
+          }
+        }
+        cur ^= 1;
+        __syncthreads();
+      }
+    }
+    for (int i = tid; i < M; i += kDrLargeThreads) {
+      const double2 v = ys[i];
+      ob[(int64_t)s * M + i] = v;
+      if (!cfinite(v)) bad = true;
+    }
+  }
+  if (bad) status[0] = 1;
+}
+
+inline size_t driven_large_lds(int M, size_t tbytes) {
+  return 3 * (size_t)M * sizeof(double2) + 2 * (size_t)M * tbytes;
+}
+
+template <typename T, bool MIXED>
+int launch_driven_large(const void* A, int64_t strideA, const void* diag, int64_t strideD, int K,
+                        const int* Ecol, const void* Ecoef, int64_t strideE, int P,
+                        const void* F, int64_t strideF, int64_t nF, int M, const void* y0,
+                        int R, int B, const int* nsub, const double* h, int nsave, void* out,
+                        int* status, hipStream_t stream) {
+  using T2 = cplx_t<T>;
+  const size_t shmem = driven_large_lds(M, sizeof(T2));
+  auto kern = driven_large_kernel<T, MIXED>;
+  if (shmem > 64 * 1024)
+    QSX_CHECK_HIP(hipFuncSetAttribute((const void*)kern,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+  hipLaunchKernelGGL(kern, dim3(R, B), dim3(kDrLargeThreads), shmem, stream, (const T2*)A,
+                     strideA, (const double2*)diag, strideD, K, Ecol, (const T2*)Ecoef, strideE,
+                     P, (const double2*)F, strideF, nF, M, (const double2*)y0, R, nsub, h, nsave,
+                     (double2*)out, status);
+  QSX_CHECK_HIP(hipGetLastError());
+  return QSX_OK;
+}
+
 struct DrShape {
   int NT, S, W, KW;
 };
@@ -234,9 +358,23 @@ extern "C" int32_t qsx_propagate_driven(int32_t precision, const void* A, int64_
                 "qsx_propagate_driven: bad field record (nF=%lld, strideF=%lld)", (long long)nF,
                 (long long)strideF);
   DrShape sh;
-  QSX_CHECK_ARG(pick_driven_shape(M, K, sh),
-                "qsx_propagate_driven: M=%d exceeds 64 or K=%d ELL slots exceed the row", M, K);
   hipStream_t s = as_stream(stream);
+  if (!pick_driven_shape(M, K, sh)) {
+    // beyond the register-resident shapes: the LDS-resident kernel
+    QSX_CHECK_ARG(M <= kDrivenLargeMax && K <= M,
+                  "qsx_propagate_driven: M=%d exceeds %d or K=%d ELL slots exceed the row", M,
+                  kDrivenLargeMax, K);
+    if (precision == QSX_MIXED) {
+      QSX_CHECK_ARG(diag != nullptr, "qsx_propagate_driven: MIXED needs the FP64 diagonal");
+      QSX_CHECK_ARG(strideD == 0 || strideD >= M, "qsx_propagate_driven: bad strideD");
+      return launch_driven_large<float, true>(A, strideA, diag, strideD, K, col, coef, strideE,
+                                              P, F, strideF, nF, M, y0, R, B, nsub, h, nsave,
+                                              out, status, s);
+    }
+    return launch_driven_large<double, false>(A, strideA, nullptr, 0, K, col, coef, strideE, P,
+                                              F, strideF, nF, M, y0, R, B, nsub, h, nsave, out,
+                                              status, s);
+  }
   if (precision == QSX_MIXED) {
     QSX_CHECK_ARG(diag != nullptr, "qsx_propagate_driven: MIXED needs the FP64 diagonal");
     QSX_CHECK_ARG(strideD == 0 || strideD >= M, "qsx_propagate_driven: bad strideD");

@@ -299,6 +299,30 @@ class HEOMModel(DynamicalModel):
                     bsrc=bsrc, bcoef=bcoef, ck=np.asarray(c[:K + 1], dtype=complex),
                     nrel=nrel, rel_site=rel_site, rel_occ=rel_occ)
 
+    def _hilbert_tables(self, idx, N):
+        """Device tables of the Hilbert-commutator form: the row and column
+        state of every element, the element of every (row, column) pair (-1
+        outside the subspace) and each state's block [first, end) of equal
+        excitation number; checked here so that every element the kernel
+        reads is in the subspace."""
+        rows, cols = idx % N, idx // N
+        pos = -np.ones((N, N), dtype=np.int32)
+        pos[rows, cols] = np.arange(idx.size, dtype=np.int32)
+        exc = _occupations(self.hilbert_subspace, self.hamiltonian.n_sites).sum(axis=1)
+        blk = np.zeros((N, 2), dtype=np.int32)
+        for a in range(N):
+            same = np.flatnonzero(exc == exc[a])
+            if not np.array_equal(same, np.arange(same[0], same[-1] + 1)):
+                raise NotImplementedError('Hilbert states not grouped by excitation number')
+            blk[a] = (same[0], same[-1] + 1)
+        for a, b in zip(rows, cols):
+            if np.any(pos[blk[a, 0]:blk[a, 1], b] < 0) or np.any(pos[a, blk[b, 0]:blk[b, 1]] < 0):
+                raise ValueError('the subspace must consist of whole Liouville blocks')
+        est = np.stack([rows, cols], axis=1).astype(np.int32)
+        dev = _native.device()
+        return dict(N=int(N), est=torch.as_tensor(est).to(dev), pos=torch.as_tensor(pos).to(dev),
+                    blk=torch.as_tensor(blk).to(dev))
+
     def _single_block(self, liouville_subspace):
         """Hilbert manifolds (X, Y) when the subspace is the single block
         'XY'."""
@@ -360,6 +384,7 @@ class HEOMModel(DynamicalModel):
         XY = self._single_block(liouville_subspace)
         hams = self._members()
         uc = self.unit_convert
+        hilbert = None
         if XY is not None and (tb['M'] > DENSE_BLOCK_MAX or XY[0] == XY[1]):
             # commutator form: H_X and H_Y blocks of every member
             blocks = []
@@ -371,10 +396,16 @@ class HEOMModel(DynamicalModel):
                 blocks.append(Hz.reshape(len(hams), -1))
             Hop = _native.to_device(-1j * uc * np.concatenate(blocks, axis=1))
             n_comm = int(round(np.sqrt(blocks[0].shape[1])))
+        elif tb['M'] > DENSE_BLOCK_MAX:
+            # several Liouville blocks beyond the dense form: the system part
+            # as a commutator with the Hilbert-space Hamiltonian
+            Hh = np.stack([h.H(self.hilbert_subspace) for h in hams])
+            if heisenberg_picture:
+                Hh = Hh.transpose(0, 2, 1)
+            Hop = _native.to_device(-1j * uc * Hh.reshape(len(hams), -1))
+            n_comm = 0
+            hilbert = self._hilbert_tables(tb['idx'], Hh.shape[-1])
         else:
-            if tb['M'] > DENSE_BLOCK_MAX:
-                raise NotImplementedError('HEOM on a multi-block subspace of more than %d '
-                                          'elements' % DENSE_BLOCK_MAX)
             lm = self.lspace_model.with_hamiltonians(hams) if len(hams) > 1 else self.lspace_model
             Hop = lm.liouvillian(liouville_subspace, heisenberg_picture).contiguous()
             n_comm = 0
@@ -386,7 +417,8 @@ class HEOMModel(DynamicalModel):
             bcoef=_native.to_device(tb['bcoef']), ck=_native.to_device(tb['ck']),
             nrel=tb['nrel'], rel_site=torch.as_tensor(tb['rel_site']).to(dev),
             rel_occ=torch.as_tensor(tb['rel_occ'], dtype=torch.float64).to(dev),
-            liouville_subspace=liouville_subspace, heisenberg_picture=heisenberg_picture))
+            liouville_subspace=liouville_subspace, heisenberg_picture=heisenberg_picture,
+            hilbert=hilbert))
         cache[key] = eom
         return eom
 

@@ -169,12 +169,23 @@ def test_driven_kernel_status_and_errors(gpu):
     out, status = _native.propagate_driven(A, col, coef, F, y0, nsub, h)
     assert int(status.item()) == 0
     assert torch.equal(out[0, 0, 1], y0[0, 0])                  # L = 0, no field
+    # 65 elements: the LDS-resident kernel, the same status handling
     big = torch.zeros((1, 65, 65), dtype=torch.complex128, device=dev)
     c65, k65 = _native.ell_pack(np.zeros((1, 1, 65, 65), complex))
+    y65 = torch.ones((1, 1, 65), dtype=torch.complex128, device=dev)
+    out, status = _native.propagate_driven(big, torch.as_tensor(c65, device=dev),
+                                           torch.as_tensor(k65, device=dev), F, y65, nsub, h)
+    assert int(status.item()) == 0 and torch.equal(out[0, 0, 1], y65[0, 0])
+    F3 = torch.zeros((1, 3, 3, P_), dtype=torch.complex128, device=dev)
+    _, status = _native.propagate_driven(big, torch.as_tensor(c65, device=dev),
+                                         torch.as_tensor(k65, device=dev), F3, y65, nsub, h)
+    assert int(status.item()) == 2
+    M1 = _native.MAX_DRIVEN_BLOCK + 1
     with pytest.raises(ValueError):
-        _native.propagate_driven(big, torch.as_tensor(c65, device=dev),
-                                 torch.as_tensor(k65, device=dev), F,
-                                 torch.ones((1, 1, 65), dtype=torch.complex128, device=dev),
+        _native.propagate_driven(torch.zeros((1, M1, M1), dtype=torch.complex128, device=dev),
+                                 torch.zeros((1, M1, 1), dtype=torch.int32, device=dev),
+                                 torch.zeros((1, M1, 1, 1), dtype=torch.complex128, device=dev),
+                                 F, torch.ones((1, 1, M1), dtype=torch.complex128, device=dev),
                                  nsub, h)
 
 

@@ -261,7 +261,7 @@ def test_heom_fields_match_oracle_and_reference(gpu):
                                  dipoles=gd['in_dipoles'], energy_spread_extra=float(gd['extra']))
     mp = qa.HEOMModel(hp, hilbert_subspace='gef', unit_convert=qa.CM_FS, level_cutoff=3, K=1)
     mo = Lv.HEOMOracleModel(ho, hilbert_subspace='gef', unit_convert=S.CM_FS, level_cutoff=3, K=1)
-    assert mp.time_step == float(g['time_step'])
+    assert abs(mp.time_step - float(g['time_step'])) < 1e-12
     pp = qa.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=qa.CM_FS)
     po = S.GaussianPulse(12750., 20., t_peak=0., scale=0.02, freq_convert=S.CM_FS)
     dt = mp.time_step / 4
