@@ -55,26 +55,29 @@ def parse_args(argv=None):
                          "jonas_vibronic: the further workloads to measure")
     ap.add_argument('--counters', action='store_true',
                     help='no timing: run a fixed kernel sequence for hardware-counter '
-                         'profiles (a 256 MiB device copy for calibration, the 2-D spectrum '
-                         'launch sequence, the 256-member ensemble plan, two HEOM RK4 steps '
-                         'of 256 members)')
+                         'profiles (a 256 MiB elementwise negation for calibration, the 2-D spectrum '
+                         'launch sequence, the 256-member ensemble plan, the 5120-system '
+                         'absorption ensemble plan, two HEOM RK4 steps of 256 members)')
     return ap.parse_args(argv)
 
 
 def counter_sequence(torch, qa):
     """Kernel sequence profiled with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE):
-    the calibration copy reads and writes exactly 256 MiB per dispatch; then
+    the calibration kernel (an elementwise negation of 256 MiB) reads and
+    writes exactly 256 MiB per dispatch; then
     three launch sequences of the headline 2-D plan (MIXED, the plan's
     chunk length 16), two of the 256-member ensemble plan (MIXED, rebuilt
-    from the Hamiltonians), and two HEOM RK4 steps of 256 members (FP64 and
-    MIXED)."""
+    from the Hamiltonians), two of the 5120-system absorption ensemble plan,
+    and two HEOM RK4 steps of 256 members (FP64 and MIXED)."""
     from qspectra_amd import workloads as W
     from qspectra_amd.simulate.utils import propagate_heom
     from qspectra_amd.simulate.fused import ThirdOrderPlan
-    src = torch.ones(32 * 1024 * 1024, dtype=torch.complex64, device='cuda')   # 256 MiB
+    # calibration: a kernel no other step launches at this size (device
+    # copies share one copy kernel whose grid does not depend on the size)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device='cuda')     # 256 MiB
     dst = torch.empty_like(src)
     for _ in range(3):
-        dst.copy_(src)
+        torch.neg(src, out=dst)
     torch.cuda.synchronize()
     work = W.AGG7_2D
     kw = dict(population_times=work['population_times'], geometry=work['geometry'],
@@ -90,6 +93,12 @@ def counter_sequence(torch, qa):
     ens = ThirdOrderPlan(mb, work['coherence_time_max'], ensemble_mean=True, rebuild=True, **kw)
     for _ in range(2):
         ens._launch()
+    torch.cuda.synchronize()
+    from qspectra_amd.batched import AbsorptionEnsemblePlan
+    absorb = AbsorptionEnsemblePlan(W.ensemble_absorption_hamiltonians(), 4096.0,
+                                    rw_freq=12450.0, dt=1.0, graph=False)
+    for _ in range(2):
+        absorb._launch()
     torch.cuda.synchronize()
     m = W.heom_aggregate7_model()
     eom = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(256)]) \
@@ -162,7 +171,8 @@ MFMA_LAUNCHES = ('split propagator',)
 # kernel behind each single-kernel label, as rocprofv3 names it
 KERNEL_NAMES = {'chunks': 'chunk_kernel', 'propagate powers': 'small_powers_apply_kernel',
                 'small propagator': 'small_program_kernel', 'zgemm': 'zgemm_kernel',
-                'propagate': 'step_kernel'}
+                'propagate': 'step_kernel', 'response buckets': 'response_buckets_kernel',
+                'split propagator': 'split_gemm_kernel', 'heom': 'heom_stage_kernel'}
 
 
 def dominant_kernel(summary, relaunch, timing, per):
@@ -413,12 +423,16 @@ HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 208, 'mixed': 160}
 
 # HBM bytes per launch (read + written) measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE over the `bench.py --counters` sequence, FETCH
-# scaled by 2.0 and WRITE by 1.0 (the factors that make a 256 MiB device copy
-# in the same runs read and write 256 MiB), keyed by launch label; a launch
-# of several kernels counts all of them.
+# scaled by 2.0 and WRITE by 1.0: the factors that make a 256 MiB device copy
+# read and write 256 MiB.  WRITE x 1.0 is also what the kernels with exactly
+# known outputs show (the response signals, 5120 x 4096 x 16 B, and every HEOM
+# stage's 16 B per element read back to the byte).  Keyed by launch label; a
+# launch of several kernels counts all of them.
 PMC_TRAFFIC = {
-    'heom stage fp64': 3.528e9,
-    'heom stage mixed': 2.678e9,
+    # one RK4 stage over 256 members, summed over its sample passes (FP64: 24
+    # samples per pass, 67.98 B per element-stage; MIXED: 16 samples, 52.41 B)
+    'heom stage fp64': 2.610e9,
+    'heom stage mixed': 2.012e9,
     # the headline plan's t2 operator program (one work-group, M = 49)
     'propagate powers M=49 B=1 R=7 e=0,100,200,500': 8.21e4,
     'chunks prop mixed M=147 nc=8 B=1 steps=15 nsave=16': 4.30e6,
@@ -428,6 +442,8 @@ PMC_TRAFFIC = {
     # prep 1.366e8 + three Horner GEMMs 9.716e8 each + squaring 5.307e8
     # + diagonal 1.2e6 + final split 9.25e7
     'split propagator M=147 B=256 q=2': 3.676e9,
+    # the 5120-system absorption ensemble: operators 9.6 MB in, signals out
+    'response buckets B=5120 waves=832 nt=4096': 3.452e8,
 }
 
 
@@ -553,7 +569,8 @@ def ensemble_absorption_leg(torch, replays=5):
             'propagation_kernel': {'label': label, 'ms': ms, 'algorithmic_flop': flops,
                                    'achieved_TFLOPs': achieved, 'bound': 'valu',
                                    'peak_TFLOPs': PEAK_TFLOPS['fp64'],
-                                   'frac': achieved / PEAK_TFLOPS['fp64']},
+                                   'frac': achieved / PEAK_TFLOPS['fp64'],
+                                   'traffic': launch_traffic(label)},
             'kernels': {k: {'launches': v['count'] / 2, 'mean_ms': round(v['mean_ms'], 5)}
                         for k, v in summary.items()},
             'per_bucket_path': {'seconds_mixed': res['mixed'][0], 'seconds_fp64': res['fp64'][0],

@@ -51,10 +51,13 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
 #ifndef QSX_HEOM_ELEMS_PER_THREAD
-#define QSX_HEOM_ELEMS_PER_THREAD 1
+#define QSX_HEOM_ELEMS_PER_THREAD 2
 #endif
 // Liouville elements per thread and stage: a work-group covers
-// kElemsPerThread * 256 / M ADOs (with their edge tables in LDS)
+// kElemsPerThread * 256 / M ADOs (with their edge tables in LDS).  Measured
+// on the 7-site, 3060-ADO hierarchy (MI355X, passes of 24 samples): 2
+// elements per thread 9% faster than 1 (the work-group's set-up loads are
+// amortised over more elements), 4 no faster than 2.
 constexpr int kElemsPerThread = QSX_HEOM_ELEMS_PER_THREAD;
 
 struct HeomArgs {
@@ -370,9 +373,10 @@ inline int heom_samples_per_pass(int B, int64_t bytes_per_sample) {
     const int v = atoi(env);
     return v <= 0 ? B : std::min(B, v);
   }
-  // measured on the 7-site, 3060-ADO hierarchy (MI355X): 24 samples per pass
-  // in FP64 (9.6 MB each) and 32 in MIXED (7.2 MB) are the fastest
-  const int64_t budget = 240ll << 20;
+  // measured on the 7-site, 3060-ADO hierarchy (MI355X): passes of 24
+  // samples are the fastest in FP64 (9.6 MB each) and within 2% of the
+  // fastest in MIXED (10.8 MB each); 48 are 20-25% slower in both
+  const int64_t budget = 260ll << 20;
   int c = (int)std::max<int64_t>(1, budget / std::max<int64_t>(1, bytes_per_sample));
   if (c >= kXcds) c -= c % kXcds;
   return std::min(B, c);

@@ -18,11 +18,13 @@
 // p - 1 + E_ii (complex128).
 //
 // Complex products are four real MFMAs on real and imaginary planes staged
-// in LDS (re re, im im, re im, im re); a work-group of five waves computes an
-// 80 x 80 output tile, each wave a strip of five 16 x 16 blocks (the 147-
-// element block is 2 x 2 work-groups, so each operand row is read twice);
-// K in chunks of 32, the next chunk's loads issued before the current
-// chunk's MFMAs.  L is converted once to complex64 for the GEMM operands.
+// in LDS (re re, im im, re im, im re); a work-group of four waves computes a
+// 32 x 32 output tile, each wave a 16 x 16 block; K in chunks of 32, the next
+// chunk's loads issued before the current chunk's MFMAs.  Measured on the
+// 256-member, 147-element batch (MI355X): this tiling 0.55 ms per propagator
+// launch; 80 x 80 tiles of five waves (each operand row read twice instead
+// of five times, L converted once to complex64) 0.62 ms -- fewer and longer
+// work-groups hide the operand loads less well.
 
 #include "qsx_common.h"
 
@@ -31,33 +33,34 @@ namespace {
 
 typedef float qsx_float4 __attribute__((ext_vector_type(4)));
 
-// work-group tile kTile x kTile as kWaves waves, wave w owning the 16-row
-// strip w and its kTile / 16 output blocks of 16 x 16; K in chunks of kTK
-constexpr int kTile = 80, kTK = 32, kWaves = kTile / 16, kThreads = 64 * kWaves;
-constexpr int kEwThreads = 256;
+constexpr int kTile = 32, kTK = 32, kThreads = 256;
 
 // Epilogue kinds of split_gemm_kernel
 //   HORNER: out = alpha * (acc + (i != j ? A_ij : 0) * cvec_j)    (A = L)
 //   SQUARE: out = acc + (cvec_i + cvec_j) * B_ij                    (A = B = E)
 enum { kHorner = 0, kSquare = 1 };
 
-// C[b] (M x M, complex64) from A[b] and B[b] (complex64); cvec [b][M]
-// complex128.  Each thread stages kTile * kTK / kThreads elements of each
-// operand per chunk; the next chunk's loads are issued before this chunk's
-// MFMAs.
-template <int KIND>
+__device__ inline float2 ld_c64(const double2* p) {
+  const double2 v = *p;
+  return make_float2((float)v.x, (float)v.y);
+}
+__device__ inline float2 ld_c64(const float2* p) { return *p; }
+
+// C[b] (M x M, complex64) from A[b] (complex128 or complex64) and B[b]
+// (complex64); cvec [b][M] complex128.
+template <int KIND, typename TA>
 __global__ __launch_bounds__(kThreads) void split_gemm_kernel(
-    int M, const float2* __restrict__ A, const float2* __restrict__ B,
+    int M, const TA* __restrict__ A, int64_t sA, const float2* __restrict__ B,
     const double2* __restrict__ cvec, double alpha, float2* __restrict__ C) {
   __shared__ float Ar[kTK][kTile + 1], Ai[kTK][kTile + 1];
   __shared__ float Br[kTK][kTile + 1], Bi[kTK][kTile + 1];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int wm = wave % 2, wn = wave / 2;
   const int bm = blockIdx.x * kTile, bn = blockIdx.y * kTile, b = blockIdx.z;
   const int64_t mm = (int64_t)M * M;
-  const float2* Ab = A + b * mm;
+  const TA* Ab = A + b * sA;
   const float2* Bb = B + b * mm;
-  constexpr int kPer = kTile * kTK / kThreads;          // 8 elements per operand per thread
-  static_assert(kTile * kTK % kThreads == 0, "tile and chunk sizes");
+  constexpr int kPer = kTile * kTK / kThreads;          // 4 elements per operand per thread
   float2 av[kPer], bv[kPer];
   auto load_chunk = [&](int k0) {
 #pragma unroll
@@ -65,21 +68,13 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(
       const int e = tid + kThreads * u;
       const int mi = e / kTK, kk = e % kTK;              // A: walk k (contiguous)
       const int gi = bm + mi, gk = k0 + kk;
-      av[u] = (gi < M && gk < M) ? Ab[(int64_t)gi * M + gk] : make_float2(0.f, 0.f);
+      av[u] = (gi < M && gk < M) ? ld_c64(Ab + (int64_t)gi * M + gk) : make_float2(0.f, 0.f);
       const int kb = e / kTile, nj = e % kTile;          // B: walk n (contiguous)
       const int gkb = k0 + kb, gj = bn + nj;
       bv[u] = (gkb < M && gj < M) ? Bb[(int64_t)gkb * M + gj] : make_float2(0.f, 0.f);
     }
   };
-  constexpr int NB = kTile / 16;
-  qsx_float4 crr[NB], cii[NB], cri[NB], cir[NB];
-#pragma unroll
-  for (int t = 0; t < NB; ++t) {
-    crr[t] = qsx_float4{0.f, 0.f, 0.f, 0.f};
-    cii[t] = crr[t];
-    cri[t] = crr[t];
-    cir[t] = crr[t];
-  }
+  qsx_float4 crr = {0.f, 0.f, 0.f, 0.f}, cii = crr, cri = crr, cir = crr;
   const int r16 = lane % 16, q4 = lane / 16;
   load_chunk(0);
   for (int k0 = 0; k0 < M; k0 += kTK) {
@@ -95,57 +90,50 @@ __global__ __launch_bounds__(kThreads) void split_gemm_kernel(
     if (k0 + kTK < M) load_chunk(k0 + kTK);
     const int kend = min(kTK, M - k0);
     for (int kq = 0; kq < kend; kq += 4) {
-      const float ar = Ar[kq + q4][wave * 16 + r16], ai = Ai[kq + q4][wave * 16 + r16];
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const float br = Br[kq + q4][t * 16 + r16], bi = Bi[kq + q4][t * 16 + r16];
-        crr[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, crr[t], 0, 0, 0);
-        cii[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, cii[t], 0, 0, 0);
-        cri[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cri[t], 0, 0, 0);
-        cir[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cir[t], 0, 0, 0);
-      }
+      const float ar = Ar[kq + q4][wm * 16 + r16], ai = Ai[kq + q4][wm * 16 + r16];
+      const float br = Br[kq + q4][wn * 16 + r16], bi = Bi[kq + q4][wn * 16 + r16];
+      crr = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, crr, 0, 0, 0);
+      cii = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, cii, 0, 0, 0);
+      cri = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, bi, cri, 0, 0, 0);
+      cir = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, br, cir, 0, 0, 0);
     }
     __syncthreads();
   }
+  const qsx_float4 cr = crr - cii, ci = cri + cir;
   const double2* cv = cvec + (int64_t)b * M;
 #pragma unroll
-  for (int t = 0; t < NB; ++t) {
-    const qsx_float4 cr = crr[t] - cii[t], ci = cri[t] + cir[t];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      // v_mfma_f32_16x16x4f32 result layout: lane l, element v holds row
-      // 4 (l / 16) + v, column l % 16 of the 16 x 16 block
-      const int gi = bm + wave * 16 + 4 * q4 + v, gj = bn + t * 16 + r16;
-      if (gi >= M || gj >= M) continue;
-      double2 acc = make_double2(cr[v], ci[v]);
-      double2 out;
-      if (KIND == kHorner) {
-        if (gi != gj) {
-          const float2 x = Ab[(int64_t)gi * M + gj];
-          acc = cadd(acc, cmul(make_double2(x.x, x.y), cv[gj]));
-        }
-        out = make_double2(alpha * acc.x, alpha * acc.y);
-      } else {
-        const float2 x = Bb[(int64_t)gi * M + gj];
-        out = cadd(acc, cmul(make_double2(x.x, x.y), cadd(cv[gi], cv[gj])));
+  for (int v = 0; v < 4; ++v) {
+    // v_mfma_f32_16x16x4f32 result layout: lane l, element v holds row
+    // 4 (l / 16) + v, column l % 16 of the 16 x 16 block
+    const int gi = bm + wm * 16 + 4 * q4 + v, gj = bn + wn * 16 + r16;
+    if (gi >= M || gj >= M) continue;
+    double2 acc = make_double2(cr[v], ci[v]);
+    double2 out;
+    if (KIND == kHorner) {
+      if (gi != gj) {
+        const float2 x = ld_c64(Ab + (int64_t)gi * M + gj);
+        acc = cadd(acc, cmul(make_double2(x.x, x.y), cv[gj]));
       }
-      C[b * mm + (int64_t)gi * M + gj] = make_float2((float)out.x, (float)out.y);
+      out = make_double2(alpha * acc.x, alpha * acc.y);
+    } else {
+      const float2 x = Bb[(int64_t)gi * M + gj];
+      out = cadd(acc, cmul(make_double2(x.x, x.y), cadd(cv[gi], cv[gj])));
     }
+    C[b * mm + (int64_t)gi * M + gj] = make_float2((float)out.x, (float)out.y);
   }
 }
 
 // Diagonal Horner vectors t4, t3, t2, t1 = P4(h D), stored k-major as
 // tvec[k][B][M] (k = 0..3 for t4..t1), and dT4 = (h/4) O (complex64)
-__global__ __launch_bounds__(kEwThreads) void split_prep_kernel(
+__global__ __launch_bounds__(kThreads) void split_prep_kernel(
     const double2* __restrict__ L, int64_t sL, int M, int B, double h,
-    double2* __restrict__ tvec, float2* __restrict__ dT, float2* __restrict__ L32) {
+    double2* __restrict__ tvec, float2* __restrict__ dT) {
   const int64_t mm = (int64_t)M * M, total = mm * B;
-  for (int64_t e = (int64_t)blockIdx.x * kEwThreads + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kEwThreads) {
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
     const int64_t b = e / mm, r = e % mm;
     const int i = (int)(r / M), j = (int)(r % M);
     const double2 v = L[b * sL + r];
-    L32[e] = make_float2((float)v.x, (float)v.y);
     dT[e] = i == j ? make_float2(0.f, 0.f) : make_float2((float)(h / 4 * v.x), (float)(h / 4 * v.y));
     if (i == j) {
       // Horner on the scalar d: t4 = 1 + h d / 4, t3 = 1 + h d t4 / 3, ...
@@ -164,23 +152,23 @@ __global__ __launch_bounds__(kEwThreads) void split_prep_kernel(
 }
 
 // p <- p^2 (diagonal of a squared split), per batch entry
-__global__ __launch_bounds__(kEwThreads) void square_diag_kernel(const double2* __restrict__ p,
-                                                                  double2* __restrict__ p2,
-                                                                  int64_t n) {
-  for (int64_t e = (int64_t)blockIdx.x * kEwThreads + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * kEwThreads)
+__global__ __launch_bounds__(kThreads) void square_diag_kernel(const double2* __restrict__ p,
+                                                                double2* __restrict__ p2,
+                                                                int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * kThreads)
     p2[e] = cmul(p[e], p[e]);
 }
 
 // off = E with zero diagonal (complex64), diag = p - 1 + E_ii (complex128)
-__global__ __launch_bounds__(kEwThreads) void split_final_kernel(const float2* __restrict__ E,
+__global__ __launch_bounds__(kThreads) void split_final_kernel(const float2* __restrict__ E,
                                                                 const double2* __restrict__ p,
                                                                 int M, int B,
                                                                 float2* __restrict__ off,
                                                                 double2* __restrict__ diag) {
   const int64_t mm = (int64_t)M * M, total = mm * B;
-  for (int64_t e = (int64_t)blockIdx.x * kEwThreads + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * kEwThreads) {
+  for (int64_t e = (int64_t)blockIdx.x * kThreads + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * kThreads) {
     const int64_t b = e / mm, r = e % mm;
     const int i = (int)(r / M), j = (int)(r % M);
     const float2 x = E[e];
@@ -195,7 +183,7 @@ __global__ __launch_bounds__(kEwThreads) void split_final_kernel(const float2* _
 }
 
 inline dim3 ew_grid(int64_t total) {
-  int64_t g = (total + kEwThreads - 1) / kEwThreads;
+  int64_t g = (total + kThreads - 1) / kThreads;
   return dim3((unsigned)(g < 65535 * 16 ? g : 65535 * 16));
 }
 
@@ -204,9 +192,8 @@ inline dim3 ew_grid(int64_t total) {
 
 extern "C" int64_t qsx_split_propagator_workspace_bytes(int32_t M, int32_t B) {
   if (M < 1 || B < 1) return -1;
-  // three complex64 M x M buffers (L, two ping-pong) and five complex128
-  // vectors of M per entry
-  return (int64_t)B * ((int64_t)3 * M * M * sizeof(float2) + (int64_t)5 * M * sizeof(double2));
+  // two complex64 M x M buffers and five complex128 vectors of M per entry
+  return (int64_t)B * ((int64_t)2 * M * M * sizeof(float2) + (int64_t)5 * M * sizeof(double2));
 }
 
 extern "C" int32_t qsx_rk4_propagator_split(const void* L, int64_t strideL, int32_t M, int32_t B,
@@ -227,14 +214,13 @@ extern "C" int32_t qsx_rk4_propagator_split(const void* L, int64_t strideL, int3
   const int64_t mm = (int64_t)M * M;
   float2* X0 = static_cast<float2*>(workspace);
   float2* X1 = X0 + mm * B;
-  float2* L32 = X1 + mm * B;
-  double2* tvec = reinterpret_cast<double2*>(L32 + mm * B);  // [4][B][M]: t4, t3, t2, t1
+  double2* tvec = reinterpret_cast<double2*>(X1 + mm * B);   // [4][B][M]: t4, t3, t2, t1
   const int64_t bm = (int64_t)B * M;
   double2* pv[2] = {tvec + 3 * bm, tvec + 4 * bm};           // t1, then the squarings
   const double2* Lp = static_cast<const double2*>(L);
   const int64_t sL = B > 1 ? strideL : 0;
-  hipLaunchKernelGGL(split_prep_kernel, ew_grid(mm * B), dim3(kEwThreads), 0, s, Lp, sL, M, B, h,
-                     tvec, X0, L32);
+  hipLaunchKernelGGL(split_prep_kernel, ew_grid(mm * B), dim3(kThreads), 0, s, Lp, sL, M, B, h,
+                     tvec, X0);
   QSX_CHECK_HIP(hipGetLastError());
   const dim3 grid((M + kTile - 1) / kTile, (M + kTile - 1) / kTile, B);
   // dT3 = h/3 (L dT4 + O t4), dT2 = h/2 (L dT3 + O t3), dT1 = h (L dT2 + O t2)
@@ -242,8 +228,8 @@ extern "C" int32_t qsx_rk4_propagator_split(const void* L, int64_t strideL, int3
   float2* nxt = X1;
   const double coef[3] = {h / 3.0, h / 2.0, h};
   for (int st = 0; st < 3; ++st) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(split_gemm_kernel<kHorner>), grid, dim3(kThreads), 0, s,
-                       M, L32, cur, tvec + st * bm, coef[st], nxt);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(split_gemm_kernel<kHorner, double2>), grid,
+                       dim3(kThreads), 0, s, M, Lp, sL, cur, tvec + st * bm, coef[st], nxt);
     QSX_CHECK_HIP(hipGetLastError());
     float2* t = cur;
     cur = nxt;
@@ -252,10 +238,10 @@ extern "C" int32_t qsx_rk4_propagator_split(const void* L, int64_t strideL, int3
   // P4 = diag(t1) + dT1; q = 2^k: square k times
   int pp = 0;
   for (int qq = q; qq > 1; qq >>= 1) {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(split_gemm_kernel<kSquare>), grid, dim3(kThreads), 0, s,
-                       M, cur, cur, pv[pp], 1.0, nxt);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(split_gemm_kernel<kSquare, float2>), grid, dim3(kThreads),
+                       0, s, M, cur, mm, cur, pv[pp], 1.0, nxt);
     QSX_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(square_diag_kernel, ew_grid((int64_t)M * B), dim3(kEwThreads), 0, s,
+    hipLaunchKernelGGL(square_diag_kernel, ew_grid((int64_t)M * B), dim3(kThreads), 0, s,
                        pv[pp], pv[pp ^ 1], (int64_t)M * B);
     QSX_CHECK_HIP(hipGetLastError());
     pp ^= 1;
@@ -263,7 +249,7 @@ extern "C" int32_t qsx_rk4_propagator_split(const void* L, int64_t strideL, int3
     cur = nxt;
     nxt = t;
   }
-  hipLaunchKernelGGL(split_final_kernel, ew_grid(mm * B), dim3(kEwThreads), 0, s, cur, pv[pp], M,
+  hipLaunchKernelGGL(split_final_kernel, ew_grid(mm * B), dim3(kThreads), 0, s, cur, pv[pp], M,
                      B, static_cast<float2*>(off), static_cast<double2*>(diag));
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
