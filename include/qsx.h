@@ -224,12 +224,14 @@ int32_t qsx_propagate_adjoint(const void* A, int64_t strideA, int32_t M, const v
 
 /* Linear-response signals of many problems of unequal size in one launch:
  * for problem b, y_{k+1} = P_b y_k from y0_b and s_k = v_b . y_k,
- * k = 0 .. nt-1, written to out [B][nt] complex128 (FP64 products).
- * waves int32 [nwaves][4] (device): first problem, problem count, M, G per
- * wavefront, with G a power of two above M, G * count <= 64 and M at most
- * qsx_response_buckets_max_states(); offsets int64 [B][2] (device): offset
- * of P_b (M^2 complex elements, row-major) in P and of y0_b, v_b (M each) in
- * y0 and v.  status int32 [B]: 1 where a signal value was not finite.
+ * k = 0 .. nt-1, written to row r_b of out [B][nt] complex128 (FP64
+ * products).  waves int32 [nwaves][4] (device): first problem, problem count,
+ * M, G per wavefront, with G a power of two above M, G * count <= 64 and M at
+ * most qsx_response_buckets_max_states(); offsets int64 [B][3] (device):
+ * offset of P_b (M^2 complex elements, row-major) in P, of y0_b, v_b (M each)
+ * in y0 and v, and the output row r_b (a permutation of 0 .. B-1, so the
+ * signals can leave in the caller's order).  status int32 [B]: 1 at r_b where
+ * a signal value was not finite.
  * Replaces the per-member zvode runs of linear_response / absorption_spectra
  * (qspectra/simulate/response.py:15-43, :103-154; simulate/utils.py:16-109)
  * over an ensemble: P_b = P4(h L_b)^q of the member's Heisenberg 'eg' block,

@@ -345,10 +345,12 @@ def redfield_block_vjp(H1, rw, T, lam, gam, X, Y, flags, unit_convert, gL, strea
 def response_buckets_into(waves, offsets, P, y0, v, out, status, stream=None):
     """out [B, nt] <- s_k = v_b . P_b^k y0_b for problems of unequal size in
     one launch (qsx_response_buckets; the tables are device int32 [nw, 4] and
-    int64 [B, 2], P, y0, v flat complex128)."""
+    int64 [B, 3] (P offset, y0/v offset, output row), P, y0, v flat
+    complex128)."""
     lib = require_native()
     assert waves.dtype == torch.int32 and waves.is_contiguous() and waves.shape[1] == 4
-    assert offsets.dtype == torch.int64 and offsets.is_contiguous()
+    assert offsets.dtype == torch.int64 and offsets.is_contiguous() and offsets.shape[1] == 3
+    assert offsets.shape[0] == out.shape[0]
     for X in (P, y0, v, out):
         assert X.dtype == torch.complex128 and X.is_contiguous()
     assert status.dtype == torch.int32 and status.numel() == out.shape[0]

@@ -120,7 +120,7 @@ class AbsorptionEnsemblePlan(object):
             for w0 in range(0, B, per):
                 waves.append((first + w0, min(per, B - w0), M, G))
             for k in range(B):
-                offs.append((offP + k * M * M, offY + k * M))
+                offs.append((offP + k * M * M, offY + k * M, idx[k]))
             self.buckets.append(dict(n=n, B=B, M=M, inputs=inputs, rebuild=rebuild,
                                      L=torch.empty((B, M, M), dtype=torch.complex128,
                                                    device=dev),
@@ -140,9 +140,6 @@ class AbsorptionEnsemblePlan(object):
         self.offsets = torch.as_tensor(np.asarray(offs, dtype=np.int64)).to(dev)
         self.signal = torch.empty((self.B, nt), dtype=torch.complex128, device=dev)
         self.status = torch.zeros(self.B, dtype=torch.int32, device=dev)
-        inv = np.empty(self.B, dtype=np.int64)
-        inv[self.order] = np.arange(self.B)
-        self.inverse = torch.as_tensor(inv).to(dev)        # input row -> plan row
         if correlation_decay_time is not None:
             self._decay = torch.as_tensor(np.exp(-self.t / correlation_decay_time)).to(dev)
         self._check_tables(waves, offs, offP, offY)
@@ -160,12 +157,14 @@ class AbsorptionEnsemblePlan(object):
     @staticmethod
     def _check_tables(waves, offs, nP, nY):
         """Host check that every wavefront's problems and every offset stay
-        inside the packed buffers the kernel reads."""
+        inside the packed buffers the kernel reads, and that the output rows
+        are a permutation."""
         B = len(offs)
+        assert sorted(o[2] for o in offs) == list(range(B))
         for first, count, M, G in waves:
             assert G * count <= 64 and G >= M + 1 and 0 <= first and first + count <= B
             for b in range(first, first + count):
-                p, y = offs[b]
+                p, y, _ = offs[b]
                 assert 0 <= p and p + M * M <= nP and 0 <= y and y + M <= nY
 
     def hamiltonian_inputs(self, n):
@@ -188,10 +187,13 @@ class AbsorptionEnsemblePlan(object):
             main.wait_stream(side)
         _native.response_buckets_into(self.waves, self.offsets, self.P, self.y0, self.v,
                                       self.signal, self.status)
+        # the signal rows are in input order; the response's sign goes into
+        # the transform's scale
         x = self.signal if self.decay is None else self.signal * self._decay
-        f, X = fourier_transform_device(self.t, -x, -1, self.rw_freq, self.unit_convert)
+        f, X = fourier_transform_device(self.t, x, -1, self.rw_freq, self.unit_convert,
+                                        scale=-1.0)
         self.f = f
-        return torch.index_select(X.real, 0, self.inverse)
+        return X.real
 
     def capture(self):
         """Records the whole plan into one HIP graph, replayed by run()."""

@@ -20,10 +20,11 @@
 // Work description (device, built by the host once per plan):
 //   waves   int32 [nwaves][4]: first problem, problems in the wave, M, G
 //           (every problem of a wave has the same M; G * count <= 64)
-//   offs    int64 [B][2]: offset of P_b in P (complex elements, M_b^2 of
-//           them, row-major) and of y0_b / v_b in y0 / v (M_b each)
-//   out     complex128 [B][nt]; status int32 [B]: 1 when a signal value
-//           was not finite.
+//   offs    int64 [B][3]: offset of P_b in P (complex elements, M_b^2 of
+//           them, row-major), of y0_b / v_b in y0 / v (M_b each), and the
+//           output row r_b of problem b
+//   out     complex128 [B][nt], row r_b; status int32 [B]: 1 at r_b when a
+//           signal value was not finite.
 
 #include "qsx_common.h"
 
@@ -42,17 +43,19 @@ __device__ void bucket_wave(const int4 w, const int64_t* __restrict__ offs,
                             const double2* __restrict__ P, const double2* __restrict__ y0,
                             const double2* __restrict__ v, int nt, double2* __restrict__ out,
                             int32_t* __restrict__ status, double2 (*ys)[kWave],
-                            double2 (*sbuf)[kWave / 2]) {
+                            double2 (*sbuf)[kWave / 2], int64_t* rows) {
   const int lane = threadIdx.x, G = w.w;
   const int g = lane / G, i = lane - g * G;
   const bool live = g < w.y;
   const int64_t mem = (int64_t)w.x + g;
   const bool srow = live && i < M, prow = live && i == M;
-  int64_t offP = 0, offY = 0;
+  int64_t offP = 0, offY = 0, orow = 0;
   if (live) {
-    offP = offs[2 * mem];
-    offY = offs[2 * mem + 1];
+    offP = offs[3 * mem];
+    offY = offs[3 * mem + 1];
+    orow = offs[3 * mem + 2];
   }
+  if (live && i == 0) rows[g] = orow;   // for the coalesced flushes
   double2 row[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) {
@@ -104,12 +107,12 @@ __device__ void bucket_wave(const int4 w, const int64_t* __restrict__ offs,
       const int k0 = k - kk, cnt = kk + 1;
       for (int e = lane; e < w.y * cnt; e += kWave) {
         const int gi = e / cnt, s = e - gi * cnt;
-        out[((int64_t)w.x + gi) * nt + k0 + s] = sbuf[s][gi];
+        out[rows[gi] * nt + k0 + s] = sbuf[s][gi];
       }
       __syncthreads();
     }
   }
-  if (prow) status[mem] = bad ? 1 : 0;
+  if (prow) status[orow] = bad ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kWave) void response_buckets_kernel(
@@ -121,10 +124,11 @@ __global__ __launch_bounds__(kWave) void response_buckets_kernel(
   // s_k of every group, kFlush steps at a time: the stores leave the step
   // loop, so its barriers do not wait on them, and go out coalesced
   __shared__ double2 sbuf[kFlush][kWave / 2];
+  __shared__ int64_t rows[kWave / 2];   // output row of each group
   const int4 w = waves[blockIdx.x];
   switch (w.z) {   // M, the same for every problem of the wavefront
 #define QSX_BUCKET_M(m_) \
-  case m_: bucket_wave<m_>(w, offs, P, y0, v, nt, out, status, ys, sbuf); break;
+  case m_: bucket_wave<m_>(w, offs, P, y0, v, nt, out, status, ys, sbuf, rows); break;
     QSX_BUCKET_M(1) QSX_BUCKET_M(2) QSX_BUCKET_M(3) QSX_BUCKET_M(4) QSX_BUCKET_M(5)
     QSX_BUCKET_M(6) QSX_BUCKET_M(7) QSX_BUCKET_M(8) QSX_BUCKET_M(9) QSX_BUCKET_M(10)
     QSX_BUCKET_M(11) QSX_BUCKET_M(12) QSX_BUCKET_M(13) QSX_BUCKET_M(14) QSX_BUCKET_M(15)

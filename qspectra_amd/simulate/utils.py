@@ -446,12 +446,6 @@ def _symmetrize(t, x, axis=-1):
     return t_sym, x_sym
 
 
-# transforms of at most this many multiply-adds (rows x N x samples) run as
-# one direct sum (qsx_fourier_transform, nothing compiled at run time);
-# larger batches take the library FFT
-DIRECT_DFT_MAX_MACS = 1 << 33
-
-
 class _SymmetricFourier(torch.autograd.Function):
     """The transform of the last axis onto the zero-padded symmetric grid
     (qsx_fourier_transform); backward: its conjugate transpose."""
@@ -468,11 +462,12 @@ class _SymmetricFourier(torch.autograd.Function):
 
 
 def fourier_transform_device(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1,
-                             convention='angular'):
+                             convention='angular', scale=1.0):
     """fourier_transform on a complex128 device tensor x (result stays on the
     device; differentiable): the zero-padded symmetric grid, ifftshift, DFT,
-    fftshift of utils.py:128-219, evaluated as one direct sum by
-    qsx_fourier_transform (or, above DIRECT_DFT_MAX_MACS, the library FFT)."""
+    fftshift of utils.py:128-219, evaluated by qsx_fourier_transform (a
+    direct sum for short rows, the chirp-z form with FFTs in LDS for long
+    ones), the result times ``scale``."""
     t = np.asarray(t, dtype=float)
     if t.ndim != 1:
         raise ValueError('t must be one dimensional')
@@ -490,20 +485,7 @@ def fourier_transform_device(t, x, axis=-1, rw_freq=0, unit_convert=1, sign=1,
     N = t_sym.size
     dt = t_sym[1] - t_sym[0]
     xm = x.to(torch.complex128).movedim(ax, -1)
-    rows = int(np.prod(xm.shape[:-1])) if xm.dim() > 1 else 1
-    if rows * N * t.size <= DIRECT_DFT_MAX_MACS:
-        X = _SymmetricFourier.apply(xm, N, lo, sign, float(dt)).movedim(-1, ax)
-    else:
-        # large batches: the library FFT (O(N log N) per row; a new length is
-        # compiled on its first use, which such a batch amortises)
-        shape = list(xm.shape)
-        shape[-1] = N
-        xs = torch.zeros(shape, dtype=torch.complex128, device=x.device)
-        xs[..., lo:lo + t.size] = xm * dt
-        X = torch.fft.fftshift(torch.fft.fft(torch.fft.ifftshift(xs, dim=-1), dim=-1), dim=-1)
-        if sign == 1:
-            X = torch.flip(X, dims=[-1])
-        X = X.movedim(-1, ax)
+    X = _SymmetricFourier.apply(xm, N, lo, sign, float(dt) * float(scale)).movedim(-1, ax)
     f = np.fft.fftshift(np.fft.fftfreq(N, dt * unit_convert))
     if sign == 1:
         f = -f[::-1]

@@ -39,6 +39,19 @@ def test_argument_validation_without_device():
     assert st == -1 and 'precision' in _native.last_error()
 
 
+def test_fourier_workspace_covers_the_chirp_tables():
+    """Short grids need the N-entry twiddle table only; from 2048 points on
+    the workspace also holds the 2N-entry half-angle table and one 8192-point
+    kernel spectrum per piece/chunk pair (two for 4096 samples onto 8191
+    points, the absorption grid)."""
+    from qspectra_amd import _native
+    lib = _native.load_library()
+    assert lib.qsx_fourier_workspace_bytes(0) == -1
+    assert lib.qsx_fourier_workspace_bytes(255) == 255 * 16
+    assert lib.qsx_fourier_workspace_bytes(2047) == 2047 * 16
+    assert lib.qsx_fourier_workspace_bytes(8191) >= (3 * 8191 + 2 * 8192) * 16
+
+
 def _c_kind(param):
     """ctypes class a C parameter declaration passes as."""
     import ctypes
