@@ -532,6 +532,33 @@ int32_t qsx_dense_eigh(const double* H, int32_t nb, int32_t B, double* E, double
                        void* workspace, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Complex-Hermitian Hamiltonians (check_hermitian, qspectra/hamiltonian.py:
+ * 18-22).  H1 [B][n][n] complex128: the non-rotating one-exciton Hamiltonian
+ * of each problem; Hilbert block X (0 g, 1 e, 2 f) as for the real entry
+ * points, diagonal shifted by -X rw[b] (rw may be null).
+ *
+ * qsx_block_eigh_hermitian: E [B][nb] ascending, U [B][nb][nb] complex128
+ * (eigenvectors as columns, each with its largest-magnitude component real
+ * and positive), by complex Jacobi rotations in LDS, nb <= 64.  Replaces
+ * Hamiltonian.eig (hamiltonian.py:310-328, scipy eigh) for complex H.
+ * qsx_dense_eigh_hermitian: the same for dense Hermitian H [B][nb][nb].
+ * qsx_unitary_block_hermitian: the XY block of -i unit [H, .] (complex128
+ * [B][M][M], M = nX nY, column-stacked; its transpose for heisenberg != 0),
+ * qspectra/dynamics/unitary.py:5-9 with liouville_space.py:93-131.
+ * qsx_unitary_block_dense_hermitian: the same from dense blocks HX [B][nX][nX],
+ * HY [B][nY][nY] complex128. */
+int32_t qsx_block_eigh_hermitian(const void* H1, int32_t n, int32_t B, int32_t X,
+                                 const double* rw, double* E, void* U, void* stream);
+int32_t qsx_dense_eigh_hermitian(const void* H, int32_t nb, int32_t B, double* E, void* U,
+                                 void* stream);
+int32_t qsx_unitary_block_hermitian(const void* H1, int32_t n, int32_t B, const double* rw,
+                                    double unit_convert, int32_t X, int32_t Y,
+                                    int32_t heisenberg, void* L, void* stream);
+int32_t qsx_unitary_block_dense_hermitian(const void* HX, const void* HY, int32_t nX,
+                                          int32_t nY, int32_t B, double unit_convert,
+                                          int32_t heisenberg, void* L, void* stream);
+
+/* ---------------------------------------------------------------------------
  * fourier_transform (qspectra/simulate/utils.py:128-219): x [batch][n]
  * complex128 sampled at dt, zero-padded to the grid of N points symmetric
  * about t = 0 with the samples at lo .. lo+n-1, scaled by dt, ifftshift, DFT,

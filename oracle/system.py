@@ -383,7 +383,8 @@ class VibronicHamiltonian(object):
         el = all_states(self.n_sites, subspace)
         vib = self.vib_states()
         basis = [(i, v) for i in range(len(el)) for v in vib]
-        H = np.zeros((len(basis), len(basis)))
+        # complex when the electronic part is (the reference's tensor products keep it)
+        H = np.zeros((len(basis), len(basis)), dtype=np.result_type(Hel, float))
         for r, (i, v) in enumerate(basis):
             for c, (j, u) in enumerate(basis):
                 h = Hel[i, j] if v == u else 0.0

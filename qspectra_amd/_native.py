@@ -95,6 +95,12 @@ SIGNATURES = {
                                  c_dbl, c_i32, c_dbl, c_vp, c_vp, c_vp],
     'qsx_unitary_block_dense': [c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_i32, c_vp, c_vp],
     'qsx_dense_eigh': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    'qsx_block_eigh_hermitian': [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp],
+    'qsx_dense_eigh_hermitian': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp],
+    'qsx_unitary_block_hermitian': [c_vp, c_i32, c_i32, c_vp, c_dbl, c_i32, c_i32, c_i32, c_vp,
+                                    c_vp],
+    'qsx_unitary_block_dense_hermitian': [c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_i32, c_vp,
+                                          c_vp],
     'qsx_fourier_workspace_bytes': [c_i32],
     'qsx_fourier_transform': [c_vp, c_i64, c_i32, c_i32, c_i32, c_i32, ctypes.c_double, c_i32,
                               c_vp, c_vp, c_vp],
@@ -256,11 +262,19 @@ def _workspace(n, X, Y, B, dev):
 
 
 def block_eigh(H1, X, rw=None, stream=None):
-    """E [B, nb], U [B, nb, nb] of Hilbert block X of H1 - X*rw (float64)."""
+    """E [B, nb], U [B, nb, nb] of Hilbert block X of H1 - X*rw (float64,
+    or complex128 for complex-Hermitian H1: then U is complex128)."""
     lib = require_native()
     B, n, _ = H1.shape
-    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    assert H1.is_contiguous()
     nb = block_size(X, n)
+    if H1.dtype == torch.complex128:
+        E = torch.empty((B, nb), dtype=torch.float64, device=H1.device)
+        U = torch.empty((B, nb, nb), dtype=torch.complex128, device=H1.device)
+        check(lib.qsx_block_eigh_hermitian(ptr(H1), n, B, X, ptr(rw), ptr(E), ptr(U),
+                                           stream_ptr(stream)), 'qsx_block_eigh_hermitian')
+        return E, U
+    assert H1.dtype == torch.float64
     E = torch.empty((B, nb), dtype=torch.float64, device=H1.device)
     U = torch.empty((B, nb, nb), dtype=torch.float64, device=H1.device)
     ws = _workspace(n, X, X, B, H1.device)
@@ -371,12 +385,19 @@ def redfield_workspace(n, X, Y, B, dev=None):
 
 
 def unitary_block(H1, rw, unit_convert, X, Y, heisenberg=False, stream=None):
-    """Unitary Liouvillian block XY, complex128 [B, M, M]."""
+    """Unitary Liouvillian block XY, complex128 [B, M, M] (H1 float64, or
+    complex128 for complex-Hermitian Hamiltonians)."""
     lib = require_native()
     B, n, _ = H1.shape
-    assert H1.dtype == torch.float64 and H1.is_contiguous()
+    assert H1.is_contiguous()
     M = block_size(X, n) * block_size(Y, n)
     L = torch.empty((B, M, M), dtype=torch.complex128, device=H1.device)
+    if H1.dtype == torch.complex128:
+        check(lib.qsx_unitary_block_hermitian(ptr(H1), n, B, ptr(rw), float(unit_convert), X, Y,
+                                              int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
+              'qsx_unitary_block_hermitian')
+        return L
+    assert H1.dtype == torch.float64
     check(lib.qsx_unitary_block(ptr(H1), n, B, ptr(rw), float(unit_convert), X, Y,
                                 int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
           'qsx_unitary_block')
@@ -955,8 +976,17 @@ def zofe_propagate(eom, y0, nsub, h, proj=None, V=None, F=None, stream=None):
 
 def dense_eigh(H, stream=None):
     """qsx_dense_eigh: E [B, nb], U [B, nb, nb] of real symmetric H [B, nb, nb]
-    (float64 device tensor)."""
+    (float64 device tensor); complex Hermitian H (complex128) through
+    qsx_dense_eigh_hermitian, U then complex128."""
     lib = require_native()
+    if H.is_complex():
+        H = H.to(torch.complex128).contiguous()
+        B, nb, _ = H.shape
+        E = torch.empty((B, nb), dtype=torch.float64, device=H.device)
+        U = torch.empty((B, nb, nb), dtype=torch.complex128, device=H.device)
+        check(lib.qsx_dense_eigh_hermitian(ptr(H), nb, B, ptr(E), ptr(U), stream_ptr(stream)),
+              'qsx_dense_eigh_hermitian')
+        return E, U
     H = H.to(torch.float64).contiguous()
     B, nb, _ = H.shape
     nbytes = lib.qsx_redfield_dense_workspace_bytes(1, nb, nb, B)
@@ -988,12 +1018,21 @@ def redfield_block_dense(HX, HY, occX, occY, T, lam, gam, flags, unit_convert, s
 
 
 def unitary_block_dense(HX, HY, unit_convert, heisenberg=False, stream=None):
-    """qsx_unitary_block_dense: -i unit [H, .] on the XY block."""
+    """qsx_unitary_block_dense: -i unit [H, .] on the XY block (complex128
+    blocks through qsx_unitary_block_dense_hermitian)."""
     lib = require_native()
     B, nX, _ = HX.shape
     nY = HY.shape[-1]
     M = nX * nY
     L = torch.empty((B, M, M), dtype=torch.complex128, device=HX.device)
+    if HX.is_complex() or HY.is_complex():
+        HX = HX.to(torch.complex128).contiguous()
+        HY = HY.to(torch.complex128).contiguous()
+        check(lib.qsx_unitary_block_dense_hermitian(ptr(HX), ptr(HY), nX, nY, B,
+                                                    float(unit_convert), int(bool(heisenberg)),
+                                                    ptr(L), stream_ptr(stream)),
+              'qsx_unitary_block_dense_hermitian')
+        return L
     check(lib.qsx_unitary_block_dense(ptr(HX), ptr(HY), nX, nY, B, float(unit_convert),
                                       int(bool(heisenberg)), ptr(L), stream_ptr(stream)),
           'qsx_unitary_block_dense')

@@ -8,6 +8,16 @@ from .. import _native
 from ..bath import debye_parameters, is_debye_form
 from .liouville_space import LiouvilleSpaceModel, all_liouville_subspaces
 
+# The reference moves the Redfield Liouvillian to the site basis with
+# kron(U, U) (operator_tools.py:43-56, redfield.py:95-104), which is the
+# basis change of rho -> U^H rho U only for a real U; for a complex-Hermitian
+# H its result depends on the phases eigh happens to give the eigenvectors,
+# so there is no reference result to reproduce.
+COMPLEX_REDFIELD = ('Redfield dynamics of a complex-Hermitian Hamiltonian: the reference '
+                    'transforms the Liouvillian with kron(U, U), which for complex '
+                    'eigenvectors depends on their arbitrary phases; use UnitaryModel, '
+                    'HEOMModel or ZOFEModel, or a real Hamiltonian')
+
 
 class RedfieldModel(LiouvilleSpaceModel):
     """Redfield dynamics; each site couples to an identical independent bath.
@@ -31,6 +41,8 @@ class RedfieldModel(LiouvilleSpaceModel):
                 | (_native.QSX_HEISENBERG if heisenberg else 0))
 
     def _build_block(self, X, Y, heisenberg, H1, rw):
+        if H1.is_complex():
+            raise NotImplementedError(COMPLEX_REDFIELD)
         bath = self.hamiltonian.bath
         if not is_debye_form(bath):
             return self._build_tabulated_block(X, Y, heisenberg, H1, rw)
@@ -70,6 +82,8 @@ class RedfieldModel(LiouvilleSpaceModel):
                                       '(DebyeBath or UncoupledBath)')
         T, lam, gam = debye_parameters(self.hamiltonian.bath)
         H1, rw = inputs
+        if H1.is_complex():
+            raise NotImplementedError(COMPLEX_REDFIELD)
         ws = _native.redfield_workspace(H1.shape[-1], X, Y, H1.shape[0], H1.device)
         flags = self._flags(heisenberg)
 
@@ -81,6 +95,8 @@ class RedfieldModel(LiouvilleSpaceModel):
     def _build_dense_block(self, X, Y, heisenberg, dense):
         T, lam, gam = debye_parameters(self.hamiltonian.bath)
         (HX, oX), (HY, oY) = dense[X], dense[Y]
+        if HX.is_complex() or HY.is_complex():
+            raise NotImplementedError(COMPLEX_REDFIELD)
         return _native.redfield_block_dense(HX, HY, oX, oY, T, lam, gam,
                                             self._flags(heisenberg), self.unit_convert)
 

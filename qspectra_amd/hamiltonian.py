@@ -46,6 +46,11 @@ def check_hermitian(matrix):
     return m
 
 
+def _is_complex(H):
+    """True for a matrix with a non-zero imaginary part."""
+    return np.iscomplexobj(H) and bool(np.any(np.imag(H) != 0))
+
+
 def _state_from_eig(E, U, temperature):
     """Thermal (T > 0) or ground (T <= 0) density matrix from an
     eigensystem: U diag(w) U^T / sum(w), with w = exp(-(E - E_min)/T), or the
@@ -89,7 +94,7 @@ def electronic_thermal_states(hamiltonians, subspace):
     hot = T > 0
     w = np.where(hot[:, None], np.exp(-dE / np.where(hot, T, 1.0)[:, None]), dE == 0)
     w = w / w.sum(axis=1, keepdims=True)
-    return np.einsum('bij,bj,bkj->bik', U, w, U).astype(complex)
+    return np.einsum('bij,bj,bkj->bik', U, w, U.conj()).astype(complex)
 
 
 def add_braket(labels):
@@ -381,13 +386,12 @@ class ElectronicHamiltonian(Hamiltonian):
     @property
     def H1_nonrotating(self):
         """One-exciton Hamiltonian in the lab frame (with this sample's
-        disorder), the input of the GPU eigen- and Liouvillian builders."""
-        return np.asarray(self.lab_frame.H_1exc, dtype=float)
+        disorder), the input of the GPU eigen- and Liouvillian builders:
+        float64, or complex128 for a complex-Hermitian H_1exc."""
+        H = self.lab_frame.H_1exc
+        return np.asarray(H, dtype=complex if _is_complex(H) else float)
 
     def _eig_blocks(self, subspace):
-        if np.iscomplexobj(self.H_1exc) and np.any(np.imag(self.H_1exc) != 0):
-            raise NotImplementedError('complex Hermitian H_1exc: the GPU eigensolver handles '
-                                      'real symmetric Hamiltonians')
         dev = _native.device()
         H1 = torch.as_tensor(self.H1_nonrotating[None]).to(dev)
         rw = torch.tensor([float(self.rw_freq)], dtype=torch.float64, device=dev)
@@ -403,7 +407,7 @@ class ElectronicHamiltonian(Hamiltonian):
             Us.append(U[0].cpu().numpy())
             xs.append(np.full(len(Es[-1]), X))
         E = np.concatenate(Es)
-        U = np.zeros((len(E), len(E)))
+        U = np.zeros((len(E), len(E)), dtype=np.result_type(*Us))
         k = 0
         for u in Us:
             U[k:k + len(u), k:k + len(u)] = u
@@ -552,13 +556,12 @@ class VibronicHamiltonian(Hamiltonian):
     # ---------------------------------------------------------- device data
     def dense_block(self, X, hilbert_subspace):
         """This Hamiltonian (in its frame) restricted to Hilbert block X of
-        the given subspace, as a dense float64 matrix."""
+        the given subspace, as a dense float64 matrix (complex128 when the
+        Hamiltonian is complex Hermitian)."""
         idx = self.hilbert_subspace_index(X, hilbert_subspace)
         H = np.asarray(self.H(hilbert_subspace))
-        if np.iscomplexobj(H) and np.any(np.imag(H) != 0):
-            raise NotImplementedError('complex Hermitian vibronic Hamiltonians: the GPU '
-                                      'eigensolver is real symmetric')
-        return np.ascontiguousarray(np.real(H)[idx, idx])
+        H = H if _is_complex(H) else np.real(H)
+        return np.ascontiguousarray(H[idx, idx])
 
     def _eig_blocks(self, subspace):
         dev = _native.device()
@@ -572,7 +575,7 @@ class VibronicHamiltonian(Hamiltonian):
             Us.append(U[0].cpu().numpy())
             xs.append(np.full(len(Es[-1]), 'gef'.index(s)))
         E = np.concatenate(Es)
-        U = np.zeros((len(E), len(E)))
+        U = np.zeros((len(E), len(E)), dtype=np.result_type(*Us))
         k = 0
         for u in Us:
             U[k:k + len(u), k:k + len(u)] = u
