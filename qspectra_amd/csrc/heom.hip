@@ -51,13 +51,14 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kXcds = 8;
 #ifndef QSX_HEOM_ELEMS_PER_THREAD
-#define QSX_HEOM_ELEMS_PER_THREAD 2
+#define QSX_HEOM_ELEMS_PER_THREAD 3
 #endif
 // Liouville elements per thread and stage: a work-group covers
 // kElemsPerThread * 256 / M ADOs (with their edge tables in LDS).  Measured
 // on the 7-site, 3060-ADO hierarchy (MI355X, passes of 24 samples): 2
 // elements per thread 9% faster than 1 (the work-group's set-up loads are
-// amortised over more elements), 4 no faster than 2.
+// amortised over more elements); with the compile-time commutator kernel 3
+// are 3-5% faster than 2, 4 no faster than 3.
 constexpr int kElemsPerThread = QSX_HEOM_ELEMS_PER_THREAD;
 
 struct HeomArgs {
@@ -153,6 +154,45 @@ struct HeomStageBufs {
   cplx_t<T>* z[4];
   cplx_t<T>* k[3];
 };
+
+// The stage's output for element gi given its derivative k = F(zin)[gi] and
+// zp = zin[gi] (see the file comment for the buffers): st = 4 writes
+// zout = k; otherwise the RK4 stage st.
+template <typename T>
+__device__ __forceinline__ void stage_epilogue(double2 k, double2 zp, int64_t gi, int st, double h,
+                                               const HeomStageBufs<T>& bf,
+                                               cplx_t<T>* __restrict__ zout) {
+  if (st == 4) {                           // plain application: zout = F(zin)
+    zout[gi] = from_d2<T>(k);
+    return;
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    if (st == 0) {                         // zin is y: z2 = y + h/2 k1
+      bf.z[1][gi] = caxpy(0.5 * h, k, zp);
+    } else if (st < 3) {                   // z3 = y + h/2 k2, z4 = y + h k3
+      bf.z[st + 1][gi] = caxpy(st == 1 ? 0.5 * h : h, k, bf.y[gi]);
+    } else {                               // y' from the stage vectors and k4
+      const double2 yv = bf.y[gi], z2 = bf.z[1][gi], z3 = bf.z[2][gi];
+      const double dr = ((z2.x - yv.x) + 2.0 * (z3.x - yv.x) + (zp.x - yv.x)) / 3.0;
+      const double di = ((z2.y - yv.y) + 2.0 * (z3.y - yv.y) + (zp.y - yv.y)) / 3.0;
+      bf.y[gi] = caxpy(h / 6.0, k, make_double2(yv.x + dr, yv.y + di));
+    }
+  } else {
+    if (st < 3) {                          // k_{st+1} and the next stage vector
+      bf.k[st][gi] = from_d2<T>(k);
+      const double2 y1 = st == 0 ? zp : ld<T>(bf.z[0] + gi);
+      bf.z[st + 1][gi] = from_d2<T>(caxpy(st == 2 ? h : 0.5 * h, k, y1));
+    } else {
+      const double2 k1 = ld<T>(bf.k[0] + gi), k2 = ld<T>(bf.k[1] + gi),
+                    k3 = ld<T>(bf.k[2] + gi);
+      const double2 sum = make_double2(k1.x + 2.0 * (k2.x + k3.x) + k.x,
+                                       k1.y + 2.0 * (k2.y + k3.y) + k.y);
+      const double2 yn = caxpy(h / 6.0, sum, bf.y[gi]);
+      bf.y[gi] = yn;
+      bf.z[0][gi] = from_d2<T>(yn);
+    }
+  }
+}
 
 // Stage st (0..3) of RK4 on input zin (y itself for stage 0 in FP64, z[st]
 // otherwise), or st = 4: zout = F(zin).  NR, KB > 0: the neighbour terms
@@ -261,37 +301,127 @@ __global__ __launch_bounds__(kThreads) void heom_stage_kernel(
         }
       }
     }
-    const int64_t gi = base + e;
-    if (st == 4) {                           // plain application: zout = F(zin)
-      zout[gi] = from_d2<T>(k);
-      continue;
+    stage_epilogue<T>(k, zp, base + e, st, h, bf, zout);
+  }
+}
+
+// The commutator form with its block sizes known at compile time (NX x NY
+// elements per ADO, NR occupied-site slots, KB bins per site): the same
+// stage as heom_stage_kernel with every index division by a constant, the
+// 2 (NX + NY) commutator terms unrolled with LDS offsets fixed at compile
+// time, and the per-element neighbour weights (site bin, n_j(r) - n_j(c),
+// c_k n_j(r) - conj(c_k) n_j(c)) tabulated once per work-group.  Absent
+// edges point at the element's own ADO with a zero coefficient, so the
+// gathers need no tests.  Counters on the 7-site, 3060-ADO hierarchy showed
+// the generic kernel issuing ~490 vector instructions per element (the
+// vector units busy 60% of the launch); this one issues about half as many,
+// which made the MIXED stage 13% faster and the FP64 stage 2-5%: what is
+// left is the latency of the neighbour gathers at four waves per SIMD (the
+// 120 registers; forcing six or eight waves spills).
+template <typename T, int NX, int NY, int NR, int KB>
+__global__ __launch_bounds__(kThreads) void heom_comm_kernel(
+    HeomArgs a, const cplx_t<T>* __restrict__ zin, cplx_t<T>* __restrict__ zout,
+    HeomStageBufs<T> bf, double h, int st) {
+  using T2 = cplx_t<T>;
+  constexpr int M = NX * NY;
+  constexpr int OPSZ = NX == NY ? NX * NX : NX * NX + NY * NY;
+  __shared__ int pbin[M][NR];           // 2 * (first bin of the slot's site)
+  __shared__ double pwp[M][NR];         // n_j(r) - n_j(c)
+  __shared__ double2 pwm[M][NR][KB];    // c_k n_j(r) - conj(c_k) n_j(c)
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nb2 = 2 * a.nbins;
+  double2* Hs = reinterpret_cast<double2*>(smem);
+  T2* zt = reinterpret_cast<T2*>(Hs + OPSZ);
+  size_t off = (size_t)OPSZ * sizeof(double2) + (size_t)a.ado_per_tile * M * sizeof(T2);
+  off = (off + 15) & ~(size_t)15;
+  double2* tcf = reinterpret_cast<double2*>(smem + off);
+  int* tsrc = reinterpret_cast<int*>(tcf + a.ado_per_tile * nb2);
+
+  int b, tile;
+  if (a.xcd_map) {
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    b = x + kXcds * (j / a.tiles);
+    tile = j % a.tiles;
+  } else {
+    b = blockIdx.x / a.tiles;
+    tile = blockIdx.x % a.tiles;
+  }
+  if (b >= a.B) return;                      // uniform per work-group
+  const int tid = threadIdx.x;
+  const int ado0 = tile * a.ado_per_tile;
+  const int nloc = min(a.ado_per_tile, a.n_ado - ado0);
+  const int64_t base = ((int64_t)b * a.n_ado + ado0) * M;
+  const double2* Hb = a.Hop + (int64_t)b * OPSZ;
+  for (int e = tid; e < OPSZ; e += kThreads) Hs[e] = Hb[e];
+  for (int e = tid; e < nloc * M; e += kThreads) zt[e] = zin[base + e];
+  for (int e = tid; e < nloc * nb2; e += kThreads) {
+    const int64_t g = (int64_t)ado0 * nb2 + e;
+    const int src = a.bsrc[g];
+    const bool ok = src >= 0 && src < a.n_ado;
+    tsrc[e] = ok ? src : ado0 + e / nb2;
+    tcf[e] = ok ? a.bcoef[g] : make_double2(0.0, 0.0);
+  }
+  for (int e = tid; e < M * NR; e += kThreads) {
+    const int p = e / NR, t = e % NR;
+    const int j = a.rsite[p * NR + t];
+    const bool okj = j >= 0 && (j + 1) * KB <= a.nbins;
+    const double orow = okj ? a.rocc[(p * NR + t) * 2] : 0.0;
+    const double ocol = okj ? a.rocc[(p * NR + t) * 2 + 1] : 0.0;
+    pbin[p][t] = okj ? 2 * j * KB : 0;
+    pwp[p][t] = orow - ocol;
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      const double2 c = a.ck[kk];
+      pwm[p][t][kk] = make_double2(c.x * orow - c.x * ocol, c.y * orow + c.y * ocol);
     }
-    if constexpr (std::is_same<T, double>::value) {
-      if (st == 0) {                         // zin is y: z2 = y + h/2 k1
-        bf.z[1][gi] = caxpy(0.5 * h, k, zp);
-      } else if (st < 3) {                   // z3 = y + h/2 k2, z4 = y + h k3
-        bf.z[st + 1][gi] = caxpy(st == 1 ? 0.5 * h : h, k, bf.y[gi]);
-      } else {                               // y' from the stage vectors and k4
-        const double2 yv = bf.y[gi], z2 = bf.z[1][gi], z3 = bf.z[2][gi];
-        const double dr = ((z2.x - yv.x) + 2.0 * (z3.x - yv.x) + (zp.x - yv.x)) / 3.0;
-        const double di = ((z2.y - yv.y) + 2.0 * (z3.y - yv.y) + (zp.y - yv.y)) / 3.0;
-        bf.y[gi] = caxpy(h / 6.0, k, make_double2(yv.x + dr, yv.y + di));
-      }
-    } else {
-      if (st < 3) {                          // k_{st+1} and the next stage vector
-        bf.k[st][gi] = from_d2<T>(k);
-        const double2 y1 = st == 0 ? zp : ld<T>(bf.z[0] + gi);
-        bf.z[st + 1][gi] = from_d2<T>(caxpy(st == 2 ? h : 0.5 * h, k, y1));
-      } else {
-        const double2 k1 = ld<T>(bf.k[0] + gi), k2 = ld<T>(bf.k[1] + gi),
-                      k3 = ld<T>(bf.k[2] + gi);
-        const double2 s = make_double2(k1.x + 2.0 * (k2.x + k3.x) + k.x,
-                                       k1.y + 2.0 * (k2.y + k3.y) + k.y);
-        const double2 yn = caxpy(h / 6.0, s, bf.y[gi]);
-        bf.y[gi] = yn;
-        bf.z[0][gi] = from_d2<T>(yn);
+  }
+  __syncthreads();
+
+  const double2* HY = NX == NY ? Hs : Hs + NX * NX;   // square blocks: H_Y = H_X
+  const T2* zb = zin + (int64_t)b * a.n_ado * M;
+  for (int e = tid; e < nloc * M; e += kThreads) {
+    const int la = e / M, p = e - la * M, ado = ado0 + la;
+    const int r = p % NX, c = p / NX;
+    const T2* z = zt + la * M;
+    // (H_X rho - rho H_Y)_{rc}, p = r + NX c, column-major rho
+    double2 k = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < NX; ++q) k = cadd(k, cmul(Hs[r * NX + q], ld<T>(z + q + NX * c)));
+#pragma unroll
+    for (int q = 0; q < NY; ++q) {
+      const double2 t = cmul(ld<T>(z + r + NX * q), HY[q * NY + c]);
+      k.x -= t.x;
+      k.y -= t.y;
+    }
+    const double2 zp = ld<T>(z + p);
+    const double2 dg = make_double2(a.shift[ado] + a.dvec[p].x, a.dvec[p].y);
+    k = cadd(k, cmul(dg, zp));
+    // neighbour terms: all offsets and weights, then all gathers, then the sum
+    const int* ts = tsrc + la * nb2;
+    const double2* tc = tcf + la * nb2;
+    constexpr int NE = 2 * NR * KB;
+    int src[NE];
+    double2 w[NE];
+#pragma unroll
+    for (int t = 0; t < NR; ++t) {
+      const int b0 = pbin[p][t];
+      const double wp = pwp[p][t];
+#pragma unroll
+      for (int kk = 0; kk < KB; ++kk) {
+        const int bi = b0 + 2 * kk, e0 = 2 * (t * KB + kk);
+        src[e0] = ts[bi];
+        src[e0 + 1] = ts[bi + 1];
+        const double2 c0 = tc[bi];
+        w[e0] = make_double2(c0.x * wp, c0.y * wp);
+        w[e0 + 1] = cmul(tc[bi + 1], pwm[p][t][kk]);
       }
     }
+    T2 v[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) v[q] = zb[src[q] * M + p];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) k = cadd(k, cmul(w[q], to_d2(v[q])));
+    stage_epilogue<T>(k, zp, base + e, st, h, bf, zout);
   }
 }
 
@@ -303,6 +433,13 @@ using StageKernel = void (*)(HeomArgs, const cplx_t<T>*, cplx_t<T>*, HeomStageBu
 // per element, bins per site)
 template <typename T>
 StageKernel<T> pick_stage_kernel(const HeomArgs& a) {
+  if (!a.dense && a.N == 0 && a.nX == a.nY && a.nrel == 2 && (a.K1 == 2 || a.K1 == 3)) {
+#define QSX_COMM(n_)                                                               \
+    if (a.nX == n_) return a.K1 == 2 ? heom_comm_kernel<T, n_, n_, 2, 2>           \
+                                     : heom_comm_kernel<T, n_, n_, 2, 3>;
+    QSX_COMM(2) QSX_COMM(3) QSX_COMM(4) QSX_COMM(5) QSX_COMM(6) QSX_COMM(7) QSX_COMM(8)
+#undef QSX_COMM
+  }
   if (a.nrel == 2 && a.K1 == 2) return heom_stage_kernel<T, 2, 2>;
   if (a.nrel == 1 && a.K1 == 2) return heom_stage_kernel<T, 1, 2>;
   if (a.nrel == 3 && a.K1 == 2) return heom_stage_kernel<T, 3, 2>;
