@@ -556,7 +556,10 @@ def ensemble_absorption_leg(torch, replays=5):
     summary = kt.summary()
     label = [k for k in summary if k.startswith('response buckets')][0]
     ms = summary[label]['mean_ms']
-    flops = sum(8.0 * bk['B'] * (bk['M'] ** 2 + bk['M']) * nt for bk in plan.buckets)
+    # executed: J = plan.J steps per pass, each pass M rows of P^J and J
+    # projection rows of M entries per problem
+    J = plan.J
+    flops = sum(8.0 * bk['B'] * (bk['M'] ** 2 + J * bk['M']) * -(-nt // J) for bk in plan.buckets)
     achieved = flops / (ms * 1e-3) / 1e12
     return {'workload': 'absorption of 4096 disorder members (sigma 100 cm^-1) of the 7-site '
                         'aggregate + 1024 aggregates of N ~ U{2..24} sites, Redfield ge, 4096 '

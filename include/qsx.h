@@ -225,20 +225,28 @@ int32_t qsx_propagate_adjoint(const void* A, int64_t strideA, int32_t M, const v
 /* Linear-response signals of many problems of unequal size in one launch:
  * for problem b, y_{k+1} = P_b y_k from y0_b and s_k = v_b . y_k,
  * k = 0 .. nt-1, written to row r_b of out [B][nt] complex128 (FP64
- * products).  waves int32 [nwaves][4] (device): first problem, problem count,
- * M, G per wavefront, with G a power of two above M, G * count <= 64 and M at
- * most qsx_response_buckets_max_states(); offsets int64 [B][3] (device):
- * offset of P_b (M^2 complex elements, row-major) in P, of y0_b, v_b (M each)
- * in y0 and v, and the output row r_b (a permutation of 0 .. B-1, so the
- * signals can leave in the caller's order).  status int32 [B]: 1 at r_b where
- * a signal value was not finite.
- * Replaces the per-member zvode runs of linear_response / absorption_spectra
- * (qspectra/simulate/response.py:15-43, :103-154; simulate/utils.py:16-109)
- * over an ensemble: P_b = P4(h L_b)^q of the member's Heisenberg 'eg' block,
- * y0_b = -bra, v_b = V rho0. */
+ * products), J steps (1 or 4) per pass: P holds P_b^J, W the projection rows
+ * w_j = (P_b^j)^T v_b, j < J (qsx_response_bucket_weights writes them from
+ * P_b and v_b; for J = 1, W is v).  waves int32 [nwaves][4] (device): first
+ * problem, problem count, M, G per wavefront, with G a power of two,
+ * G >= M + J, G * count <= 64 and M at most qsx_response_buckets_max_states();
+ * offsets int64 [B][3] (device): offset of P_b^J (M^2 complex elements,
+ * row-major) in P, of y0_b (M) in y0 (the rows of W at J times it, [J][M]),
+ * and the output row r_b (a permutation of 0 .. B-1, so the signals can leave
+ * in the caller's order).  status int32 [B]: 1 at r_b where a signal value
+ * was not finite.  Replaces the per-member zvode runs of linear_response /
+ * absorption_spectra (qspectra/simulate/response.py:15-43, :103-154;
+ * simulate/utils.py:16-109) over an ensemble: P_b = P4(h L_b)^q of the
+ * member's Heisenberg 'eg' block, y0_b = -bra, v_b = V rho0. */
 int32_t qsx_response_buckets(const int32_t* waves, int32_t nwaves, const int64_t* offsets,
-                             const void* P, const void* y0, const void* v, int32_t nt,
-                             void* out, int32_t* status, void* stream);
+                             const void* P, const void* y0, const void* W, int32_t J,
+                             int32_t nt, void* out, int32_t* status, void* stream);
+/* The projection rows of qsx_response_buckets: W[J o_b + j M + c] =
+ * ((P_b^j)^T v_b)[c] for j < J (1 <= J <= 4), from the one-step propagators P
+ * (same offsets) and v [o_b + c] (o_b the problem's y0 offset). */
+int32_t qsx_response_bucket_weights(const int32_t* waves, int32_t nwaves,
+                                    const int64_t* offsets, const void* P, const void* v,
+                                    int32_t J, void* W, void* stream);
 int32_t qsx_response_buckets_max_states(void);
 
 /* Batched complex GEMM, row-major storage:

@@ -58,7 +58,9 @@ SIGNATURES = {
     'qsx_redfield_block_tabulated': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_dbl, c_i32,
                                      c_i32, c_vp, c_vp, c_vp],
     'qsx_redfield_vjp_workspace_bytes': [c_i32, c_i32, c_i32, c_i32],
-    'qsx_response_buckets': [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp],
+    'qsx_response_buckets': [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp,
+                             c_vp],
+    'qsx_response_bucket_weights': [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp],
     'qsx_heom_apply_hilbert': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp],
@@ -356,27 +358,43 @@ def redfield_block_vjp(H1, rw, T, lam, gam, X, Y, flags, unit_convert, gL, strea
     return gH1, glam
 
 
-def response_buckets_into(waves, offsets, P, y0, v, out, status, stream=None):
+def response_buckets_into(waves, offsets, P, y0, W, out, status, J=1, stream=None):
     """out [B, nt] <- s_k = v_b . P_b^k y0_b for problems of unequal size in
-    one launch (qsx_response_buckets; the tables are device int32 [nw, 4] and
-    int64 [B, 3] (P offset, y0/v offset, output row), P, y0, v flat
-    complex128)."""
+    one launch, J steps (1 or 4) per pass (qsx_response_buckets; the tables
+    are device int32 [nw, 4] and int64 [B, 3] (P offset, y0 offset, output
+    row); P holds P_b^J, W the projection rows (v for J = 1; see
+    response_bucket_weights_into), all flat complex128)."""
     lib = require_native()
     assert waves.dtype == torch.int32 and waves.is_contiguous() and waves.shape[1] == 4
     assert offsets.dtype == torch.int64 and offsets.is_contiguous() and offsets.shape[1] == 3
     assert offsets.shape[0] == out.shape[0]
-    for X in (P, y0, v, out):
+    for X in (P, y0, W, out):
         assert X.dtype == torch.complex128 and X.is_contiguous()
+    assert W.numel() == J * y0.numel()
     assert status.dtype == torch.int32 and status.numel() == out.shape[0]
     nt = out.shape[1]
 
     def go():
         return lib.qsx_response_buckets(ptr(waves), waves.shape[0], ptr(offsets), ptr(P), ptr(y0),
-                                        ptr(v), nt, ptr(out), ptr(status), stream_ptr(stream))
+                                        ptr(W), int(J), nt, ptr(out), ptr(status),
+                                        stream_ptr(stream))
     with timing.launch('response buckets B=%d waves=%d nt=%d' % (out.shape[0], waves.shape[0], nt),
                        stream, go):
         st = go()
     check(st, 'qsx_response_buckets')
+
+
+def response_bucket_weights_into(waves, offsets, P, v, J, W, stream=None):
+    """W <- the projection rows (P_b^j)^T v_b, j < J, of every problem
+    (qsx_response_bucket_weights; P the one-step propagators at the bucket
+    offsets, v flat complex128)."""
+    lib = require_native()
+    for X in (P, v, W):
+        assert X.dtype == torch.complex128 and X.is_contiguous()
+    assert W.numel() == J * v.numel()
+    check(lib.qsx_response_bucket_weights(ptr(waves), waves.shape[0], ptr(offsets), ptr(P),
+                                          ptr(v), int(J), ptr(W), stream_ptr(stream)),
+          'qsx_response_bucket_weights')
 
 
 def redfield_workspace(n, X, Y, B, dev=None):

@@ -51,10 +51,15 @@ def absorption_batch(hamiltonians, time_max, rw_freq, dt, model_cls=RedfieldMode
     return f, out
 
 
-def _group_size(M):
-    """Lanes per problem in qsx_response_buckets: the power of two above M."""
+# steps per LDS round trip of qsx_response_buckets in the ensemble plan
+STEPS_PER_PASS = 4
+
+
+def _group_size(M, J=STEPS_PER_PASS):
+    """Lanes per problem in qsx_response_buckets: the power of two at or
+    above M + J (M state rows and J projection rows)."""
     G = 2
-    while G < M + 1:
+    while G < M + J:
         G *= 2
     return G
 
@@ -65,10 +70,11 @@ class AbsorptionEnsemblePlan(object):
 
     Per site count (bucket): the Redfield 'eg' blocks (Heisenberg picture)
     rebuilt from the members' Hamiltonians (qsx_redfield_block) and their
-    one-step propagators P4(dt L) (qsx_small_propagator).  Then one launch
-    propagates every member of every bucket and records its linear-response
-    signal (qsx_response_buckets), and the Fourier transform of all signals
-    gives the spectra.  The values are those of absorption_batch
+    one-step propagators P = P4(dt L) with P^4 (qsx_small_propagator).  Then
+    the projection rows (P^j)^T v, j < 4 (qsx_response_bucket_weights), one
+    launch that propagates every member of every bucket four steps per pass
+    and records its linear-response signal (qsx_response_buckets), and the
+    Fourier transform of all signals gives the spectra.  The values are those of absorption_batch
     (absorption_spectra member by member, response.py:103-154) with FP64
     propagation at the step dt.
 
@@ -130,12 +136,16 @@ class AbsorptionEnsemblePlan(object):
             offY += B * M
         self.order = np.asarray(order)
         self.B = len(order)
-        self.P = torch.empty(offP, dtype=torch.complex128, device=dev)
+        J = self.J = STEPS_PER_PASS
+        self.P = torch.empty(offP, dtype=torch.complex128, device=dev)     # P^J
+        self.P1 = torch.empty(offP, dtype=torch.complex128, device=dev)    # P
         for bk in self.buckets:
-            bk['P'] = self.P[bk['p0']:bk['p0'] + bk['B'] * bk['M'] ** 2].view(
-                bk['B'], bk['M'], bk['M'])
+            sl = slice(bk['p0'], bk['p0'] + bk['B'] * bk['M'] ** 2)
+            bk['P'] = self.P[sl].view(bk['B'], bk['M'], bk['M'])
+            bk['P1'] = self.P1[sl].view(bk['B'], bk['M'], bk['M'])
         self.y0 = _native.to_device(np.concatenate(y0s), dev=dev)
         self.v = _native.to_device(np.concatenate(vs), dev=dev)
+        self.W = torch.empty(J * offY, dtype=torch.complex128, device=dev)
         self.waves = torch.as_tensor(np.asarray(waves, dtype=np.int32)).to(dev)
         self.offsets = torch.as_tensor(np.asarray(offs, dtype=np.int64)).to(dev)
         self.signal = torch.empty((self.B, nt), dtype=torch.complex128, device=dev)
@@ -155,14 +165,14 @@ class AbsorptionEnsemblePlan(object):
             self.capture()
 
     @staticmethod
-    def _check_tables(waves, offs, nP, nY):
+    def _check_tables(waves, offs, nP, nY, J=STEPS_PER_PASS):
         """Host check that every wavefront's problems and every offset stay
         inside the packed buffers the kernel reads, and that the output rows
         are a permutation."""
         B = len(offs)
         assert sorted(o[2] for o in offs) == list(range(B))
         for first, count, M, G in waves:
-            assert G * count <= 64 and G >= M + 1 and 0 <= first and first + count <= B
+            assert G * count <= 64 and G >= M + J and 0 <= first and first + count <= B
             for b in range(first, first + count):
                 p, y, _ = offs[b]
                 assert 0 <= p and p + M * M <= nP and 0 <= y and y + M <= nY
@@ -182,11 +192,14 @@ class AbsorptionEnsemblePlan(object):
             bk = self.buckets[k]
             with torch.cuda.stream(self._sides[j % len(self._sides)]):
                 bk['rebuild'](bk['L'])
-                _native.small_propagator_into(bk['L'], self.dt, 1, 0, P=bk['P'])
+                _native.small_propagator_into(bk['L'], self.dt, 1, self.J, P=bk['P1'],
+                                              Pc=bk['P'])
         for side in self._sides:
             main.wait_stream(side)
-        _native.response_buckets_into(self.waves, self.offsets, self.P, self.y0, self.v,
-                                      self.signal, self.status)
+        _native.response_bucket_weights_into(self.waves, self.offsets, self.P1, self.v, self.J,
+                                             self.W)
+        _native.response_buckets_into(self.waves, self.offsets, self.P, self.y0, self.W,
+                                      self.signal, self.status, J=self.J)
         # the signal rows are in input order; the response's sign goes into
         # the transform's scale
         x = self.signal if self.decay is None else self.signal * self._decay

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import qspectra_amd as qa
+from qspectra_amd import _native
 from oracle import system as S
 from oracle import liouville as Lv
 from oracle import propagate as P
@@ -120,3 +121,66 @@ def test_ensemble_plan_rebuilds_from_new_hamiltonians(gpu):
     _, Xb = absorption_batch(mixed, 256, rw_freq=12500.0, dt=1.0, correlation_decay_time=300.0)
     for i in range(len(new)):
         assert rel_err(X[i], Xb[i]) < 1e-12, i
+
+
+def _bucket_tables(sizes, J):
+    """Wave table and offsets of qsx_response_buckets for problems of the
+    given sizes (grouped by size, output rows in input order)."""
+    from qspectra_amd.batched import _group_size
+    waves, offs, order = [], [None] * len(sizes), []
+    offP = offY = 0
+    for M in sorted(set(sizes)):
+        idx = [i for i, m in enumerate(sizes) if m == M]
+        G = _group_size(M, J)
+        first = len(order)
+        for w0 in range(0, len(idx), 64 // G):
+            waves.append((first + w0, min(64 // G, len(idx) - w0), M, G))
+        for i in idx:
+            offs[len(order)] = (offP, offY, i)
+            order.append(i)
+            offP += M * M
+            offY += M
+    return waves, offs, order
+
+
+def test_response_buckets_four_steps_per_pass(gpu):
+    """qsx_response_buckets at four steps per pass (P^4 and the projection
+    rows (P^j)^T v written by qsx_response_bucket_weights) and at one step
+    per pass, for problems of 1..31 states and a step count that is not a
+    multiple of 4, against the host's s_k = v . P^k y0 (1e-12)."""
+    import torch
+    rs = np.random.RandomState(3)
+    sizes = [7, 1, 31, 3, 12, 7, 13, 28, 29, 7, 2]
+    nt = 203
+    Ps = [np.eye(M) + 0.05 * (rs.randn(M, M) + 1j * rs.randn(M, M)) / np.sqrt(M) for M in sizes]
+    y0s = [rs.randn(M) + 1j * rs.randn(M) for M in sizes]
+    vs = [rs.randn(M) + 1j * rs.randn(M) for M in sizes]
+    ref = np.zeros((len(sizes), nt), complex)
+    for i, (P_, y, v) in enumerate(zip(Ps, y0s, vs)):
+        for k in range(nt):
+            ref[i, k] = v @ y
+            y = P_ @ y
+    for J in (1, 4):
+        waves, offs, order = _bucket_tables(sizes, J)
+        dev = 'cuda'
+        P1 = np.concatenate([Ps[i].ravel() for i in order])
+        PJ = np.concatenate([np.linalg.matrix_power(Ps[i], J).ravel() for i in order])
+        y0 = torch.as_tensor(np.concatenate([y0s[i] for i in order])).to(dev)
+        v = torch.as_tensor(np.concatenate([vs[i] for i in order])).to(dev)
+        wt = torch.as_tensor(np.asarray(waves, dtype=np.int32)).to(dev)
+        ot = torch.as_tensor(np.asarray(offs, dtype=np.int64)).to(dev)
+        W = torch.empty(J * y0.numel(), dtype=torch.complex128, device=dev)
+        _native.response_bucket_weights_into(wt, ot, torch.as_tensor(P1).to(dev), v, J, W)
+        Wh = W.cpu().numpy()
+        for b, i in enumerate(order):
+            M, oy = sizes[i], offs[b][1]
+            for j in range(J):
+                wj = np.linalg.matrix_power(Ps[i], j).T @ vs[i]
+                assert rel_err(Wh[J * oy + j * M:J * oy + (j + 1) * M], wj) < 1e-13, (J, i, j)
+        out = torch.empty((len(sizes), nt), dtype=torch.complex128, device=dev)
+        status = torch.ones(len(sizes), dtype=torch.int32, device=dev)
+        _native.response_buckets_into(wt, ot, torch.as_tensor(PJ).to(dev), y0, W, out, status, J=J)
+        got = out.cpu().numpy()
+        assert int(status.max().item()) == 0
+        for i in range(len(sizes)):
+            assert rel_err(got[i], ref[i]) < 1e-12, (J, i, sizes[i])
