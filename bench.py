@@ -429,10 +429,10 @@ HEOM_BYTES_PER_ELEMENT_STEP = {'fp64': 208, 'mixed': 160}
 # stage's 16 B per element read back to the byte).  Keyed by launch label; a
 # launch of several kernels counts all of them.
 PMC_TRAFFIC = {
-    # one RK4 stage over 256 members, summed over its sample passes (FP64: 24
-    # samples per pass, 67.98 B per element-stage; MIXED: 16 samples, 52.41 B)
-    'heom stage fp64': 2.610e9,
-    'heom stage mixed': 2.012e9,
+    # one RK4 stage over 256 members, summed over its sample passes (10 of 24
+    # members and one of 16; FP64 70.8 B per element-stage, MIXED 50.4 B)
+    'heom stage fp64': 2.718e9,
+    'heom stage mixed': 1.936e9,
     # the headline plan's t2 operator program (one work-group, M = 49)
     'propagate powers M=49 B=1 R=7 e=0,100,200,500': 8.21e4,
     'chunks prop mixed M=147 nc=8 B=1 steps=15 nsave=16': 4.30e6,
@@ -442,8 +442,9 @@ PMC_TRAFFIC = {
     # prep 1.366e8 + three Horner GEMMs 9.716e8 each + squaring 5.307e8
     # + diagonal 1.2e6 + final split 9.25e7
     'split propagator M=147 B=256 q=2': 3.676e9,
-    # the 5120-system absorption ensemble: operators 9.6 MB in, signals out
-    'response buckets B=5120 waves=832 nt=4096': 3.452e8,
+    # the 5120-system absorption ensemble (four steps per pass): operators
+    # and projection rows 11.7 MB in, signals 335.6 MB out
+    'response buckets B=5120 waves=1399 nt=4096': 3.473e8,
 }
 
 
