@@ -112,6 +112,27 @@ int32_t qsx_block_eigh(const double* H1, int32_t n, int32_t B, int32_t X,
                        const double* rw, double* E, double* U, void* workspace,
                        void* stream);
 
+/* The operators of linear absorption for a ragged batch of aggregates (up to
+ * qsx_absorption_operators_max_sites() sites each), one launch: per problem b
+ * the Heisenberg 'eg' Redfield block of the one-exciton Hamiltonian H1_b (Debye
+ * bath T, lam, gam; the block qsx_redfield_block builds with X = e, Y = g and
+ * QSX_HEISENBERG) in the rotating frame rw[b], times unit_convert, and its RK4
+ * propagator P1_b = P4(h L_b) with PJ_b = P1_b^J (J = 1, 2 or 4).
+ * table int64 [B][3] (device): n_b, offset of H1_b (doubles, n_b x n_b
+ * row-major) in H1, offset of P1_b and PJ_b (complex128, n_b x n_b) in P1 and
+ * PJ; rw [B]; bath [B][3]; flags: QSX_SECULAR, QSX_REAL_CORR, QSX_HEISENBERG.
+ * max_n bounds every n_b (problems above it are skipped) and sizes the
+ * launch's LDS: grouping problems by size keeps small ones at high occupancy.
+ * Replaces, for absorption_spectra over an ensemble of unequal site counts,
+ * RedfieldModel.evolution_super_operator['eg'] per member (qspectra/dynamics/
+ * redfield.py:9-152, liouville_space.py:316-341) and the RK4 step of
+ * simulate/utils.py:16-109. */
+int32_t qsx_absorption_operators(const int64_t* table, int32_t B, int32_t max_n,
+                                 const double* H1, const double* rw, const double* bath,
+                                 int32_t flags, double unit_convert, double h, int32_t J,
+                                 void* P1, void* PJ, void* stream);
+int32_t qsx_absorption_operators_max_sites(void);
+
 /* Bytes of device workspace needed by qsx_redfield_block / qsx_block_eigh
  * (or -1 for invalid arguments). */
 int64_t qsx_redfield_workspace_bytes(int32_t n, int32_t X, int32_t Y, int32_t B);

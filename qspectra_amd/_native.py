@@ -61,6 +61,9 @@ SIGNATURES = {
     'qsx_response_buckets': [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp,
                              c_vp],
     'qsx_response_bucket_weights': [c_vp, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp],
+    'qsx_absorption_operators': [c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl,
+                                 c_i32, c_vp, c_vp, c_vp],
+    'qsx_absorption_operators_max_sites': [],
     'qsx_heom_apply_hilbert': [c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                c_i32, c_i32, c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
                                c_vp],
@@ -382,6 +385,31 @@ def response_buckets_into(waves, offsets, P, y0, W, out, status, J=1, stream=Non
                        stream, go):
         st = go()
     check(st, 'qsx_response_buckets')
+
+
+def absorption_operators_into(table, max_n, H1, rw, bath, flags, unit_convert, h, J, P1, PJ,
+                              stream=None):
+    """P1 <- P4(h L_b), PJ <- P1^J for the Heisenberg 'eg' Redfield blocks L_b
+    of a ragged batch of one-exciton Hamiltonians in one launch
+    (qsx_absorption_operators; table int64 [B, 3]: n <= max_n, H1 offset,
+    P offset; H1, rw, bath [B, 3] float64 and P1, PJ complex128, all flat
+    device tensors)."""
+    lib = require_native()
+    assert table.dtype == torch.int64 and table.is_contiguous() and table.shape[1] == 3
+    for X in (H1, rw, bath):
+        assert X.dtype == torch.float64 and X.is_contiguous()
+    for X in (P1, PJ):
+        assert X.dtype == torch.complex128 and X.is_contiguous()
+    B = table.shape[0]
+    assert rw.numel() == B and bath.numel() == 3 * B
+
+    def go():
+        return lib.qsx_absorption_operators(ptr(table), B, int(max_n), ptr(H1), ptr(rw),
+                                            ptr(bath), int(flags), float(unit_convert), float(h),
+                                            int(J), ptr(P1), ptr(PJ), stream_ptr(stream))
+    with timing.launch('absorption operators B=%d n<=%d J=%d' % (B, max_n, J), stream, go):
+        st = go()
+    check(st, 'qsx_absorption_operators')
 
 
 def response_bucket_weights_into(waves, offsets, P, v, J, W, stream=None):

@@ -68,9 +68,11 @@ class AbsorptionEnsemblePlan(object):
     """Absorption spectra of many Hamiltonians of unequal site counts as one
     captured HIP graph, spectra left on the device.
 
-    Per site count (bucket): the Redfield 'eg' blocks (Heisenberg picture)
-    rebuilt from the members' Hamiltonians (qsx_redfield_block) and their
-    one-step propagators P = P4(dt L) with P^4 (qsx_small_propagator).  Then
+    The Redfield 'eg' blocks (Heisenberg picture) rebuilt from the members'
+    Hamiltonians and their one-step propagators P = P4(dt L) with P^4: for a
+    Debye bath all members of all sizes in one launch
+    (qsx_absorption_operators), otherwise per site count (bucket) with
+    qsx_redfield_block and qsx_small_propagator.  Then
     the projection rows (P^j)^T v, j < 4 (qsx_response_bucket_weights), one
     launch that propagates every member of every bucket four steps per pass
     and records its linear-response signal (qsx_response_buckets), and the
@@ -153,6 +155,7 @@ class AbsorptionEnsemblePlan(object):
         if correlation_decay_time is not None:
             self._decay = torch.as_tensor(np.exp(-self.t / correlation_decay_time)).to(dev)
         self._check_tables(waves, offs, offP, offY)
+        self._fused = self._fused_builds(offs, offP, dev)
         # the buckets' builds are independent: largest first, round-robin over
         # side streams, joined before the propagation launch
         self._build_order = sorted(range(len(self.buckets)),
@@ -163,6 +166,64 @@ class AbsorptionEnsemblePlan(object):
         self._out = None
         if graph:
             self.capture()
+
+    # largest site count of each launch of qsx_absorption_operators
+    FUSED_CLASSES = (8, 16, 24, 31)
+
+    def _fused_builds(self, offs, nP, dev):
+        """Redfield models of a Debye bath in the site basis: the operators of
+        every bucket come from one launch (qsx_absorption_operators) reading
+        flat H1 / rw buffers, and each bucket's hamiltonian_inputs become
+        views into them.  Returns the launch arguments, or None (other models
+        and baths keep one build per bucket)."""
+        from .bath import debye_parameters, is_debye_form
+        from .dynamics.redfield import RedfieldModel
+        lib = _native.require_native()
+        flags = set()
+        for bk in self.buckets:
+            m = bk['model']
+            if (type(m) is not RedfieldModel or m.evolve_basis != 'site'
+                    or not is_debye_form(m.hamiltonian.bath) or bk['inputs'][0].is_complex()
+                    or bk['n'] > int(lib.qsx_absorption_operators_max_sites())):
+                return None
+            flags.add(m._flags(True))
+        if len(flags) != 1:
+            return None
+        nH = sum(bk['B'] * bk['n'] ** 2 for bk in self.buckets)
+        H1 = torch.empty(nH, dtype=torch.float64, device=dev)
+        rw = torch.empty(self.B, dtype=torch.float64, device=dev)
+        table, bath = [], []
+        oh = ob = 0
+        for bk in self.buckets:
+            B, n = bk['B'], bk['n']
+            Hv = H1[oh:oh + B * n * n].view(B, n, n)
+            rv = rw[ob:ob + B]
+            Hv.copy_(bk['inputs'][0])
+            rv.copy_(bk['inputs'][1])
+            bk['inputs'] = (Hv, rv)
+            tb = debye_parameters(bk['model'].hamiltonian.bath)
+            for k in range(B):
+                table.append((n, oh + k * n * n, offs[ob + k][0]))
+                bath.append(tb)
+            oh += B * n * n
+            ob += B
+        for n, h, pp in table:
+            assert 0 <= h and h + n * n <= nH and 0 <= pp and pp + n * n <= nP
+        # one launch per size class (the table is in ascending n): each sizes
+        # its LDS for its largest n
+        tab = torch.as_tensor(np.asarray(table, dtype=np.int64)).to(dev)
+        bt = torch.as_tensor(np.asarray(bath, dtype=np.float64)).to(dev)
+        sizes = np.asarray([r[0] for r in table])
+        launches = []
+        lo = 0
+        for top in self.FUSED_CLASSES:
+            hi = int(np.searchsorted(sizes, top, side='right'))
+            if hi > lo:
+                launches.append((top, tab[lo:hi], rw[lo:hi], bt[lo:hi]))
+            lo = hi
+        assert lo == len(table)
+        return dict(launches=launches, H1=H1, flags=flags.pop(),
+                    unit=self.buckets[0]['model'].unit_convert)
 
     @staticmethod
     def _check_tables(waves, offs, nP, nY, J=STEPS_PER_PASS):
@@ -185,15 +246,25 @@ class AbsorptionEnsemblePlan(object):
         raise KeyError(n)
 
     def _launch(self):
+        fz = self._fused
         main = torch.cuda.current_stream()
         for side in self._sides:
             side.wait_stream(main)
-        for j, k in enumerate(self._build_order):
-            bk = self.buckets[k]
-            with torch.cuda.stream(self._sides[j % len(self._sides)]):
-                bk['rebuild'](bk['L'])
-                _native.small_propagator_into(bk['L'], self.dt, 1, self.J, P=bk['P1'],
-                                              Pc=bk['P'])
+        if fz is not None:
+            # every bucket's operators, one launch per size class, the classes
+            # (largest sites first) concurrent on the side streams
+            for j, (top, table, rw, bath) in enumerate(reversed(fz['launches'])):
+                with torch.cuda.stream(self._sides[j % len(self._sides)]):
+                    _native.absorption_operators_into(table, top, fz['H1'], rw, bath,
+                                                      fz['flags'], fz['unit'], self.dt, self.J,
+                                                      self.P1, self.P)
+        else:
+            for j, k in enumerate(self._build_order):
+                bk = self.buckets[k]
+                with torch.cuda.stream(self._sides[j % len(self._sides)]):
+                    bk['rebuild'](bk['L'])
+                    _native.small_propagator_into(bk['L'], self.dt, 1, self.J, P=bk['P1'],
+                                                  Pc=bk['P'])
         for side in self._sides:
             main.wait_stream(side)
         _native.response_bucket_weights_into(self.waves, self.offsets, self.P1, self.v, self.J,

@@ -73,46 +73,14 @@ __device__ inline void pair_of(int s, int n, int& i, int& j) {
 }
 
 // ------------------------------------------------------------------ 1. eigh
-__global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __restrict__ H1, int n,
-                                                              int X, const double* __restrict__ rw,
-                                                              const double* __restrict__ Hd,
-                                                              int nbd, double* __restrict__ ws,
-                                                              size_t rstride) {
-  __shared__ double A[kMaxBlock * kMaxBlock];
-  __shared__ double V[kMaxBlock * kMaxBlock];
-  __shared__ double rot[kMaxBlock / 2][2];
-  __shared__ int pqs[kMaxBlock / 2][2];
-  __shared__ int pi_[kMaxBlock], pj_[kMaxBlock];
-  __shared__ double red[2][kThreads];
-  __shared__ int perm[kMaxBlock];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  const int nb = Hd ? nbd : block_size(X, n);
-  const int m = nb + (nb & 1);
-  const double* H = Hd ? Hd + (size_t)b * nb * nb : H1 + (size_t)b * n * n;
-  const double shift = Hd ? 0.0 : X * (rw ? rw[b] : 0.0);
-  if (!Hd && X == 2)
-    for (int s = tid; s < nb; s += kThreads) pair_of(s, n, pi_[s], pj_[s]);
-  __syncthreads();
-  for (int e = tid; e < m * m; e += kThreads) {
-    const int r = e / m, c = e % m;
-    double v = 0.0;
-    if (r < nb && c < nb) {
-      if (Hd) {
-        v = H[r * nb + c];
-      } else if (X == 1) {
-        v = H[r * n + c];
-      } else if (X == 2) {  // operator_1_to_2 (operator_tools.py:155-191)
-        const int i = pi_[r], j = pj_[r], k = pi_[c], l = pj_[c];
-        v = (i == k ? H[j * n + l] : 0.0) + (i == l ? H[j * n + k] : 0.0) +
-            (j == k ? H[i * n + l] : 0.0) + (j == l ? H[i * n + k] : 0.0);
-      }
-      if (r == c) v -= shift;
-    }
-    A[e] = v;
-    V[e] = (r == c) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  // cyclic two-sided Jacobi, m/2 disjoint rotations per round (circle method)
+// Cyclic two-sided Jacobi on the symmetric m x m matrix A in LDS (m even),
+// accumulating the rotations into V: m/2 disjoint rotations per round
+// (circle method), sweeps until the off-diagonal norm is below 1e-30 of the
+// total.  On return A is diagonal (the eigenvalues) and the columns of V are
+// the eigenvectors.  Called by every thread of a kThreads work-group.
+__device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int (*pqs)[2],
+                              double (*red)[kThreads]) {
+  const int tid = threadIdx.x;
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < m * m; e += kThreads) {
@@ -182,6 +150,48 @@ __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __re
       __syncthreads();
     }
   }
+}
+
+__global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __restrict__ H1, int n,
+                                                              int X, const double* __restrict__ rw,
+                                                              const double* __restrict__ Hd,
+                                                              int nbd, double* __restrict__ ws,
+                                                              size_t rstride) {
+  __shared__ double A[kMaxBlock * kMaxBlock];
+  __shared__ double V[kMaxBlock * kMaxBlock];
+  __shared__ double rot[kMaxBlock / 2][2];
+  __shared__ int pqs[kMaxBlock / 2][2];
+  __shared__ int pi_[kMaxBlock], pj_[kMaxBlock];
+  __shared__ double red[2][kThreads];
+  __shared__ int perm[kMaxBlock];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int nb = Hd ? nbd : block_size(X, n);
+  const int m = nb + (nb & 1);
+  const double* H = Hd ? Hd + (size_t)b * nb * nb : H1 + (size_t)b * n * n;
+  const double shift = Hd ? 0.0 : X * (rw ? rw[b] : 0.0);
+  if (!Hd && X == 2)
+    for (int s = tid; s < nb; s += kThreads) pair_of(s, n, pi_[s], pj_[s]);
+  __syncthreads();
+  for (int e = tid; e < m * m; e += kThreads) {
+    const int r = e / m, c = e % m;
+    double v = 0.0;
+    if (r < nb && c < nb) {
+      if (Hd) {
+        v = H[r * nb + c];
+      } else if (X == 1) {
+        v = H[r * n + c];
+      } else if (X == 2) {  // operator_1_to_2 (operator_tools.py:155-191)
+        const int i = pi_[r], j = pj_[r], k = pi_[c], l = pj_[c];
+        v = (i == k ? H[j * n + l] : 0.0) + (i == l ? H[j * n + k] : 0.0) +
+            (j == k ? H[i * n + l] : 0.0) + (j == l ? H[i * n + k] : 0.0);
+      }
+      if (r == c) v -= shift;
+    }
+    A[e] = v;
+    V[e] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  jacobi_sweeps(A, V, m, rot, pqs, red);
   // ascending eigenvalues (stable ranking), canonical sign: the largest-
   // magnitude component of each eigenvector is made positive
   for (int k = tid; k < nb; k += kThreads) {
@@ -221,6 +231,25 @@ __device__ double2 matsubara_sum(double x, double T, double gam) {
   double sr = 0.0, si = 0.0, cr = 0.0, ci = 0.0;
   const double twopiT = 2.0 * M_PI * T;
   for (int k = 1; k < 1000; ++k) {
+    const double nu = twopiT * k;
+    const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
+    const double tr = nu * nu * inv - cr;
+    const double ti = nu * x * inv - ci;
+    const double ur = sr + tr, ui = si + ti;
+    cr = (ur - sr) - tr;
+    ci = (ui - si) - ti;
+    sr = ur;
+    si = ui;
+  }
+  return make_double2(sr, si);
+}
+
+// The terms k = k0, k0 + stride, ... (< 1000) of matsubara_sum, Kahan-
+// compensated: `stride` threads together evaluate one sum
+__device__ double2 matsubara_partial(double x, double T, double gam, int k0, int stride) {
+  double sr = 0.0, si = 0.0, cr = 0.0, ci = 0.0;
+  const double twopiT = 2.0 * M_PI * T;
+  for (int k = k0; k < 1000; k += stride) {
     const double nu = twopiT * k;
     const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
     const double tr = nu * nu * inv - cr;
@@ -644,6 +673,192 @@ __global__ __launch_bounds__(kThreads) void unitary_dense_kernel(const double* _
   const double2 val = make_double2(0.0, -unit * v);
   if (heisenberg) Lb[(size_t)q * M + p] = val;
   else Lb[e] = val;
+}
+
+// ------------------------------------------------- absorption operators
+// The operators of linear absorption for a ragged batch of aggregates of up
+// to kAbsMaxSites sites, one work-group per problem, everything in LDS: the
+// Heisenberg 'eg' Redfield block of the one-exciton Hamiltonian (Debye bath)
+// and its RK4 propagator P = P4(h L) with P^J.  For the eg block (Y = g: one
+// state of energy 0 that no bath couples to, so K^Y = 0 and Gs_Y = 0) the
+// pipeline of the kernels above reduces to
+//   L_eig[a][c] = unit (-Gs[a][c] - i E_a d_ac)    (secular: Gs diagonal only)
+//   Gs[a][c]    = sum_i U[i][a] U[i][c] sum_m U[i][m]^2 C(E_c - E_m)
+//   L           = U L_eig U^T  (its transpose for the Heisenberg picture)
+// (redfield.py:9-104, with the e block's couplings K_i = U^T |i><i| U).
+// The LDS layout follows each problem's n (m x m doubles A and V, m = n
+// rounded up to even, then four n x n complex matrices), the launch's LDS
+// size its largest n, so launches of small aggregates keep several
+// work-groups per CU.  Each Matsubara sum is split over the threads left
+// per pair (a power of two, at most 64, reduced by wavefront shuffles).
+// table int64 [B][3]: n, offset of H1 (doubles, n x n), offset of P and P^J
+// (complex, n x n); bath double [B][3]: T, lam, gam.
+constexpr int kAbsMaxSites = 31;
+
+__host__ __device__ inline size_t abs_lds_bytes(int n) {
+  const int m = n + (n & 1);
+  return 2 * (size_t)m * m * sizeof(double) + 4 * (size_t)n * n * sizeof(double2);
+}
+
+// out = alpha A B + gamma I (n x n row-major, LDS); the caller synchronises
+// before and after
+__device__ inline void abs_gemm(const double2* A, const double2* B, double2* out, int n,
+                                double alpha, double gamma) {
+  for (int e = threadIdx.x; e < n * n; e += kThreads) {
+    const int i = e / n, j = e % n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k = 0; k < n; ++k) cfma<double>(acc, A[i * n + k], B[k * n + j]);
+    out[i * n + j] = make_double2(fma(alpha, acc.x, i == j ? gamma : 0.0), alpha * acc.y);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
+    const int64_t* __restrict__ table, int max_n, const double* __restrict__ H1,
+    const double* __restrict__ rw, const double* __restrict__ bath, int flags, double unit,
+    double h, int J, double2* __restrict__ P1, double2* __restrict__ PJ) {
+  extern __shared__ __align__(16) unsigned char absmem[];
+  __shared__ double rot[kAbsMaxSites / 2 + 1][2];
+  __shared__ int pqs[kAbsMaxSites / 2 + 1][2];
+  __shared__ double red[2][kThreads];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int n = (int)table[3 * b];
+  if (n < 1 || n > max_n) return;                          // uniform per work-group
+  const int64_t oh = table[3 * b + 1], op = table[3 * b + 2];
+  const int m = n + (n & 1), nn = n * n;
+  double* A = reinterpret_cast<double*>(absmem);           // [m x m] H_e, then diag(E)
+  double* V = A + m * m;                                    // [m x m] eigenvectors
+  double2* C = reinterpret_cast<double2*>(V + m * m);       // C, then L_eig
+  double2* G = C + nn;                                      // Gs, then L
+  double2* X = G + nn;                                      // scratch
+  double2* Y = X + nn;                                      // P, its powers
+  const double w = rw[b];
+  for (int e = tid; e < m * m; e += kThreads) {
+    const int r = e / m, c = e % m;
+    double v = (r < n && c < n) ? H1[oh + r * n + c] : 0.0;
+    if (r == c && r < n) v -= w;
+    A[e] = v;
+    V[e] = r == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  jacobi_sweeps(A, V, m, rot, pqs, red);
+  // C[d][c] = C(E_d - E_c) over the pairs d <= c, one Matsubara sum each,
+  // split over tpp threads
+  const double T = bath[3 * b], lam = bath[3 * b + 1], gam = bath[3 * b + 2];
+  const int real_corr = (flags & kRealCorr) != 0;
+  const int npairs = n * (n + 1) / 2;
+  int tpp = 1;
+  while (tpp < 64 && 2 * tpp * npairs <= kThreads) tpp *= 2;
+  for (int e0 = 0; e0 < npairs * tpp; e0 += kThreads) {     // uniform trip count
+    const int e = e0 + tid, pair = e / tpp, part = e % tpp;
+    int d = 0, c = 0;
+    double x = 0.0;
+    double2 S = make_double2(0.0, 0.0);
+    if (pair < npairs) {
+      int r = pair;
+      while (r >= n - d) {
+        r -= n - d;
+        ++d;
+      }
+      c = d + r;
+      x = A[d * m + d] - A[c * m + c];
+      if (!real_corr && x != 0.0) S = matsubara_partial(x, T, gam, 1 + part, tpp);
+    }
+    for (int o = tpp / 2; o > 0; o >>= 1) {                  // within aligned groups of tpp lanes
+      S.x += __shfl_xor(S.x, o);
+      S.y += __shfl_xor(S.y, o);
+    }
+    if (pair < npairs && part == 0) {
+      C[d * n + c] = debye_corr_with(x, S, T, lam, gam, real_corr);
+      if (c != d) C[c * n + d] = debye_corr_with(-x, make_double2(S.x, -S.y), T, lam, gam, real_corr);
+    }
+  }
+  __syncthreads();
+  // X[i][c] = sum_m U[i][m]^2 C[c][m], then Gs[a][c] = sum_i U[i][a] U[i][c] X[i][c]
+  for (int e = tid; e < nn; e += kThreads) {
+    const int i = e / n, c = e % n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int q = 0; q < n; ++q) {
+      const double u = V[i * m + q];
+      const double2 cc = C[c * n + q];
+      acc.x = fma(u * u, cc.x, acc.x);
+      acc.y = fma(u * u, cc.y, acc.y);
+    }
+    X[i * n + c] = acc;
+  }
+  __syncthreads();
+  for (int e = tid; e < nn; e += kThreads) {
+    const int a = e / n, c = e % n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = 0; i < n; ++i) {
+      const double uu = V[i * m + a] * V[i * m + c];
+      const double2 xx = X[i * n + c];
+      acc.x = fma(uu, xx.x, acc.x);
+      acc.y = fma(uu, xx.y, acc.y);
+    }
+    G[a * n + c] = acc;
+  }
+  __syncthreads();
+  // L_eig (transposed for the Heisenberg picture) into C
+  const bool secular = (flags & kSecular) != 0, heis = (flags & kHeisenberg) != 0;
+  for (int e = tid; e < nn; e += kThreads) {
+    const int a = e / n, c = e % n;
+    double2 v = make_double2(0.0, 0.0);
+    if (!secular || a == c) v = make_double2(-G[a * n + c].x, -G[a * n + c].y);
+    if (a == c) v.y -= A[a * m + a];
+    v = make_double2(unit * v.x, unit * v.y);
+    if (heis) C[c * n + a] = v;
+    else C[a * n + c] = v;
+  }
+  __syncthreads();
+  // L = U L_eig U^T: X = L_eig U^T, G = U X
+  for (int e = tid; e < nn; e += kThreads) {
+    const int a = e / n, j = e % n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int c = 0; c < n; ++c) {
+      const double u = V[j * m + c];
+      const double2 l = C[a * n + c];
+      acc.x = fma(u, l.x, acc.x);
+      acc.y = fma(u, l.y, acc.y);
+    }
+    X[a * n + j] = acc;
+  }
+  __syncthreads();
+  for (int e = tid; e < nn; e += kThreads) {
+    const int i = e / n, j = e % n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int a = 0; a < n; ++a) {
+      const double u = V[i * m + a];
+      const double2 t = X[a * n + j];
+      acc.x = fma(u, t.x, acc.x);
+      acc.y = fma(u, t.y, acc.y);
+    }
+    G[i * n + j] = acc;
+  }
+  __syncthreads();
+  // P = P4(h L) by Horner: Y = I + h/4 L, Y = I + c L Y for c = h/3, h/2, h
+  for (int e = tid; e < nn; e += kThreads) {
+    const int i = e / n, j = e % n;
+    const double2 l = G[e];
+    Y[e] = make_double2(fma(0.25 * h, l.x, i == j ? 1.0 : 0.0), 0.25 * h * l.y);
+  }
+  __syncthreads();
+  const double coef[3] = {h / 3.0, h / 2.0, h};
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    abs_gemm(G, Y, X, n, coef[t], 1.0);
+    __syncthreads();
+    for (int e = tid; e < nn; e += kThreads) Y[e] = X[e];
+    __syncthreads();
+  }
+  for (int e = tid; e < nn; e += kThreads) P1[op + e] = Y[e];
+  // P^J by squaring (J a power of two)
+  for (int q = 1; q < J; q *= 2) {
+    abs_gemm(Y, Y, X, n, 1.0, 0.0);
+    __syncthreads();
+    for (int e = tid; e < nn; e += kThreads) Y[e] = X[e];
+    __syncthreads();
+  }
+  for (int e = tid; e < nn; e += kThreads) PJ[op + e] = Y[e];
 }
 
 // ------------------------------------------------ vector-Jacobian product
@@ -1338,6 +1553,30 @@ extern "C" int32_t qsx_unitary_block_dense(const double* HX, const double* HY, i
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }
+
+extern "C" int32_t qsx_absorption_operators(const int64_t* table, int32_t B, int32_t max_n,
+                                            const double* H1, const double* rw,
+                                            const double* bath, int32_t flags,
+                                            double unit_convert, double h, int32_t J, void* P1,
+                                            void* PJ, void* stream) {
+  QSX_CHECK_ARG(table && H1 && rw && bath && P1 && PJ, "qsx_absorption_operators: null pointer");
+  QSX_CHECK_ARG(B >= 1 && (J == 1 || J == 2 || J == 4) && max_n >= 1 && max_n <= kAbsMaxSites,
+                "qsx_absorption_operators: bad sizes B=%d J=%d max_n=%d", B, J, max_n);
+  QSX_CHECK_ARG((flags & ~(kSecular | kRealCorr | kHeisenberg)) == 0,
+                "qsx_absorption_operators: flags %d (secular, real correlation, Heisenberg only)",
+                flags);
+  const size_t lds = abs_lds_bytes(max_n);
+  if (lds > 64 * 1024)
+    QSX_CHECK_HIP(hipFuncSetAttribute((const void*)absorption_ops_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(absorption_ops_kernel, dim3(B), dim3(kThreads), lds, as_stream(stream), table,
+                     max_n, H1, rw, bath, flags, unit_convert, h, J, static_cast<double2*>(P1),
+                     static_cast<double2*>(PJ));
+  QSX_CHECK_HIP(hipGetLastError());
+  return QSX_OK;
+}
+
+extern "C" int32_t qsx_absorption_operators_max_sites(void) { return kAbsMaxSites; }
 
 // -------------------------------------------------------------------- VJP
 static size_t vjp_ws_doubles(int n, int X, int Y, int B) {

@@ -184,3 +184,64 @@ def test_response_buckets_four_steps_per_pass(gpu):
         assert int(status.max().item()) == 0
         for i in range(len(sizes)):
             assert rel_err(got[i], ref[i]) < 1e-12, (J, i, sizes[i])
+
+
+@pytest.mark.parametrize('secular,real', [(False, False), (True, False), (False, True)])
+def test_absorption_operators_match_block_builder(gpu, secular, real):
+    """qsx_absorption_operators (every member of every size in one launch)
+    equals the per-size path: qsx_redfield_block's Heisenberg 'eg' block and
+    qsx_small_propagator's P4(h L) and its fourth power (1e-12), for 1..31
+    sites, secular or not, complex or real correlation function; a launch
+    whose max_n is below some sizes leaves those problems' outputs alone."""
+    import torch
+    sizes = [1, 2, 5, 7, 13, 24, 31]
+    rw, dt, J = 12500.0, 1.0, 4
+    table, H1s, rws, baths, ref1, refJ = [], [], [], [], [], []
+    oh = op = 0
+    for n in sizes:
+        _, hams = _plan_hams([n] * 3, 11 + n)
+        hams = [h.in_rotating_frame(rw) for h in hams]
+        m = qa.RedfieldModel(hams[0], rw_freq=rw, hilbert_subspace='ge', unit_convert=qa.CM_FS,
+                             secular=secular, discard_imag_corr=real).with_hamiltonians(hams)
+        (H1, rv), rebuild = m.block_builder('eg', heisenberg_picture=True)
+        L = torch.empty((3, n, n), dtype=torch.complex128, device='cuda')
+        rebuild(L)
+        P1 = torch.empty_like(L)
+        PJ = torch.empty_like(L)
+        _native.small_propagator_into(L, dt, 1, J, P=P1, Pc=PJ)
+        ref1.append(P1.cpu().numpy())
+        refJ.append(PJ.cpu().numpy())
+        H1s.append(H1.cpu().numpy().ravel())
+        rws.append(rv.cpu().numpy())
+        flags = m._flags(True)
+        for k in range(3):
+            table.append((n, oh + k * n * n, op + k * n * n))
+            baths.append(BATH)
+        oh += 3 * n * n
+        op += 3 * n * n
+    dev = 'cuda'
+    args = (torch.as_tensor(np.asarray(table, dtype=np.int64)).to(dev),
+            torch.as_tensor(np.concatenate(H1s)).to(dev),
+            torch.as_tensor(np.concatenate(rws)).to(dev),
+            torch.as_tensor(np.asarray(baths, dtype=float)).to(dev))
+    # max_n below some sizes: those problems are skipped, the others built
+    P1 = torch.zeros(op, dtype=torch.complex128, device=dev)
+    PJ = torch.zeros(op, dtype=torch.complex128, device=dev)
+    _native.absorption_operators_into(args[0], 7, *args[1:], flags, qa.CM_FS, dt, J, P1, PJ)
+    P1, PJ = P1.cpu().numpy(), PJ.cpu().numpy()
+    for (n, _, o), k in zip(table, range(len(table))):
+        if n > 7:
+            assert not P1[o:o + n * n].any() and not PJ[o:o + n * n].any()
+        else:
+            assert rel_err(P1[o:o + n * n].reshape(n, n), ref1[sizes.index(n)][k % 3]) < 1e-12
+    P1 = torch.empty(op, dtype=torch.complex128, device=dev)
+    PJ = torch.empty(op, dtype=torch.complex128, device=dev)
+    _native.absorption_operators_into(args[0], 31, *args[1:], flags, qa.CM_FS, dt, J, P1, PJ)
+    P1, PJ = P1.cpu().numpy(), PJ.cpu().numpy()
+    row = 0
+    for s, n in enumerate(sizes):
+        for k in range(3):
+            _, _, o = table[row]
+            assert rel_err(P1[o:o + n * n].reshape(n, n), ref1[s][k]) < 1e-12, (n, k)
+            assert rel_err(PJ[o:o + n * n].reshape(n, n), refJ[s][k]) < 1e-12, (n, k)
+            row += 1
