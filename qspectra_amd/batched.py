@@ -69,14 +69,15 @@ class AbsorptionEnsemblePlan(object):
     captured HIP graph, spectra left on the device.
 
     The Redfield 'eg' blocks (Heisenberg picture) rebuilt from the members'
-    Hamiltonians and their one-step propagators P = P4(dt L) with P^4: for a
-    Debye bath all members of all sizes in one launch
-    (qsx_absorption_operators), otherwise per site count (bucket) with
-    qsx_redfield_block and qsx_small_propagator.  Then
-    the projection rows (P^j)^T v, j < 4 (qsx_response_bucket_weights), one
-    launch that propagates every member of every bucket four steps per pass
-    and records its linear-response signal (qsx_response_buckets), and the
-    Fourier transform of all signals gives the spectra.  The values are those of absorption_batch
+    Hamiltonians and their one-step propagators P = P4(dt L) with P^4, the
+    projection rows (P^j)^T v, j < 4 (qsx_response_bucket_weights), and a
+    launch that propagates the members four steps per pass and records
+    their linear-response signals (qsx_response_buckets).  For a Debye bath
+    the operators come from qsx_absorption_operators and this chain runs
+    once per size class (sites <= 8, 16, 24, 31), the classes concurrent on
+    side streams; otherwise each site count (bucket) is built with
+    qsx_redfield_block and qsx_small_propagator and one launch propagates
+    every bucket.  The Fourier transform of all signals gives the spectra.  The values are those of absorption_batch
     (absorption_spectra member by member, response.py:103-154) with FP64
     propagation at the step dt.
 
@@ -155,7 +156,7 @@ class AbsorptionEnsemblePlan(object):
         if correlation_decay_time is not None:
             self._decay = torch.as_tensor(np.exp(-self.t / correlation_decay_time)).to(dev)
         self._check_tables(waves, offs, offP, offY)
-        self._fused = self._fused_builds(offs, offP, dev)
+        self._fused = self._fused_builds(offs, waves, offP, dev)
         # the buckets' builds are independent: largest first, round-robin over
         # side streams, joined before the propagation launch
         self._build_order = sorted(range(len(self.buckets)),
@@ -170,12 +171,15 @@ class AbsorptionEnsemblePlan(object):
     # largest site count of each launch of qsx_absorption_operators
     FUSED_CLASSES = (8, 16, 24, 31)
 
-    def _fused_builds(self, offs, nP, dev):
+    def _fused_builds(self, offs, waves, nP, dev):
         """Redfield models of a Debye bath in the site basis: the operators of
-        every bucket come from one launch (qsx_absorption_operators) reading
-        flat H1 / rw buffers, and each bucket's hamiltonian_inputs become
-        views into them.  Returns the launch arguments, or None (other models
-        and baths keep one build per bucket)."""
+        every bucket come from qsx_absorption_operators, one launch per size
+        class reading flat H1 / rw buffers, and each bucket's
+        hamiltonian_inputs become views into them.  Each class then runs its
+        own projection-row and propagation launches over its wavefronts, so
+        the classes proceed independently on the side streams.  Returns the
+        launch arguments, or None (other models and baths keep one build per
+        bucket)."""
         from .bath import debye_parameters, is_debye_form
         from .dynamics.redfield import RedfieldModel
         lib = _native.require_native()
@@ -214,14 +218,16 @@ class AbsorptionEnsemblePlan(object):
         tab = torch.as_tensor(np.asarray(table, dtype=np.int64)).to(dev)
         bt = torch.as_tensor(np.asarray(bath, dtype=np.float64)).to(dev)
         sizes = np.asarray([r[0] for r in table])
+        wsizes = np.asarray([w[2] for w in waves])      # ascending M = n
         launches = []
-        lo = 0
+        lo = wlo = 0
         for top in self.FUSED_CLASSES:
             hi = int(np.searchsorted(sizes, top, side='right'))
+            whi = int(np.searchsorted(wsizes, top, side='right'))
             if hi > lo:
-                launches.append((top, tab[lo:hi], rw[lo:hi], bt[lo:hi]))
-            lo = hi
-        assert lo == len(table)
+                launches.append((top, tab[lo:hi], rw[lo:hi], bt[lo:hi], self.waves[wlo:whi]))
+            lo, wlo = hi, whi
+        assert lo == len(table) and wlo == len(waves)
         return dict(launches=launches, H1=H1, flags=flags.pop(),
                     unit=self.buckets[0]['model'].unit_convert)
 
@@ -251,13 +257,20 @@ class AbsorptionEnsemblePlan(object):
         for side in self._sides:
             side.wait_stream(main)
         if fz is not None:
-            # every bucket's operators, one launch per size class, the classes
-            # (largest sites first) concurrent on the side streams
-            for j, (top, table, rw, bath) in enumerate(reversed(fz['launches'])):
+            # per size class (largest sites first, concurrent on the side
+            # streams): the operators, the projection rows and the signals of
+            # its wavefronts, which write disjoint rows of W, signal and status
+            for j, (top, table, rw, bath, waves) in enumerate(reversed(fz['launches'])):
                 with torch.cuda.stream(self._sides[j % len(self._sides)]):
                     _native.absorption_operators_into(table, top, fz['H1'], rw, bath,
                                                       fz['flags'], fz['unit'], self.dt, self.J,
                                                       self.P1, self.P)
+                    _native.response_bucket_weights_into(waves, self.offsets, self.P1, self.v,
+                                                         self.J, self.W)
+                    _native.response_buckets_into(waves, self.offsets, self.P, self.y0, self.W,
+                                                  self.signal, self.status, J=self.J)
+            for side in self._sides:
+                main.wait_stream(side)
         else:
             for j, k in enumerate(self._build_order):
                 bk = self.buckets[k]
@@ -265,12 +278,12 @@ class AbsorptionEnsemblePlan(object):
                     bk['rebuild'](bk['L'])
                     _native.small_propagator_into(bk['L'], self.dt, 1, self.J, P=bk['P1'],
                                                   Pc=bk['P'])
-        for side in self._sides:
-            main.wait_stream(side)
-        _native.response_bucket_weights_into(self.waves, self.offsets, self.P1, self.v, self.J,
-                                             self.W)
-        _native.response_buckets_into(self.waves, self.offsets, self.P, self.y0, self.W,
-                                      self.signal, self.status, J=self.J)
+            for side in self._sides:
+                main.wait_stream(side)
+            _native.response_bucket_weights_into(self.waves, self.offsets, self.P1, self.v,
+                                                 self.J, self.W)
+            _native.response_buckets_into(self.waves, self.offsets, self.P, self.y0, self.W,
+                                          self.signal, self.status, J=self.J)
         # the signal rows are in input order; the response's sign goes into
         # the transform's scale
         x = self.signal if self.decay is None else self.signal * self._decay
