@@ -351,6 +351,36 @@ __global__ __launch_bounds__(kFftThreads) void chirp_rows_kernel(
   for (int64_t b = blockIdx.x; b < batch; b += gridDim.x) {
     const double2* row = in + b * g.n_in;
     double2* orow = out + b * g.n_out;
+    if (g.pieces == 1) {
+      // one piece: its spectrum, kept in registers (each thread its own
+      // kFftPer entries), serves every chunk
+      load_piece(buf, row, g, 0, cin);
+      fft_row(buf, tw1, tw2);
+      double2 S[kFftPer];
+#pragma unroll
+      for (int r = 0; r < kFftPer; ++r) S[r] = buf[fpad(t + r * kFftThreads)];
+      for (int j = 0; j < g.chunks; ++j) {
+        const int j0 = j * g.H, he = min(g.H, g.n_out - j0);
+        const double2* G = Gt + (size_t)j * kFftL;
+#pragma unroll
+        for (int r = 0; r < kFftPer; ++r) {
+          const int k = t + r * kFftThreads;
+          buf[fpad(k)] = cconj(cmul(S[r], G[k]));
+        }
+        __syncthreads();
+        fft_row(buf, tw1, tw2);
+#pragma unroll
+        for (int r = 0; r < kFftPer; ++r) {
+          const int u = t + r * kFftThreads;
+          if (u < he) {
+            const double2 z = cmul(cout[j0 + u], cconj(buf[fpad(u)]));
+            orow[j0 + u] = make_double2(oscale * z.x, oscale * z.y);
+          }
+        }
+        __syncthreads();
+      }
+      continue;
+    }
     for (int j = 0; j < g.chunks; ++j) {
       const int j0 = j * g.H, he = min(g.H, g.n_out - j0);
       for (int i = 0; i < g.pieces; ++i) {
