@@ -51,17 +51,21 @@ def absorption_batch(hamiltonians, time_max, rw_freq, dt, model_cls=RedfieldMode
     return f, out
 
 
-# steps per LDS round trip of qsx_response_buckets in the ensemble plan
-STEPS_PER_PASS = 4
+# steps per round trip of qsx_response_buckets in the ensemble plan
+STEPS_PER_PASS = 8
 
 
 def _group_size(M, J=STEPS_PER_PASS):
-    """Lanes per problem in qsx_response_buckets: the power of two at or
-    above M + J (M state rows and J projection rows)."""
-    G = 2
-    while G < M + J:
-        G *= 2
-    return G
+    """Lanes per problem in qsx_response_buckets (M state rows and J
+    projection rows): 16 when M + J <= 16 (the register-resident form, a
+    problem per 16-lane row); otherwise p, the power of two at or above
+    M + J, with two lanes per row (2p) when 2p fits the wavefront."""
+    if J > 1 and M + J <= 16:
+        return 16
+    p = 2
+    while p < M + J:
+        p *= 2
+    return 2 * p if J > 1 and 2 * p <= 64 else p
 
 
 class AbsorptionEnsemblePlan(object):
@@ -69,9 +73,9 @@ class AbsorptionEnsemblePlan(object):
     captured HIP graph, spectra left on the device.
 
     The Redfield 'eg' blocks (Heisenberg picture) rebuilt from the members'
-    Hamiltonians and their one-step propagators P = P4(dt L) with P^4, the
-    projection rows (P^j)^T v, j < 4 (qsx_response_bucket_weights), and a
-    launch that propagates the members four steps per pass and records
+    Hamiltonians and their one-step propagators P = P4(dt L) with P^8, the
+    projection rows (P^j)^T v, j < 8 (qsx_response_bucket_weights), and a
+    launch that propagates the members eight steps per pass and records
     their linear-response signals (qsx_response_buckets).  For a Debye bath
     the operators come from qsx_absorption_operators and this chain runs
     once per size class (sites <= 8, 16, 24, 31), the classes concurrent on

@@ -1560,7 +1560,7 @@ extern "C" int32_t qsx_absorption_operators(const int64_t* table, int32_t B, int
                                             double unit_convert, double h, int32_t J, void* P1,
                                             void* PJ, void* stream) {
   QSX_CHECK_ARG(table && H1 && rw && bath && P1 && PJ, "qsx_absorption_operators: null pointer");
-  QSX_CHECK_ARG(B >= 1 && (J == 1 || J == 2 || J == 4) && max_n >= 1 && max_n <= kAbsMaxSites,
+  QSX_CHECK_ARG(B >= 1 && (J == 1 || J == 2 || J == 4 || J == 8) && max_n >= 1 && max_n <= kAbsMaxSites,
                 "qsx_absorption_operators: bad sizes B=%d J=%d max_n=%d", B, J, max_n);
   QSX_CHECK_ARG((flags & ~(kSecular | kRealCorr | kHeisenberg)) == 0,
                 "qsx_absorption_operators: flags %d (secular, real correlation, Heisenberg only)",

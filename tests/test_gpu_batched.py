@@ -123,15 +123,23 @@ def test_ensemble_plan_rebuilds_from_new_hamiltonians(gpu):
         assert rel_err(X[i], Xb[i]) < 1e-12, i
 
 
-def _bucket_tables(sizes, J):
+def _pow2(x):
+    p = 1
+    while p < x:
+        p *= 2
+    return p
+
+
+def _bucket_tables(sizes, J, lanes=None):
     """Wave table and offsets of qsx_response_buckets for problems of the
-    given sizes (grouped by size, output rows in input order)."""
+    given sizes (grouped by size, output rows in input order); lanes(M, J)
+    gives the lanes per problem (the plan's choice by default)."""
     from qspectra_amd.batched import _group_size
     waves, offs, order = [], [None] * len(sizes), []
     offP = offY = 0
     for M in sorted(set(sizes)):
         idx = [i for i, m in enumerate(sizes) if m == M]
-        G = _group_size(M, J)
+        G = (lanes or _group_size)(M, J)
         first = len(order)
         for w0 in range(0, len(idx), 64 // G):
             waves.append((first + w0, min(64 // G, len(idx) - w0), M, G))
@@ -143,11 +151,14 @@ def _bucket_tables(sizes, J):
     return waves, offs, order
 
 
-def test_response_buckets_four_steps_per_pass(gpu):
-    """qsx_response_buckets at four steps per pass (P^4 and the projection
-    rows (P^j)^T v written by qsx_response_bucket_weights) and at one step
-    per pass, for problems of 1..31 states and a step count that is not a
-    multiple of 4, against the host's s_k = v . P^k y0 (1e-12)."""
+def test_response_buckets_several_steps_per_pass(gpu):
+    """qsx_response_buckets at J = 8 and 4 steps per pass (P^J and the
+    projection rows (P^j)^T v written by qsx_response_bucket_weights) and
+    at one step per pass, for problems of 1..31 states and a step count
+    that is not a multiple of J, against the host's s_k = v . P^k y0
+    (1e-12), in each of the kernel's forms: a problem per 16-lane row with
+    the state in registers (M + J <= 16), two lanes per row, one lane per
+    row."""
     import torch
     rs = np.random.RandomState(3)
     sizes = [7, 1, 31, 3, 12, 7, 13, 28, 29, 7, 2]
@@ -160,8 +171,11 @@ def test_response_buckets_four_steps_per_pass(gpu):
         for k in range(nt):
             ref[i, k] = v @ y
             y = P_ @ y
-    for J in (1, 4):
-        waves, offs, order = _bucket_tables(sizes, J)
+    one_lane = lambda M, J: _pow2(M + J)
+    two_lanes = lambda M, J: 2 * _pow2(M + J) if 2 * _pow2(M + J) <= 64 else _pow2(M + J)
+    for J, lanes in ((1, None), (4, None), (4, one_lane), (8, None), (8, one_lane),
+                     (8, two_lanes)):
+        waves, offs, order = _bucket_tables(sizes, J, lanes)
         dev = 'cuda'
         P1 = np.concatenate([Ps[i].ravel() for i in order])
         PJ = np.concatenate([np.linalg.matrix_power(Ps[i], J).ravel() for i in order])
