@@ -80,30 +80,37 @@ __device__ inline void pair_of(int s, int n, int& i, int& j) {
 // the eigenvectors.  Called by every thread of a kThreads work-group.
 __device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int (*pqs)[2],
                               double (*red)[kThreads]) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, h = m / 2;
+  // q / h and q / m for the small q here, exact: the quotient's fraction
+  // stays at least 1 / (2 m) from an integer
+  const float rh = 1.0f / (float)h, rm = 1.0f / (float)m;
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0.0, tot = 0.0;
     for (int e = tid; e < m * m; e += kThreads) {
       const double a = A[e];
+      const int r = __float2int_rz(((float)e + 0.5f) * rm);
       tot += a * a;
-      if (e / m != e % m) off += a * a;
+      if (e != r * (m + 1)) off += a * a;
     }
-    red[0][tid] = off;
-    red[1][tid] = tot;
+    for (int o = 32; o > 0; o >>= 1) {                    // within each wavefront
+      off += __shfl_xor(off, o);
+      tot += __shfl_xor(tot, o);
+    }
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = off;
+      red[1][tid >> 6] = tot;
+    }
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-      if (tid < w) {
-        red[0][tid] += red[0][tid + w];
-        red[1][tid] += red[1][tid + w];
-      }
-      __syncthreads();
+    off = 0.0;
+    tot = 0.0;
+    for (int q = 0; q < kThreads / 64; ++q) {
+      off += red[0][q];
+      tot += red[1][q];
     }
-    off = red[0][0];
-    tot = red[1][0];
     __syncthreads();
     if (!(off > 1e-30 * tot)) break;
     for (int round = 0; round < m - 1; ++round) {
-      if (tid < m / 2) {
+      if (tid < h) {
         auto idx = [&](int j) { return ((j + round) % (m - 1)) + 1; };
         int p = tid == 0 ? 0 : idx(tid);
         int q = tid == 0 ? idx(0) : idx(m - 1 - tid);
@@ -122,30 +129,31 @@ __device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int
         rot[tid][1] = s;
       }
       __syncthreads();
-      for (int w = tid; w < (m / 2) * m; w += kThreads) {  // A <- A J, V <- V J
-        const int k = w / m, r = w % m;
-        const int p = pqs[k][0], q = pqs[k][1];
-        const double c = rot[k][0], s = rot[k][1];
-        const double ap = A[r * m + p], aq = A[r * m + q];
-        A[r * m + p] = c * ap - s * aq;
-        A[r * m + q] = s * ap + c * aq;
-        const double vp = V[r * m + p], vq = V[r * m + q];
-        V[r * m + p] = c * vp - s * vq;
-        V[r * m + q] = s * vp + c * vq;
-      }
-      __syncthreads();
-      for (int w = tid; w < (m / 2) * m; w += kThreads) {  // A <- J^T A
-        const int k = w / m, r = w % m;
-        const int p = pqs[k][0], q = pqs[k][1];
-        const double c = rot[k][0], s = rot[k][1];
-        const double ap = A[p * m + r], aq = A[q * m + r];
-        A[p * m + r] = c * ap - s * aq;
-        A[q * m + r] = s * ap + c * aq;
-      }
-      __syncthreads();
-      if (tid < m / 2) {
-        A[pqs[tid][0] * m + pqs[tid][1]] = 0.0;
-        A[pqs[tid][1] * m + pqs[tid][0]] = 0.0;
+      // A <- J^T A J by 2 x 2 blocks (rows of rotation k1, columns of
+      // rotation k2: each block read and written by one thread; the pivot
+      // blocks k1 = k2 lose their off-diagonal pair), V <- V J by row pairs
+      for (int w = tid; w < h * h + m * h; w += kThreads) {
+        if (w < h * h) {
+          const int k1 = __float2int_rz(((float)w + 0.5f) * rh), k2 = w - k1 * h;
+          const int p1 = pqs[k1][0], q1 = pqs[k1][1], p2 = pqs[k2][0], q2 = pqs[k2][1];
+          const double c1 = rot[k1][0], s1 = rot[k1][1], c2 = rot[k2][0], s2 = rot[k2][1];
+          const double app = A[p1 * m + p2], apq = A[p1 * m + q2];
+          const double aqp = A[q1 * m + p2], aqq = A[q1 * m + q2];
+          const double bpp = c2 * app - s2 * apq, bpq = s2 * app + c2 * apq;
+          const double bqp = c2 * aqp - s2 * aqq, bqq = s2 * aqp + c2 * aqq;
+          A[p1 * m + p2] = c1 * bpp - s1 * bqp;
+          A[q1 * m + q2] = s1 * bpq + c1 * bqq;
+          A[p1 * m + q2] = k1 == k2 ? 0.0 : c1 * bpq - s1 * bqq;
+          A[q1 * m + p2] = k1 == k2 ? 0.0 : s1 * bpp + c1 * bqp;
+        } else {
+          const int v = w - h * h;
+          const int r = __float2int_rz(((float)v + 0.5f) * rh), k = v - r * h;
+          const int p = pqs[k][0], q = pqs[k][1];
+          const double c = rot[k][0], s = rot[k][1];
+          const double vp = V[r * m + p], vq = V[r * m + q];
+          V[r * m + p] = c * vp - s * vq;
+          V[r * m + q] = s * vp + c * vq;
+        }
       }
       __syncthreads();
     }
