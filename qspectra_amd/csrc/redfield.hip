@@ -77,16 +77,18 @@ __device__ inline void pair_of(int s, int n, int& i, int& j) {
 // accumulating the rotations into V: m/2 disjoint rotations per round
 // (circle method), sweeps until the off-diagonal norm is below 1e-30 of the
 // total.  On return A is diagonal (the eigenvalues) and the columns of V are
-// the eigenvectors.  Called by every thread of a kThreads work-group.
+// the eigenvectors.  Called by every thread of an NT-thread work-group; red
+// holds 2 NT / 64 doubles.
+template <int NT>
 __device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int (*pqs)[2],
-                              double (*red)[kThreads]) {
+                              double* red) {
   const int tid = threadIdx.x, h = m / 2;
   // q / h and q / m for the small q here, exact: the quotient's fraction
   // stays at least 1 / (2 m) from an integer
   const float rh = 1.0f / (float)h, rm = 1.0f / (float)m;
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0.0, tot = 0.0;
-    for (int e = tid; e < m * m; e += kThreads) {
+    for (int e = tid; e < m * m; e += NT) {
       const double a = A[e];
       const int r = __float2int_rz(((float)e + 0.5f) * rm);
       tot += a * a;
@@ -97,15 +99,15 @@ __device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int
       tot += __shfl_xor(tot, o);
     }
     if ((tid & 63) == 0) {
-      red[0][tid >> 6] = off;
-      red[1][tid >> 6] = tot;
+      red[tid >> 6] = off;
+      red[NT / 64 + (tid >> 6)] = tot;
     }
     __syncthreads();
     off = 0.0;
     tot = 0.0;
-    for (int q = 0; q < kThreads / 64; ++q) {
-      off += red[0][q];
-      tot += red[1][q];
+    for (int q = 0; q < NT / 64; ++q) {
+      off += red[q];
+      tot += red[NT / 64 + q];
     }
     __syncthreads();
     if (!(off > 1e-30 * tot)) break;
@@ -118,9 +120,12 @@ __device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int
         const double apq = A[p * m + q];
         double c = 1.0, s = 0.0;
         if (apq != 0.0) {
-          const double theta = (A[q * m + q] - A[p * m + p]) / (2.0 * apq);
-          const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          c = 1.0 / sqrt(t * t + 1.0);
+          // t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)), theta = d / e,
+          // with numerator and denominator times |e|: one division
+          const double d = A[q * m + q] - A[p * m + p], e = 2.0 * apq;
+          const double num = d > 0.0 ? e : (d < 0.0 ? -e : fabs(e));
+          const double t = num / (fabs(d) + sqrt(d * d + e * e));
+          c = rsqrt(t * t + 1.0);
           s = t * c;
         }
         pqs[tid][0] = p;
@@ -132,7 +137,7 @@ __device__ void jacobi_sweeps(double* A, double* V, int m, double (*rot)[2], int
       // A <- J^T A J by 2 x 2 blocks (rows of rotation k1, columns of
       // rotation k2: each block read and written by one thread; the pivot
       // blocks k1 = k2 lose their off-diagonal pair), V <- V J by row pairs
-      for (int w = tid; w < h * h + m * h; w += kThreads) {
+      for (int w = tid; w < h * h + m * h; w += NT) {
         if (w < h * h) {
           const int k1 = __float2int_rz(((float)w + 0.5f) * rh), k2 = w - k1 * h;
           const int p1 = pqs[k1][0], q1 = pqs[k1][1], p2 = pqs[k2][0], q2 = pqs[k2][1];
@@ -199,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __re
     V[e] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
-  jacobi_sweeps(A, V, m, rot, pqs, red);
+  jacobi_sweeps<kThreads>(A, V, m, rot, pqs, &red[0][0]);
   // ascending eigenvalues (stable ranking), canonical sign: the largest-
   // magnitude component of each eigenvector is made positive
   for (int k = tid; k < nb; k += kThreads) {
@@ -231,44 +236,39 @@ __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __re
 // ------------------------------------------------------------ 2. C(w) and K
 // Debye bath correlation function, qspectra/bath.py:84-102 (complex, 1000-term
 // Matsubara sum, k = 0 term zero) and bath.py:17-31 (real, detailed balance).
-// sum_{k=1}^{999} nu_k / ((nu_k^2 - g^2)(nu_k - i x)), nu_k = 2 pi k T,
-// Kahan-compensated; its value at -x is the complex conjugate (each term's
-// imaginary part is odd in x, its real part even), exactly so in floating
-// point since only signs change
-__device__ double2 matsubara_sum(double x, double T, double gam) {
-  double sr = 0.0, si = 0.0, cr = 0.0, ci = 0.0;
+// The terms k = k0, k0 + stride, ... (< 1000) of the Matsubara sum: four
+// interleaved Kahan-compensated sums (their divisions independent, so they
+// overlap), added at the end
+__device__ double2 matsubara_partial(double x, double T, double gam, int k0, int stride) {
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+  double cr[4] = {0.0, 0.0, 0.0, 0.0}, ci[4] = {0.0, 0.0, 0.0, 0.0};
   const double twopiT = 2.0 * M_PI * T;
-  for (int k = 1; k < 1000; ++k) {
-    const double nu = twopiT * k;
-    const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
-    const double tr = nu * nu * inv - cr;
-    const double ti = nu * x * inv - ci;
-    const double ur = sr + tr, ui = si + ti;
-    cr = (ur - sr) - tr;
-    ci = (ui - si) - ti;
-    sr = ur;
-    si = ui;
+  for (int k = k0; k < 1000; k += 4 * stride) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int ku = k + u * stride;
+      if (ku < 1000) {
+        const double nu = twopiT * ku;
+        const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
+        const double tr = nu * nu * inv - cr[u];
+        const double ti = nu * x * inv - ci[u];
+        const double ur = sr[u] + tr, ui = si[u] + ti;
+        cr[u] = (ur - sr[u]) - tr;
+        ci[u] = (ui - si[u]) - ti;
+        sr[u] = ur;
+        si[u] = ui;
+      }
+    }
   }
-  return make_double2(sr, si);
+  return make_double2((sr[0] + sr[1]) + (sr[2] + sr[3]), (si[0] + si[1]) + (si[2] + si[3]));
 }
 
-// The terms k = k0, k0 + stride, ... (< 1000) of matsubara_sum, Kahan-
-// compensated: `stride` threads together evaluate one sum
-__device__ double2 matsubara_partial(double x, double T, double gam, int k0, int stride) {
-  double sr = 0.0, si = 0.0, cr = 0.0, ci = 0.0;
-  const double twopiT = 2.0 * M_PI * T;
-  for (int k = k0; k < 1000; k += stride) {
-    const double nu = twopiT * k;
-    const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
-    const double tr = nu * nu * inv - cr;
-    const double ti = nu * x * inv - ci;
-    const double ur = sr + tr, ui = si + ti;
-    cr = (ur - sr) - tr;
-    ci = (ui - si) - ti;
-    sr = ur;
-    si = ui;
-  }
-  return make_double2(sr, si);
+// sum_{k=1}^{999} nu_k / ((nu_k^2 - g^2)(nu_k - i x)), nu_k = 2 pi k T,
+// compensated as above; its value at -x is the complex conjugate (each
+// term's imaginary part is odd in x, its real part even), exactly so in
+// floating point since only signs change
+__device__ double2 matsubara_sum(double x, double T, double gam) {
+  return matsubara_partial(x, T, gam, 1, 1);
 }
 
 // C(x) given the Matsubara sum S(x) (unused for the real correlation
@@ -710,9 +710,10 @@ __host__ __device__ inline size_t abs_lds_bytes(int n) {
 
 // out = alpha A B + gamma I (n x n row-major, LDS); the caller synchronises
 // before and after
+template <int NT>
 __device__ inline void abs_gemm(const double2* A, const double2* B, double2* out, int n,
                                 double alpha, double gamma) {
-  for (int e = threadIdx.x; e < n * n; e += kThreads) {
+  for (int e = threadIdx.x; e < n * n; e += NT) {
     const int i = e / n, j = e % n;
     double2 acc = make_double2(0.0, 0.0);
     for (int k = 0; k < n; ++k) cfma<double>(acc, A[i * n + k], B[k * n + j]);
@@ -720,14 +721,15 @@ __device__ inline void abs_gemm(const double2* A, const double2* B, double2* out
   }
 }
 
-__global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
+template <int NT>
+__global__ __launch_bounds__(NT) void absorption_ops_kernel(
     const int64_t* __restrict__ table, int max_n, const double* __restrict__ H1,
     const double* __restrict__ rw, const double* __restrict__ bath, int flags, double unit,
     double h, int J, double2* __restrict__ P1, double2* __restrict__ PJ) {
   extern __shared__ __align__(16) unsigned char absmem[];
   __shared__ double rot[kAbsMaxSites / 2 + 1][2];
   __shared__ int pqs[kAbsMaxSites / 2 + 1][2];
-  __shared__ double red[2][kThreads];
+  __shared__ double red[2 * NT / 64];
   const int b = blockIdx.x, tid = threadIdx.x;
   const int n = (int)table[3 * b];
   if (n < 1 || n > max_n) return;                          // uniform per work-group
@@ -740,7 +742,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
   double2* X = G + nn;                                      // scratch
   double2* Y = X + nn;                                      // P, its powers
   const double w = rw[b];
-  for (int e = tid; e < m * m; e += kThreads) {
+  for (int e = tid; e < m * m; e += NT) {
     const int r = e / m, c = e % m;
     double v = (r < n && c < n) ? H1[oh + r * n + c] : 0.0;
     if (r == c && r < n) v -= w;
@@ -748,15 +750,23 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
     V[e] = r == c ? 1.0 : 0.0;
   }
   __syncthreads();
-  jacobi_sweeps(A, V, m, rot, pqs, red);
+  jacobi_sweeps<NT>(A, V, m, rot, pqs, red);
   // C[d][c] = C(E_d - E_c) over the pairs d <= c, one Matsubara sum each,
   // split over tpp threads
   const double T = bath[3 * b], lam = bath[3 * b + 1], gam = bath[3 * b + 2];
   const int real_corr = (flags & kRealCorr) != 0;
   const int npairs = n * (n + 1) / 2;
-  int tpp = 1;
-  while (tpp < 64 && 2 * tpp * npairs <= kThreads) tpp *= 2;
-  for (int e0 = 0; e0 < npairs * tpp; e0 += kThreads) {     // uniform trip count
+  // threads per pair: the power of two (at most 64) with the fewest terms
+  // per thread over the rounds of NT threads
+  int tpp = 1, cost = 1 << 30;
+  for (int t = 1; t <= 64; t *= 2) {
+    const int c = ((npairs * t + NT - 1) / NT) * ((999 + t - 1) / t);
+    if (c < cost) {
+      cost = c;
+      tpp = t;
+    }
+  }
+  for (int e0 = 0; e0 < npairs * tpp; e0 += NT) {     // uniform trip count
     const int e = e0 + tid, pair = e / tpp, part = e % tpp;
     int d = 0, c = 0;
     double x = 0.0;
@@ -782,7 +792,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
   }
   __syncthreads();
   // X[i][c] = sum_m U[i][m]^2 C[c][m], then Gs[a][c] = sum_i U[i][a] U[i][c] X[i][c]
-  for (int e = tid; e < nn; e += kThreads) {
+  for (int e = tid; e < nn; e += NT) {
     const int i = e / n, c = e % n;
     double2 acc = make_double2(0.0, 0.0);
     for (int q = 0; q < n; ++q) {
@@ -794,7 +804,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
     X[i * n + c] = acc;
   }
   __syncthreads();
-  for (int e = tid; e < nn; e += kThreads) {
+  for (int e = tid; e < nn; e += NT) {
     const int a = e / n, c = e % n;
     double2 acc = make_double2(0.0, 0.0);
     for (int i = 0; i < n; ++i) {
@@ -808,7 +818,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
   __syncthreads();
   // L_eig (transposed for the Heisenberg picture) into C
   const bool secular = (flags & kSecular) != 0, heis = (flags & kHeisenberg) != 0;
-  for (int e = tid; e < nn; e += kThreads) {
+  for (int e = tid; e < nn; e += NT) {
     const int a = e / n, c = e % n;
     double2 v = make_double2(0.0, 0.0);
     if (!secular || a == c) v = make_double2(-G[a * n + c].x, -G[a * n + c].y);
@@ -819,7 +829,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
   }
   __syncthreads();
   // L = U L_eig U^T: X = L_eig U^T, G = U X
-  for (int e = tid; e < nn; e += kThreads) {
+  for (int e = tid; e < nn; e += NT) {
     const int a = e / n, j = e % n;
     double2 acc = make_double2(0.0, 0.0);
     for (int c = 0; c < n; ++c) {
@@ -831,7 +841,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
     X[a * n + j] = acc;
   }
   __syncthreads();
-  for (int e = tid; e < nn; e += kThreads) {
+  for (int e = tid; e < nn; e += NT) {
     const int i = e / n, j = e % n;
     double2 acc = make_double2(0.0, 0.0);
     for (int a = 0; a < n; ++a) {
@@ -844,7 +854,7 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
   }
   __syncthreads();
   // P = P4(h L) by Horner: Y = I + h/4 L, Y = I + c L Y for c = h/3, h/2, h
-  for (int e = tid; e < nn; e += kThreads) {
+  for (int e = tid; e < nn; e += NT) {
     const int i = e / n, j = e % n;
     const double2 l = G[e];
     Y[e] = make_double2(fma(0.25 * h, l.x, i == j ? 1.0 : 0.0), 0.25 * h * l.y);
@@ -853,20 +863,20 @@ __global__ __launch_bounds__(kThreads) void absorption_ops_kernel(
   const double coef[3] = {h / 3.0, h / 2.0, h};
 #pragma unroll
   for (int t = 0; t < 3; ++t) {
-    abs_gemm(G, Y, X, n, coef[t], 1.0);
+    abs_gemm<NT>(G, Y, X, n, coef[t], 1.0);
     __syncthreads();
-    for (int e = tid; e < nn; e += kThreads) Y[e] = X[e];
+    for (int e = tid; e < nn; e += NT) Y[e] = X[e];
     __syncthreads();
   }
-  for (int e = tid; e < nn; e += kThreads) P1[op + e] = Y[e];
+  for (int e = tid; e < nn; e += NT) P1[op + e] = Y[e];
   // P^J by squaring (J a power of two)
   for (int q = 1; q < J; q *= 2) {
-    abs_gemm(Y, Y, X, n, 1.0, 0.0);
+    abs_gemm<NT>(Y, Y, X, n, 1.0, 0.0);
     __syncthreads();
-    for (int e = tid; e < nn; e += kThreads) Y[e] = X[e];
+    for (int e = tid; e < nn; e += NT) Y[e] = X[e];
     __syncthreads();
   }
-  for (int e = tid; e < nn; e += kThreads) PJ[op + e] = Y[e];
+  for (int e = tid; e < nn; e += NT) PJ[op + e] = Y[e];
 }
 
 // ------------------------------------------------ vector-Jacobian product
@@ -1574,12 +1584,16 @@ extern "C" int32_t qsx_absorption_operators(const int64_t* table, int32_t B, int
                 "qsx_absorption_operators: flags %d (secular, real correlation, Heisenberg only)",
                 flags);
   const size_t lds = abs_lds_bytes(max_n);
+  // 256 threads per problem at every size: wider work-groups shorten a
+  // problem's Jacobi rounds but leave fewer compute units to the other
+  // size classes and the propagation running beside them
+  constexpr int kAbsThreads = 256;
   if (lds > 64 * 1024)
-    QSX_CHECK_HIP(hipFuncSetAttribute((const void*)absorption_ops_kernel,
+    QSX_CHECK_HIP(hipFuncSetAttribute((const void*)absorption_ops_kernel<kAbsThreads>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(absorption_ops_kernel, dim3(B), dim3(kThreads), lds, as_stream(stream), table,
-                     max_n, H1, rw, bath, flags, unit_convert, h, J, static_cast<double2*>(P1),
-                     static_cast<double2*>(PJ));
+  hipLaunchKernelGGL(absorption_ops_kernel<kAbsThreads>, dim3(B), dim3(kAbsThreads), lds,
+                     as_stream(stream), table, max_n, H1, rw, bath, flags, unit_convert, h, J,
+                     static_cast<double2*>(P1), static_cast<double2*>(PJ));
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }
