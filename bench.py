@@ -67,8 +67,9 @@ def counter_sequence(torch, qa):
     writes exactly 256 MiB per dispatch; then
     three launch sequences of the headline 2-D plan (MIXED, the plan's
     chunk length 16), two of the 256-member ensemble plan (MIXED, rebuilt
-    from the Hamiltonians), two of the 5120-system absorption ensemble plan,
-    and two HEOM RK4 steps of 256 members (FP64 and MIXED)."""
+    from the Hamiltonians), two of the 5120-system absorption ensemble plan
+    and two of its propagation as one launch over every wavefront, and two
+    HEOM RK4 steps of 256 members (FP64 and MIXED)."""
     from qspectra_amd import workloads as W
     from qspectra_amd.simulate.utils import propagate_heom
     from qspectra_amd.simulate.fused import ThirdOrderPlan
@@ -99,6 +100,11 @@ def counter_sequence(torch, qa):
                                     rw_freq=12450.0, dt=1.0, graph=False)
     for _ in range(2):
         absorb._launch()
+    torch.cuda.synchronize()
+    from qspectra_amd import _native
+    for _ in range(2):       # the propagation over every wavefront in one launch
+        _native.response_buckets_into(absorb.waves, absorb.offsets, absorb.P, absorb.y0,
+                                      absorb.W, absorb.signal, absorb.status, J=absorb.J)
     torch.cuda.synchronize()
     m = W.heom_aggregate7_model()
     eom = m.with_hamiltonians([m.hamiltonian.sample(n) for n in range(256)]) \
