@@ -246,11 +246,13 @@ int32_t qsx_propagate_adjoint(const void* A, int64_t strideA, int32_t M, const v
 /* Linear-response signals of many problems of unequal size in one launch:
  * for problem b, y_{k+1} = P_b y_k from y0_b and s_k = v_b . y_k,
  * k = 0 .. nt-1, written to row r_b of out [B][nt] complex128 (FP64
- * products), J steps (1 or 4) per pass: P holds P_b^J, W the projection rows
- * w_j = (P_b^j)^T v_b, j < J (qsx_response_bucket_weights writes them from
- * P_b and v_b; for J = 1, W is v).  waves int32 [nwaves][4] (device): first
- * problem, problem count, M, G per wavefront, with G a power of two,
+ * products), J steps (1, 4 or 8) per pass: P holds P_b^J, W the projection
+ * rows w_j = (P_b^j)^T v_b, j < J (qsx_response_bucket_weights writes them
+ * from P_b and v_b; for J = 1, W is v).  waves int32 [nwaves][4] (device):
+ * first problem, problem count, M, G per wavefront, with G a power of two,
  * G >= M + J, G * count <= 64 and M at most qsx_response_buckets_max_states();
+ * for J > 1, G = 16 with M + J <= 16 keeps each problem's state in registers
+ * and G twice the power of two at or above M + J gives two lanes per row;
  * offsets int64 [B][3] (device): offset of P_b^J (M^2 complex elements,
  * row-major) in P, of y0_b (M) in y0 (the rows of W at J times it, [J][M]),
  * and the output row r_b (a permutation of 0 .. B-1, so the signals can leave
