@@ -329,13 +329,65 @@ __global__ __launch_bounds__(kSmallThreads) void small_program_kernel(
     Pc[b * mm + e] = Rc[(e / M) * ld + e % M];
 }
 
+// P4(h L) with two dependent products instead of Horner's three: X = h L,
+// X2 = X X, Q = I/2 + X/6 + X2/24, P = I + X + X2 Q (the same polynomial).
+// X = L is read from Lb; P is left in Y, X is free on return.
+__device__ void small_p4_two_products(const double2* __restrict__ Lb, double h, int M, double2* X,
+                                      double2* Y, SmallGemm& g) {
+  const int ld = M + 1;
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
+    const int i = e / M, j = e % M;
+    const double2 v = Lb[e];
+    X[i * ld + j] = make_double2(h * v.x, h * v.y);
+  }
+  __syncthreads();
+  // the entries of X this thread's product block owns, for the final sum
+  double2 xo[RB][RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int c = 0; c < RB; ++c) {
+      const int i = min(g.ty + g.T * r, M - 1), j = min(g.tx + g.T * c, M - 1);
+      xo[r][c] = g.active ? X[i * ld + j] : make_double2(0.0, 0.0);
+    }
+  g.mul(X, X);
+  __syncthreads();
+  g.store(Y, 1.0, 0.0);                          // Y = X2
+  __syncthreads();
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
+    const int i = e / M, j = e % M;
+    const double2 x = X[i * ld + j], x2 = Y[i * ld + j];
+    X[i * ld + j] = make_double2((i == j ? 0.5 : 0.0) + x.x / 6.0 + x2.x / 24.0,
+                                 x.y / 6.0 + x2.y / 24.0);     // X = Q
+  }
+  __syncthreads();
+  g.mul(Y, X);
+  __syncthreads();
+  if (g.active) {
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int c = 0; c < RB; ++c) {
+        const int i = g.ty + g.T * r, j = g.tx + g.T * c;
+        if (i < M && j < M)
+          Y[i * ld + j] = make_double2((i == j ? 1.0 : 0.0) + xo[r][c].x + g.acc[r][c].x,
+                                       xo[r][c].y + g.acc[r][c].y);
+      }
+  }
+  __syncthreads();
+}
+
 // States y0 [B][R][M] propagated by P = P4(h L) to the cumulative step
 // counts ex.e[s]: out[b][s][r] = P^e_s y0[b][r], one work-group per batch
 // entry; the squarings P^(2^k) are shared by all save points (binary
-// powering), each save point's states kept in LDS.
+// powering), each save point's states kept in LDS.  With `merged`, the
+// state products of every save point whose count has bit k run on the
+// threads the product grid leaves idle, beside the squaring that reads the
+// same P^(2^k) (staging for all save points); otherwise after it, one save
+// point at a time.
 __global__ __launch_bounds__(kSmallThreads) void small_powers_apply_kernel(
     const double2* __restrict__ L, int64_t strideL, int M, double h, ExpList ex,
-    const double2* __restrict__ y0, int R, double2* __restrict__ out) {
+    const double2* __restrict__ y0, int R, double2* __restrict__ out, int merged) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int ld = M + 1;
   double2* X = reinterpret_cast<double2*>(smem);
@@ -343,14 +395,44 @@ __global__ __launch_bounds__(kSmallThreads) void small_powers_apply_kernel(
   const int b = blockIdx.x, ns = ex.n;
   const int RM = R * M;
   double2* Z = Y + M * ld;                       // [ns][R][M] states
-  double2* W = Z + ns * RM;                      // [R][M] products before copy-back
+  double2* W = Z + ns * RM;                      // products before copy-back
   long long emax = 0;
   for (int s = 0; s < ns; ++s) emax = ex.e[s] > emax ? ex.e[s] : emax;
   for (int e = threadIdx.x; e < ns * RM; e += blockDim.x)
     Z[e] = y0[(int64_t)b * RM + e % RM];
-  small_horner(L + b * strideL, h, M, X, Y);    // Y = P; X = L is free from here
   SmallGemm g(M);
+  small_p4_two_products(L + b * strideL, h, M, X, Y, g);    // Y = P
+  const int gthreads = ((g.T * g.T + 63) / 64) * 64;
+  const bool side = merged && gthreads + 64 <= (int)blockDim.x;
   for (int k = 0; k < 62 && (1LL << k) <= emax; ++k) {
+    const bool sq = (2LL << k) <= emax;
+    if (side) {
+      // the squaring's products on the grid threads; the state products
+      // W_s = P^(2^k) Z_s on the others, all from the same Y
+      if (sq) g.mul(Y, Y);
+      const int t = (int)threadIdx.x - gthreads, nt = (int)blockDim.x - gthreads;
+      if (t >= 0) {
+        for (int s = 0; s < ns; ++s) {
+          if (!(ex.e[s] & (1LL << k))) continue;
+          const double2* Zs = Z + s * RM;
+          double2* Ws = W + s * RM;
+          for (int e = t; e < RM; e += nt) {
+            const int r = e / M, i = e % M;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int j = 0; j < M; ++j) cfma<double>(acc, Y[i * ld + j], Zs[r * M + j]);
+            Ws[e] = acc;
+          }
+        }
+      }
+      __syncthreads();
+      if (sq) g.store(Y, 1.0, 0.0);
+      for (int s = 0; s < ns; ++s) {
+        if (!(ex.e[s] & (1LL << k))) continue;
+        for (int e = threadIdx.x; e < RM; e += blockDim.x) Z[s * RM + e] = W[s * RM + e];
+      }
+      __syncthreads();
+      continue;
+    }
     // states of every save point whose step count has bit k: v <- P^(2^k) v
     // (products staged in W, then copied back)
     for (int s = 0; s < ns; ++s) {
@@ -366,7 +448,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_powers_apply_kernel(
       for (int e = threadIdx.x; e < RM; e += blockDim.x) Zs[e] = W[e];
       __syncthreads();
     }
-    if ((2LL << k) <= emax) {
+    if (sq) {
       g.mul(Y, Y);
       __syncthreads();
       g.store(Y, 1.0, 0.0);
@@ -388,6 +470,10 @@ int small_threads(int M) {
 }
 size_t small_apply_lds(int M, int R, int ns) {
   return small_program_lds(M) + (size_t)(ns + 1) * R * M * sizeof(double2);
+}
+// with the state products beside the squarings: staging for every save point
+size_t small_apply_merged_lds(int M, int R, int ns) {
+  return small_program_lds(M) + (size_t)2 * ns * R * M * sizeof(double2);
 }
 
 template <typename K>
@@ -450,14 +536,19 @@ extern "C" int32_t qsx_propagate_powers(const void* L, int64_t strideL, int32_t 
                   "qsx_propagate_powers: bad step count %lld", (long long)exps[i]);
     ex.e[i] = exps[i];
   }
-  const size_t lds = small_apply_lds(M, R, ns);
+  // the state products beside the squarings when their staging fits and the
+  // product grid leaves at least a wavefront of the 1024 threads idle
+  const int T = (M + RB - 1) / RB;
+  const int merged = small_apply_merged_lds(M, R, ns) <= 160 * 1024 &&
+                     ((T * T + 63) / 64) * 64 + 64 <= kSmallThreads;
+  const size_t lds = merged ? small_apply_merged_lds(M, R, ns) : small_apply_lds(M, R, ns);
   hipStream_t s = as_stream(stream);
   const int64_t sL = B > 1 ? strideL : 0;
-auto kern = small_powers_apply_kernel;
+  auto kern = small_powers_apply_kernel;
   QSX_TRY(set_lds(kern, lds));
-  hipLaunchKernelGGL(kern, dim3(B), dim3(small_threads(M)), lds, s,
-                     static_cast<const double2*>(L), sL, M, h, ex, static_cast<const double2*>(y0), R,
-                     static_cast<double2*>(out));
+  hipLaunchKernelGGL(kern, dim3(B), dim3(merged ? kSmallThreads : small_threads(M)), lds, s,
+                     static_cast<const double2*>(L), sL, M, h, ex, static_cast<const double2*>(y0),
+                     R, static_cast<double2*>(out), merged);
   QSX_CHECK_HIP(hipGetLastError());
   return QSX_OK;
 }
