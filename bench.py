@@ -440,17 +440,18 @@ PMC_TRAFFIC = {
     'heom stage fp64': 2.718e9,
     'heom stage mixed': 1.936e9,
     # the headline plan's t2 operator program (one work-group, M = 49)
-    'propagate powers M=49 B=1 R=7 e=0,100,200,500': 8.21e4,
+    'propagate powers M=49 B=1 R=7 e=0,100,200,500': 8.64e4,
     'chunks prop mixed M=147 nc=8 B=1 steps=15 nsave=16': 4.30e6,
     # the 256-member ensemble plan
-    'propagate powers M=49 B=256 R=7 e=0,100,200,500': 1.704e7,
+    'propagate powers M=49 B=256 R=7 e=0,100,200,500': 1.708e7,
     'propagate prop mixed M=147 R=1 B=256 steps=127 nsave=128': 1.235e8,
     # prep 1.366e8 + three Horner GEMMs 9.716e8 each + squaring 5.307e8
     # + diagonal 1.2e6 + final split 9.25e7
     'split propagator M=147 B=256 q=2': 3.676e9,
-    # the 5120-system absorption ensemble (four steps per pass): operators
-    # and projection rows 11.7 MB in, signals 335.6 MB out
-    'response buckets B=5120 waves=1399 nt=4096': 3.473e8,
+    # the 5120-system absorption ensemble, one launch over every wavefront
+    # (eight steps per pass): P^8 and projection rows 15.0 MB in, signals
+    # 335.6 MB out
+    'response buckets B=5120 waves=1800 nt=4096': 3.506e8,
 }
 
 

@@ -536,17 +536,21 @@ extern "C" int32_t qsx_propagate_powers(const void* L, int64_t strideL, int32_t 
                   "qsx_propagate_powers: bad step count %lld", (long long)exps[i]);
     ex.e[i] = exps[i];
   }
-  // the state products beside the squarings when their staging fits and the
-  // product grid leaves at least a wavefront of the 1024 threads idle
+  // the state products beside the squarings (on up to six more wavefronts)
+  // for blocks whose squarings are long enough to hide them (M >= 16), when
+  // their staging fits and the product grid leaves a wavefront of the 1024
+  // threads
   const int T = (M + RB - 1) / RB;
-  const int merged = small_apply_merged_lds(M, R, ns) <= 160 * 1024 &&
-                     ((T * T + 63) / 64) * 64 + 64 <= kSmallThreads;
+  const int gthreads = ((T * T + 63) / 64) * 64;
+  const int merged = M >= 16 && small_apply_merged_lds(M, R, ns) <= 160 * 1024 &&
+                     gthreads + 64 <= kSmallThreads;
+  const int threads = merged ? std::min(kSmallThreads, gthreads + 384) : small_threads(M);
   const size_t lds = merged ? small_apply_merged_lds(M, R, ns) : small_apply_lds(M, R, ns);
   hipStream_t s = as_stream(stream);
   const int64_t sL = B > 1 ? strideL : 0;
   auto kern = small_powers_apply_kernel;
   QSX_TRY(set_lds(kern, lds));
-  hipLaunchKernelGGL(kern, dim3(B), dim3(merged ? kSmallThreads : small_threads(M)), lds, s,
+  hipLaunchKernelGGL(kern, dim3(B), dim3(threads), lds, s,
                      static_cast<const double2*>(L), sL, M, h, ex, static_cast<const double2*>(y0),
                      R, static_cast<double2*>(out), merged);
   QSX_CHECK_HIP(hipGetLastError());

@@ -236,9 +236,20 @@ __global__ __launch_bounds__(kThreads) void block_eigh_kernel(const double* __re
 // ------------------------------------------------------------ 2. C(w) and K
 // Debye bath correlation function, qspectra/bath.py:84-102 (complex, 1000-term
 // Matsubara sum, k = 0 term zero) and bath.py:17-31 (real, detailed balance).
+// 1 / d from the hardware reciprocal and two Newton steps: no shared
+// condition register, unlike the IEEE division sequence, so several of them
+// overlap; for the well-scaled denominators of the Matsubara terms (no
+// overflow, underflow or zero) it is within an ulp of the quotient
+__device__ __forceinline__ double recip_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
+
 // The terms k = k0, k0 + stride, ... (< 1000) of the Matsubara sum: four
-// interleaved Kahan-compensated sums (their divisions independent, so they
-// overlap), added at the end
+// interleaved Kahan-compensated sums (their reciprocals independent, so
+// they overlap), added at the end
 __device__ double2 matsubara_partial(double x, double T, double gam, int k0, int stride) {
   double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
   double cr[4] = {0.0, 0.0, 0.0, 0.0}, ci[4] = {0.0, 0.0, 0.0, 0.0};
@@ -249,7 +260,7 @@ __device__ double2 matsubara_partial(double x, double T, double gam, int k0, int
       const int ku = k + u * stride;
       if (ku < 1000) {
         const double nu = twopiT * ku;
-        const double inv = 1.0 / ((nu * nu - gam * gam) * (nu * nu + x * x));
+        const double inv = recip_nr((nu * nu - gam * gam) * (nu * nu + x * x));
         const double tr = nu * nu * inv - cr[u];
         const double ti = nu * x * inv - ci[u];
         const double ur = sr[u] + tr, ui = si[u] + ti;
