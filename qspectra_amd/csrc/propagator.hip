@@ -37,17 +37,20 @@ inline dim3 ew_grid(int64_t total) {
   return dim3((unsigned)(g < 65535 * 16 ? g : 65535 * 16));
 }
 
-// T[b] = s * L[b] + I
-__global__ __launch_bounds__(kEwThreads) void scaled_plus_identity_kernel(
-    const double2* __restrict__ L, int64_t strideL, int M, int B, double s,
-    double2* __restrict__ T) {
+// Q[b] = I/2 + (h/6) L[b] + X2[b]/24 and S[b] = I + h L[b], X2 = (h L)^2:
+// with P = S + X2 Q the two-product form of P4(h L)
+__global__ __launch_bounds__(kEwThreads) void p4_terms_kernel(
+    const double2* __restrict__ L, int64_t strideL, const double2* __restrict__ X2, int M, int B,
+    double h, double2* __restrict__ Q, double2* __restrict__ S) {
   const int64_t mm = (int64_t)M * M, total = mm * B;
   for (int64_t e = (int64_t)blockIdx.x * kEwThreads + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * kEwThreads) {
     const int64_t b = e / mm, r = e % mm;
-    const double2 v = L[b * strideL + r];
+    const double2 v = L[b * strideL + r], x2 = X2[e];
     const bool d = (r / M) == (r % M);
-    T[e] = make_double2(fma(s, v.x, d ? 1.0 : 0.0), s * v.y);
+    Q[e] = make_double2((d ? 0.5 : 0.0) + (h / 6.0) * v.x + x2.x / 24.0,
+                        (h / 6.0) * v.y + x2.y / 24.0);
+    S[e] = make_double2(fma(h, v.x, d ? 1.0 : 0.0), h * v.y);
   }
 }
 
@@ -574,18 +577,17 @@ int rk4_propagator_impl(const void* L, int64_t strideL, int M, int B, double h, 
   const double2* Lp = static_cast<const double2*>(L);
   double2* Pp = static_cast<double2*>(P);
   const int64_t sL = B > 1 ? strideL : 0;
-  hipLaunchKernelGGL(scaled_plus_identity_kernel, ew_grid(mm * B), dim3(kEwThreads), 0, s, Lp,
-                     sL, M, B, h / 4.0, Ta);
-  QSX_CHECK_HIP(hipGetLastError());
-  const int64_t sT = B > 1 ? mm : 0;
-  // Tb = I + (h/3) L Ta;  Ta = I + (h/2) L Tb;  P' = I + h L Ta
-  QSX_TRY(zgemm_enqueue(QSX_FP64, 0, M, M, M, h / 3.0, 0.0, Lp, M, sL, Ta, M, sT, 0.0, 0.0, Tb,
-                        M, sT, 1.0, 0.0, B, s));
-  QSX_TRY(zgemm_enqueue(QSX_FP64, 0, M, M, M, h / 2.0, 0.0, Lp, M, sL, Tb, M, sT, 0.0, 0.0, Ta,
-                        M, sT, 1.0, 0.0, B, s));
+  const int64_t sT = mm;
+  // two dependent products: Ta = X2 = h^2 L L;  Tc = Q = I/2 + X/6 + X2/24,
+  // P' = I + X;  P' += X2 Q  (P4(hL) = I + X + X2 Q, X = h L)
   double2* P1 = q == 1 ? Pp : Tb;
-  QSX_TRY(zgemm_enqueue(QSX_FP64, 0, M, M, M, h, 0.0, Lp, M, sL, Ta, M, sT, 0.0, 0.0, P1, M, sT,
-                        1.0, 0.0, B, s));
+  QSX_TRY(zgemm_enqueue(QSX_FP64, 0, M, M, M, h * h, 0.0, Lp, M, sL, Lp, M, sL, 0.0, 0.0, Ta, M,
+                        sT, 0.0, 0.0, B, s));
+  hipLaunchKernelGGL(p4_terms_kernel, ew_grid(mm * B), dim3(kEwThreads), 0, s, Lp, sL, Ta, M, B,
+                     h, Tc, P1);
+  QSX_CHECK_HIP(hipGetLastError());
+  QSX_TRY(zgemm_enqueue(QSX_FP64, 0, M, M, M, 1.0, 0.0, Ta, M, sT, Tc, M, sT, 1.0, 0.0, P1, M,
+                        sT, 0.0, 0.0, B, s));
   if (q == 1) return QSX_OK;
   const int64_t e = q;
   return powers_impl(Tb, mm, M, B, &e, 1, Pp, 0, mm, Ta, Tc, Td, s);

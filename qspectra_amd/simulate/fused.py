@@ -357,6 +357,9 @@ class ThirdOrderPlan(object):
                                 key=lambda i: -self._t3_list[i]['stage'].M)
         self._sides = [torch.cuda.Stream() for _ in self.t3_stages]
         self._t2side = torch.cuda.Stream()
+        # the largest t2 group runs on the t2 stream; the others propagate on
+        # the main stream beside it and add to W after it
+        self._t2_keys = sorted(self.groups, key=lambda k: -int(self.groups[k]['L'].shape[-1]))
         self._graph = None
         self._statuses = []
 
@@ -425,7 +428,11 @@ class ThirdOrderPlan(object):
         S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
         _native.zgemm(self.F1, S1, self.S1F)
         S1F = self.S1F if B == 1 else self.S1F.repeat_interleave(n2, dim=0)
+        for k12 in self._t2_keys[1:]:
+            self._t2_propagate(B, k12)
         main.wait_stream(t2side)
+        for k12 in self._t2_keys[1:]:
+            self._t2_accumulate(self.groups[k12], self.Q[k12], 1.0)
         _native.zgemm(S1F, self.W, self.U)                                       # [B n2, N1, w]
         for side in self._sides:
             main.wait_stream(side)
@@ -458,7 +465,11 @@ class ThirdOrderPlan(object):
         stage1.build()
         S1 = stage1.run(y1, statuses, self.status[0:1])[:, 0, :n1]             # [B, n1, M1]
         self.S1cat.copy_(S1.permute(1, 0, 2))
+        for k12 in self._t2_keys[1:]:
+            self._t2_propagate(B, k12)
         main.wait_stream(t2side)
+        for k12 in self._t2_keys[1:]:
+            self._t2_accumulate(self.groups[k12], self.Q[k12], 1.0)
         for side in self._sides:
             main.wait_stream(side)
         W = self.W.view(B, n2, M1, -1)
@@ -475,31 +486,41 @@ class ThirdOrderPlan(object):
         return self.XE.permute(1, 0, 2)[None]
 
     def _t2_stage(self, B):
-        """W = sum over t2 groups of Q_g V2_g^T, Q_g the rows of V1^T
-        propagated to every population time."""
-        for i, (k12, g) in enumerate(self.groups.items()):
-            if g['rebuild'] is not None:
-                g['rebuild']()
-            if g['rows'] is not None:
-                _native.propagate_powers_into(g['L'], g['h'], self.N2, g['rows'], self.Q[k12])
+        """W = Q_g V2_g^T for the largest t2 group g, Q_g the rows of V1^T
+        propagated to every population time (the other groups add theirs
+        after it, from the main stream)."""
+        k12 = self._t2_keys[0]
+        self._t2_propagate(B, k12)
+        self._t2_accumulate(self.groups[k12], self.Q[k12], 0.0)
+
+    def _t2_propagate(self, B, k12):
+        """Q[k12]: the rows of the group's V1^T propagated to every
+        population time (its block rebuilt first when the plan rebuilds)."""
+        g = self.groups[k12]
+        if g['rebuild'] is not None:
+            g['rebuild']()
+        if g['rows'] is not None:
+            _native.propagate_powers_into(g['L'], g['h'], self.N2, g['rows'], self.Q[k12])
+        else:
+            _native.rk4_propagator_into(g['L'], g['h'], 1, g['P'], g['ws'])
+            Pn = g['Pn']
+            if g['P'].shape[0] == 1 and B > 1:
+                _native.matrix_powers_into(g['P'], self.N2, Pn[:1], g['ws'])
+                Pn[1:].copy_(Pn[:1].expand(B - 1, -1, -1, -1))
             else:
-                _native.rk4_propagator_into(g['L'], g['h'], 1, g['P'], g['ws'])
-                Pn = g['Pn']
-                if g['P'].shape[0] == 1 and B > 1:
-                    _native.matrix_powers_into(g['P'], self.N2, Pn[:1], g['ws'])
-                    Pn[1:].copy_(Pn[:1].expand(B - 1, -1, -1, -1))
-                else:
-                    _native.matrix_powers_into(g['P'], self.N2, Pn, g['ws'])
-                _native.zgemm(g['V1T'], Pn.view(-1, *Pn.shape[2:]), self.Q[k12], transb=True)
-            beta = 0.0 if i == 0 else 1.0
-            if g['V2T'].shape[0] == 1:
-                # a shared V2 map: one tall GEMM over the stacked rows of every
-                # member and population time
-                Q, W = self.Q[k12], self.W
-                _native.zgemm(Q.view(1, -1, Q.shape[-1]), g['V2T'], W.view(1, -1, W.shape[-1]),
-                              beta=beta)
-            else:
-                _native.zgemm(self.Q[k12], g['V2T'], self.W, beta=beta)
+                _native.matrix_powers_into(g['P'], self.N2, Pn, g['ws'])
+            _native.zgemm(g['V1T'], Pn.view(-1, *Pn.shape[2:]), self.Q[k12], transb=True)
+
+    def _t2_accumulate(self, g, Q, beta):
+        """W <- Q V2^T + beta W for one t2 group."""
+        if g['V2T'].shape[0] == 1:
+            # a shared V2 map: one tall GEMM over the stacked rows of every
+            # member and population time
+            W = self.W
+            _native.zgemm(Q.view(1, -1, Q.shape[-1]), g['V2T'], W.view(1, -1, W.shape[-1]),
+                          beta=beta)
+        else:
+            _native.zgemm(Q, g['V2T'], self.W, beta=beta)
 
     def capture(self):
         """Records the launch sequence into a HIP graph, replayed by run()."""
