@@ -253,85 +253,6 @@ __device__ inline void lds_copy(const double2* src, double2* dst, int M) {
     dst[(e / M) * ld + e % M] = src[(e / M) * ld + e % M];
 }
 
-// Horner in LDS: X = L (loaded), Y = P4(h L) on return (X unchanged)
-__device__ void small_horner(const double2* __restrict__ Lb, double h, int M, double2* X,
-                             double2* Y) {
-  const int ld = M + 1;
-  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
-    const int i = e / M, j = e % M;
-    const double2 v = Lb[e];
-    X[i * ld + j] = v;
-    Y[i * ld + j] = make_double2(fma(h / 4.0, v.x, i == j ? 1.0 : 0.0), h / 4.0 * v.y);
-  }
-  __syncthreads();
-  SmallGemm g(M);
-  const double coef[3] = {h / 3.0, h / 2.0, h};
-  for (int t = 0; t < 3; ++t) {
-    g.mul(X, Y);
-    __syncthreads();
-    g.store(Y, coef[t], 1.0);
-    __syncthreads();
-  }
-}
-
-// S (in LDS) raised to the power q: the result is left in S or in R (the
-// returned pointer); R and S are both overwritten.  q >= 1.
-__device__ double2* small_power(double2* S, double2* R, long long q, int M) {
-  SmallGemm g(M);
-  bool have = false;
-  for (int k = 0; k < 62 && (1LL << k) <= q; ++k) {
-    if (q & (1LL << k)) {
-      if (!have) {
-        lds_copy(S, R, M);
-        have = true;
-      } else {
-        g.mul(R, S);
-        __syncthreads();
-        g.store(R, 1.0, 0.0);
-      }
-      __syncthreads();
-    }
-    if ((2LL << k) <= q) {
-      g.mul(S, S);
-      __syncthreads();
-      g.store(S, 1.0, 0.0);
-      __syncthreads();
-    }
-  }
-  return R;
-}
-
-// P = P4(h L)^q (optionally stored), Pc = P^c (c > 0, optional), MIXED split
-// of P (optional): one work-group per batch entry.
-__global__ __launch_bounds__(kSmallThreads) void small_program_kernel(
-    const double2* __restrict__ L, int64_t strideL, int M, double h, long long q, long long c,
-    double2* __restrict__ P, double2* __restrict__ Pc, float2* __restrict__ off,
-    double2* __restrict__ diag) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int ld = M + 1;
-  double2* X = reinterpret_cast<double2*>(smem);
-  double2* Y = X + M * ld;
-  const int b = blockIdx.x;
-  const int64_t mm = (int64_t)M * M;
-  small_horner(L + b * strideL, h, M, X, Y);
-  double2* Pq = q == 1 ? Y : small_power(Y, X, q, M);
-  double2* other = Pq == Y ? X : Y;
-  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
-    const int i = e / M, j = e % M;
-    const double2 v = Pq[i * ld + j];
-    if (P != nullptr) P[b * mm + e] = v;
-    if (off != nullptr) {
-      off[b * mm + e] = i == j ? make_float2(0.f, 0.f) : make_float2((float)v.x, (float)v.y);
-      if (i == j) diag[(int64_t)b * M + i] = make_double2(v.x - 1.0, v.y);
-    }
-  }
-  if (Pc == nullptr || c < 1) return;
-  __syncthreads();
-  double2* Rc = small_power(Pq, other, c, M);
-  for (int e = threadIdx.x; e < M * M; e += blockDim.x)
-    Pc[b * mm + e] = Rc[(e / M) * ld + e % M];
-}
-
 // P4(h L) with two dependent products instead of Horner's three: X = h L,
 // X2 = X X, Q = I/2 + X/6 + X2/24, P = I + X + X2 Q (the same polynomial).
 // X = L is read from Lb; P is left in Y, X is free on return.
@@ -378,6 +299,65 @@ __device__ void small_p4_two_products(const double2* __restrict__ Lb, double h, 
       }
   }
   __syncthreads();
+}
+
+// S (in LDS) raised to the power q: the result is left in S or in R (the
+// returned pointer); R and S are both overwritten.  q >= 1.
+__device__ double2* small_power(double2* S, double2* R, long long q, int M) {
+  SmallGemm g(M);
+  bool have = false;
+  for (int k = 0; k < 62 && (1LL << k) <= q; ++k) {
+    if (q & (1LL << k)) {
+      if (!have) {
+        lds_copy(S, R, M);
+        have = true;
+      } else {
+        g.mul(R, S);
+        __syncthreads();
+        g.store(R, 1.0, 0.0);
+      }
+      __syncthreads();
+    }
+    if ((2LL << k) <= q) {
+      g.mul(S, S);
+      __syncthreads();
+      g.store(S, 1.0, 0.0);
+      __syncthreads();
+    }
+  }
+  return R;
+}
+
+// P = P4(h L)^q (optionally stored), Pc = P^c (c > 0, optional), MIXED split
+// of P (optional): one work-group per batch entry.
+__global__ __launch_bounds__(kSmallThreads) void small_program_kernel(
+    const double2* __restrict__ L, int64_t strideL, int M, double h, long long q, long long c,
+    double2* __restrict__ P, double2* __restrict__ Pc, float2* __restrict__ off,
+    double2* __restrict__ diag) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ld = M + 1;
+  double2* X = reinterpret_cast<double2*>(smem);
+  double2* Y = X + M * ld;
+  const int b = blockIdx.x;
+  const int64_t mm = (int64_t)M * M;
+  SmallGemm g(M);
+  small_p4_two_products(L + b * strideL, h, M, X, Y, g);   // Y = P4(h L)
+  double2* Pq = q == 1 ? Y : small_power(Y, X, q, M);
+  double2* other = Pq == Y ? X : Y;
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
+    const int i = e / M, j = e % M;
+    const double2 v = Pq[i * ld + j];
+    if (P != nullptr) P[b * mm + e] = v;
+    if (off != nullptr) {
+      off[b * mm + e] = i == j ? make_float2(0.f, 0.f) : make_float2((float)v.x, (float)v.y);
+      if (i == j) diag[(int64_t)b * M + i] = make_double2(v.x - 1.0, v.y);
+    }
+  }
+  if (Pc == nullptr || c < 1) return;
+  __syncthreads();
+  double2* Rc = small_power(Pq, other, c, M);
+  for (int e = threadIdx.x; e < M * M; e += blockDim.x)
+    Pc[b * mm + e] = Rc[(e / M) * ld + e % M];
 }
 
 // States y0 [B][R][M] propagated by P = P4(h L) to the cumulative step
