@@ -559,11 +559,13 @@ def ensemble_absorption_leg(torch, replays=5):
     errp = np.max(np.abs(Xp - X64), axis=1) / np.max(np.abs(X64), axis=1)
     steps = len(hams) * (int(tmax) - 1)
     nt = len(plan.t)
-    plan._launch()                          # untimed: the streams start idle
+    # per-launch durations with the launches one after another on one stream
+    # (in the graph the size classes overlap)
+    plan._launch(sequential=True)
     torch.cuda.synchronize()
     with timing.kernel_timer() as kt:
         for _ in range(2):
-            plan._launch()
+            plan._launch(sequential=True)
     summary = kt.summary()
     # the graph propagates each size class in its own launch, concurrent with
     # the other classes' builds; the kernel's roofline is taken on one launch

@@ -255,17 +255,20 @@ class AbsorptionEnsemblePlan(object):
                 return bk['inputs']
         raise KeyError(n)
 
-    def _launch(self):
+    def _launch(self, sequential=False):
+        """The plan's launches; ``sequential`` puts them all on the current
+        stream (one after the other, for per-launch timing)."""
         fz = self._fused
         main = torch.cuda.current_stream()
-        for side in self._sides:
+        sides = [main] if sequential else self._sides
+        for side in sides:
             side.wait_stream(main)
         if fz is not None:
             # per size class (largest sites first, concurrent on the side
             # streams): the operators, the projection rows and the signals of
             # its wavefronts, which write disjoint rows of W, signal and status
             for j, (top, table, rw, bath, waves) in enumerate(reversed(fz['launches'])):
-                with torch.cuda.stream(self._sides[j % len(self._sides)]):
+                with torch.cuda.stream(sides[j % len(sides)]):
                     _native.absorption_operators_into(table, top, fz['H1'], rw, bath,
                                                       fz['flags'], fz['unit'], self.dt, self.J,
                                                       self.P1, self.P)
@@ -273,16 +276,16 @@ class AbsorptionEnsemblePlan(object):
                                                          self.J, self.W)
                     _native.response_buckets_into(waves, self.offsets, self.P, self.y0, self.W,
                                                   self.signal, self.status, J=self.J)
-            for side in self._sides:
+            for side in sides:
                 main.wait_stream(side)
         else:
             for j, k in enumerate(self._build_order):
                 bk = self.buckets[k]
-                with torch.cuda.stream(self._sides[j % len(self._sides)]):
+                with torch.cuda.stream(sides[j % len(sides)]):
                     bk['rebuild'](bk['L'])
                     _native.small_propagator_into(bk['L'], self.dt, 1, self.J, P=bk['P1'],
                                                   Pc=bk['P'])
-            for side in self._sides:
+            for side in sides:
                 main.wait_stream(side)
             _native.response_bucket_weights_into(self.waves, self.offsets, self.P1, self.v,
                                                  self.J, self.W)

@@ -210,12 +210,13 @@ __device__ __forceinline__ void dft16(double2 (&v)[16]) {
   t = v[11]; v[11] = v[14]; v[14] = t;
 }
 
-// One radix-16 Stockham pass over the padded LDS row (in place: every
-// thread's reads finish before the barrier, its writes follow it).
-__device__ __forceinline__ void radix16_pass(double2* buf, int Ns, const double2* tw1,
-                                             const double2* tw2) {
+// The butterflies of one radix-16 Stockham pass over the padded LDS row:
+// thread j reads entries j + r L/16 and leaves in v[r] the pass's outputs
+// for entries (j - j mod Ns) 16 + j mod Ns + r Ns.
+__device__ __forceinline__ void radix16_butterflies(const double2* buf, int Ns,
+                                                    const double2* tw1, const double2* tw2,
+                                                    double2 (&v)[16]) {
   const int j = threadIdx.x;   // butterfly index, kFftL / 16 of them
-  double2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = buf[fpad(j + r * (kFftL / 16))];
   const int jm = j & (Ns - 1);
@@ -228,11 +229,30 @@ __device__ __forceinline__ void radix16_pass(double2* buf, int Ns, const double2
     w = cmul(w, w1);
   }
   dft16(v);
+}
+
+// One radix-16 Stockham pass, in place: every thread's reads finish before
+// the barrier, its writes follow it.
+__device__ __forceinline__ void radix16_pass(double2* buf, int Ns, const double2* tw1,
+                                             const double2* tw2) {
+  double2 v[16];
+  radix16_butterflies(buf, Ns, tw1, tw2, v);
+  const int j = threadIdx.x, jm = j & (Ns - 1);
   const int idx = (j - jm) * 16 + jm;
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < 16; ++r) buf[fpad(idx + r * Ns)] = v[r];
   __syncthreads();
+}
+
+// The last three passes of an FFT whose radix-2 pass is in LDS: the final
+// pass (Ns = L / 16) leaves entry t + r L/16 of the transform in v[r] of
+// thread t, the entries that thread read, so it needs no write-back.
+__device__ __forceinline__ void fft_tail_to_registers(double2* buf, const double2* tw1,
+                                                      const double2* tw2, double2 (&v)[16]) {
+  radix16_pass(buf, 2, tw1, tw2);
+  radix16_pass(buf, 32, tw1, tw2);
+  radix16_butterflies(buf, kFftL / 16, tw1, tw2, v);
 }
 
 // Forward FFT of the L-point row in LDS (natural order in and out).  Expects
@@ -352,23 +372,59 @@ __global__ __launch_bounds__(kFftThreads) void chirp_rows_kernel(
     const double2* row = in + b * g.n_in;
     double2* orow = out + b * g.n_out;
     if (g.pieces == 1) {
-      // one piece: its spectrum, kept in registers (each thread its own
-      // kFftPer entries), serves every chunk
-      load_piece(buf, row, g, 0, cin);
-      fft_row(buf, tw1, tw2);
-      double2 S[kFftPer];
+      // one piece (P = L/2 samples, zero-padded to L): its spectrum, kept in
+      // registers (entry t + r L/16 in S[r] of thread t), serves every
+      // chunk.  The radix-2 pass of its FFT only duplicates each sample (the
+      // upper half is zero), so the samples go straight to both outputs.
+      __syncthreads();                 // the previous row's reads are done
 #pragma unroll
-      for (int r = 0; r < kFftPer; ++r) S[r] = buf[fpad(t + r * kFftThreads)];
+      for (int r0 = 0; r0 < kFftPer / 2; r0 += 4) {
+        double2 x[4], c[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int v = t + (r0 + q) * kFftThreads;
+          x[q] = make_double2(0.0, 0.0);
+          c[q] = make_double2(0.0, 0.0);
+          if (v < g.n_in) {
+            x[q] = row[v];
+            c[q] = cin[v];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int v = t + (r0 + q) * kFftThreads;
+          const double2 y = cmul(x[q], c[q]);
+          buf[fpad(2 * v)] = y;
+          buf[fpad(2 * v + 1)] = y;
+        }
+      }
+      __syncthreads();
+      double2 S[kFftPer];
+      fft_tail_to_registers(buf, tw1, tw2, S);
       for (int j = 0; j < g.chunks; ++j) {
         const int j0 = j * g.H, he = min(g.H, g.n_out - j0);
         const double2* G = Gt + (size_t)j * kFftL;
+        // the inverse FFT (conj(FFT(conj(.))) / L) of S times the kernel
+        // spectrum, its radix-2 pass from registers: entries t + q L/16 and
+        // t + (q + 8) L/16 are this thread's S[q], S[q + 8]
+        __syncthreads();               // every thread's reads of buf are done
 #pragma unroll
-        for (int r = 0; r < kFftPer; ++r) {
-          const int k = t + r * kFftThreads;
-          buf[fpad(k)] = cconj(cmul(S[r], G[k]));
+        for (int q = 0; q < kFftPer / 2; ++q) {
+          const int jj = t + q * kFftThreads;
+          const double2 a = cconj(cmul(S[q], G[jj]));
+          const double2 b = cconj(cmul(S[q + kFftPer / 2], G[jj + kFftL / 2]));
+          buf[fpad(2 * jj)] = cadd(a, b);
+          buf[fpad(2 * jj + 1)] = csub(a, b);
         }
         __syncthreads();
-        fft_row(buf, tw1, tw2);
+        // the last pass reads and writes only this thread's entries t + r
+        // L/16 (no barrier), which the output then reads back
+        {
+          double2 y[kFftPer];
+          fft_tail_to_registers(buf, tw1, tw2, y);
+#pragma unroll
+          for (int r = 0; r < kFftPer; ++r) buf[fpad(t + r * kFftThreads)] = y[r];
+        }
 #pragma unroll
         for (int r = 0; r < kFftPer; ++r) {
           const int u = t + r * kFftThreads;
@@ -377,7 +433,6 @@ __global__ __launch_bounds__(kFftThreads) void chirp_rows_kernel(
             orow[j0 + u] = make_double2(oscale * z.x, oscale * z.y);
           }
         }
-        __syncthreads();
       }
       continue;
     }
