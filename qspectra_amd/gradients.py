@@ -254,3 +254,126 @@ class RedfieldAbsorption(object):
             x = x * torch.as_tensor(np.exp(-self.t / self.decay)).to(x.device)
         f, X = fourier_transform_device(self.t, -x, -1, self.rw_freq, self.unit_convert)
         return f, X.real
+
+
+_BLOCK_INDEX = {'g': 0, 'e': 1, 'f': 2}
+
+
+class RedfieldThirdOrder(object):
+    """Differentiable third-order response R(t1, t2, t3) and 2-D spectrum of
+    a Redfield aggregate ('gef' Hilbert space), optionally averaged over a
+    static-disorder ensemble evaluated as one batch.
+
+        tds = RedfieldThirdOrder(dipoles, cutoff_freq, temperature, rw_freq,
+                                 coherence_time_max, population_times, dt)
+        R = tds.response(E, J, lam)        # [n1, n2, n3] complex128
+        (f1, t2, f3), X = tds(E, J, lam)   # the 2-D spectrum
+
+    Each Liouville pathway (response.py:269-337, the pathways of
+    THIRD_ORDER_PATHWAYS[geometry]) is evaluated as the reference orders it:
+    the coherence rho1 = V0 rho0 propagated over t1, V1 applied at every t1,
+    those states propagated to the population times, V2 applied, and the
+    contraction with the t3 bra propagated in the Heisenberg picture.  Every
+    block is the differentiable Redfield block (qsx_redfield_block, reverse
+    mode qsx_redfield_block_vjp), every propagation the differentiable
+    one-step-propagator recursion (qsx_propagate, reverse mode
+    qsx_propagate_adjoint; blocks up to 64 elements, so up to five sites).
+    The forward values equal third_order_response / two_dimensional_spectra
+    of the RedfieldModel with the same parameters at the step dt
+    (two_dimensional_spectra: response.py:430-455, the t1 transform with
+    sign -1, the t3 transform with sign +1)."""
+
+    def __init__(self, dipoles, cutoff_freq, temperature, rw_freq, coherence_time_max,
+                 population_times, dt, geometry='-++', polarization='xxxx', include_signal=None,
+                 disorder=None, ensemble_size=None, random_seed=0, secular=False,
+                 discard_imag_corr=False, unit_convert=CM_FS, ckpt=64):
+        from .simulate.response import THIRD_ORDER_PATHWAYS, _parse_pathways
+        self.dipoles = np.asarray(dipoles, dtype=float)
+        self.N = self.dipoles.shape[0]
+        self.gamma, self.T = float(cutoff_freq), float(temperature)
+        self.rw_freq, self.dt = float(rw_freq), float(dt)
+        self.secular, self.discard_imag_corr = secular, discard_imag_corr
+        self.unit_convert, self.ckpt = unit_convert, ckpt
+        self.size = ensemble_size or 1
+        self.shifts = disorder_shifts(self.N, disorder if ensemble_size else None, self.size,
+                                      random_seed)
+        self.t1 = np.arange(0, coherence_time_max, dt)
+        self.t3 = self.t1
+        self.t2 = np.asarray(population_times, dtype=float)
+        steps2 = np.round(self.t2 / dt).astype(np.int64)
+        if np.any(np.abs(steps2 * dt - self.t2) > 1e-9 * max(1.0, dt)) or \
+                np.any(np.diff(steps2) < 0) or steps2[0] < 0:
+            raise ValueError('population times must be ascending multiples of dt')
+        self.nsub1 = np.ones(len(self.t1), dtype=np.int64)
+        self.nsub1[0] = 0
+        self.nsub2 = np.diff(np.concatenate([[0], steps2]))
+        dev = _native.device()
+        self.dev = dev
+        # dipole maps and the ground state in the site basis do not depend on
+        # theta
+        ham = ElectronicHamiltonian(np.eye(self.N), bath=DebyeBath(self.T, 1.0, self.gamma),
+                                    dipoles=self.dipoles)
+        model = RedfieldModel(ham, rw_freq=self.rw_freq, hilbert_subspace='gef',
+                              unit_convert=unit_convert)
+        rho0 = model.thermal_state('gg')
+        trans = geometry + '-'
+        dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.complex128)).to(dev)
+        self.paths = []
+        for path in _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal):
+            s = path.split('->')
+            V = [_op_matrices(model, '%s->%s' % (s[i], s[i + 1]), polarization[i], trans[i],
+                              'commutator')[0] for i in range(3)]
+            b3 = _op_matrices(model, '%s->%s' % (s[3], s[4]), polarization[3], trans[3],
+                              'bra')[0]
+            y1 = (V[0] @ rho0)[None, None, :]
+            self.paths.append(dict(
+                blocks=(s[1], s[2], s[3]),
+                y1=dv(np.broadcast_to(y1, (self.size, 1, y1.shape[-1]))),
+                V1T=dv(V[1].T), V2T=dv(V[2].T),
+                b3=dv(np.broadcast_to(b3[None, None, :], (self.size, 1, b3.shape[-1])))))
+
+    def hamiltonians(self, E, J):
+        H = site_hamiltonian(E, J)
+        return H[None] + torch.diag_embed(torch.as_tensor(self.shifts).to(H.device))
+
+    def _propagator(self, cache, H1, lam, block, heisenberg):
+        """P4(dt L) of Liouville block `block` ('eg', 'fe', ...) of every member."""
+        key = (block, heisenberg)
+        if key not in cache:
+            flags = ((_native.QSX_SECULAR if self.secular else 0)
+                     | (_native.QSX_REAL_CORR if self.discard_imag_corr else 0)
+                     | (_native.QSX_HEISENBERG if heisenberg else 0))
+            rw = torch.full((H1.shape[0],), self.rw_freq, dtype=torch.float64,
+                            device=H1.device)
+            L = redfield_block(H1.contiguous(), lam, self.gamma, self.T, rw,
+                               _BLOCK_INDEX[block[0]], _BLOCK_INDEX[block[1]], flags,
+                               self.unit_convert)
+            cache[key] = rk4_propagator(L, self.dt)
+        return cache[key]
+
+    def response(self, E, J, lam):
+        """Ensemble-mean third-order response [len(t1), len(t2), len(t3)]."""
+        H1 = self.hamiltonians(E, J)
+        B, n1, n2, n3 = self.size, len(self.t1), len(self.t2), len(self.t3)
+        cache, total = {}, None
+        for p in self.paths:
+            s1, s2, s3 = p['blocks']
+            P1 = self._propagator(cache, H1, lam, s1, False)
+            P2 = self._propagator(cache, H1, lam, s2, False)
+            P3 = self._propagator(cache, H1, lam, s3, True)
+            S1 = propagate(P1, p['y1'], self.nsub1, self.ckpt)[:, 0]           # [B, n1, M1]
+            y2 = torch.matmul(S1, p['V1T'])                                    # [B, n1, M2]
+            S2 = propagate(P2, y2.contiguous(), self.nsub2, self.ckpt)          # [B, n1, n2, M2]
+            y3 = torch.matmul(S2, p['V2T'])                                    # [B, n1, n2, M3]
+            G3 = propagate(P3, p['b3'], self.nsub1, self.ckpt)[:, 0]           # [B, n3, M3]
+            R = torch.matmul(y3.reshape(B, n1 * n2, -1),
+                             G3.transpose(1, 2).contiguous()).reshape(B, n1, n2, n3)
+            total = R if total is None else total + R
+        return total.mean(dim=0)
+
+    def __call__(self, E, J, lam):
+        R = self.response(E, J, lam)
+        f1, X = fourier_transform_device(self.t1, R, 0, self.rw_freq, self.unit_convert,
+                                         sign=-1)
+        f3, X = fourier_transform_device(self.t3, X, 2, self.rw_freq, self.unit_convert)
+        return (f1, self.t2, f3), X
