@@ -1,5 +1,6 @@
-"""Gradients of Redfield absorption spectra with respect to site energies,
-couplings and the reorganization energy (reverse mode on the GPU).
+"""Gradients of Redfield absorption spectra, third-order responses and 2-D
+spectra with respect to site energies, couplings and the reorganization
+energy (reverse mode on the GPU; RedfieldAbsorption, RedfieldThirdOrder).
 
 The reference computes spectra only (response.py:103-154); this module makes
 the same computation differentiable:
