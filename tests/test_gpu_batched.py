@@ -101,6 +101,28 @@ def test_ensemble_plan_matches_batch_and_oracle(gpu):
         assert rel_err(got[i], Xo) < 1e-10, i
 
 
+def test_ensemble_plan_per_bucket_builds(gpu):
+    """Models the one-launch operator builder does not take (eigen-basis
+    Redfield blocks here): one build per site count on the side streams and
+    one bucketed launch at eight steps per pass; every replay equals
+    absorption_batch (1e-12)."""
+    import torch
+    from qspectra_amd.batched import absorption_batch, AbsorptionEnsemblePlan
+    sizes = [7, 2, 24, 7, 13, 31, 5, 9]
+    _, hams = _plan_hams(sizes, 6)
+    kw = dict(evolve_basis='eigen', secular=False)
+    plan = AbsorptionEnsemblePlan(hams, 256, rw_freq=12500.0, dt=1.0, model_kwargs=kw)
+    assert plan._fused is None
+    f, Xb = absorption_batch(hams, 256, rw_freq=12500.0, dt=1.0, model_kwargs=kw)
+    assert np.allclose(plan.f, f)
+    for rep in range(2):
+        got = plan.run().cpu().numpy()
+        plan.check_status()
+        for i in range(len(hams)):
+            assert rel_err(got[i], Xb[i]) < 1e-12, (rep, i, sizes[i])
+        torch.ones(1000, device='cuda').sum().item()
+
+
 def test_ensemble_plan_rebuilds_from_new_hamiltonians(gpu):
     """New Hamiltonians written into a bucket's inputs in place: the replay
     rebuilds every operator and returns their spectra."""
