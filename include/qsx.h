@@ -232,7 +232,8 @@ int32_t qsx_propagate_chunks(int32_t precision, const void* Pc, int64_t strideP,
  * states are recomputed from checkpoints every ckpt steps held in the
  * workspace (qsx_adjoint_workspace_bytes; total_steps = sum of nsub).
  * status: 1 non-finite gradient, 2 workspace smaller than the checkpoints.
- * Blocks up to M = 64; ckpt * M * 16 bytes must fit the LDS.  The gradient
+ * Blocks up to M = 160 (above 64 the workspace also holds the operators'
+ * conjugate transposes); ckpt * M * 16 bytes must fit the LDS.  The gradient
  * path of the reference's spectra (absorption_spectra,
  * qspectra/simulate/response.py:103-154), which has no reverse mode. */
 int64_t qsx_adjoint_workspace_bytes(int32_t M, int32_t R, int32_t B, int64_t total_steps,

@@ -113,7 +113,7 @@ def propagate(A, y0, nsub, ckpt=64):
     """Differentiable propagation with the one-step propagator A [B|1, M, M]:
     states [B, R, len(nsub), M] (nsub: host step counts per save interval).
     The reverse pass recomputes the forward states from checkpoints every
-    ckpt steps (blocks up to M = 64)."""
+    ckpt steps (blocks up to M = 160)."""
     states, status = _Propagate.apply(A, y0, nsub, int(ckpt))
     if int(status.item()) != 0:
         raise FloatingPointError('propagation produced a non-finite state')
@@ -278,7 +278,7 @@ class RedfieldThirdOrder(object):
     block is the differentiable Redfield block (qsx_redfield_block, reverse
     mode qsx_redfield_block_vjp), every propagation the differentiable
     one-step-propagator recursion (qsx_propagate, reverse mode
-    qsx_propagate_adjoint; blocks up to 64 elements, so up to five sites).
+    qsx_propagate_adjoint; blocks up to 160 elements, so up to seven sites).
     The forward values equal third_order_response / two_dimensional_spectra
     of the RedfieldModel with the same parameters at the step dt
     (two_dimensional_spectra: response.py:430-455, the t1 transform with
@@ -318,7 +318,7 @@ class RedfieldThirdOrder(object):
                               unit_convert=unit_convert)
         rho0 = model.thermal_state('gg')
         trans = geometry + '-'
-        dv = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.complex128)).to(dev)
+        dv = lambda a: torch.as_tensor(np.array(a, dtype=np.complex128)).to(dev)
         self.paths = []
         for path in _parse_pathways(THIRD_ORDER_PATHWAYS[geometry], include_signal):
             s = path.split('->')

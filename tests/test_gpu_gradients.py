@@ -35,7 +35,8 @@ def _cpu_loop(A, y0, nsub):
 
 
 @pytest.mark.parametrize('M,B,R,shared', [(5, 3, 2, False), (7, 4, 1, True), (13, 2, 3, False),
-                                          (24, 2, 1, False), (49, 1, 2, False)])
+                                          (24, 2, 1, False), (49, 1, 2, False),
+                                          (100, 1, 2, False), (147, 2, 1, True)])
 def test_adjoint_kernel_matches_autograd(gpu, M, B, R, shared):
     rng = np.random.RandomState(M)
     A = _cplx(rng, 1 if shared else B, M, M)
@@ -58,19 +59,21 @@ def test_adjoint_kernel_matches_autograd(gpu, M, B, R, shared):
 
 
 def test_adjoint_rejects_oversized_block_and_short_workspace(gpu):
-    A = torch.zeros((1, 65, 65), dtype=torch.complex128, device='cuda')
-    y0 = torch.zeros((1, 1, 65), dtype=torch.complex128, device='cuda')
+    A = torch.zeros((1, 161, 161), dtype=torch.complex128, device='cuda')
+    y0 = torch.zeros((1, 1, 161), dtype=torch.complex128, device='cuda')
     n = torch.ones(2, dtype=torch.int32, device='cuda')
-    g = torch.zeros((1, 1, 2, 65), dtype=torch.complex128, device='cuda')
+    g = torch.zeros((1, 1, 2, 161), dtype=torch.complex128, device='cuda')
     with pytest.raises(_native.NativeError):
         _native.propagate_adjoint(A, y0, n, 2, g)
-    # checkpoints that do not fit the workspace set status 2 (no access beyond it)
-    A = torch.eye(4, dtype=torch.complex128, device='cuda')[None]
-    y0 = torch.ones((1, 1, 4), dtype=torch.complex128, device='cuda')
-    n = torch.tensor([0, 40, 40], dtype=torch.int32, device='cuda')
-    g = torch.ones((1, 1, 3, 4), dtype=torch.complex128, device='cuda')
-    _, _, st = _native.propagate_adjoint(A, y0, n, 8, g, ckpt=4)   # sized for 8 steps
-    assert int(st.item()) == 2
+    # checkpoints that do not fit the workspace set status 2 (no access beyond
+    # it), in the register-resident and in the large-block kernel
+    for M in (4, 100):
+        A = torch.eye(M, dtype=torch.complex128, device='cuda')[None]
+        y0 = torch.ones((1, 1, M), dtype=torch.complex128, device='cuda')
+        n = torch.tensor([0, 40, 40], dtype=torch.int32, device='cuda')
+        g = torch.ones((1, 1, 3, M), dtype=torch.complex128, device='cuda')
+        _, _, st = _native.propagate_adjoint(A, y0, n, 8, g, ckpt=4)   # sized for 8 steps
+        assert int(st.item()) == 2, M
 
 
 def test_rk4_propagator_vjp(gpu):
