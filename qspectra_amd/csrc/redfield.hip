@@ -282,6 +282,60 @@ __device__ double2 matsubara_sum(double x, double T, double gam) {
   return matsubara_partial(x, T, gam, 1, 1);
 }
 
+// digamma of a complex argument away from the poles 0, -1, -2, ...: the
+// recurrence psi(z) = psi(z + 1) - 1/z until Re z >= 20 (at most 4096
+// steps), then the asymptotic series ln z - 1/(2z) - sum_k B_2k / (2k z^2k)
+// through k = 6 (the first omitted term is below 1e-19 there)
+__device__ double2 digamma_c(double zr, double zi) {
+  double sr = 0.0, si = 0.0;
+  for (int i = 0; i < 4096 && zr < 20.0; ++i) {
+    const double d = 1.0 / (zr * zr + zi * zi);
+    sr -= zr * d;
+    si += zi * d;
+    zr += 1.0;
+  }
+  const double d = 1.0 / (zr * zr + zi * zi);
+  const double wr = zr * d, wi = -zi * d;                     // w = 1/z
+  const double w2r = wr * wr - wi * wi, w2i = 2.0 * wr * wi;  // w^2
+  double tr = 691.0 / 32760.0, ti = 0.0;
+  const double cf[5] = {1.0 / 132.0, 1.0 / 240.0, 1.0 / 252.0, 1.0 / 120.0, 1.0 / 12.0};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {                                // t <- c_k - w^2 t
+    const double ur = cf[k] - (w2r * tr - w2i * ti);
+    const double ui = -(w2r * ti + w2i * tr);
+    tr = ur;
+    ti = ui;
+  }
+  const double fr = w2r * tr - w2i * ti, fi = w2r * ti + w2i * tr;   // w^2 t
+  return make_double2(sr + 0.5 * log(zr * zr + zi * zi) - 0.5 * wr - fr,
+                      si + atan2(zi, zr) - 0.5 * wi - fi);
+}
+
+// p = gam / (2 pi T) inside which matsubara_closed is used (relative error
+// below 1e-13 of the summed form there; outside, the sum)
+__device__ __forceinline__ bool matsubara_closed_ok(double T, double gam) {
+  const double p = gam / (2.0 * M_PI * T);
+  return T > 0.0 && p >= 0.01 && p <= 32.0;
+}
+
+// The same sum in closed form: with nu_k = a k, a = 2 pi T, p = gam / a and
+// q = i x / a, each term is (k / a^2) / ((k - p)(k + p)(k - q)); in partial
+// fractions A/(k - p) + B/(k + p) + C/(k - q) with A = 1/(2(p - q)),
+// B = -1/(2(p + q)), C = q/(q^2 - p^2), and sum_{k=1}^{999} 1/(k - z) =
+// psi(1000 - z) - psi(1 - z).  dm and dp are the brackets at z = p and
+// z = -p (real, per problem).
+__device__ double2 matsubara_closed(double x, double T, double dm, double dp, double gam) {
+  const double a = 2.0 * M_PI * T, p = gam / a, y = x / a;
+  const double r2 = p * p + y * y;
+  const double Ar = 0.5 * p / r2, Ai = 0.5 * y / r2;          // 1/(2(p - iy))
+  const double Br = -0.5 * p / r2, Bi = 0.5 * y / r2;         // -1/(2(p + iy))
+  const double Ci = -y / r2;                                  // iy / (-y^2 - p^2)
+  const double2 hi = digamma_c(1000.0, -y), lo = digamma_c(1.0, -y);
+  const double Dr = hi.x - lo.x, Di = hi.y - lo.y;
+  const double inv = 1.0 / (a * a);
+  return make_double2((Ar * dm + Br * dp - Ci * Di) * inv, (Ai * dm + Bi * dp + Ci * Dr) * inv);
+}
+
 // C(x) given the Matsubara sum S(x) (unused for the real correlation
 // function and at x = 0)
 __device__ double2 debye_corr_with(double x, double2 S, double T, double lam, double gam,
@@ -767,10 +821,18 @@ __global__ __launch_bounds__(NT) void absorption_ops_kernel(
   const double T = bath[3 * b], lam = bath[3 * b + 1], gam = bath[3 * b + 2];
   const int real_corr = (flags & kRealCorr) != 0;
   const int npairs = n * (n + 1) / 2;
-  // threads per pair: the power of two (at most 64) with the fewest terms
-  // per thread over the rounds of NT threads
+  // the closed form where it is accurate (one thread per pair); else threads
+  // per pair: the power of two (at most 64) with the fewest terms per thread
+  // over the rounds of NT threads
+  const bool closed = !real_corr && matsubara_closed_ok(T, gam);
+  double dm = 0.0, dp = 0.0;
+  if (closed) {
+    const double pg = gam / (2.0 * M_PI * T);
+    dm = digamma_c(1000.0 - pg, 0.0).x - digamma_c(1.0 - pg, 0.0).x;
+    dp = digamma_c(1000.0 + pg, 0.0).x - digamma_c(1.0 + pg, 0.0).x;
+  }
   int tpp = 1, cost = 1 << 30;
-  for (int t = 1; t <= 64; t *= 2) {
+  for (int t = 1; t <= (closed ? 1 : 64); t *= 2) {
     const int c = ((npairs * t + NT - 1) / NT) * ((999 + t - 1) / t);
     if (c < cost) {
       cost = c;
@@ -790,7 +852,8 @@ __global__ __launch_bounds__(NT) void absorption_ops_kernel(
       }
       c = d + r;
       x = A[d * m + d] - A[c * m + c];
-      if (!real_corr && x != 0.0) S = matsubara_partial(x, T, gam, 1 + part, tpp);
+      if (!real_corr && x != 0.0)
+        S = closed ? matsubara_closed(x, T, dm, dp, gam) : matsubara_partial(x, T, gam, 1 + part, tpp);
     }
     for (int o = tpp / 2; o > 0; o >>= 1) {                  // within aligned groups of tpp lanes
       S.x += __shfl_xor(S.x, o);

@@ -222,20 +222,32 @@ def test_response_buckets_several_steps_per_pass(gpu):
             assert rel_err(got[i], ref[i]) < 1e-12, (J, i, sizes[i])
 
 
-@pytest.mark.parametrize('secular,real', [(False, False), (True, False), (False, True)])
-def test_absorption_operators_match_block_builder(gpu, secular, real):
+# baths of gamma / (2 pi T) = 0.315 and 28.6 (the fused builder's Matsubara
+# sum in closed form, through digamma differences) and 0.0038 (the summed
+# form, outside the closed form's window)
+NARROW_WARM = (S.CM_K * 300, 35.0, 5.0)
+BROAD_COLD = (S.CM_K * 4, 35.0, 500.0)
+
+
+@pytest.mark.parametrize('secular,real,bath', [(False, False, BATH), (True, False, BATH),
+                                               (False, True, BATH), (False, False, NARROW_WARM),
+                                               (False, False, BROAD_COLD)])
+def test_absorption_operators_match_block_builder(gpu, secular, real, bath):
     """qsx_absorption_operators (every member of every size in one launch)
     equals the per-size path: qsx_redfield_block's Heisenberg 'eg' block and
     qsx_small_propagator's P4(h L) and its fourth power (1e-12), for 1..31
-    sites, secular or not, complex or real correlation function; a launch
-    whose max_n is below some sizes leaves those problems' outputs alone."""
+    sites, secular or not, complex or real correlation function, with the
+    Matsubara sum in closed form or summed; a launch whose max_n is below some
+    sizes leaves those problems' outputs alone."""
     import torch
     sizes = [1, 2, 5, 7, 13, 24, 31]
     rw, dt, J = 12500.0, 1.0, 4
     table, H1s, rws, baths, ref1, refJ = [], [], [], [], [], []
     oh = op = 0
     for n in sizes:
-        _, hams = _plan_hams([n] * 3, 11 + n)
+        systems = _aggregates([n] * 3, seed=11 + n)
+        hams = [qa.ElectronicHamiltonian(H, bath=qa.DebyeBath(*bath), dipoles=d)
+                for H, d in systems]
         hams = [h.in_rotating_frame(rw) for h in hams]
         m = qa.RedfieldModel(hams[0], rw_freq=rw, hilbert_subspace='ge', unit_convert=qa.CM_FS,
                              secular=secular, discard_imag_corr=real).with_hamiltonians(hams)
@@ -252,7 +264,7 @@ def test_absorption_operators_match_block_builder(gpu, secular, real):
         flags = m._flags(True)
         for k in range(3):
             table.append((n, oh + k * n * n, op + k * n * n))
-            baths.append(BATH)
+            baths.append(bath)
         oh += 3 * n * n
         op += 3 * n * n
     dev = 'cuda'
